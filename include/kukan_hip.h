@@ -1,0 +1,242 @@
+/*
+ * kukan_hip.h — C-ABI of libkukan_hip.so, the MI355X (gfx950) implementation of the
+ * RSPACE "kukan" v8f real-space Kohn-Sham hot path.
+ *
+ * Every entry point named kukan_<routine> replaces the Fortran routine <routine> of the
+ * reference with the same argument list, order and meaning (cited per function as
+ * file:line under RSPACEes.v08.01/kukan/).  The Fortran shim modules in
+ * rspacedist_amd/fortran/ keep the reference module and routine names and forward here.
+ *
+ * Conventions
+ *  - Scalars are passed by value: Fortran `integer` = int32_t, `real*8` = double,
+ *    `logical` = int (0 = .false., nonzero = .true.).  Output scalars are pointers.
+ *  - Arrays are Fortran explicit-shape, column-major, first index fastest, passed as a
+ *    pointer to their first element.  Index lists (lstvec2) stay 1-based as in Fortran.
+ *  - An array pointer may be (a) device memory (hipMalloc / torch tensor): used in place,
+ *    (b) host memory registered with kukan_bind(): the device mirror is used and stays
+ *    authoritative until kukan_sync_to_host(), (c) any other host memory: copied to the
+ *    device on entry and (when the routine writes it) back on exit.
+ *  - Return value: 0 on success, nonzero error code; kukan_last_error() gives the message,
+ *    whose text matches the reference's `stopp` message for the same condition.
+ *  - The library is single-GPU: every MPI reduction of the reference is an identity here,
+ *    and the shim requires nprocx = nprocy = nprocz = nprock = 1.
+ */
+#ifndef KUKAN_HIP_H
+#define KUKAN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ library control */
+
+/* Select the device and create the library stream and BLAS/solver handles.
+ * No reference counterpart (the reference has no device); called once by the host. */
+int kukan_init(int device);
+int kukan_finalize(void);
+const char *kukan_last_error(void);
+/* Tunables outside the reference signatures.  Keys:
+ *   "kscg_mode"   0 = band-sequential CG exactly as kscg8f.F90:98 (default),
+ *                 1 = block CG (all bands advance together; subspace work as GEMMs)
+ *   "precision"   0 = FP64 everywhere (default), 1 = split-FP16 MFMA subspace GEMMs with
+ *                 FP32 accumulation inside K-chunks and FP64 across chunks
+ *   "verbose"     0/1: per-band report lines like the reference's unit ndisp output  */
+int kukan_set_param(const char *key, double value);
+double kukan_get_param(const char *key);
+/* Device-side stream all entry points run on (hipStream_t cast to void*). */
+void *kukan_stream(void);
+
+/* ------------------------------------------------------------------ residency */
+
+/* Create a device mirror of host array [host_ptr, host_ptr+bytes) and copy it in.
+ * Replaces nothing in the reference: it keeps the Fortran-owned arrays of
+ * dim_global8f.F90:28-45 resident in HBM across calls. */
+int kukan_bind(void *host_ptr, size_t bytes);
+int kukan_unbind(void *host_ptr);            /* copies back, then frees the mirror */
+int kukan_sync_to_host(void *host_ptr);      /* device mirror -> host */
+int kukan_sync_to_device(void *host_ptr);    /* host -> device mirror */
+
+/* ------------------------------------------------------------------ leaf routines
+ * The reference calls these from inside OpenMP parallel regions; the library exports
+ * them with single-call semantics for unit tests and for hosts that call them serially. */
+
+/* tools8f.F90:144-237 tools_definefdcoef: acfd[0..8] = central 2nd-derivative weights. */
+int kukan_tools_definefdcoef(int nf, double *acfd);
+
+/* overlap_finitedifference8f.F90:36-324 overlap_finitedifference_r followed by :648-827
+ * overlap_fdcheck_r on one rank: periodic wrap of nsnd face layers in each periodic
+ * direction (x if nperi>=1, y if >=2, z if >=3).  v_cc(-nf1:ncpa+nf, -nf1:ncpb+nf, -nf1:ncpc+nf). */
+int kukan_overlap_finitedifference_r(int nperi, int ncpa, int ncpb, int ncpc, int nf, int nf1,
+                                     int nsnd, double *v_cc);
+int kukan_overlap_fdcheck_r(int nperi, int ncpa, int ncpb, int ncpc, int nf, int nf1, int nsnd,
+                            double *v_cc);
+
+/* kslaplacian8f.F90:23-557 kslaplacian_r:
+ * avre(:,:,:,l) += (-1/2 lap_nf + veff) vre, vre(-(nf-1):ncpx+nf, ...) with filled halo. */
+int kukan_kslaplacian_r(int ncpx, int ncpy, int ncpz, int neigmx, int nf, double dx, double dy,
+                        double dz, const double *vre, const double *veff, int l, double *avre);
+
+/* ksprecondition8f.F90:8-39 ksprecondition: avre = 1/2 vre + 1/12 sum of 6 neighbours. */
+int kukan_ksprecondition(int ncpx, int ncpy, int ncpz, int nf, int ncol, const double *vre,
+                         double *avre);
+
+/* nonlocaloperation8f.F90:9-53 nonlocaloperation_r_01: rpsep(i,a) += sum_k vnlocp(k,i,a) vre2(lstvec2(k,a)). */
+int kukan_nonlocaloperation_r_01(int natom, int nprjmx, int num_spe, int num_ppcell, int num_list,
+                                 int ncpx, int ncpy, int ncpz, int key_natpri_in,
+                                 int key_natpri_inps, const int *nprj, const int *indspe,
+                                 const int *natinf, const int *natpri, const int *naps,
+                                 const int *lstvec2, const double *vnlocp, const double *vre2,
+                                 double *rpsep);
+
+/* nonlocaloperation8f.F90:149-235 nonlocaloperation_r_02:
+ * avre(lstvec2(k,a)) += sum_j vnlocp(k,j,a) [sum_i rpsep(i,a) X(i,j)] / dV,
+ * X = dijsss(:,:,ns,a) (nhs=1, H) or dijsss(:,:,1,species) (nhs=0, S).  avr is workspace. */
+int kukan_nonlocaloperation_r_02(int nhs, int natom, int num_spe, int nums, int nprjmx, int ns,
+                                 int num_list, int num_ppcell, int ncpx, int ncpy, int ncpz,
+                                 int key_natpri_in, int key_natpri_inps, double dx, double dy,
+                                 double dz, const int *nprj, const int *indspe, const int *natinf,
+                                 const int *lstvec2, const int *natpri, const int *naps,
+                                 const double *vnlocp, const double *rpsep, const double *dijsss,
+                                 double *avre, double *avr);
+
+/* ------------------------------------------------------------------ coarse routines
+ * These are the drop-in entry points: each is called once per use from one host thread. */
+
+/* kscg8f.F90:7-489 kscg_r: preconditioned CG band update with 2x2 Rayleigh-Ritz steps.
+ * ksconv (logical, inout), ksitmax (inout).  kscg_mode selects band-sequential / block. */
+int kukan_kscg_r(int natom, int num_spe, int neigmx, int nums, int nspv, int numk, int nperi,
+                 int northo, int ns, int nk, int nf, int nkscg, int nsdmax, int nprecon_cg,
+                 int nprjmx, int num_list, int num_ppcell, int ndisp, int ncpx, int ncpy, int ncpz,
+                 int key_ortho_nocmpt_innerproduct, int key_ortho_cmpt_innerproduct,
+                 int key_natpri_in, int key_natpri_inps, int key_natpri_out, double dx, double dy,
+                 double dz, double epssd, const int *indspe, const int *natpri, const int *naps,
+                 const int *nprj, const int *natinf, const int *lstvec2, const int *latom,
+                 const int *ntyppp, const double *dij, const double *veff, const double *vnlocp,
+                 const double *sss, int *ksconv, int *ksitmax, double *sval, double *rspsep,
+                 double *svecre, double *ssvre, double *residual_states, double *hsvre);
+
+/* ksdiis8f.F90:8-712 ksdiis_r: RMM-DIIS band update (bands are independent, so all bands
+ * of a pass run together on the device with the reference's per-band arithmetic). */
+int kukan_ksdiis_r(int l_hsv, int natom, int num_spe, int neigmx, int nums, int nspv, int numk,
+                   int nperi, int ndiismax, int nprecon_diis, int nk, int ns, int nf, int nprjmx,
+                   int num_list, int num_ppcell, int ndisp, int northo, int ncpx, int ncpy,
+                   int ncpz, int key_ortho_cmpt_innerproduct, int key_natpri_in,
+                   int key_natpri_inps, double eps_eig_diis, double alambda_diis,
+                   double ratio_diis, double alambda_min, double alambda_max, double epssd,
+                   double xmax, double ymax, double zmax, const int *indspe, const int *natpri,
+                   const int *naps, const int *nprj, const int *natinf, const int *lstvec2,
+                   const int *latom, const int *ntyppp, const double *dij, const double *veff,
+                   const double *vnlocp, const double *sss, int *ksconv, int *ksitmax,
+                   double *rspsep, double *sval, double *svecre, double *ssvre,
+                   double *residual_states, double *hsvre);
+
+/* orthogonalization8f.F90:8-44 orthogonalization_r: S-orthonormalise all bands, northo
+ * passes.  Implemented as Cholesky-QR (same Q as the reference's Gram-Schmidt). */
+int kukan_orthogonalization_r(int natom, int neigmx, int nprjmx, int nums, int numk,
+                              int num_ppcell, int num_spe, int northo, int num_list, int nflag,
+                              int ndisp, int ncpx, int ncpy, int ncpz,
+                              int key_ortho_cmpt_innerproduct, int key_natpri_in,
+                              int key_natpri_inps, double dx, double dy, double dz,
+                              const int *nprj, const int *indspe, const int *natpri,
+                              const int *naps, const int *natinf, const int *lstvec2,
+                              const int *latom, const int *ntyppp, const double *vnlocp,
+                              const double *sss, double *svecre, double *ssvre, double *rspsep);
+
+/* orthogonalization8f.F90:93-376 orthogonalization_r_01: one pass for bands l_inp (or all
+ * bands when l_inp = 0) of spin ns, k-point nk. */
+int kukan_orthogonalization_r_01(int natom, int neigmx, int nprjmx, int nums, int numk, int nk,
+                                 int ns, int l_inp, int num_list, int num_ppcell, int num_spe,
+                                 int nflag, int ndisp, int ncpx, int ncpy, int ncpz,
+                                 int key_ortho_cmpt_innerproduct, int key_natpri_in,
+                                 int key_natpri_inps, double dx, double dy, double dz,
+                                 const int *nprj, const int *indspe, const int *natpri,
+                                 const int *naps, const int *natinf, const int *lstvec2,
+                                 const int *latom, const double *vnlocp, const double *sss,
+                                 double *svecre, double *ssvre, double *rspsep);
+
+/* rayleigh-ritz8f.F90:22-125 rayleighritz_r (same signature as rayleigh-ritz8f_sclpck.F90:127-138):
+ * A = dV Psi^T H Psi, symmetric eigensolve, rotate Psi, S Psi, <p|psi>.  On exit hsvre
+ * holds S Psi rotated, as in the reference. */
+int kukan_rayleighritz_r(int natom, int num_ppcell, int num_spe, int nprjmx, int neigmx, int numk,
+                         int nums, int nspv, int num_list, int ncpx, int ncpy, int ncpz,
+                         int nperi, int nk, int ns, int nf, int ndisp, int key_natpri_in,
+                         int key_natpri_inps, double dx, double dy, double dz, const int *nprj,
+                         const int *natinf, const int *lstvec2, const int *indspe,
+                         const int *natpri, const int *naps, const int *ntyppp, const double *dij,
+                         const double *veff, const double *vnlocp, int l_rrb, double *rspsep,
+                         double *svecre, double *ssvre, double *hsvre, double *sval);
+/* rayleigh-ritz8f.F90:9-19: BLACS set-up and tear-down, no-ops on one GPU. */
+int kukan_rayleighritz_blacs_ini(int neigmx, int nrc);
+int kukan_rayleighritz_blacs_fin(void);
+
+/* reorderstates8f.F90:8-124 reorderstates (real path nrc = 0): stable rank sort by sval. */
+int kukan_reorderstates(int nrc, int num_ppcell, int nprjmx, int neigmx, int nums, int ncol,
+                        int numk, int ncpx, int ncpy, int ncpz, int ns1, int nk, double *sval,
+                        double *residual_states, double *rspsep, void *cspsep, double *svecre,
+                        void *sveccm, double *ssvre, void *ssvcm, double *hsvre, void *hsvcm);
+
+/* scf_diag8f.F90:6-316 scf_diag, real path (nrc = 0, nso = 0, ncol = 1): optional RR
+ * before (nrrz > 0), CG or DIIS, optional RR after (nrrz < 0), optional reorder. */
+int kukan_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, int iscf,
+                   int ncgscf, int nretcg, int natom, int neigmx, int nums, int ncol, int nspv,
+                   int numk, int nperi, int northo, int nf, int ndisp, int num_list, int ncpx,
+                   int ncpy, int ncpz, int npxmax, int npymax, int npzmax, int nchange, int nrrz,
+                   int key_ortho_nocmpt_innerproduct, int key_ortho_cmpt_innerproduct,
+                   int key_natpri_in, int key_natpri_inps, int key_natpri_out, int key_soc_calc,
+                   int key_pp_paw, const double *skpxx, const double *skpyy, const double *skpzz,
+                   double dx, double dy, double dz, double xmax, double ymax, double zmax,
+                   const double *dij, const double *dijsoc, const double *veff,
+                   const double *vnlocp, const double *sss, int nkscg, int nsdmax, int nprecon_cg,
+                   double epssd, int ndiismax, int nprecon_diis, double eps_eig_diis,
+                   double alambda_diis, double ratio_diis, double alambda_min,
+                   double alambda_max, const int *indspe, const int *natpri, const int *naps,
+                   const int *nprj, const int *natinf, const int *lstvec2, const int *latom,
+                   const int *natsoc, const int *ntyppp, const int *lstx, const int *lsty,
+                   const int *lstz, const int *natx, const int *naty, const int *natz,
+                   double *rspsep, void *cspsep, double *svecre, void *sveccm, double *hsvre,
+                   void *hsvcm, double *ssvre, void *ssvcm, double *sval, double *svalsoc,
+                   void *cspsepsoc, void *svecsoc, double *residual_states, int *ksconv,
+                   int *ksitmax);
+
+/* scf_hartree8f.F90:11-611 scf_hartree (+ scf_hartree_cg :626-812): solve
+ * -1/2 lap_nfh V = 2 pi rho by CG, warm-started from vh_dense, until
+ * sqrt(sum r^2 dV / Omega) < epsvh.  Periodic (nperi = 3) and isolated cells with
+ * Dirichlet boundary planes boundx/y/z (nd = 0) are supported. */
+int kukan_scf_hartree(int nd, int ncpx_d, int ncpy_d, int ncpz_d, int nfh, int ndisp, int nperi,
+                      int ncgres, int ncgmin, int ncgmax, double xmax, double ymax, double zmax,
+                      double epsvh, const double *rho_aug_dense, const double *boundx,
+                      const double *boundy, const double *boundz, double *vh_dense);
+
+/* tools8f.F90:354-378 tools_correctpotential: vh_dense -= its cell average. */
+int kukan_tools_correctpotential(int ncpx_d, int ncpy_d, int ncpz_d, double xmax, double ymax,
+                                 double zmax, double ddx, double ddy, double ddz,
+                                 double *vh_dense);
+
+/* scf_charge8f.F90:290-321 scf_charge_smt_r: rhosmt(:,ns) = sum_{k,l} fnele psi^2. */
+int kukan_scf_charge_smt_r(int neigmx, int nums, int nspv, int numk, int ncpx, int ncpy, int ncpz,
+                           const double *fnele, const double *svecre, double *rhosmt);
+
+/* force8f.F90:9-195 force_eig_r: nonlocal band forces
+ * F_a = -2 sum_n f_n sum_ij <p_i|psi_n> <p_j|d psi_n> (D_ij - eps_n S_ij), d = FD order nf. */
+int kukan_force_eig_r(int nperi, int numk, int nums, int neigmx, int nf, int ncpx, int ncpy,
+                      int ncpz, int num_list, int natom, int num_spe, int num_ppcell, int nprjmx,
+                      int key_natpri_in, int key_natpri_inps, const int *lstvec2,
+                      const int *latom, const int *lstx, const int *lsty, const int *lstz,
+                      const int *nprj, const int *natpri, const int *naps, const int *indspe,
+                      const int *natinf, double xmax, double ymax, double zmax,
+                      const double *sss, const double *vnlocp, const double *dij,
+                      const double *svecre, const double *sval, const double *fnele,
+                      double *fatx, double *faty, double *fatz);
+
+/* totalenergy8f.F90:151-199 totalenergy_eig: band energy and Fermi smearing entropy term. */
+int kukan_totalenergy_eig(int nums, int ncol, int numk, int neigmx, const int *nwskk,
+                          int nwskptot, double tf, const double *sval, const double *fnele,
+                          double *eneel, double *eneth);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KUKAN_HIP_H */

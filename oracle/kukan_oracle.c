@@ -1,0 +1,1499 @@
+/*
+ * kukan_oracle.c — FP64 CPU restatement of the RSPACE kukan v8f Kohn-Sham hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by the tests, by the smoke check and by
+ * the CPU-baseline leg of bench.py; never by the product library.
+ *
+ * Every routine follows the reference routine cited above it (file:line relative to
+ * RSPACEes.v08.01/kukan/), with the same loop order and summation order, on one MPI rank
+ * (all mpi_allreduce calls are identities).  OpenMP is used only inside data-parallel
+ * loops and in dot products (reduction clauses), mirroring the reference's orphaned
+ * `!$omp do` work-sharing; run with OMP_NUM_THREADS=1 for a fixed summation order.
+ *
+ * Parity status of each function (what pins it) is listed in the tests and in the
+ * oracle-pinning table of the design notes: closed forms (FD weights, plane waves,
+ * FFT Poisson), brute-force dense eigensolves, invariants.  The exact iterate sequence of
+ * orc_kscg_r / orc_ksdiis_r is not pinned by any reference output (the reference ships no
+ * tests or fixtures); their converged eigenvalues are pinned by brute force.
+ */
+#include "kukan_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static char g_err[512];
+static int fail(int code, const char *msg) {
+  snprintf(g_err, sizeof g_err, "%s", msg);
+  return code;
+}
+const char *orc_last_error(void) { return g_err; }
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------ small helpers */
+
+static void *xcalloc(size_t n, size_t sz) {
+  void *p = calloc(n ? n : 1, sz);
+  if (!p) {
+    fprintf(stderr, "kukan_oracle: out of memory (%zu x %zu)\n", n, sz);
+    abort();
+  }
+  return p;
+}
+
+/* dot over n doubles (orthogonalization_r_05 / kscg_r_01 pattern, orthogonalization8f.F90:797-811) */
+static double dot(long n, const double *a, const double *b) {
+  double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (long i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+/* Halo'd array index, lower bound -nf1, extent n+nf+nf1+1 per axis. */
+#define HX(ix, iy, iz) \
+  ((size_t)((ix) + nf1) + (size_t)ex * ((size_t)((iy) + nf1) + (size_t)ey * (size_t)((iz) + nf1)))
+#define IX3(ix, iy, iz) ((size_t)((ix)-1) + (size_t)ncpx * ((size_t)((iy)-1) + (size_t)ncpy * (size_t)((iz)-1)))
+
+/* ------------------------------------------------------------------ tools8f.F90:144-237 */
+int orc_tools_definefdcoef(int nf, double *a) {
+  static const double t[9][9] = {
+      {0},
+      {-2.0, 1.0},
+      {-5.0 / 2.0, 4.0 / 3.0, -1.0 / 12.0},
+      {-49.0 / 18.0, 3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0},
+      {-205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0},
+      {-36883.0 / 12600.0, 5.0 / 3.0, -5.0 / 21.0, 5.0 / 126.0, -5.0 / 1008.0, 1.0 / 3150.0},
+      {-5369.0 / 1800.0, 12.0 / 7.0, -15.0 / 56.0, 10.0 / 189.0, -1.0 / 112.0, 2.0 / 1925.0,
+       -1.0 / 16632.0},
+      {-3.0235941043083900, 1.7500000000000000, -0.2916666666666667, 6.4814814814814815e-2,
+       -1.3257575757575758e-2, 2.1212121212121212e-3, -2.2662522662522663e-4,
+       1.1892869035726179e-5},
+      {-3.0548441043083900, 1.7777777777777778, -0.3111111111111111, 7.5420875420875421e-2,
+       -1.7676767676767677e-2, 3.4809634809634810e-3, -5.1800051800051800e-4,
+       5.0742907885765029e-5, -2.4281274281274281e-6}};
+  for (int m = 0; m < 9; m++) a[m] = 0.0;
+  if (nf < 1 || nf > 8) return 0; /* the reference leaves the weights untouched (zero here) */
+  for (int m = 0; m < 9; m++) a[m] = t[nf][m];
+  return 0;
+}
+
+/* ------------------------------------------------------------------ halo
+ * overlap_finitedifference8f.F90:36-324 (pack) + :648-827 (unpack), one rank per axis:
+ * each periodic direction copies its own opposite faces (local wrap, :97-111); faces only,
+ * interior ranges of the other two axes; isolated directions are left untouched. */
+int orc_overlap_finitedifference_r(int nperi, int na, int nb, int nc, int nf, int nf1, int nsnd,
+                                   double *v) {
+  const long ex = na + nf + nf1 + 1, ey = nb + nf + nf1 + 1;
+  if (nperi >= 1)
+    for (int ic = 1; ic <= nc; ic++)
+      for (int ib = 1; ib <= nb; ib++)
+        for (int iov = 1; iov <= nsnd; iov++) {
+          double lo = v[HX(na - nsnd + iov, ib, ic)], hi = v[HX(iov, ib, ic)];
+          v[HX(-nsnd + iov, ib, ic)] = lo;
+          v[HX(na + iov, ib, ic)] = hi;
+        }
+  if (nperi >= 2)
+    for (int ic = 1; ic <= nc; ic++)
+      for (int iov = 1; iov <= nsnd; iov++)
+        for (int ia = 1; ia <= na; ia++) {
+          double lo = v[HX(ia, nb - nsnd + iov, ic)], hi = v[HX(ia, iov, ic)];
+          v[HX(ia, -nsnd + iov, ic)] = lo;
+          v[HX(ia, nb + iov, ic)] = hi;
+        }
+  if (nperi >= 3)
+    for (int iov = 1; iov <= nsnd; iov++)
+      for (int ib = 1; ib <= nb; ib++)
+        for (int ia = 1; ia <= na; ia++) {
+          double lo = v[HX(ia, ib, nc - nsnd + iov)], hi = v[HX(ia, ib, iov)];
+          v[HX(ia, ib, -nsnd + iov)] = lo;
+          v[HX(ia, ib, nc + iov)] = hi;
+        }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ kslaplacian8f.F90:23-557
+ * Three sweeps in the reference order: x neighbours + centre (with veff), then y, then z,
+ * each a read-modify-write of avre(:,:,:,l); terms added left to right as written there
+ * (offset -nf .. -1, centre, +1 .. +nf). */
+int orc_kslaplacian_r(int ncpx, int ncpy, int ncpz, int neigmx, int nf, double dx, double dy,
+                      double dz, const double *vre, const double *veff, int l, double *avre) {
+  (void)neigmx;
+  if (nf < 1 || nf > 8) return fail(1, "error in kslaplacian_r! This subroutine is for nf in 1-8.");
+  double acfd[9], xx[9], yy[9], zz[9];
+  orc_tools_definefdcoef(nf, acfd);
+  for (int m = 0; m < 9; m++) {
+    xx[m] = acfd[m] / dx / dx * (-0.5);
+    yy[m] = acfd[m] / dy / dy * (-0.5);
+    zz[m] = acfd[m] / dz / dz * (-0.5);
+  }
+  const int nf1 = nf - 1;
+  const long ex = ncpx + 2 * nf, ey = ncpy + 2 * nf;
+  double *out = avre + (size_t)(l - 1) * ncpx * ncpy * ncpz;
+  const double c0 = xx[0] + yy[0] + zz[0];
+#pragma omp parallel for schedule(static)
+  for (int iz = 1; iz <= ncpz; iz++)
+    for (int iy = 1; iy <= ncpy; iy++)
+      for (int ix = 1; ix <= ncpx; ix++) {
+        double s = out[IX3(ix, iy, iz)];
+        for (int m = nf; m >= 1; m--) s += xx[m] * vre[HX(ix - m, iy, iz)];
+        s += (c0 + veff[IX3(ix, iy, iz)]) * vre[HX(ix, iy, iz)];
+        for (int m = 1; m <= nf; m++) s += xx[m] * vre[HX(ix + m, iy, iz)];
+        out[IX3(ix, iy, iz)] = s;
+      }
+#pragma omp parallel for schedule(static)
+  for (int iz = 1; iz <= ncpz; iz++)
+    for (int iy = 1; iy <= ncpy; iy++)
+      for (int ix = 1; ix <= ncpx; ix++) {
+        double s = out[IX3(ix, iy, iz)];
+        for (int m = nf; m >= 1; m--) s += yy[m] * vre[HX(ix, iy - m, iz)];
+        for (int m = 1; m <= nf; m++) s += yy[m] * vre[HX(ix, iy + m, iz)];
+        out[IX3(ix, iy, iz)] = s;
+      }
+#pragma omp parallel for schedule(static)
+  for (int iz = 1; iz <= ncpz; iz++)
+    for (int iy = 1; iy <= ncpy; iy++)
+      for (int ix = 1; ix <= ncpx; ix++) {
+        double s = out[IX3(ix, iy, iz)];
+        for (int m = nf; m >= 1; m--) s += zz[m] * vre[HX(ix, iy, iz - m)];
+        for (int m = 1; m <= nf; m++) s += zz[m] * vre[HX(ix, iy, iz + m)];
+        out[IX3(ix, iy, iz)] = s;
+      }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ ksprecondition8f.F90:8-39 */
+int orc_ksprecondition(int ncpx, int ncpy, int ncpz, int nf, int ncol, const double *vre,
+                       double *avre) {
+  const int nf1 = nf - 1;
+  const long ex = ncpx + 2 * nf, ey = ncpy + 2 * nf;
+  const double w = 1.0 / 12.0;
+  const size_t hstride = (size_t)ex * ey * (ncpz + 2 * nf), n = (size_t)ncpx * ncpy * ncpz;
+  for (int is = 0; is < ncol; is++) {
+    const double *v = vre + is * hstride;
+    double *o = avre + is * n;
+#pragma omp parallel for schedule(static)
+    for (int iz = 1; iz <= ncpz; iz++)
+      for (int iy = 1; iy <= ncpy; iy++)
+        for (int ix = 1; ix <= ncpx; ix++)
+          o[IX3(ix, iy, iz)] = w * v[HX(ix - 1, iy, iz)] + w * v[HX(ix, iy - 1, iz)] +
+                               w * v[HX(ix, iy, iz - 1)] + 0.5 * v[HX(ix, iy, iz)] +
+                               w * v[HX(ix, iy, iz + 1)] + w * v[HX(ix, iy + 1, iz)] +
+                               w * v[HX(ix + 1, iy, iz)];
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ nonlocaloperation8f.F90:9-53 */
+int orc_nonlocaloperation_r_01(int natom, int nprjmx, int num_spe, int num_ppcell, int num_list,
+                               int ncpx, int ncpy, int ncpz, int key_natpri_in,
+                               int key_natpri_inps, const int *nprj, const int *indspe,
+                               const int *natinf, const int *natpri, const int *naps,
+                               const int *lstvec2, const double *vnlocp, const double *vre2,
+                               double *rpsep) {
+  (void)num_spe; (void)num_ppcell; (void)ncpx; (void)ncpy; (void)ncpz;
+  for (int na = 1; na <= natom; na++) {
+    if (natpri[na - 1] != key_natpri_in && natpri[na - 1] != key_natpri_inps) continue;
+    const int iaps = naps[na - 1], iatinf = natinf[na - 1], iprj = nprj[indspe[na - 1] - 1];
+    const int *lst = lstvec2 + (size_t)(iaps - 1) * num_list;
+    for (int i = 1; i <= iprj; i++) {
+      const double *p = vnlocp + (size_t)num_list * ((i - 1) + (size_t)nprjmx * (iaps - 1));
+      double t = 0.0;
+      for (int ii = 0; ii < iatinf; ii++) t += p[ii] * vre2[lst[ii] - 1];
+      rpsep[(i - 1) + (size_t)nprjmx * (iaps - 1)] += t;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ nonlocaloperation8f.F90:149-235 */
+int orc_nonlocaloperation_r_02(int nhs, int natom, int num_spe, int nums, int nprjmx, int ns,
+                               int num_list, int num_ppcell, int ncpx, int ncpy, int ncpz,
+                               int key_natpri_in, int key_natpri_inps, double dx, double dy,
+                               double dz, const int *nprj, const int *indspe, const int *natinf,
+                               const int *lstvec2, const int *natpri, const int *naps,
+                               const double *vnlocp, const double *rpsep, const double *dijsss,
+                               double *avre, double *avr) {
+  (void)num_spe; (void)num_ppcell; (void)ncpx; (void)ncpy; (void)ncpz;
+  if (nhs != 1 && nhs != 0) return fail(1, "error occurs in nonlocaloperation_r_02");
+  if (nhs == 0 && ns != 1) return fail(1, "nonlocaloperation_r_02: nhs=0 and ns/=1");
+  const int d3 = nums * nhs - nhs + 1;
+  const double dxyzin = 1.0 / (dx * dy * dz);
+  double t[64];
+  for (int na = 1; na <= natom; na++) {
+    if (natpri[na - 1] != key_natpri_in && natpri[na - 1] != key_natpri_inps) continue;
+    const int iaps = naps[na - 1], iatinf = natinf[na - 1], ispe = indspe[na - 1];
+    const int iprj = nprj[ispe - 1];
+    const int i4 = nhs == 1 ? na : ispe, i3 = nhs == 1 ? ns : 1;
+    const double *X = dijsss + (size_t)nprjmx * nprjmx * ((i3 - 1) + (size_t)d3 * (i4 - 1));
+    for (int i = 1; i <= iprj; i++) {
+      double s = 0.0;
+      for (int j = 1; j <= iprj; j++)
+        s += rpsep[(j - 1) + (size_t)nprjmx * (iaps - 1)] * X[(j - 1) + (size_t)nprjmx * (i - 1)];
+      t[i - 1] = s * dxyzin;
+    }
+    const double *p = vnlocp + (size_t)num_list * nprjmx * (iaps - 1);
+    for (int i = 0; i < iatinf; i++) avr[i] = t[0] * p[i];
+    for (int j = 2; j <= iprj; j++)
+      for (int i = 0; i < iatinf; i++) avr[i] = avr[i] + t[j - 1] * p[i + (size_t)num_list * (j - 1)];
+    const int *lst = lstvec2 + (size_t)(iaps - 1) * num_list;
+    for (int i = 0; i < iatinf; i++) avre[lst[i] - 1] += avr[i];
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ symmetric eigensolver */
+/* Householder reduction to tridiagonal form (tred2) followed by implicit QL with shifts
+ * (tql2); the published Wilkinson-Reinsch algorithms.  z(n,n) column-major holds the
+ * matrix on entry (lower triangle used) and the eigenvectors on exit. */
+static void tred2(int n, double *z, double *d, double *e) {
+#define Z(i, j) z[(size_t)(i) + (size_t)n * (j)]
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < j; i++) Z(i, j) = Z(j, i); /* symmetrise from the lower triangle */
+  for (int i = n - 1; i > 0; i--) {
+    int l = i - 1;
+    double h = 0.0, scale = 0.0;
+    if (l > 0) {
+      for (int k = 0; k <= l; k++) scale += fabs(Z(i, k));
+      if (scale == 0.0) {
+        e[i] = Z(i, l);
+      } else {
+        for (int k = 0; k <= l; k++) {
+          Z(i, k) /= scale;
+          h += Z(i, k) * Z(i, k);
+        }
+        double f = Z(i, l);
+        double g = (f >= 0.0 ? -sqrt(h) : sqrt(h));
+        e[i] = scale * g;
+        h -= f * g;
+        Z(i, l) = f - g;
+        f = 0.0;
+        for (int j = 0; j <= l; j++) {
+          Z(j, i) = Z(i, j) / h;
+          g = 0.0;
+          for (int k = 0; k <= j; k++) g += Z(j, k) * Z(i, k);
+          for (int k = j + 1; k <= l; k++) g += Z(k, j) * Z(i, k);
+          e[j] = g / h;
+          f += e[j] * Z(i, j);
+        }
+        double hh = f / (h + h);
+        for (int j = 0; j <= l; j++) {
+          f = Z(i, j);
+          e[j] = g = e[j] - hh * f;
+          for (int k = 0; k <= j; k++) Z(j, k) -= (f * e[k] + g * Z(i, k));
+        }
+      }
+    } else {
+      e[i] = Z(i, l);
+    }
+    d[i] = h;
+  }
+  d[0] = 0.0;
+  e[0] = 0.0;
+  for (int i = 0; i < n; i++) {
+    int l = i - 1;
+    if (d[i] != 0.0) {
+      for (int j = 0; j <= l; j++) {
+        double g = 0.0;
+        for (int k = 0; k <= l; k++) g += Z(i, k) * Z(k, j);
+        for (int k = 0; k <= l; k++) Z(k, j) -= g * Z(k, i);
+      }
+    }
+    d[i] = Z(i, i);
+    Z(i, i) = 1.0;
+    for (int j = 0; j <= l; j++) Z(j, i) = Z(i, j) = 0.0;
+  }
+#undef Z
+}
+
+static int tql2(int n, double *d, double *e, double *z) {
+  for (int i = 1; i < n; i++) e[i - 1] = e[i];
+  e[n - 1] = 0.0;
+  for (int l = 0; l < n; l++) {
+    int iter = 0, m;
+    do {
+      for (m = l; m < n - 1; m++) {
+        double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= 1e-300 || fabs(e[m]) + dd == dd) break;
+      }
+      if (m != l) {
+        if (iter++ == 60) return 1;
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = hypot(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        for (i = m - 1; i >= l; i--) {
+          double f = s * e[i], b = c * e[i];
+          e[i + 1] = (r = hypot(f, g));
+          if (r == 0.0) {
+            d[i + 1] -= p;
+            e[m] = 0.0;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - p;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          d[i + 1] = g + (p = s * r);
+          g = c * r - b;
+          for (int k = 0; k < n; k++) {
+            f = z[k + (size_t)n * (i + 1)];
+            z[k + (size_t)n * (i + 1)] = s * z[k + (size_t)n * i] + c * f;
+            z[k + (size_t)n * i] = c * z[k + (size_t)n * i] - s * f;
+          }
+        }
+        if (r == 0.0 && i >= l) continue;
+        d[l] -= p;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+    } while (m != l);
+  }
+  return 0;
+}
+
+int orc_dsyev(int n, double *a, double *w) {
+  if (n <= 0) return 0;
+  double *e = (double *)xcalloc(n, sizeof(double));
+  tred2(n, a, w, e);
+  int rc = tql2(n, w, e, a);
+  free(e);
+  if (rc) return fail(1, "dsyev: no convergence");
+  /* ascending selection sort of eigenpairs */
+  for (int i = 0; i < n - 1; i++) {
+    int k = i;
+    for (int j = i + 1; j < n; j++)
+      if (w[j] < w[k]) k = j;
+    if (k != i) {
+      double t = w[i];
+      w[i] = w[k];
+      w[k] = t;
+      for (int r = 0; r < n; r++) {
+        t = a[r + (size_t)n * i];
+        a[r + (size_t)n * i] = a[r + (size_t)n * k];
+        a[r + (size_t)n * k] = t;
+      }
+    }
+  }
+  /* sign convention: largest-magnitude component positive (lowest index on ties) */
+  for (int j = 0; j < n; j++) {
+    double *c = a + (size_t)n * j;
+    int im = 0;
+    for (int r = 1; r < n; r++)
+      if (fabs(c[r]) > fabs(c[im])) im = r;
+    if (c[im] < 0.0)
+      for (int r = 0; r < n; r++) c[r] = -c[r];
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ shared context */
+typedef struct {
+  int ncpx, ncpy, ncpz, nf, nperi;
+  double dx, dy, dz, dv;
+  int natom, num_spe, nums, nprjmx, num_list, num_ppcell;
+  int key_in, key_inps;
+  const int *nprj, *indspe, *natinf, *lstvec2, *natpri, *naps;
+  const double *vnlocp, *dij, *sss, *veff;
+  long n, nh, n1; /* grid points, halo'd points, nprjmx*num_ppcell */
+  double *vre, *avr;
+} ctx_t;
+
+static void ctx_alloc(ctx_t *c) {
+  c->n = (long)c->ncpx * c->ncpy * c->ncpz;
+  c->nh = (long)(c->ncpx + 2 * c->nf) * (c->ncpy + 2 * c->nf) * (c->ncpz + 2 * c->nf);
+  c->n1 = (long)c->nprjmx * c->num_ppcell;
+  c->dv = c->dx * c->dy * c->dz;
+  c->vre = (double *)xcalloc(c->nh, sizeof(double));
+  c->avr = (double *)xcalloc(c->num_list, sizeof(double));
+}
+static void ctx_free(ctx_t *c) {
+  free(c->vre);
+  free(c->avr);
+}
+
+/* vre = 0; vre(1:n,1:n,1:n) = psi; halo fill with nsnd layers (kscg8f.F90:114-133). */
+static void load_halo(ctx_t *c, const double *psi, int nsnd) {
+  const int nf = c->nf, nf1 = nf - 1, ncpx = c->ncpx, ncpy = c->ncpy, ncpz = c->ncpz;
+  const long ex = ncpx + 2 * nf, ey = ncpy + 2 * nf;
+  memset(c->vre, 0, sizeof(double) * c->nh);
+#pragma omp parallel for schedule(static)
+  for (int iz = 1; iz <= ncpz; iz++)
+    for (int iy = 1; iy <= ncpy; iy++)
+      for (int ix = 1; ix <= ncpx; ix++) c->vre[HX(ix, iy, iz)] = psi[IX3(ix, iy, iz)];
+  orc_overlap_finitedifference_r(c->nperi, ncpx, ncpy, ncpz, nf, nf1, nsnd, c->vre);
+}
+
+/* out = H psi with given <p|psi> (kscg8f.F90:114-153): zero, nonlocal r_02 (nhs=1),
+ * then kslaplacian_r, in that order. */
+static int apply_h(ctx_t *c, int ns, int nsv, const double *psi, const double *rp, double *out) {
+  memset(out, 0, sizeof(double) * c->n);
+  load_halo(c, psi, c->nf);
+  int rc = orc_nonlocaloperation_r_02(1, c->natom, c->num_spe, c->nums, c->nprjmx, ns,
+                                      c->num_list, c->num_ppcell, c->ncpx, c->ncpy, c->ncpz,
+                                      c->key_in, c->key_inps, c->dx, c->dy, c->dz, c->nprj,
+                                      c->indspe, c->natinf, c->lstvec2, c->natpri, c->naps,
+                                      c->vnlocp, rp, c->dij, out, c->avr);
+  if (rc) return rc;
+  return orc_kslaplacian_r(c->ncpx, c->ncpy, c->ncpz, 1, c->nf, c->dx, c->dy, c->dz, c->vre,
+                           c->veff + (size_t)(nsv - 1) * c->n, 1, out);
+}
+
+/* rp = <p|v> (nonlocaloperation_r_01 on zeroed rp). */
+static void proj(ctx_t *c, const double *v, double *rp) {
+  memset(rp, 0, sizeof(double) * c->n1);
+  orc_nonlocaloperation_r_01(c->natom, c->nprjmx, c->num_spe, c->num_ppcell, c->num_list, c->ncpx,
+                             c->ncpy, c->ncpz, c->key_in, c->key_inps, c->nprj, c->indspe,
+                             c->natinf, c->natpri, c->naps, c->lstvec2, c->vnlocp, v, rp);
+}
+
+/* sv = v + S-part with given <p|v> (nonlocaloperation_r_02 with nhs=0, sss). */
+static void apply_s(ctx_t *c, const double *v, const double *rp, double *sv) {
+  memcpy(sv, v, sizeof(double) * c->n);
+  orc_nonlocaloperation_r_02(0, c->natom, c->num_spe, c->nums, c->nprjmx, 1, c->num_list,
+                             c->num_ppcell, c->ncpx, c->ncpy, c->ncpz, c->key_in, c->key_inps,
+                             c->dx, c->dy, c->dz, c->nprj, c->indspe, c->natinf, c->lstvec2,
+                             c->natpri, c->naps, c->vnlocp, rp, c->sss, sv, c->avr);
+}
+
+/* out = K r (kscg8f.F90:206-234): 1-layer halo, ksprecondition. */
+static void precond(ctx_t *c, const double *r, double *out) {
+  load_halo(c, r, 1);
+  orc_ksprecondition(c->ncpx, c->ncpy, c->ncpz, c->nf, 1, c->vre, out);
+}
+
+/* orthogonalization_r_03 (:727-744): v -= sum_j ar(j) a(:,j), j = 1..lmax, in j order. */
+static void r03(long n, int lmax, double *v, const double *a, const double *ar) {
+  for (int j = 0; j < lmax; j++) {
+    const double *aj = a + (size_t)n * j;
+    const double s = ar[j];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) v[i] = v[i] - s * aj[i];
+  }
+}
+
+/* multr (:891-912): a0(:,j) -= sum_k b(j,k) a1(:,k). */
+static void multr(long n, int m, int l, double *a0, const double *a1, const double *b) {
+  for (int k = 0; k < l; k++)
+    for (int j = 0; j < m; j++)
+      for (long i = 0; i < n; i++) a0[i + n * j] -= b[j + (size_t)m * k] * a1[i + n * k];
+}
+
+/* Normalise band (v, sv, rp) with vecnor = dV <v|sv>; stopp on non-positive norm. */
+static int normalise(ctx_t *c, double *v, double *sv, double *rp, const char *msg) {
+  double vecnor = dot(c->n, v, sv) * c->dv;
+  if (vecnor <= 0.0) return fail(2, msg);
+  double t = 1.0 / sqrt(vecnor);
+  for (long i = 0; i < c->n; i++) v[i] *= t;
+  for (long i = 0; i < c->n; i++) sv[i] *= t;
+  for (long i = 0; i < c->n1; i++) rp[i] *= t;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ orthogonalization8f.F90:93-376 */
+static int ortho_r_01(ctx_t *c, int neigmx, int l_inp, int nflag, int key_cmpt, double *svecre,
+                      double *ssvre, double *rspsep) {
+  const long n = c->n, n1 = c->n1;
+  int mstep = 32;
+  if (neigmx > 1024) mstep = 64;
+  if (neigmx > 2048) mstep = 128;
+  if (neigmx > 4096) mstep = 256;
+  const int l_stt = l_inp == 0 ? 1 : l_inp, l_end = l_inp == 0 ? neigmx : l_inp;
+#define PSI(l) (svecre + (size_t)n * ((l)-1))
+#define SPSI(l) (ssvre + (size_t)n * ((l)-1))
+#define RPS(l) (rspsep + (size_t)n1 * ((l)-1))
+  if (nflag == key_cmpt) {
+    for (int l = l_stt; l <= l_end; l++) proj(c, PSI(l), RPS(l));
+    for (int l = l_stt; l <= l_end; l++) apply_s(c, PSI(l), RPS(l), SPSI(l));
+  }
+  double *avreal = (double *)xcalloc(neigmx, sizeof(double));
+  double *amatr = (double *)xcalloc((size_t)mstep * mstep, sizeof(double));
+  int rc = 0, l = l_stt;
+  while (l < l_end + 1) {
+    if (l > mstep && l_end - l + 1 >= mstep) {
+      for (int j = 1; j <= l - 1; j += mstep) {
+        for (int a = 0; a < mstep; a++)
+          for (int b = 0; b < mstep; b++) amatr[a + mstep * b] = dot(n, PSI(l + a), SPSI(j + b)) * c->dv;
+        for (int a = 0; a < mstep; a++) {
+          double *va = PSI(l + a), *sa = SPSI(l + a);
+          for (int b = 0; b < mstep; b++) {
+            const double cab = amatr[a + mstep * b];
+            const double *vb = PSI(j + b), *sb = SPSI(j + b);
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n; i++) {
+              va[i] -= vb[i] * cab;
+              sa[i] -= sb[i] * cab;
+            }
+          }
+        }
+        multr(n1, mstep, mstep, RPS(l), RPS(j), amatr);
+      }
+      if ((rc = normalise(c, PSI(l), SPSI(l), RPS(l), "orthogonalization_r_01 (1): wf has negative norm"))) break;
+      for (int l0 = l + 1; l0 <= l + mstep - 1; l0++) {
+        for (int i = 0; i < l0 - l; i++) avreal[i] = dot(n, SPSI(l + i), PSI(l0)) * c->dv;
+        r03(n, l0 - l, PSI(l0), PSI(l), avreal);
+        r03(n, l0 - l, SPSI(l0), SPSI(l), avreal);
+        multr(n1, 1, l0 - l, RPS(l0), RPS(l), avreal);
+        if ((rc = normalise(c, PSI(l0), SPSI(l0), RPS(l0), "orthogonalization_r_01 (2): wf has negative norm"))) break;
+      }
+      if (rc) break;
+      l += mstep;
+    } else {
+      for (int j = 1; j <= l - 1; j++) avreal[j - 1] = dot(n, SPSI(j), PSI(l)) * c->dv;
+      r03(n, l - 1, PSI(l), PSI(1), avreal);
+      r03(n, l - 1, SPSI(l), SPSI(1), avreal);
+      multr(n1, 1, l - 1, RPS(l), RPS(1), avreal);
+      if ((rc = normalise(c, PSI(l), SPSI(l), RPS(l), "orthogonalization_r_01 (3): wf has negative norm"))) break;
+      l += 1;
+    }
+  }
+  free(avreal);
+  free(amatr);
+  return rc;
+#undef PSI
+#undef SPSI
+#undef RPS
+}
+
+int orc_orthogonalization_r_01(int natom, int neigmx, int nprjmx, int nums, int numk, int nk,
+                               int ns, int l_inp, int num_list, int num_ppcell, int num_spe,
+                               int nflag, int ndisp, int ncpx, int ncpy, int ncpz,
+                               int key_ortho_cmpt_innerproduct, int key_natpri_in,
+                               int key_natpri_inps, double dx, double dy, double dz,
+                               const int *nprj, const int *indspe, const int *natpri,
+                               const int *naps, const int *natinf, const int *lstvec2,
+                               const int *latom, const double *vnlocp, const double *sss,
+                               double *svecre, double *ssvre, double *rspsep) {
+  (void)numk; (void)ndisp; (void)latom;
+  ctx_t c = {0};
+  c.ncpx = ncpx; c.ncpy = ncpy; c.ncpz = ncpz; c.nf = 1; c.nperi = 0;
+  c.dx = dx; c.dy = dy; c.dz = dz;
+  c.natom = natom; c.num_spe = num_spe; c.nums = nums; c.nprjmx = nprjmx;
+  c.num_list = num_list; c.num_ppcell = num_ppcell;
+  c.key_in = key_natpri_in; c.key_inps = key_natpri_inps;
+  c.nprj = nprj; c.indspe = indspe; c.natinf = natinf; c.lstvec2 = lstvec2;
+  c.natpri = natpri; c.naps = naps; c.vnlocp = vnlocp; c.sss = sss;
+  ctx_alloc(&c);
+  const size_t off = (size_t)c.n * neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
+  const size_t off1 = (size_t)c.n1 * neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
+  int rc = ortho_r_01(&c, neigmx, l_inp, nflag, key_ortho_cmpt_innerproduct, svecre + off,
+                      ssvre + off, rspsep + off1);
+  ctx_free(&c);
+  return rc;
+}
+
+/* orthogonalization8f.F90:8-44 */
+int orc_orthogonalization_r(int natom, int neigmx, int nprjmx, int nums, int numk, int num_ppcell,
+                            int num_spe, int northo, int num_list, int nflag, int ndisp, int ncpx,
+                            int ncpy, int ncpz, int key_ortho_cmpt_innerproduct,
+                            int key_natpri_in, int key_natpri_inps, double dx, double dy,
+                            double dz, const int *nprj, const int *indspe, const int *natpri,
+                            const int *naps, const int *natinf, const int *lstvec2,
+                            const int *latom, const int *ntyppp, const double *vnlocp,
+                            const double *sss, double *svecre, double *ssvre, double *rspsep) {
+  (void)ntyppp;
+  for (int nk = 1; nk <= numk; nk++)
+    for (int ns = 1; ns <= nums; ns++)
+      for (int loop = 1; loop <= northo; loop++) {
+        int rc = orc_orthogonalization_r_01(
+            natom, neigmx, nprjmx, nums, numk, nk, ns, 0, num_list, num_ppcell, num_spe, nflag,
+            ndisp, ncpx, ncpy, ncpz, key_ortho_cmpt_innerproduct, key_natpri_in, key_natpri_inps,
+            dx, dy, dz, nprj, indspe, natpri, naps, natinf, lstvec2, latom, vnlocp, sss, svecre,
+            ssvre, rspsep);
+        if (rc) return rc;
+      }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ kscg8f.F90:7-489, 1035-1230 */
+/* kscg_r_05 (:1197-1230): 2x2 Rayleigh-Ritz rotation of (psi, p). */
+static void kscg_r_05(long n, long n1, const double *pvecre, double *hsvre, const double *avre,
+                      const double *asvre, double *svecre, double *ssvre, const double *rpsep,
+                      double *rspsep, double pap, double sapre, double sval, int *l_converged) {
+  const double eps = 1.0e-14;
+  double alph = 1.0, beta = 0.0;
+  double gamma = (sval + pap) * 0.5 - sqrt((sval - pap) * (sval - pap) * 0.25 + sapre * sapre);
+  if (fabs((sval - gamma) / gamma) > eps) {
+    alph = sapre / sqrt(sapre * sapre + (sval - gamma) * (sval - gamma));
+    beta = -(sval - gamma) / sqrt(sapre * sapre + (sval - gamma) * (sval - gamma));
+  }
+  if (alph < eps) {
+    alph = -alph;
+    beta = -beta;
+    *l_converged = 1;
+  }
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; i++) {
+    svecre[i] = alph * svecre[i] + beta * pvecre[i];
+    hsvre[i] = alph * hsvre[i] + beta * avre[i];
+    ssvre[i] = alph * ssvre[i] + beta * asvre[i];
+  }
+  for (long i = 0; i < n1; i++) rspsep[i] = alph * rspsep[i] + beta * rpsep[i];
+}
+
+int orc_kscg_r(int natom, int num_spe, int neigmx, int nums, int nspv, int numk, int nperi,
+               int northo, int ns, int nk, int nf, int nkscg, int nsdmax, int nprecon_cg,
+               int nprjmx, int num_list, int num_ppcell, int ndisp, int ncpx, int ncpy, int ncpz,
+               int key_ortho_nocmpt_innerproduct, int key_ortho_cmpt_innerproduct,
+               int key_natpri_in, int key_natpri_inps, int key_natpri_out, double dx, double dy,
+               double dz, double epssd, const int *indspe, const int *natpri, const int *naps,
+               const int *nprj, const int *natinf, const int *lstvec2, const int *latom,
+               const int *ntyppp, const double *dij, const double *veff, const double *vnlocp,
+               const double *sss, int *ksconv, int *ksitmax, double *sval, double *rspsep,
+               double *svecre, double *ssvre, double *residual_states, double *hsvre) {
+  (void)numk; (void)ndisp; (void)key_natpri_out; (void)latom; (void)ntyppp;
+  ctx_t c = {0};
+  c.ncpx = ncpx; c.ncpy = ncpy; c.ncpz = ncpz; c.nf = nf; c.nperi = nperi;
+  c.dx = dx; c.dy = dy; c.dz = dz;
+  c.natom = natom; c.num_spe = num_spe; c.nums = nums; c.nprjmx = nprjmx;
+  c.num_list = num_list; c.num_ppcell = num_ppcell;
+  c.key_in = key_natpri_in; c.key_inps = key_natpri_inps;
+  c.nprj = nprj; c.indspe = indspe; c.natinf = natinf; c.lstvec2 = lstvec2;
+  c.natpri = natpri; c.naps = naps; c.vnlocp = vnlocp; c.sss = sss; c.dij = dij; c.veff = veff;
+  ctx_alloc(&c);
+  const long n = c.n, n1 = c.n1;
+  const int nsv = ns < nspv ? ns : nspv;
+  const size_t off = (size_t)n * neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
+  const size_t off1 = (size_t)n1 * neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
+  double *Psi = svecre + off, *SPsi = ssvre + off, *Rps = rspsep + off1;
+  double *sv = sval + (size_t)neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
+  double *rvecre = (double *)xcalloc(n, sizeof(double)), *pvecre = (double *)xcalloc(n, sizeof(double));
+  double *pvre = (double *)xcalloc(n, sizeof(double)), *rvre = (double *)xcalloc(n, sizeof(double));
+  double *spvre = (double *)xcalloc(n, sizeof(double)), *avre = (double *)xcalloc(n, sizeof(double));
+  double *asvre = (double *)xcalloc(n, sizeof(double));
+  double *avreal = (double *)xcalloc(neigmx, sizeof(double));
+  double *rrpsep = (double *)xcalloc(n1, sizeof(double)), *rppsep = (double *)xcalloc(n1, sizeof(double));
+  double *rpsep = (double *)xcalloc(n1, sizeof(double));
+  const int nsdmax1 = 10, nsdmax2 = 200;
+  int rc = 0;
+  for (int l = 1; l <= neigmx && !rc; l++) {
+    double *psi = Psi + (size_t)n * (l - 1), *spsi = SPsi + (size_t)n * (l - 1);
+    double *hpsi = hsvre + (size_t)n * (l - 1), *rps = Rps + (size_t)n1 * (l - 1);
+    const int key = northo == 2 ? key_ortho_cmpt_innerproduct : key_ortho_nocmpt_innerproduct;
+    for (int loop = 1; loop <= northo && !rc; loop++)
+      rc = ortho_r_01(&c, neigmx, l, key, key_ortho_cmpt_innerproduct, Psi, SPsi, Rps);
+    if (rc) break;
+    memcpy(rpsep, rps, sizeof(double) * n1);
+    if ((rc = apply_h(&c, ns, nsv, psi, rpsep, hpsi))) break;
+    int iiter = 0, l_converged = 0;
+    double r2rr = 0.0, r1rr;
+    while (iiter < nsdmax && !l_converged) {
+      iiter++;
+      int doreset = (iiter == 1) || (iiter == nsdmax1);
+      if (nsdmax2 > 0) doreset = doreset || ((iiter - 1) % nsdmax2 == 0);
+      if (doreset) {
+        r2rr = 1.0;
+        memset(pvecre, 0, sizeof(double) * n);
+        memset(rvecre, 0, sizeof(double) * n);
+        memset(pvre, 0, sizeof(double) * n);
+        memset(spvre, 0, sizeof(double) * n);
+        memset(rppsep, 0, sizeof(double) * n1);
+      }
+      /* eigenvalue and residual (:186-201) */
+      r1rr = r2rr;
+      double valre = dot(n, psi, hpsi) * c.dv;
+      sv[l - 1] = valre;
+      for (long i = 0; i < n; i++) rvecre[i] = -(hpsi[i] - valre * spsi[i]);
+      /* precondition (:204-234) */
+      if (nprecon_cg == 1) precond(&c, rvecre, rvre);
+      else memcpy(rvre, rvecre, sizeof(double) * n);
+      /* Kr orthogonal to psi_1..l in the S metric (:237-264) */
+      for (int loop = 1; loop <= northo; loop++) {
+        for (int j = 1; j <= l; j++) avreal[j - 1] = dot(n, rvre, SPsi + (size_t)n * (j - 1)) * c.dv;
+        r03(n, l, rvre, Psi, avreal);
+      }
+      /* <p|Kr>, S Kr (:265-297) */
+      proj(&c, rvre, rpsep);
+      memcpy(rrpsep, rpsep, sizeof(double) * n1);
+      apply_s(&c, rvre, rrpsep, asvre);
+      r2rr = dot(n, rvecre, rvre) * c.dv;
+      residual_states[l - 1] = r2rr;
+      /* conjugate direction (:300-333) */
+      if (nkscg == 1) {
+        const double t = r2rr / r1rr;
+        for (long i = 0; i < n; i++) {
+          pvre[i] = rvre[i] + t * pvre[i];
+          pvecre[i] = pvre[i];
+          spvre[i] = asvre[i] + t * spvre[i];
+        }
+        for (long i = 0; i < n1; i++) rppsep[i] = rrpsep[i] + r2rr / r1rr * rppsep[i];
+      } else {
+        memcpy(pvecre, rvre, sizeof(double) * n);
+        memcpy(spvre, asvre, sizeof(double) * n);
+        memcpy(rppsep, rrpsep, sizeof(double) * n1);
+      }
+      /* p orthogonal to psi_1..l, normalised (:336-404) */
+      memcpy(rpsep, rppsep, sizeof(double) * n1);
+      memcpy(asvre, spvre, sizeof(double) * n);
+      for (int loop = 1; loop <= northo; loop++) {
+        for (int j = 1; j <= l; j++) avreal[j - 1] = dot(n, pvecre, SPsi + (size_t)n * (j - 1)) * c.dv;
+        r03(n, l, pvecre, Psi, avreal);
+        r03(n, l, asvre, SPsi, avreal);
+        multr(n1, 1, l, rpsep, Rps, avreal);
+      }
+      if ((rc = normalise(&c, pvecre, asvre, rpsep, "kscg_r: pvec has negative norm"))) break;
+      /* H p (:407-442) */
+      if ((rc = apply_h(&c, ns, nsv, pvecre, rpsep, avre))) break;
+      const double pap = dot(n, pvecre, avre) * c.dv;
+      const double sapre = dot(n, psi, avre) * c.dv;
+      kscg_r_05(n, n1, pvecre, hpsi, avre, asvre, psi, spsi, rpsep, rps, pap, sapre, sv[l - 1],
+                &l_converged);
+      l_converged = sqrt(residual_states[l - 1]) < epssd;
+      *ksconv = *ksconv && (l_converged || (iiter < nsdmax));
+    }
+    if (iiter > *ksitmax) *ksitmax = iiter;
+  }
+  free(rvecre); free(pvecre); free(pvre); free(rvre); free(spvre); free(avre); free(asvre);
+  free(avreal); free(rrpsep); free(rppsep); free(rpsep);
+  ctx_free(&c);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ ksdiis8f.F90:8-712 */
+int orc_ksdiis_r(int l_hsv, int natom, int num_spe, int neigmx, int nums, int nspv, int numk,
+                 int nperi, int ndiismax, int nprecon_diis, int nk, int ns, int nf, int nprjmx,
+                 int num_list, int num_ppcell, int ndisp, int northo, int ncpx, int ncpy,
+                 int ncpz, int key_ortho_cmpt_innerproduct, int key_natpri_in,
+                 int key_natpri_inps, double eps_eig_diis, double alambda_diis,
+                 double ratio_diis, double alambda_min, double alambda_max, double epssd,
+                 double xmax, double ymax, double zmax, const int *indspe, const int *natpri,
+                 const int *naps, const int *nprj, const int *natinf, const int *lstvec2,
+                 const int *latom, const int *ntyppp, const double *dij, const double *veff,
+                 const double *vnlocp, const double *sss, int *ksconv, int *ksitmax,
+                 double *rspsep, double *sval, double *svecre, double *ssvre,
+                 double *residual_states, double *hsvre) {
+  (void)numk; (void)ndisp; (void)latom; (void)ntyppp; (void)ratio_diis;
+  if (nspv != nums) return 0; /* the reference skips the real DIIS routine (:78-81) */
+  ctx_t c = {0};
+  c.ncpx = ncpx; c.ncpy = ncpy; c.ncpz = ncpz; c.nf = nf; c.nperi = nperi;
+  c.dx = 2.0 * xmax / ncpx; c.dy = 2.0 * ymax / ncpy; c.dz = 2.0 * zmax / ncpz;
+  c.natom = natom; c.num_spe = num_spe; c.nums = nums; c.nprjmx = nprjmx;
+  c.num_list = num_list; c.num_ppcell = num_ppcell;
+  c.key_in = key_natpri_in; c.key_inps = key_natpri_inps;
+  c.nprj = nprj; c.indspe = indspe; c.natinf = natinf; c.lstvec2 = lstvec2;
+  c.natpri = natpri; c.naps = naps; c.vnlocp = vnlocp; c.sss = sss; c.dij = dij; c.veff = veff;
+  ctx_alloc(&c);
+  const long n = c.n, n1 = c.n1;
+  enum { NIT = 2 }; /* nitermax (:31) */
+  const size_t off = (size_t)n * neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
+  const size_t off1 = (size_t)n1 * neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
+  double *Psi = svecre + off, *SPsi = ssvre + off, *Rps = rspsep + off1;
+  double *sv = sval + (size_t)neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
+  double *phire = (double *)xcalloc(n * (NIT + 1), sizeof(double));
+  double *sphire = (double *)xcalloc(n * (NIT + 1), sizeof(double));
+  double *hphire = (double *)xcalloc(n * (NIT + 1), sizeof(double));
+  double *rvecre = (double *)xcalloc(n * (NIT + 1), sizeof(double));
+  double *rphisep = (double *)xcalloc(n1 * (NIT + 1), sizeof(double));
+  double *rvre = (double *)xcalloc(n, sizeof(double)), *avre = (double *)xcalloc(n, sizeof(double));
+  double *asvre = (double *)xcalloc(n, sizeof(double));
+  double *rpsep = (double *)xcalloc(n1, sizeof(double));
+#define PH(j) (phire + (size_t)n * (j))
+#define SPH(j) (sphire + (size_t)n * (j))
+#define HPH(j) (hphire + (size_t)n * (j))
+#define RV(j) (rvecre + (size_t)n * (j))
+#define RPH(j) (rphisep + (size_t)n1 * (j))
+  int rc = 0;
+  int jiter = 0;
+  double alambda = 0.0;
+  while (jiter + 1 < ndiismax && !rc) {
+    jiter++;
+    for (int l = 1; l <= neigmx && !rc; l++) {
+      memcpy(PH(0), Psi + (size_t)n * (l - 1), sizeof(double) * n);
+      memcpy(SPH(0), SPsi + (size_t)n * (l - 1), sizeof(double) * n);
+      memcpy(RPH(0), Rps + (size_t)n1 * (l - 1), sizeof(double) * n1);
+      int iiter = 0, l_converged = 0;
+      while (iiter + 1 < NIT && !l_converged) {
+        iiter++;
+        if (iiter == 1) {
+          memcpy(rpsep, RPH(iiter - 1), sizeof(double) * n1);
+          if ((rc = apply_h(&c, ns, ns, PH(iiter - 1), rpsep, avre))) break;
+          memcpy(HPH(iiter - 1), avre, sizeof(double) * n);
+        }
+        double valre = dot(n, PH(iiter - 1), HPH(iiter - 1)) * c.dv;
+        sv[l - 1] = valre;
+        for (long i = 0; i < n; i++) RV(iiter - 1)[i] = HPH(iiter - 1)[i] + (-valre) * SPH(iiter - 1)[i];
+        if (nprecon_diis == 1) precond(&c, RV(iiter - 1), rvre);
+        else memcpy(rvre, RV(iiter - 1), sizeof(double) * n);
+        proj(&c, rvre, rpsep);
+        if ((rc = apply_h(&c, ns, ns, rvre, rpsep, avre))) break;
+        apply_s(&c, rvre, rpsep, asvre);
+        if (iiter == 1) {
+          const double a1 = alambda_diis;
+          for (long i = 0; i < n1; i++) RPH(1)[i] = RPH(0)[i] + a1 * rpsep[i];
+          for (long i = 0; i < n; i++) PH(1)[i] = PH(0)[i] + a1 * rvre[i];
+          for (long i = 0; i < n; i++) HPH(1)[i] = HPH(0)[i] + a1 * avre[i];
+          for (long i = 0; i < n; i++) SPH(1)[i] = SPH(0)[i] + a1 * asvre[i];
+          const double s1 = dot(n, rvre, HPH(0)), s2 = dot(n, rvre, SPH(0));
+          const double s3 = dot(n, PH(1), HPH(1)), s4 = dot(n, PH(1), SPH(1));
+          const double aaa = 2.0 * (s1 - sv[l - 1] * s2) * c.dv;
+          const double vr = s3 / s4;
+          const double bbb = (vr - sv[l - 1] - aaa * a1) / (a1 * a1);
+          alambda = 0.5 * aaa / bbb;
+          if (alambda > alambda_max) alambda = alambda_max;
+          if (alambda < alambda_min) alambda = alambda_min;
+        }
+        /* trial phi (:326-336) */
+        for (long i = 0; i < n; i++) PH(iiter)[i] = PH(iiter - 1)[i] + alambda * rvre[i];
+        for (long i = 0; i < n; i++) HPH(iiter)[i] = HPH(iiter - 1)[i] + alambda * avre[i];
+        for (long i = 0; i < n; i++) SPH(iiter)[i] = SPH(iiter - 1)[i] + alambda * asvre[i];
+        for (long i = 0; i < n1; i++) RPH(iiter)[i] = RPH(iiter - 1)[i] + alambda * rpsep[i];
+        for (int j = 0; j <= iiter; j++)
+          for (long i = 0; i < n; i++) RV(j)[i] = HPH(j)[i] + (-sv[l - 1]) * SPH(j)[i];
+        /* DIIS matrices (:375-408) */
+        const int m = iiter + 1;
+        double rrmat[(NIT + 1) * (NIT + 1)], rsmat[(NIT + 1) * (NIT + 1)];
+        double rsmat0[(NIT + 1) * (NIT + 1)], rvect[(NIT + 1) * (NIT + 1)], eigen[NIT + 1];
+        double work[NIT + 1];
+#define M2(a, i, j) a[((i)-1) + m * ((j)-1)]
+        for (int j = 0; j <= iiter; j++)
+          for (int i = 0; i <= j; i++) {
+            const double s = dot(n, SPH(j), PH(i)) * c.dv, r = dot(n, RV(j), RV(i)) * c.dv;
+            M2(rsmat, i + 1, j + 1) = s; M2(rsmat, j + 1, i + 1) = s;
+            M2(rrmat, i + 1, j + 1) = r; M2(rrmat, j + 1, i + 1) = r;
+          }
+        const double tmp = 1.0 / sqrt(M2(rsmat, m, m));
+        for (long i = 0; i < n; i++) {
+          PH(iiter)[i] *= tmp; HPH(iiter)[i] *= tmp; SPH(iiter)[i] *= tmp; RV(iiter)[i] *= tmp;
+        }
+        for (long i = 0; i < n1; i++) RPH(iiter)[i] *= tmp;
+        for (int i = 1; i <= m; i++) { M2(rsmat, m, i) *= tmp; M2(rrmat, m, i) *= tmp; }
+        for (int i = 1; i <= m; i++) { M2(rsmat, i, m) *= tmp; M2(rrmat, i, m) *= tmp; }
+        memcpy(rsmat0, rsmat, sizeof(double) * m * m);
+        /* eigen-decomposition of the overlap, descending order (:410-420) */
+        if ((rc = orc_dsyev(m, rsmat, eigen))) break;
+        for (int i = 1; i <= m; i++) work[i - 1] = eigen[i - 1];
+        for (int j = 1; j <= m; j++) {
+          for (int i = 1; i <= m; i++) M2(rvect, i, j) = M2(rsmat, i, m + 1 - j);
+          eigen[j - 1] = work[m + 1 - j - 1];
+        }
+        /* drop non-positive overlap eigenvalues (:421-435) */
+        int nmat = 0, k = 0;
+        while (k < m) {
+          nmat++;
+          k++;
+          if (eigen[nmat - 1] < eps_eig_diis) {
+            for (int i = nmat; i <= iiter - 1; i++) {
+              for (int r = 1; r <= m; r++) M2(rvect, r, i) = M2(rvect, r, i + 1);
+              eigen[i - 1] = eigen[i];
+            }
+            nmat--;
+          }
+        }
+        if (nmat > 1) {
+          for (int i = 1; i <= nmat; i++)
+            for (int r = 1; r <= m; r++) M2(rvect, r, i) /= sqrt(eigen[i - 1]);
+          double rrmat2[(NIT + 1) * (NIT + 1)], rvect2[(NIT + 1) * (NIT + 1)];
+          for (int kk = 1; kk <= m; kk++)
+            for (int j = 1; j <= nmat; j++) {
+              double s = 0.0;
+              for (int i = 1; i <= m; i++) s += M2(rrmat, kk, i) * M2(rvect, i, j);
+              M2(rsmat, kk, j) = s;
+            }
+          for (int kk = 1; kk <= nmat; kk++)
+            for (int j = 1; j <= nmat; j++) {
+              double s = 0.0;
+              for (int i = 1; i <= m; i++) s += M2(rvect, i, kk) * M2(rsmat, i, j);
+              rrmat2[(kk - 1) + nmat * (j - 1)] = s;
+            }
+          if ((rc = orc_dsyev(nmat, rrmat2, eigen))) break;
+          for (int i = 1; i <= nmat; i++) work[i - 1] = eigen[i - 1];
+          for (int j = 1; j <= nmat; j++) {
+            for (int i = 1; i <= nmat; i++)
+              rvect2[(i - 1) + nmat * (j - 1)] = rrmat2[(i - 1) + nmat * (nmat + 1 - j - 1)];
+            eigen[j - 1] = work[nmat + 1 - j - 1];
+          }
+          residual_states[l - 1] = eigen[nmat - 1];
+          memset(rrmat, 0, sizeof rrmat);
+          for (int kk = 1; kk <= nmat; kk++)
+            for (int j = 1; j <= m; j++) {
+              double s = 0.0;
+              for (int i = 1; i <= nmat; i++) s += M2(rvect, j, i) * rvect2[(i - 1) + nmat * (kk - 1)];
+              M2(rrmat, j, kk) = s;
+            }
+          double t2 = 0.0;
+          for (int j = 1; j <= m; j++)
+            for (int i = 1; i <= m; i++) t2 += M2(rrmat, i, nmat) * M2(rsmat0, i, j) * M2(rrmat, j, nmat);
+          for (int j = 1; j <= m; j++) M2(rrmat, j, nmat) /= sqrt(t2);
+        } else {
+          if (nmat < 1) nmat = 1;
+          for (int j = 1; j <= m; j++) M2(rrmat, j, nmat) = 0.0;
+          M2(rrmat, m, nmat) = 1.0;
+        }
+        /* sign convention: coefficient of the incoming band non-negative (the reference
+         * inherits the arbitrary sign of the LAPACK eigenvector here) */
+        if (M2(rrmat, 1, nmat) < 0.0)
+          for (int j = 1; j <= m; j++) M2(rrmat, j, nmat) = -M2(rrmat, j, nmat);
+        /* new band (:524-570) */
+        memset(avre, 0, sizeof(double) * n);
+        for (int j = 1; j <= m; j++) {
+          const double cj = M2(rrmat, j, nmat);
+          for (long i = 0; i < n; i++) avre[i] += cj * PH(j - 1)[i];
+        }
+        memcpy(PH(iiter), avre, sizeof(double) * n);
+        memset(avre, 0, sizeof(double) * n);
+        for (int j = 1; j <= m; j++) {
+          const double cj = M2(rrmat, j, nmat);
+          for (long i = 0; i < n; i++) avre[i] += cj * SPH(j - 1)[i];
+        }
+        memcpy(SPH(iiter), avre, sizeof(double) * n);
+        memset(avre, 0, sizeof(double) * n);
+        for (int j = 1; j <= m; j++) {
+          const double cj = M2(rrmat, j, nmat);
+          for (long i = 0; i < n; i++) avre[i] += cj * HPH(j - 1)[i];
+        }
+        memcpy(HPH(iiter), avre, sizeof(double) * n);
+        memset(rpsep, 0, sizeof(double) * n1);
+        for (int j = 1; j <= m; j++) {
+          const double cj = M2(rrmat, j, nmat);
+          for (long i = 0; i < n1; i++) rpsep[i] += cj * RPH(j - 1)[i];
+        }
+        memcpy(RPH(iiter), rpsep, sizeof(double) * n1);
+#undef M2
+      }
+      if (rc) break;
+      memcpy(Psi + (size_t)n * (l - 1), PH(iiter), sizeof(double) * n);
+      memcpy(SPsi + (size_t)n * (l - 1), SPH(iiter), sizeof(double) * n);
+      memcpy(hsvre + (size_t)n * (l - 1), HPH(iiter), sizeof(double) * n);
+      memcpy(Rps + (size_t)n1 * (l - 1), RPH(iiter), sizeof(double) * n1);
+      if (iiter > *ksitmax) *ksitmax = iiter;
+      *ksconv = *ksconv && (sqrt(residual_states[l - 1]) <= epssd);
+    }
+    if (rc) break;
+    if (l_hsv) {
+      for (int i = 1; i <= northo && !rc; i++)
+        rc = ortho_r_01(&c, neigmx, 0, key_ortho_cmpt_innerproduct, key_ortho_cmpt_innerproduct,
+                        Psi, SPsi, Rps);
+      for (int l = 1; l <= neigmx && !rc; l++) {
+        memcpy(rpsep, Rps + (size_t)n1 * (l - 1), sizeof(double) * n1);
+        rc = apply_h(&c, ns, ns, Psi + (size_t)n * (l - 1), rpsep, hsvre + (size_t)n * (l - 1));
+      }
+    }
+  }
+#undef PH
+#undef SPH
+#undef HPH
+#undef RV
+#undef RPH
+  free(phire); free(sphire); free(hphire); free(rvecre); free(rphisep);
+  free(rvre); free(avre); free(asvre); free(rpsep);
+  ctx_free(&c);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ rayleigh-ritz8f.F90:22-125, 493-575 */
+int orc_rayleighritz_r(int natom, int num_ppcell, int num_spe, int nprjmx, int neigmx, int numk,
+                       int nums, int nspv, int num_list, int ncpx, int ncpy, int ncpz, int nperi,
+                       int nk, int ns, int nf, int ndisp, int key_natpri_in, int key_natpri_inps,
+                       double dx, double dy, double dz, const int *nprj, const int *natinf,
+                       const int *lstvec2, const int *indspe, const int *natpri, const int *naps,
+                       const int *ntyppp, const double *dij, const double *veff,
+                       const double *vnlocp, int l_rrb, double *rspsep, double *svecre,
+                       double *ssvre, double *hsvre, double *sval) {
+  (void)numk; (void)ndisp; (void)ntyppp;
+  ctx_t c = {0};
+  c.ncpx = ncpx; c.ncpy = ncpy; c.ncpz = ncpz; c.nf = nf; c.nperi = nperi;
+  c.dx = dx; c.dy = dy; c.dz = dz;
+  c.natom = natom; c.num_spe = num_spe; c.nums = nums; c.nprjmx = nprjmx;
+  c.num_list = num_list; c.num_ppcell = num_ppcell;
+  c.key_in = key_natpri_in; c.key_inps = key_natpri_inps;
+  c.nprj = nprj; c.indspe = indspe; c.natinf = natinf; c.lstvec2 = lstvec2;
+  c.natpri = natpri; c.naps = naps; c.vnlocp = vnlocp; c.dij = dij; c.veff = veff;
+  ctx_alloc(&c);
+  const long n = c.n, n1 = c.n1;
+  const int nsv = ns < nspv ? ns : nspv;
+  const size_t off = (size_t)n * neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
+  const size_t off1 = (size_t)n1 * neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
+  double *Psi = svecre + off, *SPsi = ssvre + off, *Rps = rspsep + off1;
+  double *sv = sval + (size_t)neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
+  int rc = 0;
+  if (l_rrb)
+    for (int l = 1; l <= neigmx && !rc; l++)
+      rc = apply_h(&c, ns, nsv, Psi + (size_t)n * (l - 1), Rps + (size_t)n1 * (l - 1),
+                   hsvre + (size_t)n * (l - 1));
+  if (rc) { ctx_free(&c); return rc; }
+  const int M = neigmx;
+  double *raqr = (double *)xcalloc((size_t)M * M, sizeof(double));
+#pragma omp parallel for schedule(dynamic)
+  for (int j = 0; j < M; j++)
+    for (int i = 0; i < M; i++) {
+      double s = 0.0;
+      const double *a = Psi + (size_t)n * i, *b = hsvre + (size_t)n * j;
+      for (long k = 0; k < n; k++) s += a[k] * b[k];
+      raqr[i + (size_t)M * j] = c.dv * s;
+    }
+  if (orc_dsyev(M, raqr, sv)) {
+    free(raqr);
+    ctx_free(&c);
+    return fail(3, "rayleighritz_diag_r: error in matrix diagonalization");
+  }
+  /* rotate: hsvre = Psi Z; Psi = hsvre; hsvre = SPsi Z; SPsi = hsvre; rspsep = rspsep Z */
+  double *work = (double *)xcalloc((size_t)(n > n1 ? n : n1) * M, sizeof(double));
+  for (int pass = 0; pass < 3; pass++) {
+    double *X = pass == 0 ? Psi : pass == 1 ? SPsi : Rps;
+    const long len = pass == 2 ? n1 : n;
+    double *Y = pass == 2 ? work : hsvre;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < len; i++) {
+      double row[4096];
+      double *rr = M <= 4096 ? row : NULL;
+      if (!rr) continue;
+      for (int k = 0; k < M; k++) rr[k] = X[i + (size_t)len * k];
+      for (int j = 0; j < M; j++) {
+        double s = 0.0;
+        for (int k = 0; k < M; k++) s += rr[k] * raqr[k + (size_t)M * j];
+        Y[i + (size_t)len * j] = s;
+      }
+    }
+    memcpy(X, Y, sizeof(double) * len * M);
+  }
+  free(work);
+  free(raqr);
+  ctx_free(&c);
+  if (M > 4096) return fail(4, "rayleighritz_r oracle: neigmx > 4096 not supported");
+  return 0;
+}
+
+/* ------------------------------------------------------------------ reorderstates8f.F90:8-124 */
+int orc_reorderstates(int nrc, int num_ppcell, int nprjmx, int neigmx, int nums, int ncol,
+                      int numk, int ncpx, int ncpy, int ncpz, int ns1, int nk, double *sval,
+                      double *residual_states, double *rspsep, void *cspsep, double *svecre,
+                      void *sveccm, double *ssvre, void *ssvcm, double *hsvre, void *hsvcm) {
+  (void)numk; (void)cspsep; (void)sveccm; (void)ssvcm; (void)hsvcm;
+  if (nrc != 0) return fail(5, "reorderstates: complex path (nrc=1) is not part of this build");
+  const int M = neigmx, d2 = nums + 1 - ncol;
+  const long n = (long)ncpx * ncpy * ncpz, n1 = (long)nprjmx * num_ppcell;
+  double *sv = sval + (size_t)M * ((ns1 - 1) + (size_t)d2 * (nk - 1));
+  double *rs = residual_states + (size_t)M * ((ns1 - 1) + (size_t)d2 * (nk - 1));
+  const size_t off = (size_t)n * M * ((ns1 - 1) + (size_t)nums * (nk - 1));
+  const size_t off1 = (size_t)n1 * M * ((ns1 - 1) + (size_t)nums * (nk - 1));
+  int *lank = (int *)xcalloc(M, sizeof(int));
+  double *st = (double *)xcalloc(M, sizeof(double)), *rt = (double *)xcalloc(M, sizeof(double));
+  double *rp = (double *)xcalloc((size_t)n1 * M, sizeof(double));
+  lank[0] = 1;
+  for (int j = 2; j <= M; j++) {
+    lank[j - 1] = 1;
+    for (int i = 1; i <= j - 1; i++) {
+      if (sv[i - 1] > sv[j - 1]) lank[i - 1]++;
+      else lank[j - 1]++;
+    }
+  }
+  for (int j = 1; j <= M; j++) {
+    st[lank[j - 1] - 1] = sv[j - 1];
+    rt[lank[j - 1] - 1] = rs[j - 1];
+    memcpy(rp + (size_t)n1 * (lank[j - 1] - 1), rspsep + off1 + (size_t)n1 * (j - 1), sizeof(double) * n1);
+    memcpy(hsvre + (size_t)n * (lank[j - 1] - 1), svecre + off + (size_t)n * (j - 1), sizeof(double) * n);
+  }
+  for (int j = 1; j <= M; j++) {
+    sv[j - 1] = st[j - 1];
+    rs[j - 1] = rt[j - 1];
+  }
+  memcpy(rspsep + off1, rp, sizeof(double) * n1 * M);
+  memcpy(svecre + off, hsvre, sizeof(double) * n * M);
+  for (int j = 1; j <= M; j++)
+    memcpy(hsvre + (size_t)n * (lank[j - 1] - 1), ssvre + off + (size_t)n * (j - 1), sizeof(double) * n);
+  memcpy(ssvre + off, hsvre, sizeof(double) * n * M);
+  free(lank); free(st); free(rt); free(rp);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ scf_hartree8f.F90:11-1143 */
+/* scf_hartee_laplacian (:901-1143): av = -1/2 lap v, x sweep with diagonal, then y, z. */
+static void hartree_lap(int nx, int ny, int nz, int nfh, const double *w, const double *v,
+                        double *av, double diag) {
+  const int nf = nfh, nf1 = nfh - 1, ncpx = nx, ncpy = ny;
+  const long ex = nx + 2 * nf, ey = ny + 2 * nf;
+  (void)ncpx;
+#pragma omp parallel for schedule(static)
+  for (int iz = 1; iz <= nz; iz++)
+    for (int iy = 1; iy <= ny; iy++)
+      for (int ix = 1; ix <= nx; ix++) {
+        double s = 0.0;
+        for (int m = nf; m >= 1; m--) s += w[m] * v[HX(ix - m, iy, iz)];
+        s += diag * v[HX(ix, iy, iz)];
+        for (int m = 1; m <= nf; m++) s += w[m] * v[HX(ix + m, iy, iz)];
+        av[IX3(ix, iy, iz)] = s;
+      }
+#pragma omp parallel for schedule(static)
+  for (int iz = 1; iz <= nz; iz++)
+    for (int iy = 1; iy <= ny; iy++)
+      for (int ix = 1; ix <= nx; ix++) {
+        double s = av[IX3(ix, iy, iz)];
+        for (int m = nf; m >= 1; m--) s += w[9 + m] * v[HX(ix, iy - m, iz)];
+        for (int m = 1; m <= nf; m++) s += w[9 + m] * v[HX(ix, iy + m, iz)];
+        av[IX3(ix, iy, iz)] = s;
+      }
+#pragma omp parallel for schedule(static)
+  for (int iz = 1; iz <= nz; iz++)
+    for (int iy = 1; iy <= ny; iy++)
+      for (int ix = 1; ix <= nx; ix++) {
+        double s = av[IX3(ix, iy, iz)];
+        for (int m = nf; m >= 1; m--) s += w[18 + m] * v[HX(ix, iy, iz - m)];
+        for (int m = 1; m <= nf; m++) s += w[18 + m] * v[HX(ix, iy, iz + m)];
+        av[IX3(ix, iy, iz)] = s;
+      }
+}
+
+int orc_scf_hartree(int nd, int ncpx_d, int ncpy_d, int ncpz_d, int nfh, int ndisp, int nperi,
+                    int ncgres, int ncgmin, int ncgmax, double xmax, double ymax, double zmax,
+                    double epsvh, const double *rho, const double *boundx, const double *boundy,
+                    const double *boundz, double *vh) {
+  (void)ndisp;
+  if (nfh != 4 && nfh != 5 && nfh != 6)
+    return fail(6, "error in scf_hartree! This subroutine is for nfh ne 4, 5 or 6.");
+  const int nx = ncpx_d, ny = ncpy_d, nz = ncpz_d, ncpx = nx, ncpy = ny;
+  const long n = (long)nx * ny * nz;
+  const double pi = acos(-1.0);
+  const double ddx = 2.0 * xmax / nx, ddy = 2.0 * ymax / ny, ddz = 2.0 * zmax / nz;
+  double acfdh[9], w[27];
+  orc_tools_definefdcoef(nfh, acfdh);
+  for (int m = 0; m < 9; m++) {
+    w[m] = acfdh[m] / ddx / ddx * (-0.5);
+    w[9 + m] = acfdh[m] / ddy / ddy * (-0.5);
+    w[18 + m] = acfdh[m] / ddz / ddz * (-0.5);
+  }
+  const double diag = acfdh[0] / (ddx * ddx) * (-0.5) + acfdh[0] / (ddy * ddy) * (-0.5) +
+                      acfdh[0] / (ddz * ddz) * (-0.5);
+  double *b = (double *)xcalloc(n, sizeof(double));
+  for (long i = 0; i < n; i++) b[i] = 2.0 * pi * rho[i];
+  /* Dirichlet boundary planes of isolated directions (:88-595).  bound arrays hold the
+   * nfh outer planes on each side: index -(nfh-1)..0 below, 1..nfh above (nd = 0). */
+  if (nd == 0) {
+    const int nb = 2 * nfh;
+    if (nperi <= 2) /* z */
+      for (int iy = 1; iy <= ny; iy++)
+        for (int ix = 1; ix <= nx; ix++)
+          for (int m = 1; m <= nfh; m++)
+            for (int q = 0; q <= nfh - m; q++) {
+              /* b(z=1+q) -= c_{m+q} bound(z=1-m) ; b(z=nz-q) -= c_{m+q} bound(z=nz+m) */
+              const int mm = m + q;
+              if (mm > nfh) continue;
+              const double lo = boundz[(ix - 1) + (size_t)nx * ((iy - 1) + (size_t)ny * (size_t)(1 - m + nfh - 1))];
+              const double hi = boundz[(ix - 1) + (size_t)nx * ((iy - 1) + (size_t)ny * (size_t)(m + nfh - 1))];
+              b[IX3(ix, iy, 1 + q)] -= w[18 + mm] * lo;
+              b[IX3(ix, iy, nz - q)] -= w[18 + mm] * hi;
+            }
+    if (nperi <= 1) /* y */
+      for (int iz = 1; iz <= nz; iz++)
+        for (int ix = 1; ix <= nx; ix++)
+          for (int m = 1; m <= nfh; m++)
+            for (int q = 0; q <= nfh - m; q++) {
+              const int mm = m + q;
+              const double lo = boundy[(ix - 1) + (size_t)nx * ((size_t)(1 - m + nfh - 1) + (size_t)nb * (iz - 1))];
+              const double hi = boundy[(ix - 1) + (size_t)nx * ((size_t)(m + nfh - 1) + (size_t)nb * (iz - 1))];
+              b[IX3(ix, 1 + q, iz)] -= w[9 + mm] * lo;
+              b[IX3(ix, ny - q, iz)] -= w[9 + mm] * hi;
+            }
+    if (nperi == 0) /* x */
+      for (int iz = 1; iz <= nz; iz++)
+        for (int iy = 1; iy <= ny; iy++)
+          for (int m = 1; m <= nfh; m++)
+            for (int q = 0; q <= nfh - m; q++) {
+              const int mm = m + q;
+              const double lo = boundx[(size_t)(1 - m + nfh - 1) + (size_t)nb * ((iy - 1) + (size_t)ny * (iz - 1))];
+              const double hi = boundx[(size_t)(m + nfh - 1) + (size_t)nb * ((iy - 1) + (size_t)ny * (iz - 1))];
+              b[IX3(1 + q, iy, iz)] -= w[mm] * lo;
+              b[IX3(nx - q, iy, iz)] -= w[mm] * hi;
+            }
+  }
+  /* scf_hartree_cg (:626-812) */
+  if (nperi == 3) {
+    double bve = 0.0;
+    for (long i = 0; i < n; i++) bve += b[i] * ddx * ddy * ddz;
+    bve = bve / (8.0 * xmax * ymax * zmax);
+    for (long i = 0; i < n; i++) b[i] -= bve;
+  }
+  const int nf = nfh, nf1 = nfh - 1;
+  const long ex = nx + 2 * nf, ey = ny + 2 * nf, nh = ex * ey * (nz + 2 * nf);
+  double *v = (double *)xcalloc(nh, sizeof(double)), *av = (double *)xcalloc(n, sizeof(double));
+  double *r = (double *)xcalloc(n, sizeof(double)), *p = (double *)xcalloc(n, sizeof(double));
+  double *avhs = (double *)xcalloc(n, sizeof(double));
+  for (int iz = 1; iz <= nz; iz++)
+    for (int iy = 1; iy <= ny; iy++)
+      for (int ix = 1; ix <= nx; ix++) v[HX(ix, iy, iz)] = vh[IX3(ix, iy, iz)];
+  orc_overlap_finitedifference_r(nperi, nx, ny, nz, nf, nf1, nf, v);
+  hartree_lap(nx, ny, nz, nfh, w, v, av, diag);
+  for (long i = 0; i < n; i++) avhs[i] = av[i];
+  for (long i = 0; i < n; i++) r[i] = b[i] - av[i];
+  for (long i = 0; i < n; i++) p[i] = r[i];
+  int loopcg = 0;
+  for (;;) {
+    loopcg++;
+    for (int iz = 1; iz <= nz; iz++)
+      for (int iy = 1; iy <= ny; iy++)
+        for (int ix = 1; ix <= nx; ix++) v[HX(ix, iy, iz)] = p[IX3(ix, iy, iz)];
+    orc_overlap_finitedifference_r(nperi, nx, ny, nz, nf, nf1, nf, v);
+    hartree_lap(nx, ny, nz, nfh, w, v, av, diag);
+    double r1r = 0.0, pap = 0.0;
+#pragma omp parallel for reduction(+ : r1r, pap) schedule(static)
+    for (long i = 0; i < n; i++) {
+      r1r += r[i] * r[i];
+      pap += p[i] * av[i];
+    }
+    const double alph = r1r / pap;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+      vh[i] = vh[i] + alph * p[i];
+      avhs[i] = avhs[i] + alph * av[i];
+    }
+    double r2r = 0.0;
+#pragma omp parallel for reduction(+ : r2r) schedule(static)
+    for (long i = 0; i < n; i++) {
+      r[i] = b[i] - avhs[i];
+      r2r += r[i] * r[i];
+    }
+    double beta = r2r / r1r;
+    if (loopcg % ncgres == 0) beta = 0.0;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) p[i] = r[i] + beta * p[i];
+    const double dp = sqrt(r2r * ddx * ddy * ddz / (8.0 * xmax * ymax * zmax));
+    if (loopcg < ncgmin) continue;
+    if (loopcg >= ncgmax) break;
+    if (dp > epsvh) continue;
+    break;
+  }
+  free(b); free(v); free(av); free(r); free(p); free(avhs);
+  return 0;
+}
+
+/* tools8f.F90:354-378 */
+int orc_tools_correctpotential(int ncpx_d, int ncpy_d, int ncpz_d, double xmax, double ymax,
+                               double zmax, double ddx, double ddy, double ddz, double *vh) {
+  const long n = (long)ncpx_d * ncpy_d * ncpz_d;
+  const double omegain = 1.0 / (8.0 * xmax * ymax * zmax);
+  double s = 0.0;
+  for (long i = 0; i < n; i++) s += vh[i];
+  const double ave = s * ddx * ddy * ddz * omegain;
+  for (long i = 0; i < n; i++) vh[i] = vh[i] - ave;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ scf_charge8f.F90:290-321 */
+int orc_scf_charge_smt_r(int neigmx, int nums, int nspv, int numk, int ncpx, int ncpy, int ncpz,
+                         const double *fnele, const double *svecre, double *rhosmt) {
+  const long n = (long)ncpx * ncpy * ncpz;
+  memset(rhosmt, 0, sizeof(double) * n * nspv);
+  for (int ns = 1; ns <= nums; ns++) {
+    const int nsr = ns < nspv ? ns : nspv;
+    for (int nk = 1; nk <= numk; nk++)
+      for (int l = 1; l <= neigmx; l++) {
+        const double t = fnele[(l - 1) + (size_t)neigmx * ((ns - 1) + (size_t)nums * (nk - 1))];
+        const double *w = svecre + (size_t)n * ((l - 1) + (size_t)neigmx * ((ns - 1) + (size_t)nums * (nk - 1)));
+        double *rho = rhosmt + (size_t)n * (nsr - 1);
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; i++) rho[i] = rho[i] + t * w[i] * w[i];
+      }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ force8f.F90:9-195, 409-1010 */
+/* First-derivative weights of force_r_02 (:408-608): d/dx at order nf, returned so that
+ * dpsi(i) = sum_{m=1..nf} a[m] (psi(i+m) - psi(i-m)) / h. */
+static int fd1_weights(int nf, double *a) {
+  for (int m = 0; m < 9; m++) a[m] = 0.0;
+  switch (nf) {
+  case 1: a[1] = 0.5; break;
+  case 2: a[1] = 8.0 / 12.0; a[2] = -1.0 / 12.0; break;
+  case 3: a[1] = 45.0 / 60.0; a[2] = -9.0 / 60.0; a[3] = 1.0 / 60.0; break;
+  case 4: a[1] = 672.0 / 840.0; a[2] = -168.0 / 840.0; a[3] = 32.0 / 840.0; a[4] = -3.0 / 840.0; break;
+  default: return 1;
+  }
+  return 0;
+}
+
+int orc_force_eig_r(int nperi, int numk, int nums, int neigmx, int nf, int ncpx, int ncpy,
+                    int ncpz, int num_list, int natom, int num_spe, int num_ppcell, int nprjmx,
+                    int key_natpri_in, int key_natpri_inps, const int *lstvec2, const int *latom,
+                    const int *lstx, const int *lsty, const int *lstz, const int *nprj,
+                    const int *natpri, const int *naps, const int *indspe, const int *natinf,
+                    double xmax, double ymax, double zmax, const double *sss,
+                    const double *vnlocp, const double *dij, const double *svecre,
+                    const double *sval, const double *fnele, double *fatx, double *faty,
+                    double *fatz) {
+  (void)latom; (void)lstx; (void)lsty; (void)lstz;
+  double a[9];
+  if (fd1_weights(nf, a)) return fail(7, "force_eig_r: nf outside 1-4 is not part of this build");
+  const long n = (long)ncpx * ncpy * ncpz, n1 = (long)nprjmx * num_ppcell;
+  const double dx = 2.0 * xmax / ncpx, dy = 2.0 * ymax / ncpy, dz = 2.0 * zmax / ncpz;
+  const int nf1 = nf - 1;
+  const long ex = ncpx + 2 * nf, ey = ncpy + 2 * nf, nh = ex * ey * (ncpz + 2 * nf);
+  double *vre = (double *)xcalloc(nh, sizeof(double)), *avre = (double *)xcalloc(n, sizeof(double));
+  double *rps = (double *)xcalloc(n1 * neigmx, sizeof(double)), *rdp = (double *)xcalloc(n1, sizeof(double));
+  for (int na = 0; na < natom; na++) fatx[na] = faty[na] = fatz[na] = 0.0;
+  for (int nk = 1; nk <= numk; nk++)
+    for (int ns = 1; ns <= nums; ns++) {
+      const size_t boff = (size_t)neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
+      for (int l = 1; l <= neigmx; l++) {
+        memset(rps + n1 * (l - 1), 0, sizeof(double) * n1);
+        orc_nonlocaloperation_r_01(natom, nprjmx, num_spe, num_ppcell, num_list, ncpx, ncpy, ncpz,
+                                   key_natpri_in, key_natpri_inps, nprj, indspe, natinf, natpri,
+                                   naps, lstvec2, vnlocp, svecre + (size_t)n * (boff + l - 1),
+                                   rps + n1 * (l - 1));
+      }
+      for (int lmn = 1; lmn <= 3; lmn++) {
+        const double h = lmn == 1 ? dx : lmn == 2 ? dy : dz;
+        for (int l = 1; l <= neigmx; l++) {
+          const double *psi = svecre + (size_t)n * (boff + l - 1);
+          memset(vre, 0, sizeof(double) * nh);
+          for (int iz = 1; iz <= ncpz; iz++)
+            for (int iy = 1; iy <= ncpy; iy++)
+              for (int ix = 1; ix <= ncpx; ix++) vre[HX(ix, iy, iz)] = psi[IX3(ix, iy, iz)];
+          orc_overlap_finitedifference_r(nperi, ncpx, ncpy, ncpz, nf, nf1, nf, vre);
+          const int sx = lmn == 1, sy = lmn == 2, sz = lmn == 3;
+#pragma omp parallel for schedule(static)
+          for (int iz = 1; iz <= ncpz; iz++)
+            for (int iy = 1; iy <= ncpy; iy++)
+              for (int ix = 1; ix <= ncpx; ix++) {
+                /* reference order: -a_nf psi(+nf) ... -a_1 psi(+1) + a_1 psi(-1) ... + a_nf psi(-nf)
+                 * with a_m = -weight_m / h, i.e. sum_m weight_m (psi(+m) - psi(-m)) / h */
+                double s = 0.0;
+                for (int m = nf; m >= 1; m--) s += (a[m] / h) * vre[HX(ix + sx * m, iy + sy * m, iz + sz * m)];
+                for (int m = 1; m <= nf; m++) s -= (a[m] / h) * vre[HX(ix - sx * m, iy - sy * m, iz - sz * m)];
+                avre[IX3(ix, iy, iz)] = s;
+              }
+          memset(rdp, 0, sizeof(double) * n1);
+          orc_nonlocaloperation_r_01(natom, nprjmx, num_spe, num_ppcell, num_list, ncpx, ncpy, ncpz,
+                                     key_natpri_in, key_natpri_inps, nprj, indspe, natinf, natpri,
+                                     naps, lstvec2, vnlocp, avre, rdp);
+          const double f = fnele[boff + l - 1], ev = sval[boff + l - 1];
+          for (int na = 1; na <= natom; na++) {
+            if (natpri[na - 1] != key_natpri_in) continue;
+            const int iaps = naps[na - 1], ispe = indspe[na - 1], ip = nprj[ispe - 1];
+            const double *rp = rps + n1 * (l - 1) + (size_t)nprjmx * (iaps - 1);
+            const double *rd = rdp + (size_t)nprjmx * (iaps - 1);
+            const double *D = dij + (size_t)nprjmx * nprjmx * ((ns - 1) + (size_t)nums * (na - 1));
+            const double *S = sss + (size_t)nprjmx * nprjmx * (ispe - 1);
+            double *fa = lmn == 1 ? fatx : lmn == 2 ? faty : fatz;
+            for (int j = 0; j < ip; j++)
+              for (int i = 0; i < ip; i++)
+                fa[na - 1] = fa[na - 1] - 2.0 * f * (rp[i] * rd[j]) *
+                                              (D[i + (size_t)nprjmx * j] - ev * S[i + (size_t)nprjmx * j]);
+          }
+        }
+      }
+    }
+  free(vre); free(avre); free(rps); free(rdp);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ totalenergy8f.F90:151-199 */
+int orc_totalenergy_eig(int nums, int ncol, int numk, int neigmx, const int *nwskk, int nwskptot,
+                        double tf, const double *sval, const double *fnele, double *eneel,
+                        double *eneth) {
+  const int d2 = nums + 1 - ncol;
+  double e = 0.0, t = 0.0;
+  for (int nk = 1; nk <= numk; nk++)
+    for (int ns = 1; ns <= d2; ns++)
+      for (int l = 1; l <= neigmx; l++) {
+        const size_t k = (l - 1) + (size_t)neigmx * ((ns - 1) + (size_t)d2 * (nk - 1));
+        e = e + fnele[k] * sval[k];
+      }
+  for (int nk = 1; nk <= numk; nk++) {
+    const double tmpfac = (double)(nwskptot * nums) / (double)(2 * nwskk[nk - 1]);
+    for (int ns = 1; ns <= d2; ns++)
+      for (int l = 1; l <= neigmx; l++) {
+        const size_t k = (l - 1) + (size_t)neigmx * ((ns - 1) + (size_t)d2 * (nk - 1));
+        const double te = fnele[k] * tmpfac;
+        double t1 = 0.0, t2 = 0.0;
+        if (te > 1.0e-8) t1 = log(te);
+        if (1.0 - te > 1.0e-8) t2 = log(1.0 - te);
+        t = t + (te * t1 + (1.0 - te) * t2) / tmpfac;
+      }
+  }
+  *eneel = e;
+  *eneth = t * tf;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ scf_diag8f.F90:6-316 (nrc = 0) */
+int orc_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, int iscf, int ncgscf,
+                 int nretcg, int natom, int neigmx, int nums, int ncol, int nspv, int numk,
+                 int nperi, int northo, int nf, int ndisp, int num_list, int ncpx, int ncpy,
+                 int ncpz, int npxmax, int npymax, int npzmax, int nchange, int nrrz,
+                 int key_ortho_nocmpt_innerproduct, int key_ortho_cmpt_innerproduct,
+                 int key_natpri_in, int key_natpri_inps, int key_natpri_out, int key_soc_calc,
+                 int key_pp_paw, const double *skpxx, const double *skpyy, const double *skpzz,
+                 double dx, double dy, double dz, double xmax, double ymax, double zmax,
+                 const double *dij, const double *dijsoc, const double *veff,
+                 const double *vnlocp, const double *sss, int nkscg, int nsdmax, int nprecon_cg,
+                 double epssd, int ndiismax, int nprecon_diis, double eps_eig_diis,
+                 double alambda_diis, double ratio_diis, double alambda_min, double alambda_max,
+                 const int *indspe, const int *natpri, const int *naps, const int *nprj,
+                 const int *natinf, const int *lstvec2, const int *latom, const int *natsoc,
+                 const int *ntyppp, const int *lstx, const int *lsty, const int *lstz,
+                 const int *natx, const int *naty, const int *natz, double *rspsep, void *cspsep,
+                 double *svecre, void *sveccm, double *hsvre, void *hsvcm, double *ssvre,
+                 void *ssvcm, double *sval, double *svalsoc, void *cspsepsoc, void *svecsoc,
+                 double *residual_states, int *ksconv, int *ksitmax) {
+  (void)npxmax; (void)npymax; (void)npzmax; (void)key_soc_calc; (void)key_pp_paw;
+  (void)skpxx; (void)skpyy; (void)skpzz; (void)dijsoc; (void)natsoc; (void)lstx; (void)lsty;
+  (void)lstz; (void)natx; (void)naty; (void)natz; (void)svalsoc; (void)cspsepsoc; (void)svecsoc;
+  if (nrc != 0 || nso != 0 || ncol != 1)
+    return fail(5, "scf_diag: only the real path (nrc=0, nso=0, ncol=1) is part of this build");
+  const int l_soc2nd = (nso == 1) && (ncol == 1);
+  int l_rrb, l_rra, l_hsv, l_reo;
+  if (nrrz != 0) {
+    const int a = nrrz < 0 ? -nrrz : nrrz;
+    l_rrb = (iscf % a == 0) && (nrrz > 0);
+    l_rra = (iscf % a == 0) && (nrrz < 0);
+    l_hsv = l_rra || l_soc2nd || ((nrrz > 0) && ((iscf + 1) % nrrz == 0));
+  } else {
+    l_rrb = l_rra = 0;
+    l_hsv = l_soc2nd;
+  }
+  l_reo = nchange > 0 ? ((iscf % nchange == 0) && !l_rra && !l_soc2nd) : 0;
+  *ksconv = 1;
+  *ksitmax = 0;
+  const int d2 = nums + 1 - ncol;
+  int rc = 0;
+  for (int nk = 1; nk <= numk && !rc; nk++) {
+    for (int ns = 1; ns <= nums && !rc; ns++) {
+      double *rs = residual_states + (size_t)neigmx * ((ns - 1) + (size_t)d2 * (nk - 1));
+      if (l_rrb)
+        rc = orc_rayleighritz_r(natom, num_ppcell, num_spe, nprjmx, neigmx, numk, nums, nspv,
+                                num_list, ncpx, ncpy, ncpz, nperi, nk, ns, nf, ndisp,
+                                key_natpri_in, key_natpri_inps, dx, dy, dz, nprj, natinf, lstvec2,
+                                indspe, natpri, naps, ntyppp, dij, veff, vnlocp, l_rrb, rspsep,
+                                svecre, ssvre, hsvre, sval);
+      if (rc) break;
+      if (iscf <= ncgscf || iscf % nretcg == 0)
+        rc = orc_kscg_r(natom, num_spe, neigmx, nums, nspv, numk, nperi, northo, ns, nk, nf,
+                        nkscg, nsdmax, nprecon_cg, nprjmx, num_list, num_ppcell, ndisp, ncpx, ncpy,
+                        ncpz, key_ortho_nocmpt_innerproduct, key_ortho_cmpt_innerproduct,
+                        key_natpri_in, key_natpri_inps, key_natpri_out, dx, dy, dz, epssd, indspe,
+                        natpri, naps, nprj, natinf, lstvec2, latom, ntyppp, dij, veff, vnlocp, sss,
+                        ksconv, ksitmax, sval, rspsep, svecre, ssvre, rs, hsvre);
+      else
+        rc = orc_ksdiis_r(l_hsv, natom, num_spe, neigmx, nums, nspv, numk, nperi, ndiismax,
+                          nprecon_diis, nk, ns, nf, nprjmx, num_list, num_ppcell, ndisp, northo,
+                          ncpx, ncpy, ncpz, key_ortho_cmpt_innerproduct, key_natpri_in,
+                          key_natpri_inps, eps_eig_diis, alambda_diis, ratio_diis, alambda_min,
+                          alambda_max, epssd, xmax, ymax, zmax, indspe, natpri, naps, nprj, natinf,
+                          lstvec2, latom, ntyppp, dij, veff, vnlocp, sss, ksconv, ksitmax, rspsep,
+                          sval, svecre, ssvre, rs, hsvre);
+      if (rc) break;
+      if (l_rra)
+        rc = orc_rayleighritz_r(natom, num_ppcell, num_spe, nprjmx, neigmx, numk, nums, nspv,
+                                num_list, ncpx, ncpy, ncpz, nperi, nk, ns, nf, ndisp,
+                                key_natpri_in, key_natpri_inps, dx, dy, dz, nprj, natinf, lstvec2,
+                                indspe, natpri, naps, ntyppp, dij, veff, vnlocp, l_rrb, rspsep,
+                                svecre, ssvre, hsvre, sval);
+    }
+    if (!rc && l_reo)
+      for (int ns = 1; ns <= nums && !rc; ns++)
+        rc = orc_reorderstates(nrc, num_ppcell, nprjmx, neigmx, nums, ncol, numk, ncpx, ncpy, ncpz,
+                               ns, nk, sval, residual_states, rspsep, cspsep, svecre, sveccm,
+                               ssvre, ssvcm, hsvre, hsvcm);
+  }
+  return rc;
+}

@@ -1,0 +1,145 @@
+/*
+ * kukan_oracle.h — FP64 CPU restatement of the RSPACE kukan v8f Kohn-Sham hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for libkukan_hip.so: only the
+ * tests, the smoke check and the CPU-baseline leg of bench.py load it.  It is never part
+ * of the product path.
+ *
+ * Each orc_<routine> has exactly the argument list of kukan_<routine> in
+ * include/kukan_hip.h, which in turn is the reference Fortran routine's list.  Arrays are
+ * Fortran-ordered (column-major, first index fastest); lstvec2 stays 1-based.
+ * Routines return 0 on success and a nonzero code (message via orc_last_error()) where the
+ * reference calls stopp().
+ */
+#ifndef KUKAN_ORACLE_H
+#define KUKAN_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *orc_last_error(void);
+int orc_num_threads(void);
+
+int orc_tools_definefdcoef(int nf, double *acfd);
+int orc_overlap_finitedifference_r(int nperi, int ncpa, int ncpb, int ncpc, int nf, int nf1,
+                                   int nsnd, double *v_cc);
+int orc_kslaplacian_r(int ncpx, int ncpy, int ncpz, int neigmx, int nf, double dx, double dy,
+                      double dz, const double *vre, const double *veff, int l, double *avre);
+int orc_ksprecondition(int ncpx, int ncpy, int ncpz, int nf, int ncol, const double *vre,
+                       double *avre);
+int orc_nonlocaloperation_r_01(int natom, int nprjmx, int num_spe, int num_ppcell, int num_list,
+                               int ncpx, int ncpy, int ncpz, int key_natpri_in,
+                               int key_natpri_inps, const int *nprj, const int *indspe,
+                               const int *natinf, const int *natpri, const int *naps,
+                               const int *lstvec2, const double *vnlocp, const double *vre2,
+                               double *rpsep);
+int orc_nonlocaloperation_r_02(int nhs, int natom, int num_spe, int nums, int nprjmx, int ns,
+                               int num_list, int num_ppcell, int ncpx, int ncpy, int ncpz,
+                               int key_natpri_in, int key_natpri_inps, double dx, double dy,
+                               double dz, const int *nprj, const int *indspe, const int *natinf,
+                               const int *lstvec2, const int *natpri, const int *naps,
+                               const double *vnlocp, const double *rpsep, const double *dijsss,
+                               double *avre, double *avr);
+int orc_kscg_r(int natom, int num_spe, int neigmx, int nums, int nspv, int numk, int nperi,
+               int northo, int ns, int nk, int nf, int nkscg, int nsdmax, int nprecon_cg,
+               int nprjmx, int num_list, int num_ppcell, int ndisp, int ncpx, int ncpy, int ncpz,
+               int key_ortho_nocmpt_innerproduct, int key_ortho_cmpt_innerproduct,
+               int key_natpri_in, int key_natpri_inps, int key_natpri_out, double dx, double dy,
+               double dz, double epssd, const int *indspe, const int *natpri, const int *naps,
+               const int *nprj, const int *natinf, const int *lstvec2, const int *latom,
+               const int *ntyppp, const double *dij, const double *veff, const double *vnlocp,
+               const double *sss, int *ksconv, int *ksitmax, double *sval, double *rspsep,
+               double *svecre, double *ssvre, double *residual_states, double *hsvre);
+int orc_ksdiis_r(int l_hsv, int natom, int num_spe, int neigmx, int nums, int nspv, int numk,
+                 int nperi, int ndiismax, int nprecon_diis, int nk, int ns, int nf, int nprjmx,
+                 int num_list, int num_ppcell, int ndisp, int northo, int ncpx, int ncpy,
+                 int ncpz, int key_ortho_cmpt_innerproduct, int key_natpri_in,
+                 int key_natpri_inps, double eps_eig_diis, double alambda_diis,
+                 double ratio_diis, double alambda_min, double alambda_max, double epssd,
+                 double xmax, double ymax, double zmax, const int *indspe, const int *natpri,
+                 const int *naps, const int *nprj, const int *natinf, const int *lstvec2,
+                 const int *latom, const int *ntyppp, const double *dij, const double *veff,
+                 const double *vnlocp, const double *sss, int *ksconv, int *ksitmax,
+                 double *rspsep, double *sval, double *svecre, double *ssvre,
+                 double *residual_states, double *hsvre);
+int orc_orthogonalization_r(int natom, int neigmx, int nprjmx, int nums, int numk, int num_ppcell,
+                            int num_spe, int northo, int num_list, int nflag, int ndisp, int ncpx,
+                            int ncpy, int ncpz, int key_ortho_cmpt_innerproduct,
+                            int key_natpri_in, int key_natpri_inps, double dx, double dy,
+                            double dz, const int *nprj, const int *indspe, const int *natpri,
+                            const int *naps, const int *natinf, const int *lstvec2,
+                            const int *latom, const int *ntyppp, const double *vnlocp,
+                            const double *sss, double *svecre, double *ssvre, double *rspsep);
+int orc_orthogonalization_r_01(int natom, int neigmx, int nprjmx, int nums, int numk, int nk,
+                               int ns, int l_inp, int num_list, int num_ppcell, int num_spe,
+                               int nflag, int ndisp, int ncpx, int ncpy, int ncpz,
+                               int key_ortho_cmpt_innerproduct, int key_natpri_in,
+                               int key_natpri_inps, double dx, double dy, double dz,
+                               const int *nprj, const int *indspe, const int *natpri,
+                               const int *naps, const int *natinf, const int *lstvec2,
+                               const int *latom, const double *vnlocp, const double *sss,
+                               double *svecre, double *ssvre, double *rspsep);
+int orc_rayleighritz_r(int natom, int num_ppcell, int num_spe, int nprjmx, int neigmx, int numk,
+                       int nums, int nspv, int num_list, int ncpx, int ncpy, int ncpz, int nperi,
+                       int nk, int ns, int nf, int ndisp, int key_natpri_in, int key_natpri_inps,
+                       double dx, double dy, double dz, const int *nprj, const int *natinf,
+                       const int *lstvec2, const int *indspe, const int *natpri, const int *naps,
+                       const int *ntyppp, const double *dij, const double *veff,
+                       const double *vnlocp, int l_rrb, double *rspsep, double *svecre,
+                       double *ssvre, double *hsvre, double *sval);
+int orc_reorderstates(int nrc, int num_ppcell, int nprjmx, int neigmx, int nums, int ncol,
+                      int numk, int ncpx, int ncpy, int ncpz, int ns1, int nk, double *sval,
+                      double *residual_states, double *rspsep, void *cspsep, double *svecre,
+                      void *sveccm, double *ssvre, void *ssvcm, double *hsvre, void *hsvcm);
+int orc_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, int iscf,
+                 int ncgscf, int nretcg, int natom, int neigmx, int nums, int ncol, int nspv,
+                 int numk, int nperi, int northo, int nf, int ndisp, int num_list, int ncpx,
+                 int ncpy, int ncpz, int npxmax, int npymax, int npzmax, int nchange, int nrrz,
+                 int key_ortho_nocmpt_innerproduct, int key_ortho_cmpt_innerproduct,
+                 int key_natpri_in, int key_natpri_inps, int key_natpri_out, int key_soc_calc,
+                 int key_pp_paw, const double *skpxx, const double *skpyy, const double *skpzz,
+                 double dx, double dy, double dz, double xmax, double ymax, double zmax,
+                 const double *dij, const double *dijsoc, const double *veff,
+                 const double *vnlocp, const double *sss, int nkscg, int nsdmax, int nprecon_cg,
+                 double epssd, int ndiismax, int nprecon_diis, double eps_eig_diis,
+                 double alambda_diis, double ratio_diis, double alambda_min,
+                 double alambda_max, const int *indspe, const int *natpri, const int *naps,
+                 const int *nprj, const int *natinf, const int *lstvec2, const int *latom,
+                 const int *natsoc, const int *ntyppp, const int *lstx, const int *lsty,
+                 const int *lstz, const int *natx, const int *naty, const int *natz,
+                 double *rspsep, void *cspsep, double *svecre, void *sveccm, double *hsvre,
+                 void *hsvcm, double *ssvre, void *ssvcm, double *sval, double *svalsoc,
+                 void *cspsepsoc, void *svecsoc, double *residual_states, int *ksconv,
+                 int *ksitmax);
+int orc_scf_hartree(int nd, int ncpx_d, int ncpy_d, int ncpz_d, int nfh, int ndisp, int nperi,
+                    int ncgres, int ncgmin, int ncgmax, double xmax, double ymax, double zmax,
+                    double epsvh, const double *rho_aug_dense, const double *boundx,
+                    const double *boundy, const double *boundz, double *vh_dense);
+int orc_tools_correctpotential(int ncpx_d, int ncpy_d, int ncpz_d, double xmax, double ymax,
+                               double zmax, double ddx, double ddy, double ddz, double *vh_dense);
+int orc_scf_charge_smt_r(int neigmx, int nums, int nspv, int numk, int ncpx, int ncpy, int ncpz,
+                         const double *fnele, const double *svecre, double *rhosmt);
+int orc_force_eig_r(int nperi, int numk, int nums, int neigmx, int nf, int ncpx, int ncpy,
+                    int ncpz, int num_list, int natom, int num_spe, int num_ppcell, int nprjmx,
+                    int key_natpri_in, int key_natpri_inps, const int *lstvec2, const int *latom,
+                    const int *lstx, const int *lsty, const int *lstz, const int *nprj,
+                    const int *natpri, const int *naps, const int *indspe, const int *natinf,
+                    double xmax, double ymax, double zmax, const double *sss,
+                    const double *vnlocp, const double *dij, const double *svecre,
+                    const double *sval, const double *fnele, double *fatx, double *faty,
+                    double *fatz);
+int orc_totalenergy_eig(int nums, int ncol, int numk, int neigmx, const int *nwskk, int nwskptot,
+                        double tf, const double *sval, const double *fnele, double *eneel,
+                        double *eneth);
+
+/* Symmetric eigensolver used wherever the reference calls dsyev: Householder
+ * tridiagonalisation + implicit QL (tred2/tql2 of Wilkinson & Reinsch, Handbook for
+ * Automatic Computation II, 1971).  a(n,n) column-major, lower triangle read; on exit the
+ * columns of a are eigenvectors for ascending w, each signed so that its component of
+ * largest magnitude (lowest index on ties) is positive. */
+int orc_dsyev(int n, double *a, double *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

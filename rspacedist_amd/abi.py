@@ -1,0 +1,213 @@
+"""Argument tables of the C-ABI entry points (include/kukan_hip.h) and a ctypes binder.
+
+Each entry is the reference Fortran routine's argument list in order, with a type code:
+``i`` int32 scalar, ``d`` double scalar, ``I`` int32 array, ``D`` double array,
+``V`` opaque array (complex-path dummy), ``oi``/``od`` int/double output scalar
+(passed by pointer).  ``bind(lib, prefix)`` attaches ctypes prototypes for every routine
+present in ``lib`` under ``prefix + name``; ``call`` marshals keyword arguments:
+numpy arrays (host memory, Fortran order) and torch tensors (``data_ptr()``, host or
+device) are passed as pointers, scalars by value.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+_I, _D = "i", "d"
+
+SIGNATURES = {
+    "tools_definefdcoef": [("nf", "i"), ("acfd", "D")],
+    "overlap_finitedifference_r": [("nperi", "i"), ("ncpa", "i"), ("ncpb", "i"), ("ncpc", "i"),
+                                   ("nf", "i"), ("nf1", "i"), ("nsnd", "i"), ("v_cc", "D")],
+    "kslaplacian_r": [("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("neigmx", "i"), ("nf", "i"),
+                      ("dx", "d"), ("dy", "d"), ("dz", "d"), ("vre", "D"), ("veff", "D"),
+                      ("l", "i"), ("avre", "D")],
+    "ksprecondition": [("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("nf", "i"), ("ncol", "i"),
+                       ("vre", "D"), ("avre", "D")],
+    "nonlocaloperation_r_01": [("natom", "i"), ("nprjmx", "i"), ("num_spe", "i"),
+                               ("num_ppcell", "i"), ("num_list", "i"), ("ncpx", "i"),
+                               ("ncpy", "i"), ("ncpz", "i"), ("key_natpri_in", "i"),
+                               ("key_natpri_inps", "i"), ("nprj", "I"), ("indspe", "I"),
+                               ("natinf", "I"), ("natpri", "I"), ("naps", "I"), ("lstvec2", "I"),
+                               ("vnlocp", "D"), ("vre2", "D"), ("rpsep", "D")],
+    "nonlocaloperation_r_02": [("nhs", "i"), ("natom", "i"), ("num_spe", "i"), ("nums", "i"),
+                               ("nprjmx", "i"), ("ns", "i"), ("num_list", "i"),
+                               ("num_ppcell", "i"), ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"),
+                               ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("dx", "d"),
+                               ("dy", "d"), ("dz", "d"), ("nprj", "I"), ("indspe", "I"),
+                               ("natinf", "I"), ("lstvec2", "I"), ("natpri", "I"), ("naps", "I"),
+                               ("vnlocp", "D"), ("rpsep", "D"), ("dijsss", "D"), ("avre", "D"),
+                               ("avr", "D")],
+    "kscg_r": [("natom", "i"), ("num_spe", "i"), ("neigmx", "i"), ("nums", "i"), ("nspv", "i"),
+               ("numk", "i"), ("nperi", "i"), ("northo", "i"), ("ns", "i"), ("nk", "i"),
+               ("nf", "i"), ("nkscg", "i"), ("nsdmax", "i"), ("nprecon_cg", "i"),
+               ("nprjmx", "i"), ("num_list", "i"), ("num_ppcell", "i"), ("ndisp", "i"),
+               ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"),
+               ("key_ortho_nocmpt_innerproduct", "i"), ("key_ortho_cmpt_innerproduct", "i"),
+               ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("key_natpri_out", "i"),
+               ("dx", "d"), ("dy", "d"), ("dz", "d"), ("epssd", "d"), ("indspe", "I"),
+               ("natpri", "I"), ("naps", "I"), ("nprj", "I"), ("natinf", "I"), ("lstvec2", "I"),
+               ("latom", "I"), ("ntyppp", "I"), ("dij", "D"), ("veff", "D"), ("vnlocp", "D"),
+               ("sss", "D"), ("ksconv", "oi"), ("ksitmax", "oi"), ("sval", "D"), ("rspsep", "D"),
+               ("svecre", "D"), ("ssvre", "D"), ("residual_states", "D"), ("hsvre", "D")],
+    "ksdiis_r": [("l_hsv", "i"), ("natom", "i"), ("num_spe", "i"), ("neigmx", "i"), ("nums", "i"),
+                 ("nspv", "i"), ("numk", "i"), ("nperi", "i"), ("ndiismax", "i"),
+                 ("nprecon_diis", "i"), ("nk", "i"), ("ns", "i"), ("nf", "i"), ("nprjmx", "i"),
+                 ("num_list", "i"), ("num_ppcell", "i"), ("ndisp", "i"), ("northo", "i"),
+                 ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"),
+                 ("key_ortho_cmpt_innerproduct", "i"), ("key_natpri_in", "i"),
+                 ("key_natpri_inps", "i"), ("eps_eig_diis", "d"), ("alambda_diis", "d"),
+                 ("ratio_diis", "d"), ("alambda_min", "d"), ("alambda_max", "d"), ("epssd", "d"),
+                 ("xmax", "d"), ("ymax", "d"), ("zmax", "d"), ("indspe", "I"), ("natpri", "I"),
+                 ("naps", "I"), ("nprj", "I"), ("natinf", "I"), ("lstvec2", "I"), ("latom", "I"),
+                 ("ntyppp", "I"), ("dij", "D"), ("veff", "D"), ("vnlocp", "D"), ("sss", "D"),
+                 ("ksconv", "oi"), ("ksitmax", "oi"), ("rspsep", "D"), ("sval", "D"),
+                 ("svecre", "D"), ("ssvre", "D"), ("residual_states", "D"), ("hsvre", "D")],
+    "orthogonalization_r": [("natom", "i"), ("neigmx", "i"), ("nprjmx", "i"), ("nums", "i"),
+                            ("numk", "i"), ("num_ppcell", "i"), ("num_spe", "i"), ("northo", "i"),
+                            ("num_list", "i"), ("nflag", "i"), ("ndisp", "i"), ("ncpx", "i"),
+                            ("ncpy", "i"), ("ncpz", "i"), ("key_ortho_cmpt_innerproduct", "i"),
+                            ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("dx", "d"),
+                            ("dy", "d"), ("dz", "d"), ("nprj", "I"), ("indspe", "I"),
+                            ("natpri", "I"), ("naps", "I"), ("natinf", "I"), ("lstvec2", "I"),
+                            ("latom", "I"), ("ntyppp", "I"), ("vnlocp", "D"), ("sss", "D"),
+                            ("svecre", "D"), ("ssvre", "D"), ("rspsep", "D")],
+    "orthogonalization_r_01": [("natom", "i"), ("neigmx", "i"), ("nprjmx", "i"), ("nums", "i"),
+                               ("numk", "i"), ("nk", "i"), ("ns", "i"), ("l_inp", "i"),
+                               ("num_list", "i"), ("num_ppcell", "i"), ("num_spe", "i"),
+                               ("nflag", "i"), ("ndisp", "i"), ("ncpx", "i"), ("ncpy", "i"),
+                               ("ncpz", "i"), ("key_ortho_cmpt_innerproduct", "i"),
+                               ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("dx", "d"),
+                               ("dy", "d"), ("dz", "d"), ("nprj", "I"), ("indspe", "I"),
+                               ("natpri", "I"), ("naps", "I"), ("natinf", "I"), ("lstvec2", "I"),
+                               ("latom", "I"), ("vnlocp", "D"), ("sss", "D"), ("svecre", "D"),
+                               ("ssvre", "D"), ("rspsep", "D")],
+    "rayleighritz_r": [("natom", "i"), ("num_ppcell", "i"), ("num_spe", "i"), ("nprjmx", "i"),
+                       ("neigmx", "i"), ("numk", "i"), ("nums", "i"), ("nspv", "i"),
+                       ("num_list", "i"), ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"),
+                       ("nperi", "i"), ("nk", "i"), ("ns", "i"), ("nf", "i"), ("ndisp", "i"),
+                       ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("dx", "d"), ("dy", "d"),
+                       ("dz", "d"), ("nprj", "I"), ("natinf", "I"), ("lstvec2", "I"),
+                       ("indspe", "I"), ("natpri", "I"), ("naps", "I"), ("ntyppp", "I"),
+                       ("dij", "D"), ("veff", "D"), ("vnlocp", "D"), ("l_rrb", "i"),
+                       ("rspsep", "D"), ("svecre", "D"), ("ssvre", "D"), ("hsvre", "D"),
+                       ("sval", "D")],
+    "reorderstates": [("nrc", "i"), ("num_ppcell", "i"), ("nprjmx", "i"), ("neigmx", "i"),
+                      ("nums", "i"), ("ncol", "i"), ("numk", "i"), ("ncpx", "i"), ("ncpy", "i"),
+                      ("ncpz", "i"), ("ns1", "i"), ("nk", "i"), ("sval", "D"),
+                      ("residual_states", "D"), ("rspsep", "D"), ("cspsep", "V"),
+                      ("svecre", "D"), ("sveccm", "V"), ("ssvre", "D"), ("ssvcm", "V"),
+                      ("hsvre", "D"), ("hsvcm", "V")],
+    "scf_diag": [("nrc", "i"), ("nso", "i"), ("nprjmx", "i"), ("num_ppcell", "i"),
+                 ("num_spe", "i"), ("iscf", "i"), ("ncgscf", "i"), ("nretcg", "i"),
+                 ("natom", "i"), ("neigmx", "i"), ("nums", "i"), ("ncol", "i"), ("nspv", "i"),
+                 ("numk", "i"), ("nperi", "i"), ("northo", "i"), ("nf", "i"), ("ndisp", "i"),
+                 ("num_list", "i"), ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("npxmax", "i"),
+                 ("npymax", "i"), ("npzmax", "i"), ("nchange", "i"), ("nrrz", "i"),
+                 ("key_ortho_nocmpt_innerproduct", "i"), ("key_ortho_cmpt_innerproduct", "i"),
+                 ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("key_natpri_out", "i"),
+                 ("key_soc_calc", "i"), ("key_pp_paw", "i"), ("skpxx", "D"), ("skpyy", "D"),
+                 ("skpzz", "D"), ("dx", "d"), ("dy", "d"), ("dz", "d"), ("xmax", "d"),
+                 ("ymax", "d"), ("zmax", "d"), ("dij", "D"), ("dijsoc", "D"), ("veff", "D"),
+                 ("vnlocp", "D"), ("sss", "D"), ("nkscg", "i"), ("nsdmax", "i"),
+                 ("nprecon_cg", "i"), ("epssd", "d"), ("ndiismax", "i"), ("nprecon_diis", "i"),
+                 ("eps_eig_diis", "d"), ("alambda_diis", "d"), ("ratio_diis", "d"),
+                 ("alambda_min", "d"), ("alambda_max", "d"), ("indspe", "I"), ("natpri", "I"),
+                 ("naps", "I"), ("nprj", "I"), ("natinf", "I"), ("lstvec2", "I"), ("latom", "I"),
+                 ("natsoc", "I"), ("ntyppp", "I"), ("lstx", "I"), ("lsty", "I"), ("lstz", "I"),
+                 ("natx", "I"), ("naty", "I"), ("natz", "I"), ("rspsep", "D"), ("cspsep", "V"),
+                 ("svecre", "D"), ("sveccm", "V"), ("hsvre", "D"), ("hsvcm", "V"),
+                 ("ssvre", "D"), ("ssvcm", "V"), ("sval", "D"), ("svalsoc", "D"),
+                 ("cspsepsoc", "V"), ("svecsoc", "V"), ("residual_states", "D"),
+                 ("ksconv", "oi"), ("ksitmax", "oi")],
+    "scf_hartree": [("nd", "i"), ("ncpx_d", "i"), ("ncpy_d", "i"), ("ncpz_d", "i"), ("nfh", "i"),
+                    ("ndisp", "i"), ("nperi", "i"), ("ncgres", "i"), ("ncgmin", "i"),
+                    ("ncgmax", "i"), ("xmax", "d"), ("ymax", "d"), ("zmax", "d"), ("epsvh", "d"),
+                    ("rho_aug_dense", "D"), ("boundx", "D"), ("boundy", "D"), ("boundz", "D"),
+                    ("vh_dense", "D")],
+    "tools_correctpotential": [("ncpx_d", "i"), ("ncpy_d", "i"), ("ncpz_d", "i"), ("xmax", "d"),
+                               ("ymax", "d"), ("zmax", "d"), ("ddx", "d"), ("ddy", "d"),
+                               ("ddz", "d"), ("vh_dense", "D")],
+    "scf_charge_smt_r": [("neigmx", "i"), ("nums", "i"), ("nspv", "i"), ("numk", "i"),
+                         ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("fnele", "D"),
+                         ("svecre", "D"), ("rhosmt", "D")],
+    "force_eig_r": [("nperi", "i"), ("numk", "i"), ("nums", "i"), ("neigmx", "i"), ("nf", "i"),
+                    ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("num_list", "i"),
+                    ("natom", "i"), ("num_spe", "i"), ("num_ppcell", "i"), ("nprjmx", "i"),
+                    ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("lstvec2", "I"),
+                    ("latom", "I"), ("lstx", "I"), ("lsty", "I"), ("lstz", "I"), ("nprj", "I"),
+                    ("natpri", "I"), ("naps", "I"), ("indspe", "I"), ("natinf", "I"),
+                    ("xmax", "d"), ("ymax", "d"), ("zmax", "d"), ("sss", "D"), ("vnlocp", "D"),
+                    ("dij", "D"), ("svecre", "D"), ("sval", "D"), ("fnele", "D"), ("fatx", "D"),
+                    ("faty", "D"), ("fatz", "D")],
+    "totalenergy_eig": [("nums", "i"), ("ncol", "i"), ("numk", "i"), ("neigmx", "i"),
+                        ("nwskk", "I"), ("nwskptot", "i"), ("tf", "d"), ("sval", "D"),
+                        ("fnele", "D"), ("eneel", "od"), ("eneth", "od")],
+}
+
+_CT = {"i": ctypes.c_int, "d": ctypes.c_double, "I": ctypes.c_void_p, "D": ctypes.c_void_p,
+       "V": ctypes.c_void_p, "oi": ctypes.POINTER(ctypes.c_int),
+       "od": ctypes.POINTER(ctypes.c_double)}
+
+
+def bind(lib, prefix):
+    """Attach prototypes for every routine of SIGNATURES found in lib."""
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, prefix + name, None)
+        if fn is None:
+            continue
+        fn.restype = ctypes.c_int
+        fn.argtypes = [_CT[t] for _, t in args]
+    return lib
+
+
+def _ptr(x, code, name):
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        want = np.int32 if code == "I" else np.float64
+        if code in ("I", "D") and x.dtype != want:
+            raise TypeError(f"argument {name}: dtype {x.dtype}, expected {want}")
+        if not (x.flags.f_contiguous or x.flags.c_contiguous):
+            raise TypeError(f"argument {name}: array must be contiguous")
+        if x.ndim > 1 and not x.flags.f_contiguous:
+            raise TypeError(f"argument {name}: multi-dimensional array must be Fortran-ordered")
+        return x.ctypes.data
+    if hasattr(x, "data_ptr"):  # torch tensor
+        return x.data_ptr()
+    if isinstance(x, int):
+        return x
+    raise TypeError(f"argument {name}: cannot pass {type(x)}")
+
+
+class Outs(dict):
+    """Output scalars of a call (ksconv, ksitmax, eneel, ...)."""
+
+
+def call(lib, prefix, name, err=None, **kw):
+    """Call prefix+name with keyword arguments named as in the reference.  Scalar outputs
+    (``oi``/``od``) may be given as initial values; they are returned in an Outs dict
+    together with the return code under key ``rc``."""
+    args = SIGNATURES[name]
+    fn = getattr(lib, prefix + name)
+    cargs, outs = [], Outs()
+    for an, t in args:
+        if t in ("i", "d"):
+            if an not in kw:
+                raise TypeError(f"{name}: missing scalar {an}")
+            cargs.append(kw[an])
+        elif t in ("oi", "od"):
+            box = (ctypes.c_int if t == "oi" else ctypes.c_double)(kw.get(an, 0))
+            outs[an] = box
+            cargs.append(ctypes.byref(box))
+        else:
+            if an not in kw and t != "V":
+                raise TypeError(f"{name}: missing array {an}")
+            cargs.append(_ptr(kw.get(an), t, an))
+    rc = fn(*cargs)
+    res = Outs({k: v.value for k, v in outs.items()})
+    res["rc"] = rc
+    if rc != 0 and err is not None:
+        raise RuntimeError(f"{prefix}{name} failed ({rc}): {err()}")
+    return res

@@ -1,0 +1,92 @@
+"""Shared test helpers: dense reference operators built from the definitions (not from the
+oracle's code path) and problem builders."""
+import numpy as np
+
+from rspacedist_amd import synth
+
+
+def fd_weights(nf):
+    from oracle import oracle
+    import ctypes
+    a = np.zeros(9)
+    oracle.call("tools_definefdcoef", nf=nf, acfd=a)
+    return a
+
+
+def circulant_d2(n, h, nf):
+    """-1/2 d^2/dx^2 as a dense periodic n x n matrix from the FD weights."""
+    a = fd_weights(nf)
+    m = np.zeros((n, n))
+    for i in range(n):
+        m[i, i] += a[0]
+        for k in range(1, nf + 1):
+            m[i, (i + k) % n] += a[k]
+            m[i, (i - k) % n] += a[k]
+    return -0.5 * m / h ** 2
+
+
+def dense_projectors(s):
+    """P (N x 9*natom) with P[r, 9a+i] = vnlocp(k,i,a) at r = lstvec2(k,a)-1."""
+    P = np.zeros((s.npts, 9 * s.natom))
+    for a in range(s.natom):
+        m = s.natinf[a]
+        idx = s.lstvec2[:m, a] - 1
+        for i in range(9):
+            np.add.at(P[:, 9 * a + i], idx, s.vnlocp[:m, i, a])
+    return P
+
+
+def dense_h(s):
+    """Dense H = T_x + T_y + T_z + diag(veff) + P D P^T / dV (x fastest index order)."""
+    n = s.n
+    T = circulant_d2(n, s.h, s.nf)
+    I = np.eye(n)
+    H = np.kron(I, np.kron(I, T)) + np.kron(I, np.kron(T, I)) + np.kron(T, np.kron(I, I))
+    H += np.diag(s.veff[..., 0].ravel(order="F"))
+    P = dense_projectors(s)
+    Dbig = np.kron(np.eye(s.natom), s.dmat)
+    H += P @ Dbig.T @ P.T / s.dv
+    return H
+
+
+def halo(v, nf, nsnd, nperi=3):
+    """Halo'd copy of interior v (n,n,n) with periodic faces filled (np.pad wrap, faces only)."""
+    n1, n2, n3 = v.shape
+    out = np.zeros((n1 + 2 * nf, n2 + 2 * nf, n3 + 2 * nf), order="F")
+    out[nf:nf + n1, nf:nf + n2, nf:nf + n3] = v
+    if nperi >= 1:
+        out[nf - nsnd:nf, nf:nf + n2, nf:nf + n3] = v[n1 - nsnd:, :, :]
+        out[nf + n1:nf + n1 + nsnd, nf:nf + n2, nf:nf + n3] = v[:nsnd, :, :]
+    if nperi >= 2:
+        out[nf:nf + n1, nf - nsnd:nf, nf:nf + n3] = v[:, n2 - nsnd:, :]
+        out[nf:nf + n1, nf + n2:nf + n2 + nsnd, nf:nf + n3] = v[:, :nsnd, :]
+    if nperi >= 3:
+        out[nf:nf + n1, nf:nf + n2, nf - nsnd:nf] = v[:, :, n3 - nsnd:]
+        out[nf:nf + n1, nf:nf + n2, nf + n3:nf + n3 + nsnd] = v[:, :, :nsnd]
+    return np.asfortranarray(out)
+
+
+def plane_wave(n, h, kvec):
+    j = np.arange(1, n + 1) * h
+    X, Y, Z = np.meshgrid(j, j, j, indexing="ij")
+    return np.asfortranarray(np.cos(kvec[0] * X + kvec[1] * Y + kvec[2] * Z))
+
+
+def fd_symbol(k, h, nf):
+    a = fd_weights(nf)
+    return -0.5 * (a[0] + 2.0 * sum(a[m] * np.cos(m * k * h) for m in range(1, nf + 1))) / h ** 2
+
+
+def init_state(s, orc, nbands=None, seed=None, smooth=False):
+    """Seeded N(0,1) bands (optionally low-pass filtered) orthonormalised by the oracle's
+    orthogonalization_r (nflag=cmpt), which also sets S psi and <p|psi>."""
+    m = s.nbands if nbands is None else nbands
+    psi = synth.random_bands(s, seed=seed, nbands=m)
+    if smooth:
+        psi = synth.smooth_bands(psi, s.n, s.h)
+    spsi = np.zeros_like(psi, order="F")
+    rsp = np.zeros((9, s.natom, m), order="F")
+    kw = synth.abi_args(s, neigmx=m)
+    orc.call("orthogonalization_r", **kw, nflag=1, svecre=psi, ssvre=spsi, rspsep=rsp)
+    return dict(svecre=psi, ssvre=spsi, rspsep=rsp, hsvre=np.zeros_like(psi, order="F"),
+                sval=np.zeros(m), residual_states=np.zeros(m))

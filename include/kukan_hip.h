@@ -59,6 +59,17 @@ int kukan_unbind(void *host_ptr);            /* copies back, then frees the mirr
 int kukan_sync_to_host(void *host_ptr);      /* device mirror -> host */
 int kukan_sync_to_device(void *host_ptr);    /* host -> device mirror */
 
+/* Device memory owned by the caller through the library's own HIP runtime, for hosts that
+ * keep the solver arrays resident in HBM and pass device pointers to the entry points.
+ * No reference counterpart (the reference has no device).  kind: 1 host->device,
+ * 2 device->host, 3 device->device.  kukan_fill_normal writes n seeded N(0,1) doubles
+ * (counter-based: value i depends only on (seed, i)). */
+int kukan_device_malloc(void **ptr, size_t bytes);
+int kukan_device_free(void *ptr);
+int kukan_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int kukan_fill_normal(double *dev, size_t n, uint64_t seed);
+int kukan_device_synchronize(void);
+
 /* ------------------------------------------------------------------ leaf routines
  * The reference calls these from inside OpenMP parallel regions; the library exports
  * them with single-call semantics for unit tests and for hosts that call them serially. */

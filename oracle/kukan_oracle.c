@@ -869,6 +869,8 @@ int orc_ksdiis_r(int l_hsv, int natom, int num_spe, int neigmx, int nums, int ns
         for (int i = 1; i <= m; i++) { M2(rsmat, m, i) *= tmp; M2(rrmat, m, i) *= tmp; }
         for (int i = 1; i <= m; i++) { M2(rsmat, i, m) *= tmp; M2(rrmat, i, m) *= tmp; }
         memcpy(rsmat0, rsmat, sizeof(double) * m * m);
+        for (int j = 1; j <= m; j++) /* dsyev('v','u',...) reads the upper triangle */
+          for (int i = j + 1; i <= m; i++) M2(rsmat, i, j) = M2(rsmat, j, i);
         /* eigen-decomposition of the overlap, descending order (:410-420) */
         if ((rc = orc_dsyev(m, rsmat, eigen))) break;
         for (int i = 1; i <= m; i++) work[i - 1] = eigen[i - 1];
@@ -905,6 +907,9 @@ int orc_ksdiis_r(int l_hsv, int natom, int num_spe, int neigmx, int nums, int ns
               for (int i = 1; i <= m; i++) s += M2(rvect, i, kk) * M2(rsmat, i, j);
               rrmat2[(kk - 1) + nmat * (j - 1)] = s;
             }
+          /* dsyev('v','u',...): the upper triangle defines the matrix */
+          for (int j = 0; j < nmat; j++)
+            for (int i = j + 1; i < nmat; i++) rrmat2[i + nmat * j] = rrmat2[j + nmat * i];
           if ((rc = orc_dsyev(nmat, rrmat2, eigen))) break;
           for (int i = 1; i <= nmat; i++) work[i - 1] = eigen[i - 1];
           for (int j = 1; j <= nmat; j++) {

@@ -174,6 +174,8 @@ def _ptr(x, code, name):
         if x.ndim > 1 and not x.flags.f_contiguous:
             raise TypeError(f"argument {name}: multi-dimensional array must be Fortran-ordered")
         return x.ctypes.data
+    if hasattr(x, "device_ptr"):  # kukan.DeviceArray
+        return x.device_ptr
     if hasattr(x, "data_ptr"):  # torch tensor
         return x.data_ptr()
     if isinstance(x, int):

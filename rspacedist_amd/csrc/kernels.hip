@@ -1,0 +1,747 @@
+// kernels.hip — device kernels of libkukan_hip.so for gfx950 (CDNA4, wave64).
+//
+// Layouts follow the reference's Fortran arrays: a band is ncpx*ncpy*ncpz doubles, x
+// fastest; bands are contiguous columns (svecre(:,:,:,l)).  Halos are never materialised
+// for the solver paths: stencil kernels read neighbours with a periodic wrap (or zero, for
+// isolated directions) straight from the unhaloed band, which replaces the reference's
+// pack/copy halo exchange (overlap_finitedifference8f.F90:97-137) on one GPU.
+#include <hipcub/hipcub.hpp>
+
+#include "kukan_internal.hpp"
+
+namespace kukan {
+namespace {
+
+constexpr int WAVE = 64;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, WAVE);
+  return v;
+}
+
+// Block-wide sum for blockDim.x*blockDim.y*blockDim.z == NT threads; result valid in thread 0.
+template <int NT>
+__device__ inline double block_sum(double v, double *red) {
+  const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  v = wave_sum(v);
+  if ((t & (WAVE - 1)) == 0) red[t / WAVE] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (t == 0)
+    for (int w = 0; w < NT / WAVE; w++) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+// ----------------------------------------------------------------------------- stencil
+struct SP {
+  int nx, ny, nz, px, py, pz, halo, nfin;
+  long ex, ey;
+  const double *in;
+  long ld_in;
+  double *out;
+  long ld_out;
+  const double *veff;
+  double cx[9], cy[9], cz[9], c0;
+  int accumulate;
+  const int *row_ptr;
+  const int2 *ent;
+  const double *vnl;
+  const double *coef;
+  long ld_coef;
+  int num_list, nprjmx;
+  const int *iprj_slot;
+  int zc, nzch;
+  double *dotp;
+};
+
+__device__ inline double val(const SP &p, const double *base, int x, int y, int z) {
+  if (p.halo) return base[(long)(x + p.nfin) + p.ex * ((long)(y + p.nfin) + p.ey * (long)(z + p.nfin))];
+  if (x < 0 || x >= p.nx) {
+    if (!p.px) return 0.0;
+    x = x < 0 ? x + p.nx : x - p.nx;
+  }
+  if (y < 0 || y >= p.ny) {
+    if (!p.py) return 0.0;
+    y = y < 0 ? y + p.ny : y - p.ny;
+  }
+  if (z < 0 || z >= p.nz) {
+    if (!p.pz) return 0.0;
+    z = z < 0 ? z + p.nz : z - p.nz;
+  }
+  return base[(long)x + (long)p.nx * ((long)y + (long)p.ny * z)];
+}
+
+// One thread per (x,y) column marching over a z-chunk; the z-neighbours live in a register
+// queue, x/y neighbours come through the vector L1.  Block 64 x 4 (one wavefront per x-row).
+template <int NF>
+__global__ __launch_bounds__(256) void k_stencil(SP p) {
+  __shared__ double red[4];
+  const int nxb = (p.nx + 63) / 64;
+  const int x = (blockIdx.x % nxb) * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int zch = blockIdx.x / nxb, b = blockIdx.z;
+  const int z0 = zch * p.zc;
+  const int z1 = min(z0 + p.zc, p.nz);
+  const bool act = x < p.nx && y < p.ny;
+  const double *in = p.in + (long)b * p.ld_in;
+  double *out = p.out + (long)b * p.ld_out;
+  double q[2 * NF + 1];
+  double dsum = 0.0;
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < 2 * NF; m++) q[m + 1] = val(p, in, x, y, z0 - NF + m);
+    for (int z = z0; z < z1; z++) {
+#pragma unroll
+      for (int m = 0; m < 2 * NF; m++) q[m] = q[m + 1];
+      q[2 * NF] = val(p, in, x, y, z + NF);
+      const long r = (long)x + (long)p.nx * ((long)y + (long)p.ny * z);
+      const double c = q[NF];
+      double s = (p.c0 + (p.veff ? p.veff[r] : 0.0)) * c;
+#pragma unroll
+      for (int m = 1; m <= NF; m++) s += p.cx[m] * (val(p, in, x - m, y, z) + val(p, in, x + m, y, z));
+#pragma unroll
+      for (int m = 1; m <= NF; m++) s += p.cy[m] * (val(p, in, x, y - m, z) + val(p, in, x, y + m, z));
+#pragma unroll
+      for (int m = 1; m <= NF; m++) s += p.cz[m] * (q[NF - m] + q[NF + m]);
+      if (p.row_ptr) {
+        const double *cf = p.coef + (long)b * p.ld_coef;
+        const int e1 = p.row_ptr[r + 1];
+        for (int e = p.row_ptr[r]; e < e1; e++) {
+          const int2 en = p.ent[e];
+          const int ip = p.iprj_slot[en.x];
+          const double *v = p.vnl + en.y + (long)p.num_list * p.nprjmx * en.x;
+          const double *cc = cf + (long)p.nprjmx * en.x;
+          for (int j = 0; j < ip; j++) s += cc[j] * v[(long)p.num_list * j];
+        }
+      }
+      if (p.dotp) dsum += c * s;
+      out[r] = p.accumulate ? out[r] + s : s;
+    }
+  }
+  if (p.dotp) {
+    const double t = block_sum<256>(dsum, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      const long part = blockIdx.x + (long)gridDim.x * blockIdx.y;
+      const long nparts = (long)gridDim.x * gridDim.y;
+      p.dotp[(long)b * nparts + part] = t;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------- preconditioner
+struct PP {
+  int nx, ny, nz, px, py, pz, halo, nfin;
+  long ex, ey;
+  const double *in;
+  long ld_in;
+  double *out;
+  long ld_out;
+};
+__device__ inline double pval(const PP &p, const double *base, int x, int y, int z) {
+  SP s;
+  s.nx = p.nx; s.ny = p.ny; s.nz = p.nz; s.px = p.px; s.py = p.py; s.pz = p.pz;
+  s.halo = p.halo; s.nfin = p.nfin; s.ex = p.ex; s.ey = p.ey;
+  return val(s, base, x, y, z);
+}
+// ksprecondition8f.F90:17-32, same term order.
+__global__ __launch_bounds__(256) void k_precond(PP p) {
+  const int nxb = (p.nx + 63) / 64;
+  const int x = (blockIdx.x % nxb) * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.x / nxb, b = blockIdx.z;
+  if (x >= p.nx || y >= p.ny) return;
+  const double *v = p.in + (long)b * p.ld_in;
+  const double w = 1.0 / 12.0;
+  const double o = w * pval(p, v, x - 1, y, z) + w * pval(p, v, x, y - 1, z) +
+                   w * pval(p, v, x, y, z - 1) + 0.5 * pval(p, v, x, y, z) +
+                   w * pval(p, v, x, y, z + 1) + w * pval(p, v, x, y + 1, z) +
+                   w * pval(p, v, x + 1, y, z);
+  p.out[(long)b * p.ld_out + x + (long)p.nx * (y + (long)p.ny * z)] = o;
+}
+
+// Fused residual + preconditioner: r = sign*(h - eps*s) at the 7 points, out = K r.
+struct PRP {
+  int nx, ny, nz, px, py, pz;
+  const double *h, *s;
+  long ld;
+  const double *eps;
+  double sign;
+  double *r, *out;
+};
+__device__ inline double rval(const PRP &p, const double *h, const double *s, double e, int x,
+                              int y, int z) {
+  if (x < 0 || x >= p.nx) { if (!p.px) return 0.0; x = x < 0 ? x + p.nx : x - p.nx; }
+  if (y < 0 || y >= p.ny) { if (!p.py) return 0.0; y = y < 0 ? y + p.ny : y - p.ny; }
+  if (z < 0 || z >= p.nz) { if (!p.pz) return 0.0; z = z < 0 ? z + p.nz : z - p.nz; }
+  const long i = (long)x + (long)p.nx * ((long)y + (long)p.ny * z);
+  return p.sign * (h[i] - e * s[i]);
+}
+__global__ __launch_bounds__(256) void k_precond_resid(PRP p) {
+  const int nxb = (p.nx + 63) / 64;
+  const int x = (blockIdx.x % nxb) * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.x / nxb, b = blockIdx.z;
+  if (x >= p.nx || y >= p.ny) return;
+  const double *h = p.h + (long)b * p.ld, *s = p.s + (long)b * p.ld;
+  const double e = p.eps[b];
+  const double w = 1.0 / 12.0;
+  const double c = rval(p, h, s, e, x, y, z);
+  const double o = w * rval(p, h, s, e, x - 1, y, z) + w * rval(p, h, s, e, x, y - 1, z) +
+                   w * rval(p, h, s, e, x, y, z - 1) + 0.5 * c + w * rval(p, h, s, e, x, y, z + 1) +
+                   w * rval(p, h, s, e, x, y + 1, z) + w * rval(p, h, s, e, x + 1, y, z);
+  const long i = (long)b * p.ld + x + (long)p.nx * (y + (long)p.ny * z);
+  if (p.r) p.r[i] = c;
+  p.out[i] = o;
+}
+
+// ----------------------------------------------------------------------------- halo (leaf)
+__global__ void k_halo_axis(double *v, int axis, int na, int nb, int nc, int nf, int nf1,
+                            int nsnd) {
+  const long ex = na + nf + nf1 + 1, ey = nb + nf + nf1 + 1;
+  auto H = [&](int ia, int ib, int ic) -> long {
+    return (long)(ia + nf1) + ex * ((long)(ib + nf1) + ey * (long)(ic + nf1));
+  };
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  // (face index u in 1..d1, v in 1..d2, iov in 1..nsnd)
+  long d1, d2;
+  if (axis == 0) { d1 = nb; d2 = nc; } else if (axis == 1) { d1 = na; d2 = nc; } else { d1 = na; d2 = nb; }
+  if (t >= d1 * d2 * nsnd) return;
+  const int u = 1 + (int)(t % d1), w = 1 + (int)((t / d1) % d2), iov = 1 + (int)(t / (d1 * d2));
+  if (axis == 0) {
+    v[H(-nsnd + iov, u, w)] = v[H(na - nsnd + iov, u, w)];
+    v[H(na + iov, u, w)] = v[H(iov, u, w)];
+  } else if (axis == 1) {
+    v[H(u, -nsnd + iov, w)] = v[H(u, nb - nsnd + iov, w)];
+    v[H(u, nb + iov, w)] = v[H(u, iov, w)];
+  } else {
+    v[H(u, w, -nsnd + iov)] = v[H(u, w, nc - nsnd + iov)];
+    v[H(u, w, nc + iov)] = v[H(u, w, iov)];
+  }
+}
+
+// ----------------------------------------------------------------------------- nonlocal
+// <p|psi> for active atom blockIdx.x and BB bands starting at blockIdx.y*BB.
+template <int IPM, int BB>
+__global__ __launch_bounds__(256) void k_gather(const int *act, int num_list, int nprjmx,
+                                                const double *vnl, const int *lst,
+                                                const double *psi, long ld, int nb,
+                                                double *rp, long ld_rp, int accumulate) {
+  __shared__ double red[4][IPM * BB];
+  const int a = blockIdx.x;
+  const int slot = act[5 * a], natinf = act[5 * a + 1], iprj = act[5 * a + 2];
+  const int b0 = blockIdx.y * BB;
+  double acc[BB][IPM];
+#pragma unroll
+  for (int q = 0; q < BB; q++)
+#pragma unroll
+    for (int j = 0; j < IPM; j++) acc[q][j] = 0.0;
+  const int *ls = lst + (long)num_list * slot;
+  const double *pv = vnl + (long)num_list * nprjmx * slot;
+  for (int k = threadIdx.x; k < natinf; k += 256) {
+    const long g = ls[k] - 1;
+    double w[BB];
+#pragma unroll
+    for (int q = 0; q < BB; q++) w[q] = (b0 + q < nb) ? psi[(long)(b0 + q) * ld + g] : 0.0;
+#pragma unroll
+    for (int j = 0; j < IPM; j++) {
+      if (j < iprj) {
+        const double v = pv[k + (long)num_list * j];
+#pragma unroll
+        for (int q = 0; q < BB; q++) acc[q][j] += v * w[q];
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < BB; q++)
+#pragma unroll
+    for (int j = 0; j < IPM; j++) {
+      const double s = wave_sum(acc[q][j]);
+      if (lane == 0) red[wv][q * IPM + j] = s;
+    }
+  __syncthreads();
+  for (int t = threadIdx.x; t < IPM * BB; t += 256) {
+    const int q = t / IPM, j = t % IPM;
+    if (j < iprj && b0 + q < nb) {
+      const double s = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+      double *o = rp + (long)(b0 + q) * ld_rp + j + (long)nprjmx * slot;
+      *o = accumulate ? *o + s : s;
+    }
+  }
+}
+
+// coef(j, slot, b) = dvinv * sum_i rp(i, slot, b) X(i, j)   (nonlocaloperation8f.F90:180-198)
+__global__ void k_coef(const int *act, int nact, int nprjmx, int nums, int ns, const double *X,
+                       int use_sss, const double *rp, long ld_rp, int nb, double dvinv,
+                       double *coef, long ld_coef) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long total = (long)nact * nprjmx * nb;
+  if (t >= total) return;
+  const int j = (int)(t % nprjmx), a = (int)((t / nprjmx) % nact), b = (int)(t / ((long)nprjmx * nact));
+  const int slot = act[5 * a], iprj = act[5 * a + 2], na = act[5 * a + 3], ispe = act[5 * a + 4];
+  if (j >= iprj) return;
+  const double *Xa = use_sss ? X + (long)nprjmx * nprjmx * ispe
+                             : X + (long)nprjmx * nprjmx * ((ns - 1) + (long)nums * na);
+  const double *r = rp + (long)b * ld_rp + (long)nprjmx * slot;
+  double s = 0.0;
+  for (int i = 0; i < iprj; i++) s += r[i] * Xa[i + (long)nprjmx * j];
+  coef[(long)b * ld_coef + j + (long)nprjmx * slot] = s * dvinv;
+}
+
+__global__ void k_expand(const int *row_ptr, const int2 *ent, const int *iprj_slot,
+                         const double *vnl, int num_list, int nprjmx, long n, const double *coef,
+                         long ld_coef, int nb, double *out, long ld_out) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n * nb) return;
+  const long r = t % n;
+  const int b = (int)(t / n);
+  const int e0 = row_ptr[r], e1 = row_ptr[r + 1];
+  if (e0 == e1) return;
+  const double *cf = coef + (long)b * ld_coef;
+  double s = 0.0;
+  for (int e = e0; e < e1; e++) {
+    const int2 en = ent[e];
+    const int ip = iprj_slot[en.x];
+    const double *v = vnl + en.y + (long)num_list * nprjmx * en.x;
+    const double *cc = cf + (long)nprjmx * en.x;
+    for (int j = 0; j < ip; j++) s += cc[j] * v[(long)num_list * j];
+  }
+  out[(long)b * ld_out + r] += s;
+}
+
+// inverse map construction
+// lstvec2 entries outside [1, n] raise *bad and are skipped (never dereferenced).
+__global__ void k_csr_count(const int *act, int nact, const int *lst, int num_list, long n,
+                            int *cnt, int *bad) {
+  const int a = blockIdx.y;
+  const int slot = act[5 * a], natinf = act[5 * a + 1];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < natinf; k += gridDim.x * blockDim.x) {
+    const int r = lst[k + (long)num_list * slot] - 1;
+    if (r < 0 || r >= n) {
+      atomicOr(bad, 1);
+      continue;
+    }
+    atomicAdd(&cnt[r], 1);
+  }
+}
+__global__ void k_csr_fill(const int *act, const int *lst, int num_list, long n,
+                           const int *row_ptr, int *cursor, int2 *ent) {
+  const int a = blockIdx.y;
+  const int slot = act[5 * a], natinf = act[5 * a + 1];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < natinf; k += gridDim.x * blockDim.x) {
+    const int r = lst[k + (long)num_list * slot] - 1;
+    if (r < 0 || r >= n) continue;
+    const int pos = atomicAdd(&cursor[r], 1);
+    ent[row_ptr[r] + pos] = make_int2(slot, k);
+  }
+}
+__global__ void k_csr_sort(const int *row_ptr, int2 *ent, long n) {
+  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int e0 = row_ptr[r], e1 = row_ptr[r + 1];
+  for (int i = e0 + 1; i < e1; i++) {
+    const int2 v = ent[i];
+    int j = i - 1;
+    while (j >= e0 && (ent[j].x > v.x || (ent[j].x == v.x && ent[j].y > v.y))) {
+      ent[j + 1] = ent[j];
+      j--;
+    }
+    ent[j + 1] = v;
+  }
+}
+
+// ----------------------------------------------------------------------------- reductions
+__global__ __launch_bounds__(256) void k_dot_part(long n, const double *x, long ldx,
+                                                  const double *y, long ldy, double *part,
+                                                  int nparts) {
+  __shared__ double red[4];
+  const int b = blockIdx.y;
+  const long chunk = (n + nparts - 1) / nparts;
+  const long i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  const double *xb = x + (long)b * ldx, *yb = y + (long)b * ldy;
+  double s = 0.0;
+  for (long i = i0 + threadIdx.x; i < i1; i += 256) s += xb[i] * yb[i];
+  s = block_sum<256>(s, red);
+  if (threadIdx.x == 0) part[(long)b * nparts + blockIdx.x] = s;
+}
+__global__ __launch_bounds__(256) void k_reduce_parts(const double *part, int nparts, double scale,
+                                                      double *res) {
+  __shared__ double red[4];
+  const int b = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) s += part[(long)b * nparts + i];
+  s = block_sum<256>(s, red);
+  if (threadIdx.x == 0) res[b] = scale * s;
+}
+
+// ----------------------------------------------------------------------------- elementwise
+__global__ void k_axpby(long n, int nb, const double *a, double ca, const double *x, long ldx,
+                        const double *bb, double cb, const double *y, long ldy, double *out,
+                        long ldo) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n * nb) return;
+  const long i = t % n;
+  const int b = (int)(t / n);
+  const double fa = ca * (a ? a[b] : 1.0), fb = cb * (bb ? bb[b] : 1.0);
+  double v = fa * x[(long)b * ldx + i];
+  if (y) v += fb * y[(long)b * ldy + i];
+  out[(long)b * ldo + i] = v;
+}
+__global__ void k_scale(long n, int nb, const double *f, double *x, long ldx) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n * nb) return;
+  const long i = t % n;
+  const int b = (int)(t / n);
+  x[(long)b * ldx + i] *= f[b];
+}
+__global__ void k_copy2d(long n, int nb, const double *x, long ldx, double *y, long ldy) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n * nb) return;
+  const long i = t % n;
+  const int b = (int)(t / n);
+  y[(long)b * ldy + i] = x[(long)b * ldx + i];
+}
+__global__ void k_fill(long n, double v, double *x) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t < n) x[t] = v;
+}
+__global__ void k_permute(long n, int nb, const int *perm, const double *x, long ldx, double *y,
+                          long ldy) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n * nb) return;
+  const long i = t % n;
+  const int b = (int)(t / n);
+  y[(long)perm[b] * ldy + i] = x[(long)b * ldx + i];
+}
+__global__ __launch_bounds__(256) void k_canon_sign(int n, double *z) {
+  __shared__ double bv[256];
+  __shared__ int bi[256];
+  double *c = z + (long)n * blockIdx.x;
+  double best = -1.0;
+  int idx = 0x7fffffff;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double a = fabs(c[i]);
+    if (a > best || (a == best && i < idx)) { best = a; idx = i; }
+  }
+  bv[threadIdx.x] = best;
+  bi[threadIdx.x] = idx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const double o = bv[threadIdx.x + s];
+      const int oi = bi[threadIdx.x + s];
+      if (o > bv[threadIdx.x] || (o == bv[threadIdx.x] && oi < bi[threadIdx.x])) {
+        bv[threadIdx.x] = o;
+        bi[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  const bool flip = c[bi[0]] < 0.0;
+  __syncthreads();
+  if (flip)
+    for (int i = threadIdx.x; i < n; i += 256) c[i] = -c[i];
+}
+
+inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// splitmix64 finaliser as a counter-based generator; Box-Muller on two 53-bit uniforms.
+__device__ inline unsigned long long mix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__global__ void k_fill_normal(double *x, long n, unsigned long long seed) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long a = mix64(seed * 0x100000001B3ull + 2ull * (unsigned long long)i);
+  const unsigned long long b = mix64(seed * 0x100000001B3ull + 2ull * (unsigned long long)i + 1ull);
+  const double u1 = ((a >> 11) + 1.0) * (1.0 / 9007199254740993.0);  // (0, 1]
+  const double u2 = (b >> 11) * (1.0 / 9007199254740992.0);          // [0, 1)
+  x[i] = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+}
+
+}  // namespace
+
+void launch_fill_normal(double *x, long n, unsigned long long seed, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_normal, dim3(nblk(n, 256)), dim3(256), 0, s, x, n, seed);
+  HIPCK(hipGetLastError());
+}
+
+// ============================================================================= launchers
+static void fd_weights(int nf, double *a) {
+  static const double t[9][9] = {
+      {0},
+      {-2.0, 1.0},
+      {-5.0 / 2.0, 4.0 / 3.0, -1.0 / 12.0},
+      {-49.0 / 18.0, 3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0},
+      {-205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0},
+      {-36883.0 / 12600.0, 5.0 / 3.0, -5.0 / 21.0, 5.0 / 126.0, -5.0 / 1008.0, 1.0 / 3150.0},
+      {-5369.0 / 1800.0, 12.0 / 7.0, -15.0 / 56.0, 10.0 / 189.0, -1.0 / 112.0, 2.0 / 1925.0,
+       -1.0 / 16632.0},
+      {-3.0235941043083900, 1.7500000000000000, -0.2916666666666667, 6.4814814814814815e-2,
+       -1.3257575757575758e-2, 2.1212121212121212e-3, -2.2662522662522663e-4,
+       1.1892869035726179e-5},
+      {-3.0548441043083900, 1.7777777777777778, -0.3111111111111111, 7.5420875420875421e-2,
+       -1.7676767676767677e-2, 3.4809634809634810e-3, -5.1800051800051800e-4,
+       5.0742907885765029e-5, -2.4281274281274281e-6}};
+  for (int m = 0; m < 9; m++) a[m] = (nf >= 1 && nf <= 8) ? t[nf][m] : 0.0;
+}
+void fd_coefficients(int nf, double *acfd) { fd_weights(nf, acfd); }
+
+int stencil_nparts(const Grid &g) {
+  const int zc = 8;
+  return (int)(nblk(g.nx, 64) * nblk(g.ny, 4) * nblk(g.nz, zc));
+}
+
+void launch_stencil(const StencilCall &c, hipStream_t s) {
+  const Grid &g = *c.g;
+  const int nf = c.nfh_coeffs > 0 ? c.nfh_coeffs : g.nf;
+  if (!c.halo_in && (g.nx < nf || g.ny < nf || g.nz < nf))
+    throw Error(1, "grid smaller than the finite-difference radius");
+  SP p{};
+  p.nx = g.nx; p.ny = g.ny; p.nz = g.nz;
+  p.px = g.nperi >= 1; p.py = g.nperi >= 2; p.pz = g.nperi >= 3;
+  p.halo = c.halo_in ? 1 : 0;
+  p.nfin = g.nf;
+  p.ex = g.nx + 2 * g.nf;
+  p.ey = g.ny + 2 * g.nf;
+  p.in = c.in; p.ld_in = c.ld_in; p.out = c.out; p.ld_out = c.ld_out;
+  p.veff = c.veff;
+  double a[9];
+  fd_weights(nf, a);
+  for (int m = 0; m < 9; m++) {
+    p.cx[m] = c.scale_lap * a[m] / g.dx / g.dx * (-0.5);
+    p.cy[m] = c.scale_lap * a[m] / g.dy / g.dy * (-0.5);
+    p.cz[m] = c.scale_lap * a[m] / g.dz / g.dz * (-0.5);
+  }
+  p.c0 = p.cx[0] + p.cy[0] + p.cz[0];
+  p.accumulate = c.accumulate ? 1 : 0;
+  if (c.pl && c.coef && c.pl->nent > 0) {
+    p.row_ptr = c.pl->row_ptr; p.ent = c.pl->ent; p.vnl = c.pl->vnl;
+    p.num_list = c.pl->num_list; p.nprjmx = c.pl->nprjmx; p.iprj_slot = c.pl->iprj_slot;
+    p.coef = c.coef; p.ld_coef = c.ld_coef;
+  }
+  p.zc = 8;
+  p.nzch = (int)nblk(g.nz, p.zc);
+  p.dotp = c.dot_partial;
+  if (c.nb > 65535) throw Error(1, "too many bands in one launch");
+  dim3 grid(nblk(g.nx, 64) * p.nzch, nblk(g.ny, 4), (unsigned)c.nb);
+  dim3 blk(64, 4, 1);
+  switch (nf) {
+#define CASE(N) case N: hipLaunchKernelGGL(k_stencil<N>, grid, blk, 0, s, p); break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default: throw Error(1, "error in kslaplacian_r! This subroutine is for nf in 1-8.");
+  }
+  HIPCK(hipGetLastError());
+}
+
+void launch_halo_fill(double *v, int nperi, int na, int nb, int nc, int nf, int nf1, int nsnd,
+                      hipStream_t s) {
+  if (nsnd > nf1 + 1 || nsnd > nf || (nperi >= 1 && nsnd > na) || (nperi >= 2 && nsnd > nb) ||
+      (nperi >= 3 && nsnd > nc))
+    throw Error(1, "overlap_finitedifference_r: nsnd exceeds the halo or the grid");
+  for (int axis = 0; axis < 3; axis++) {
+    if (nperi < axis + 1) continue;
+    long faces = axis == 0 ? (long)nb * nc : axis == 1 ? (long)na * nc : (long)na * nb;
+    long tot = faces * nsnd;
+    hipLaunchKernelGGL(k_halo_axis, dim3(nblk(tot, 256)), dim3(256), 0, s, v, axis, na, nb, nc,
+                       nf, nf1, nsnd);
+  }
+  HIPCK(hipGetLastError());
+}
+
+void launch_precond(const Grid &g, const double *in, long ld_in, bool halo_in, int nf_halo,
+                    double *out, long ld_out, int nb, hipStream_t s) {
+  PP p{};
+  p.nx = g.nx; p.ny = g.ny; p.nz = g.nz;
+  p.px = g.nperi >= 1; p.py = g.nperi >= 2; p.pz = g.nperi >= 3;
+  p.halo = halo_in; p.nfin = nf_halo;
+  p.ex = g.nx + 2 * nf_halo; p.ey = g.ny + 2 * nf_halo;
+  p.in = in; p.ld_in = ld_in; p.out = out; p.ld_out = ld_out;
+  if (nb > 65535) throw Error(1, "too many bands in one launch");
+  hipLaunchKernelGGL(k_precond, dim3(nblk(g.nx, 64) * g.nz, nblk(g.ny, 4), (unsigned)nb),
+                     dim3(64, 4), 0, s, p);
+  HIPCK(hipGetLastError());
+}
+
+void launch_precond_resid(const Grid &g, const double *h, const double *sp, long ld,
+                          const double *eps, double sign, double *r_out, double *out, int nb,
+                          hipStream_t s) {
+  PRP p{};
+  p.nx = g.nx; p.ny = g.ny; p.nz = g.nz;
+  p.px = g.nperi >= 1; p.py = g.nperi >= 2; p.pz = g.nperi >= 3;
+  p.h = h; p.s = sp; p.ld = ld; p.eps = eps; p.sign = sign; p.r = r_out; p.out = out;
+  if (nb > 65535) throw Error(1, "too many bands in one launch");
+  hipLaunchKernelGGL(k_precond_resid, dim3(nblk(g.nx, 64) * g.nz, nblk(g.ny, 4), (unsigned)nb),
+                     dim3(64, 4), 0, s, p);
+  HIPCK(hipGetLastError());
+}
+
+void launch_gather(const NonlocalPlan &pl, const double *psi, long ld, int nb, double *rp,
+                   long ld_rp, bool accumulate, hipStream_t s) {
+  if (pl.nact == 0 || nb == 0) return;
+  if (pl.max_iprj <= 9) {
+    hipLaunchKernelGGL((k_gather<9, 4>), dim3(pl.nact, nblk(nb, 4)), dim3(256), 0, s, pl.act,
+                       pl.num_list, pl.nprjmx, pl.vnl, pl.lst, psi, ld, nb, rp, ld_rp,
+                       accumulate ? 1 : 0);
+  } else if (pl.max_iprj <= 32) {
+    hipLaunchKernelGGL((k_gather<32, 1>), dim3(pl.nact, nb), dim3(256), 0, s, pl.act,
+                       pl.num_list, pl.nprjmx, pl.vnl, pl.lst, psi, ld, nb, rp, ld_rp,
+                       accumulate ? 1 : 0);
+  } else {
+    throw Error(1, "nonlocal projectors: more than 32 per atom are not supported");
+  }
+  HIPCK(hipGetLastError());
+}
+
+void launch_coef(const NonlocalPlan &pl, const double *rp, long ld_rp, int nb, const double *X,
+                 bool use_sss, int ns, double dvinv, double *coef, long ld_coef, hipStream_t s) {
+  if (pl.nact == 0 || nb == 0) return;
+  const long tot = (long)pl.nact * pl.nprjmx * nb;
+  hipLaunchKernelGGL(k_coef, dim3(nblk(tot, 256)), dim3(256), 0, s, pl.act, pl.nact, pl.nprjmx,
+                     pl.nums, ns, X, use_sss ? 1 : 0, rp, ld_rp, nb, dvinv, coef, ld_coef);
+  HIPCK(hipGetLastError());
+}
+
+void launch_expand(const NonlocalPlan &pl, long n, const double *coef, long ld_coef, int nb,
+                   double *out, long ld_out, hipStream_t s) {
+  if (pl.nent == 0 || nb == 0) return;
+  hipLaunchKernelGGL(k_expand, dim3(nblk(n * nb, 256)), dim3(256), 0, s, pl.row_ptr, pl.ent,
+                     pl.iprj_slot, pl.vnl, pl.num_list, pl.nprjmx, n, coef, ld_coef, nb, out,
+                     ld_out);
+  HIPCK(hipGetLastError());
+}
+
+static int dot_nparts(long n) {
+  long p = (n + 8191) / 8192;
+  if (p < 1) p = 1;
+  if (p > 1024) p = 1024;
+  return (int)p;
+}
+
+void launch_dots(long n, const double *x, long ldx, const double *y, long ldy, int nb,
+                 double scale, double *res, hipStream_t s) {
+  if (nb == 0) return;
+  const int np = dot_nparts(n);
+  double *part = (double *)scratch("dots.part", sizeof(double) * (size_t)np * nb);
+  hipLaunchKernelGGL(k_dot_part, dim3(np, nb), dim3(256), 0, s, n, x, ldx, y, ldy, part, np);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part, np, scale, res);
+  HIPCK(hipGetLastError());
+}
+
+void launch_reduce_partials(const double *part, int nparts, int nb, double scale, double *res,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part, nparts, scale, res);
+  HIPCK(hipGetLastError());
+}
+
+void launch_axpby(long n, int nb, const double *a, double ca, const double *x, long ldx,
+                  const double *b, double cb, const double *y, long ldy, double *out, long ldo,
+                  hipStream_t s) {
+  if (n * nb == 0) return;
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n * nb, 256)), dim3(256), 0, s, n, nb, a, ca, x, ldx, b,
+                     cb, y, ldy, out, ldo);
+  HIPCK(hipGetLastError());
+}
+void launch_scale(long n, int nb, const double *f, double *x, long ldx, hipStream_t s) {
+  if (n * nb == 0) return;
+  hipLaunchKernelGGL(k_scale, dim3(nblk(n * nb, 256)), dim3(256), 0, s, n, nb, f, x, ldx);
+  HIPCK(hipGetLastError());
+}
+void launch_copy2d(long n, int nb, const double *x, long ldx, double *y, long ldy, hipStream_t s) {
+  if (n * nb == 0) return;
+  hipLaunchKernelGGL(k_copy2d, dim3(nblk(n * nb, 256)), dim3(256), 0, s, n, nb, x, ldx, y, ldy);
+  HIPCK(hipGetLastError());
+}
+void launch_fill(long n, double v, double *x, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, s, n, v, x);
+  HIPCK(hipGetLastError());
+}
+void launch_permute_cols(long n, int nb, const int *perm, const double *x, long ldx, double *y,
+                         long ldy, hipStream_t s) {
+  if (n * nb == 0) return;
+  hipLaunchKernelGGL(k_permute, dim3(nblk(n * nb, 256)), dim3(256), 0, s, n, nb, perm, x, ldx, y,
+                     ldy);
+  HIPCK(hipGetLastError());
+}
+void launch_canon_sign(int n, double *z, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_canon_sign, dim3(n), dim3(256), 0, s, n, z);
+  HIPCK(hipGetLastError());
+}
+
+// ----------------------------------------------------------------------------- plan build
+void build_plan(NonlocalPlan &pl, const Grid &g, int natom, int nprjmx, int num_spe,
+                int num_ppcell, int num_list, int nums, int key_in, int key_inps,
+                const int *nprj, const int *indspe, const int *natinf, const int *natpri,
+                const int *naps, const double *vnlocp_dev, const int *lstvec2_dev,
+                const std::string &tag) {
+  hipStream_t s = lib().stream;
+  pl.natom = natom; pl.nprjmx = nprjmx; pl.num_list = num_list; pl.num_ppcell = num_ppcell;
+  pl.nums = nums; pl.num_spe = num_spe; pl.vnl = vnlocp_dev; pl.lst = lstvec2_dev;
+  std::vector<int> h_nprj = host_ints(nprj, num_spe), h_ind = host_ints(indspe, natom),
+                   h_inf = host_ints(natinf, natom), h_pri = host_ints(natpri, natom),
+                   h_aps = host_ints(naps, natom);
+  pl.act_h.clear();
+  std::vector<int> iprj_slot(num_ppcell > 0 ? num_ppcell : 1, 0);
+  pl.max_iprj = 0;
+  pl.max_natinf = 0;
+  for (int na = 0; na < natom; na++) {
+    if (h_pri[na] != key_in && h_pri[na] != key_inps) continue;
+    const int slot = h_aps[na] - 1, ispe = h_ind[na] - 1;
+    const int ip = h_nprj[ispe];
+    if (slot < 0 || slot >= num_ppcell) throw Error(1, "naps out of range");
+    if (h_inf[na] > num_list) throw Error(1, "error! num_list is too small.");
+    pl.act_h.insert(pl.act_h.end(), {slot, h_inf[na], ip, na, ispe});
+    iprj_slot[slot] = ip;
+    pl.max_iprj = std::max(pl.max_iprj, ip);
+    pl.max_natinf = std::max(pl.max_natinf, h_inf[na]);
+  }
+  pl.nact = (int)(pl.act_h.size() / 5);
+  pl.act = (int *)scratch(tag + ".act", sizeof(int) * std::max<size_t>(1, pl.act_h.size()));
+  pl.iprj_slot = (int *)scratch(tag + ".iprj", sizeof(int) * iprj_slot.size());
+  if (!pl.act_h.empty())
+    HIPCK(hipMemcpyAsync(pl.act, pl.act_h.data(), sizeof(int) * pl.act_h.size(),
+                         hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(pl.iprj_slot, iprj_slot.data(), sizeof(int) * iprj_slot.size(),
+                       hipMemcpyHostToDevice, s));
+  const long n = g.n();
+  long nent = 0;
+  for (int a = 0; a < pl.nact; a++) nent += pl.act_h[5 * a + 1];
+  pl.nent = nent;
+  pl.row_ptr = (int *)scratch(tag + ".rowptr", sizeof(int) * (n + 1));
+  int *cnt = (int *)scratch(tag + ".cnt", sizeof(int) * (n + 1));
+  pl.ent = (int2 *)scratch(tag + ".ent", sizeof(int2) * std::max<long>(1, nent));
+  HIPCK(hipMemsetAsync(cnt, 0, sizeof(int) * (n + 1), s));
+  if (pl.nact > 0) {
+    int *bad = (int *)scratch(tag + ".bad", sizeof(int));
+    HIPCK(hipMemsetAsync(bad, 0, sizeof(int), s));
+    dim3 grid(nblk(pl.max_natinf, 256), pl.nact);
+    hipLaunchKernelGGL(k_csr_count, grid, dim3(256), 0, s, pl.act, pl.nact, pl.lst, num_list, n,
+                       cnt, bad);
+    HIPCK(hipGetLastError());
+    int hbad = 0;
+    HIPCK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    if (hbad) throw Error(1, "lstvec2 holds a grid index outside 1..ncpx*ncpy*ncpz");
+  }
+  size_t tmp_bytes = 0;
+  HIPCK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, pl.row_ptr, (int)(n + 1), s));
+  void *tmp = scratch(tag + ".scan", tmp_bytes);
+  HIPCK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, pl.row_ptr, (int)(n + 1), s));
+  if (pl.nact > 0) {
+    HIPCK(hipMemsetAsync(cnt, 0, sizeof(int) * (n + 1), s));
+    dim3 grid(nblk(pl.max_natinf, 256), pl.nact);
+    hipLaunchKernelGGL(k_csr_fill, grid, dim3(256), 0, s, pl.act, pl.lst, num_list, n, pl.row_ptr,
+                       cnt, pl.ent);
+    hipLaunchKernelGGL(k_csr_sort, dim3(nblk(n, 256)), dim3(256), 0, s, pl.row_ptr, pl.ent, n);
+    HIPCK(hipGetLastError());
+  }
+}
+
+}  // namespace kukan

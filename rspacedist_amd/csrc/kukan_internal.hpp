@@ -1,0 +1,172 @@
+// kukan_internal.hpp — internal declarations of libkukan_hip.so (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace kukan {
+
+// ----------------------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+void hip_check(hipError_t e, const char *what, const char *file, int line);
+#define HIPCK(x) ::kukan::hip_check((x), #x, __FILE__, __LINE__)
+void blas_check(rocblas_status s, const char *what, const char *file, int line);
+#define BLASCK(x) ::kukan::blas_check((x), #x, __FILE__, __LINE__)
+
+// ----------------------------------------------------------------------------- library state
+struct Lib {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  std::map<std::string, double> params;
+  std::string last_error;
+  // host pointer -> device mirror
+  struct Mirror {
+    void *dev;
+    size_t bytes;
+  };
+  std::map<const void *, Mirror> bound;
+  // grow-only scratch buffers keyed by tag
+  std::map<std::string, std::pair<void *, size_t>> scratch;
+};
+Lib &lib();
+void ensure_init();
+double param(const char *key, double dflt);
+void *scratch(const std::string &tag, size_t bytes);  // device memory, reused across calls
+void sync();
+
+// Resolves an argument array to device memory for the duration of one entry-point call.
+// Device pointers are used in place; bound host arrays use their mirror; other host
+// arrays are copied in (if read) and back (if written) around the call.
+class Arr {
+ public:
+  enum Intent { In = 1, Out = 2, InOut = 3 };
+  Arr() = default;
+  Arr(const void *p, size_t bytes, Intent it);
+  Arr(const Arr &) = delete;
+  Arr &operator=(const Arr &) = delete;
+  Arr(Arr &&o) noexcept { *this = std::move(o); }
+  Arr &operator=(Arr &&o) noexcept;
+  ~Arr();
+  void finish();  // copy back now (called on success paths)
+  double *d() const { return (double *)dev_; }
+  int *i() const { return (int *)dev_; }
+  void *v() const { return dev_; }
+  bool is_host_copy() const { return temp_; }
+
+ private:
+  const void *host_ = nullptr;
+  void *dev_ = nullptr;
+  size_t bytes_ = 0;
+  Intent intent_ = In;
+  bool temp_ = false;
+};
+// Copy a (host or device) int array to host.
+std::vector<int> host_ints(const int *p, size_t n);
+std::vector<double> host_doubles(const double *p, size_t n);
+bool is_device_ptr(const void *p);
+
+// ----------------------------------------------------------------------------- grid + nonlocal
+struct Grid {
+  int nx, ny, nz, nf, nperi;
+  double dx, dy, dz;
+  long n() const { return (long)nx * ny * nz; }
+  double dv() const { return dx * dy * dz; }
+};
+
+// Active-atom tables and the inverse (grid point -> projector entries) map used for the
+// deterministic owner-computes expansion.  Built on the device once per coarse call.
+struct NonlocalPlan {
+  int natom = 0, nprjmx = 0, num_list = 0, num_ppcell = 0, nums = 1, num_spe = 1;
+  int nact = 0;                  // active atoms (natpri in {in, inps})
+  const double *vnl = nullptr;   // vnlocp (num_list, nprjmx, num_ppcell), device
+  const int *lst = nullptr;      // lstvec2 (num_list, num_ppcell), device, 1-based
+  int *act = nullptr;            // per active atom: {iaps-1, natinf, iprj, na-1, ispe-1} x nact, device
+  int *iprj_slot = nullptr;      // nprj for each ppcell slot (0 if unused), device
+  int *row_ptr = nullptr;        // N+1
+  int2 *ent = nullptr;           // {slot, k}
+  long nent = 0;
+  std::vector<int> act_h;        // host copy of act
+  int max_iprj = 0, max_natinf = 0;
+};
+void build_plan(NonlocalPlan &pl, const Grid &g, int natom, int nprjmx, int num_spe,
+                int num_ppcell, int num_list, int nums, int key_in, int key_inps,
+                const int *nprj, const int *indspe, const int *natinf, const int *natpri,
+                const int *naps, const double *vnlocp_dev, const int *lstvec2_dev,
+                const std::string &tag);
+
+// ----------------------------------------------------------------------------- kernels (launchers)
+// Hpsi for nb bands: out_b (=|+=) (-1/2 lap + veff) in_b + expansion with coef_b.
+// in: interior arrays (wrap/zero halos per nperi) unless halo_in (explicit halo of width nf).
+struct StencilCall {
+  const Grid *g;
+  const double *in;
+  long ld_in;        // band stride (elements)
+  bool halo_in = false;
+  double *out;
+  long ld_out;
+  const double *veff;  // may be null
+  int nb = 1;
+  bool accumulate = false;
+  const NonlocalPlan *pl = nullptr;  // expansion if non-null and coef non-null
+  const double *coef = nullptr;      // (nprjmx, num_ppcell) per band, ld = ld_coef
+  long ld_coef = 0;
+  double scale_lap = 1.0;            // multiplies the kinetic part (1 for H)
+  double *dot_partial = nullptr;     // optional: per-block partial sums of in_b . out_b
+  int nfh_coeffs = 0;                // if > 0: use this FD order (Hartree), no veff
+};
+void launch_stencil(const StencilCall &c, hipStream_t s);
+// partial-sum layout of launch_stencil's dot_partial: nb * nparts
+int stencil_nparts(const Grid &g);
+
+void launch_halo_fill(double *v, int nperi, int na, int nb, int nc, int nf, int nf1, int nsnd,
+                      hipStream_t s);
+// out = K in (7-point smoother); halo_in: explicit halo width nf; else wrap/zero by nperi
+void launch_precond(const Grid &g, const double *in, long ld_in, bool halo_in, int nf_halo,
+                    double *out, long ld_out, int nb, hipStream_t s);
+// out_b = K r_b with r_b = -(h_b - eps_b s_b) computed on the fly (fused residual).
+void launch_precond_resid(const Grid &g, const double *h, const double *sp, long ld,
+                          const double *eps, double sign, double *r_out, double *out, int nb,
+                          hipStream_t s);
+
+// <p|psi>: rp(i, slot, b) (=|+=) sum_k vnl(k,i,slot) psi_b(lst(k,slot)-1)
+void launch_gather(const NonlocalPlan &pl, const double *psi, long ld, int nb, double *rp,
+                   long ld_rp, bool accumulate, hipStream_t s);
+// coef(j, slot, b) = sum_i rp(i, slot, b) X(i, j; atom) / dV, X = dij(:,:,ns,na) or sss(:,:,1,ispe)
+void launch_coef(const NonlocalPlan &pl, const double *rp, long ld_rp, int nb, const double *X,
+                 bool use_sss, int ns, double dvinv, double *coef, long ld_coef, hipStream_t s);
+// out_b(r) += sum_{entries of r} sum_j coef_b(j, slot) vnl(k, j, slot)
+void launch_expand(const NonlocalPlan &pl, long n, const double *coef, long ld_coef, int nb,
+                   double *out, long ld_out, hipStream_t s);
+
+// batched reductions: res[b] = scale * sum_i x_b[i] * y_b[i] (deterministic two-stage)
+void launch_dots(long n, const double *x, long ldx, const double *y, long ldy, int nb,
+                 double scale, double *res, hipStream_t s);
+// res[b] = scale * sum over partials (nb x nparts) in fixed order
+void launch_reduce_partials(const double *part, int nparts, int nb, double scale, double *res,
+                            hipStream_t s);
+
+// elementwise helpers (per-band scalars are device arrays)
+void launch_axpby(long n, int nb, const double *a, double ca, const double *x, long ldx,
+                  const double *b, double cb, const double *y, long ldy, double *out, long ldo,
+                  hipStream_t s);  // out = (ca*a_b) x + (cb*b_b) y ; a/b may be null => 1
+void launch_scale(long n, int nb, const double *f, double *x, long ldx, hipStream_t s);
+void launch_copy2d(long n, int nb, const double *x, long ldx, double *y, long ldy, hipStream_t s);
+void launch_fill(long n, double v, double *x, hipStream_t s);
+void launch_fill_normal(double *x, long n, unsigned long long seed, hipStream_t s);
+void launch_permute_cols(long n, int nb, const int *perm_dev, const double *x, long ldx,
+                         double *y, long ldy, hipStream_t s);  // y(:,perm[j]) = x(:,j)
+// sign-canonicalise eigenvector columns (largest |.| positive, lowest index on ties)
+void launch_canon_sign(int n, double *z, hipStream_t s);
+
+}  // namespace kukan

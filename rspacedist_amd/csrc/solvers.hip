@@ -1,0 +1,976 @@
+// solvers.hip — the Kohn-Sham hot path on device pointers (libkukan_hip.so, gfx950).
+//
+// Each routine restates the reference routine named in its comment; the entry points in
+// abi.hip resolve the caller's arrays (host, bound or device) and call these.
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "kukan_internal.hpp"
+#include "solvers.hpp"
+
+namespace kukan {
+namespace {
+
+inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// ----------------------------------------------------------------------------- scalar kernels
+__global__ void k_rsqrt_norm(int nb, const double *vecnor, double *tmp, int *flag) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const double v = vecnor[b];
+  if (!(v > 0.0)) {
+    atomicOr(flag, 1);
+    tmp[b] = 0.0;
+  } else {
+    tmp[b] = 1.0 / sqrt(v);
+  }
+}
+
+// kscg8f.F90:1197-1230 (kscg_r_05) for each band; inactive bands keep alph=1, beta=0.
+__global__ void k_kscg05(int nb, const double *sval, const double *pap, const double *sapre,
+                         const int *active, double *alph, double *beta) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double a = 1.0, be = 0.0;
+  if (!active || active[b]) {
+    const double eps = 1.0e-14, sv = sval[b], pp = pap[b], sa = sapre[b];
+    const double gamma = (sv + pp) * 0.5 - sqrt((sv - pp) * (sv - pp) * 0.25 + sa * sa);
+    if (fabs((sv - gamma) / gamma) > eps) {
+      a = sa / sqrt(sa * sa + (sv - gamma) * (sv - gamma));
+      be = -(sv - gamma) / sqrt(sa * sa + (sv - gamma) * (sv - gamma));
+    }
+    if (a < eps) {
+      a = -a;
+      be = -be;
+    }
+  }
+  alph[b] = a;
+  beta[b] = be;
+}
+
+// t[b] = reset ? 0 : num[b]/den[b]
+__global__ void k_ratio(int nb, const double *num, const double *den, int reset, double *t) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  t[b] = reset ? 0.0 : num[b] / den[b];
+}
+
+// block CG direction scalars: d = <S psi|P>, then normalisation of P - d psi.
+//   q = <P|S P> - 2 d <psi|S P>... with S psi, psi orthonormal-ish we use the exact
+//   expression n2 = <P|SP> - 2 d e + d^2 g, e = <Spsi|P>=d, g = <psi|Spsi>.
+__global__ void k_pnorm(int nb, const double *d, const double *pp, const double *g,
+                        double *fac, double *mdfac, int *flag) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const double n2 = pp[b] - 2.0 * d[b] * d[b] + d[b] * d[b] * g[b];
+  if (!(n2 > 0.0)) {
+    atomicOr(flag, 1);
+    fac[b] = 0.0;
+    mdfac[b] = 0.0;
+    return;
+  }
+  const double f = 1.0 / sqrt(n2);
+  fac[b] = f;
+  mdfac[b] = -d[b] * f;
+}
+
+// ksdiis8f.F90:285-600 per band, nitermax = 2 (one trial step), expressed on the basis
+// (phi0, Kr): returns coefficients (ca, cb) of the new band and its residual measure.
+// Inputs per band (already multiplied by dV): e = eps, hkk=<Kr|HKr>, h0k=<phi0|HKr>,
+// hk0=<Kr|Hphi0>, s00=<phi0|Sphi0>, sk0=<Kr|Sphi0>, skk=<Kr|SKr>, r00=<R0|R0>, r0q=<R0|Q>,
+// qq=<Q|Q> with R0 = Hphi0 - e Sphi0, Q = HKr - e SKr.
+struct DiisIn {
+  double h00, hkk, h0k, hk0, s00, sk0, skk, r00, r0q, qq;
+};
+__device__ void sym2_eig(double a, double b, double c, double *w, double *v) {
+  // [[a, b],[b, c]]: ascending eigenvalues, eigenvectors canonical (largest |.| positive)
+  const double tr = 0.5 * (a + c), df = 0.5 * (a - c);
+  const double r = hypot(df, b);
+  w[0] = tr - r;
+  w[1] = tr + r;
+  for (int k = 0; k < 2; k++) {
+    double x, y;
+    // (A - w I) v = 0
+    if (fabs(a - w[k]) + fabs(b) >= fabs(c - w[k]) + fabs(b)) {
+      x = -b;
+      y = a - w[k];
+    } else {
+      x = c - w[k];
+      y = -b;
+    }
+    double nn = hypot(x, y);
+    if (nn == 0.0) {
+      x = k == 0 ? 1.0 : 0.0;
+      y = k == 0 ? 0.0 : 1.0;
+      if (b == 0.0 && a > c) { x = 1.0 - x; y = 1.0 - y; }
+      nn = 1.0;
+    }
+    x /= nn;
+    y /= nn;
+    const bool flip = (fabs(x) >= fabs(y)) ? (x < 0.0) : (y < 0.0);
+    if (flip) { x = -x; y = -y; }
+    v[2 * k] = x;
+    v[2 * k + 1] = y;
+  }
+}
+
+__global__ void k_diis_step(int nb, const double *dots /*10 per band*/, double dv, double a1,
+                            double lmin, double lmax, double eps_eig, double *sval_io,
+                            double *lambda, double *ca, double *cb, double *resid) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const double *d = dots + 10 * b;
+  const double e = d[0] * dv;  // eps = dV <phi0|H phi0>
+  sval_io[b] = e;
+  // line search (ksdiis8f.F90:296-320): phi1' = phi0 + a1 Kr
+  const double s1 = d[3], s2 = d[5];
+  const double s3 = d[0] + a1 * (d[2] + d[3]) + a1 * a1 * d[1];
+  const double s4 = d[4] + 2.0 * a1 * d[5] + a1 * a1 * d[6];
+  const double aaa = 2.0 * (s1 - e * s2) * dv;
+  const double vr = s3 / s4;
+  const double bbb = (vr - e - aaa * a1) / (a1 * a1);
+  double lam = 0.5 * aaa / bbb;
+  if (lam > lmax) lam = lmax;
+  if (lam < lmin) lam = lmin;
+  lambda[b] = lam;
+  // rsmat/rrmat of {phi0, phi1 = phi0 + lam Kr} (:375-408), then phi1 normalised
+  double S00 = d[4] * dv, S01 = (d[4] + lam * d[5]) * dv,
+         S11 = (d[4] + 2.0 * lam * d[5] + lam * lam * d[6]) * dv;
+  double R00 = d[7] * dv, R01 = (d[7] + lam * d[8]) * dv,
+         R11 = (d[7] + 2.0 * lam * d[8] + lam * lam * d[9]) * dv;
+  const double tmp = 1.0 / sqrt(S11);
+  S01 *= tmp; R01 *= tmp; S11 *= tmp * tmp; R11 *= tmp * tmp;
+  // overlap eigenvectors, descending (:410-420)
+  double w[2], v[4], rv[4], eg[2];
+  sym2_eig(S00, S01, S11, w, v);
+  rv[0] = v[2]; rv[1] = v[3]; eg[0] = w[1];  // column 1 = largest
+  rv[2] = v[0]; rv[3] = v[1]; eg[1] = w[0];
+  int nmat = 0, k = 0;
+  while (k < 2) {
+    nmat++;
+    k++;
+    if (eg[nmat - 1] < eps_eig) {
+      // columns nmat..iiter-1 shift left (iiter = 1: nothing moves)
+      nmat--;
+    }
+  }
+  double c0, c1;
+  double res = resid[b];
+  if (nmat > 1) {
+    for (int i = 0; i < nmat; i++) {
+      const double f = 1.0 / sqrt(eg[i]);
+      rv[2 * i] *= f;
+      rv[2 * i + 1] *= f;
+    }
+    // rrmat2 = rv^T R rv
+    const double R[4] = {R00, R01, R01, R11};
+    double T[4], M2[4];
+    for (int kk = 0; kk < 2; kk++)
+      for (int j = 0; j < 2; j++) T[kk + 2 * j] = R[kk + 0] * rv[0 + 2 * j] + R[kk + 2] * rv[1 + 2 * j];
+    for (int kk = 0; kk < 2; kk++)
+      for (int j = 0; j < 2; j++)
+        M2[kk + 2 * j] = rv[0 + 2 * kk] * T[0 + 2 * j] + rv[1 + 2 * kk] * T[1 + 2 * j];
+    double w2[2], v2[4];
+    sym2_eig(M2[0], M2[2], M2[3], w2, v2);  // dsyev('v','u'): upper triangle
+    // descending: column nmat (=2) is the smallest eigenvalue
+    res = w2[0];
+    const double x = v2[0], y = v2[1];
+    c0 = rv[0] * x + rv[2] * y;
+    c1 = rv[1] * x + rv[3] * y;
+    const double S[4] = {S00, S01, S01, S11};
+    const double q = c0 * (S[0] * c0 + S[2] * c1) + c1 * (S[1] * c0 + S[3] * c1);
+    const double f = 1.0 / sqrt(q);
+    c0 *= f;
+    c1 *= f;
+  } else {
+    c0 = 0.0;
+    c1 = 1.0;
+  }
+  if (c0 < 0.0) { c0 = -c0; c1 = -c1; }
+  resid[b] = res;
+  // new = c0 phi0 + c1 tmp (phi0 + lam Kr)
+  ca[b] = c0 + c1 * tmp;
+  cb[b] = c1 * tmp * lam;
+}
+
+// ten DIIS dots per band in one pass: see DiisIn ordering in k_diis_step
+__global__ __launch_bounds__(256) void k_diis_dots(long n, const double *phi, const double *hphi,
+                                                   const double *sphi, const double *kr,
+                                                   const double *hkr, const double *skr, long ld,
+                                                   const double *sval_dv /*eps per band*/,
+                                                   double *part, int nparts) {
+  __shared__ double red[10][4];
+  const int b = blockIdx.y;
+  const long chunk = (n + nparts - 1) / nparts;
+  const long i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  const long o = (long)b * ld;
+  const double e = sval_dv[b];
+  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (long i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double p = phi[o + i], hp = hphi[o + i], sp = sphi[o + i];
+    const double k = kr[o + i], hk = hkr[o + i], sk = skr[o + i];
+    const double r0 = hp - e * sp, q = hk - e * sk;
+    acc[0] += p * hp;
+    acc[1] += k * hk;
+    acc[2] += p * hk;
+    acc[3] += k * hp;
+    acc[4] += p * sp;
+    acc[5] += k * sp;
+    acc[6] += k * sk;
+    acc[7] += r0 * r0;
+    acc[8] += r0 * q;
+    acc[9] += q * q;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j = 0; j < 10; j++) {
+    double v = acc[j];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[j][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    const int j = threadIdx.x;
+    part[((long)b * 10 + j) * nparts + blockIdx.x] = ((red[j][0] + red[j][1]) + red[j][2]) + red[j][3];
+  }
+}
+__global__ __launch_bounds__(256) void k_diis_reduce(const double *part, int nparts, double *dots) {
+  __shared__ double red[4];
+  const int bj = blockIdx.x;  // b*10 + j
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) s += part[(long)bj * nparts + i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) dots[bj] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+// eps per band (dV <phi0|Hphi0>) is needed inside k_diis_dots; computed by a dot first.
+
+// FD first derivative along axis (force8f.F90:408-1010): out = sum_m w_m (psi(+m)-psi(-m))/h
+__global__ void k_fd1(int nx, int ny, int nz, int px, int py, int pz, int axis, int nf,
+                      const double *w, double hinv, const double *in, long ld, double *out,
+                      long ldo) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = (long)nx * ny * nz;
+  const int b = blockIdx.y;
+  if (t >= n) return;
+  const int x = (int)(t % nx), y = (int)((t / nx) % ny), z = (int)(t / ((long)nx * ny));
+  const double *v = in + (long)b * ld;
+  auto at = [&](int dxx, int dyy, int dzz) -> double {
+    int X = x + dxx, Y = y + dyy, Z = z + dzz;
+    if (X < 0 || X >= nx) { if (!px) return 0.0; X = X < 0 ? X + nx : X - nx; }
+    if (Y < 0 || Y >= ny) { if (!py) return 0.0; Y = Y < 0 ? Y + ny : Y - ny; }
+    if (Z < 0 || Z >= nz) { if (!pz) return 0.0; Z = Z < 0 ? Z + nz : Z - nz; }
+    return v[(long)X + (long)nx * ((long)Y + (long)ny * Z)];
+  };
+  const int sx = axis == 0, sy = axis == 1, sz = axis == 2;
+  double s = 0.0;
+  for (int m = nf; m >= 1; m--) s += (w[m] * hinv) * at(sx * m, sy * m, sz * m);
+  for (int m = 1; m <= nf; m++) s -= (w[m] * hinv) * at(-sx * m, -sy * m, -sz * m);
+  out[(long)b * ldo + t] = s;
+}
+
+// nonlocal band force accumulation (force8f.F90:149-165), one thread per active atom,
+// bands in order; only natpri == key_natpri_in atoms contribute.
+__global__ void k_force_acc(const int *act, int nact, const int *is_in, int nprjmx, int nums,
+                            int ns, const double *dij, const double *sss, const double *rp,
+                            const double *rd, long ld_rp, int nb, const double *fnele,
+                            const double *sval, double *fa) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nact || !is_in[a]) return;
+  const int slot = act[5 * a], iprj = act[5 * a + 2], na = act[5 * a + 3], ispe = act[5 * a + 4];
+  const double *D = dij + (long)nprjmx * nprjmx * ((ns - 1) + (long)nums * na);
+  const double *S = sss + (long)nprjmx * nprjmx * ispe;
+  double f = fa[na];
+  for (int l = 0; l < nb; l++) {
+    const double *p = rp + (long)l * ld_rp + (long)nprjmx * slot;
+    const double *q = rd + (long)l * ld_rp + (long)nprjmx * slot;
+    const double fl = fnele[l], ev = sval[l];
+    for (int j = 0; j < iprj; j++)
+      for (int i = 0; i < iprj; i++)
+        f = f - 2.0 * fl * (p[i] * q[j]) * (D[i + (long)nprjmx * j] - ev * S[i + (long)nprjmx * j]);
+  }
+  fa[na] = f;
+}
+
+// rho(:,s) += sum_l f_l psi_l^2 over a band chunk, band order (scf_charge8f.F90:303-318)
+__global__ void k_charge(long n, int nb, const double *psi, long ld, const double *f, double *rho) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r = rho[i];
+  for (int l = 0; l < nb; l++) {
+    const double w = psi[(long)l * ld + i];
+    r = r + f[l] * w * w;
+  }
+  rho[i] = r;
+}
+
+// Hartree: b = 2 pi rho; boundary planes of isolated directions (scf_hartree8f.F90:88-595)
+__global__ void k_hartree_b(long n, const double *rho, double *b) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = 2.0 * 3.14159265358979323846 * rho[i];
+}
+__global__ void k_hartree_bound(int nx, int ny, int nz, int nfh, int axis, const double *w,
+                                const double *bound, double *b) {
+  // one thread per face point; lower and upper faces
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long d1, d2;
+  if (axis == 0) { d1 = ny; d2 = nz; } else if (axis == 1) { d1 = nx; d2 = nz; } else { d1 = nx; d2 = ny; }
+  if (t >= d1 * d2) return;
+  const int u = (int)(t % d1), v = (int)(t / d1);
+  const int nb2 = 2 * nfh;
+  for (int m = 1; m <= nfh; m++)
+    for (int q = 0; q <= nfh - m; q++) {
+      const int mm = m + q;
+      long ilo, ihi, olo, ohi;
+      if (axis == 2) {
+        ilo = u + (long)nx * (v + (long)ny * (1 - m + nfh - 1));
+        ihi = u + (long)nx * (v + (long)ny * (m + nfh - 1));
+        olo = u + (long)nx * (v + (long)ny * q);
+        ohi = u + (long)nx * (v + (long)ny * (nz - 1 - q));
+      } else if (axis == 1) {
+        ilo = u + (long)nx * ((1 - m + nfh - 1) + (long)nb2 * v);
+        ihi = u + (long)nx * ((m + nfh - 1) + (long)nb2 * v);
+        olo = u + (long)nx * (q + (long)ny * v);
+        ohi = u + (long)nx * ((ny - 1 - q) + (long)ny * v);
+      } else {
+        ilo = (1 - m + nfh - 1) + (long)nb2 * (u + (long)ny * v);
+        ihi = (m + nfh - 1) + (long)nb2 * (u + (long)ny * v);
+        olo = q + (long)nx * (u + (long)ny * v);
+        ohi = (nx - 1 - q) + (long)nx * (u + (long)ny * v);
+      }
+      b[olo] -= w[mm] * bound[ilo];
+      b[ohi] -= w[mm] * bound[ihi];
+    }
+}
+__global__ void k_sub_scalar(long n, const double *s, double f, double *x) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x[i] - s[0] * f;
+}
+// Hartree CG updates (scf_hartree8f.F90:849-891); scalars on the device: sc[0]=r1r, sc[1]=pap
+__global__ void k_hcg_update(long n, const double *sc, double *vh, const double *p, const double *ap,
+                             double *avhs, const double *b, double *r, double *part, int nparts) {
+  __shared__ double red[4];
+  const double alph = sc[0] / sc[1];
+  const long chunk = (n + nparts - 1) / nparts;
+  const long i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  double s = 0.0;
+  for (long i = i0 + threadIdx.x; i < i1; i += 256) {
+    vh[i] = vh[i] + alph * p[i];
+    avhs[i] = avhs[i] + alph * ap[i];
+    const double ri = b[i] - avhs[i];
+    r[i] = ri;
+    s += ri * ri;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+__global__ void k_hcg_dir(long n, const double *r2, const double *r1, int reset, const double *r,
+                          double *p) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const double beta = reset ? 0.0 : r2[0] / r1[0];
+  if (i < n) p[i] = r[i] + beta * p[i];
+}
+
+}  // namespace
+
+// ============================================================================= helpers
+Ctx::Ctx(const Grid &g_) : g(g_), s(lib().stream) {}
+
+double *Ctx::buf(const char *tag, size_t count) {
+  return (double *)scratch(std::string("ctx.") + tag, count * sizeof(double));
+}
+
+int *Ctx::flag(int i) {
+  int *f = (int *)scratch("ctx.flag", sizeof(int) * 4);
+  return f + i;
+}
+
+void Ctx::clear_flag(int i) { HIPCK(hipMemsetAsync(flag(i), 0, sizeof(int), s)); }
+
+int Ctx::read_flag(int i) {
+  int h = 0;
+  HIPCK(hipMemcpyAsync(&h, flag(i), sizeof(int), hipMemcpyDeviceToHost, s));
+  sync();
+  return h;
+}
+
+void Ctx::proj(const double *psi, long ld, int nb, double *rp, long ld_rp) {
+  launch_gather(pl, psi, ld, nb, rp, ld_rp, false, s);
+}
+
+void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, const double *rp,
+               long ld_rp, double *out, long ld_out, bool accumulate, double *eps_out) {
+  const long n1 = (long)pl.nprjmx * pl.num_ppcell;
+  double *coef = buf("coef", (size_t)std::max<long>(1, n1) * nb);
+  launch_coef(pl, rp, ld_rp, nb, dij, false, ns, 1.0 / g.dv(), coef, n1, s);
+  StencilCall c;
+  c.g = &g; c.in = psi; c.ld_in = ld; c.out = out; c.ld_out = ld_out; c.veff = veffs; c.nb = nb;
+  c.accumulate = accumulate; c.pl = &pl; c.coef = coef; c.ld_coef = n1;
+  double *part = nullptr;
+  const int np = stencil_nparts(g);
+  if (eps_out) {
+    part = buf("hpsi.part", (size_t)np * nb);
+    c.dot_partial = part;
+  }
+  launch_stencil(c, s);
+  if (eps_out) launch_reduce_partials(part, np, nb, g.dv(), eps_out, s);
+}
+
+void Ctx::spsi(const double *psi, long ld, int nb, const double *rp, long ld_rp, double *out,
+               long ld_out) {
+  if (out != psi) launch_copy2d(g.n(), nb, psi, ld, out, ld_out, s);
+  if (q_zero) return;
+  const long n1 = (long)pl.nprjmx * pl.num_ppcell;
+  double *coef = buf("scoef", (size_t)std::max<long>(1, n1) * nb);
+  launch_coef(pl, rp, ld_rp, nb, sss, true, 1, 1.0 / g.dv(), coef, n1, s);
+  launch_expand(pl, g.n(), coef, n1, nb, out, ld_out, s);
+}
+
+void Ctx::dots(const double *x, long ldx, const double *y, long ldy, int nb, double *res) {
+  launch_dots(g.n(), x, ldx, y, ldy, nb, g.dv(), res, s);
+}
+
+// ============================================================================= Cholesky-QR
+// One pass of S-orthonormalisation of bands [0, M): G = dV Psi^T S Psi = R^T R,
+// Psi, S Psi, <p|psi> (and H Psi if given) <- X R^-1.  Same Q as Gram-Schmidt in exact
+// arithmetic (orthogonalization8f.F90:93-376 with l_inp = 0).
+void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi) {
+  const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
+  rocblas_handle h = lib().blas;
+  double *G = c.buf("cholqr.G", (size_t)M * M);
+  const double dv = c.g.dv(), zero = 0.0, one = 1.0;
+  BLASCK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, &dv,
+                       psi, (int)n, spsi, (int)n, &zero, G, M));
+  int *info = (int *)scratch("cholqr.info", sizeof(int));
+  BLASCK(rocsolver_dpotrf(h, rocblas_fill_upper, M, G, M, info));
+  int hinfo = 0;
+  HIPCK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c.s));
+  sync();
+  if (hinfo != 0) throw Error(2, "orthogonalization_r_01 (1): wf has negative norm");
+  auto tr = [&](double *X, long ldx, long rows) {
+    BLASCK(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                         rocblas_diagonal_non_unit, (int)rows, M, &one, G, M, X, (int)ldx));
+  };
+  tr(psi, n, n);
+  if (spsi != psi) tr(spsi, n, n);
+  if (rsp && n1 > 0) tr(rsp, n1, n1);
+  if (hpsi) tr(hpsi, n, n);
+}
+
+// ============================================================================= single-band GS
+// orthogonalization8f.F90:316-369: band l (0-based) against bands 0..l-1, then S-normalise.
+void gs_one(Ctx &c, int l, double *psi, double *spsi, double *rsp) {
+  const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
+  rocblas_handle h = lib().blas;
+  double *av = c.buf("gs1.av", std::max(1, l));
+  const double dv = c.g.dv(), zero = 0.0, one = 1.0, mone = -1.0;
+  double *v = psi + n * l, *sv = spsi + n * l, *rv = rsp + n1 * l;
+  if (l > 0) {
+    BLASCK(rocblas_dgemv(h, rocblas_operation_transpose, (int)n, l, &dv, spsi, (int)n, v, 1, &zero,
+                         av, 1));
+    BLASCK(rocblas_dgemv(h, rocblas_operation_none, (int)n, l, &mone, psi, (int)n, av, 1, &one, v, 1));
+    BLASCK(rocblas_dgemv(h, rocblas_operation_none, (int)n, l, &mone, spsi, (int)n, av, 1, &one, sv, 1));
+    if (n1 > 0)
+      BLASCK(rocblas_dgemv(h, rocblas_operation_none, (int)n1, l, &mone, rsp, (int)n1, av, 1, &one, rv, 1));
+  }
+  double *vn = c.buf("gs1.vn", 2);
+  c.dots(v, n, sv, n, 1, vn);
+  hipLaunchKernelGGL(k_rsqrt_norm, dim3(1), dim3(64), 0, c.s, 1, vn, vn + 1, c.flag());
+  launch_scale(n, 1, vn + 1, v, n, c.s);
+  launch_scale(n, 1, vn + 1, sv, n, c.s);
+  if (n1 > 0) launch_scale(n1, 1, vn + 1, rv, n1, c.s);
+}
+
+void ortho_r_01(Ctx &c, int M, int l_inp, int nflag, int key_cmpt, double *psi, double *spsi,
+                double *rsp) {
+  const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
+  const int l0 = l_inp == 0 ? 0 : l_inp - 1, l1 = l_inp == 0 ? M : l_inp;
+  if (nflag == key_cmpt) {
+    c.proj(psi + n * l0, n, l1 - l0, rsp + n1 * l0, n1);
+    c.spsi(psi + n * l0, n, l1 - l0, rsp + n1 * l0, n1, spsi + n * l0, n);
+  }
+  if (l_inp == 0) {
+    cholqr(c, M, psi, spsi, rsp, nullptr);
+  } else {
+    c.clear_flag();
+    gs_one(c, l_inp - 1, psi, spsi, rsp);
+    if (c.read_flag()) throw Error(2, "orthogonalization_r_01 (3): wf has negative norm");
+  }
+}
+
+// ============================================================================= Rayleigh-Ritz
+// rayleigh-ritz8f.F90:22-125: A = dV Psi^T (H Psi), eigh (lower), rotate Psi, S Psi, <p|psi>.
+void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, double *psi,
+                   double *spsi, double *rsp, double *hpsi, double *sval, bool keep_h) {
+  const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
+  rocblas_handle h = lib().blas;
+  if (l_rrb) c.hpsi(ns, veffs, psi, n, M, rsp, n1, hpsi, n, false, nullptr);
+  double *A = c.buf("rr.A", (size_t)M * M);
+  double *E = c.buf("rr.E", (size_t)M);
+  const double dv = c.g.dv(), zero = 0.0, one = 1.0;
+  BLASCK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, &dv,
+                       psi, (int)n, hpsi, (int)n, &zero, A, M));
+  int *info = (int *)scratch("rr.info", sizeof(int));
+  BLASCK(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, M, A, M, sval, E, info));
+  int hinfo = 0;
+  HIPCK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c.s));
+  sync();
+  if (hinfo != 0) throw Error(3, "rayleighritz_diag_r: error in matrix diagonalization");
+  launch_canon_sign(M, A, c.s);
+  // rotations through hsvre as the reference does (:96-119)
+  double *W = hpsi;
+  double *Wh = nullptr;
+  if (keep_h) {  // block path: rotate H Psi too, through a separate buffer
+    Wh = c.buf("rr.Wh", (size_t)n * M);
+    BLASCK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)n, M, M, &one, hpsi,
+                         (int)n, A, M, &zero, Wh, (int)n));
+  }
+  BLASCK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)n, M, M, &one, psi,
+                       (int)n, A, M, &zero, W, (int)n));
+  HIPCK(hipMemcpyAsync(psi, W, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+  if (c.q_zero && spsi != psi) {
+    HIPCK(hipMemcpyAsync(spsi, psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+  } else if (spsi != psi) {
+    BLASCK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)n, M, M, &one,
+                         spsi, (int)n, A, M, &zero, W, (int)n));
+    HIPCK(hipMemcpyAsync(spsi, W, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+  }
+  if (n1 > 0) {
+    double *R = c.buf("rr.R", (size_t)n1 * M);
+    BLASCK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)n1, M, M, &one, rsp,
+                         (int)n1, A, M, &zero, R, (int)n1));
+    HIPCK(hipMemcpyAsync(rsp, R, sizeof(double) * n1 * M, hipMemcpyDeviceToDevice, c.s));
+  }
+  if (keep_h) {
+    HIPCK(hipMemcpyAsync(hpsi, Wh, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+  } else if (!(c.q_zero && spsi != psi)) {
+    // W already holds S Psi rotated
+  } else {
+    HIPCK(hipMemcpyAsync(hpsi, spsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+  }
+}
+
+// ============================================================================= kscg_r
+// Band-sequential preconditioned CG exactly as kscg8f.F90:98-472.
+void kscg_sequential(Ctx &c, const KscgArgs &a) {
+  const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
+  const int M = a.neigmx;
+  rocblas_handle h = lib().blas;
+  double *rvecre = c.buf("cg.rvecre", n), *pvecre = c.buf("cg.pvecre", n), *pvre = c.buf("cg.pvre", n);
+  double *rvre = c.buf("cg.rvre", n), *spvre = c.buf("cg.spvre", n), *avre = c.buf("cg.avre", n);
+  double *asvre = c.buf("cg.asvre", n);
+  double *rrpsep = c.buf("cg.rrpsep", std::max<long>(1, n1)), *rppsep = c.buf("cg.rppsep", std::max<long>(1, n1));
+  double *rpsep = c.buf("cg.rpsep", std::max<long>(1, n1));
+  double *avreal = c.buf("cg.avreal", M);
+  // device scalars: 0 r1rr, 1 r2rr, 2 pap, 3 sapre, 4 vecnor, 5 tmp, 6 alph, 7 beta, 8 t, 9 eps
+  double *sc = c.buf("cg.sc", 16);
+  const double dv = c.g.dv(), zero = 0.0, one = 1.0, mone = -1.0;
+  const int key = a.northo == 2 ? a.key_cmpt : a.key_nocmpt;
+  std::vector<double> res_h(1);
+  c.clear_flag(1);
+  for (int l = 0; l < M; l++) {
+    double *psi = a.psi + n * l, *spsi = a.spsi + n * l, *hp = a.hpsi + n * l, *rps = a.rsp + n1 * l;
+    for (int loop = 0; loop < a.northo; loop++) ortho_r_01(c, M, l + 1, key, a.key_cmpt, a.psi, a.spsi, a.rsp);
+    if (n1 > 0) HIPCK(hipMemcpyAsync(rpsep, rps, sizeof(double) * n1, hipMemcpyDeviceToDevice, c.s));
+    c.hpsi(a.ns, a.veffs, psi, n, 1, rpsep, n1, hp, n, false, nullptr);
+    int iiter = 0;
+    bool conv = false;
+    while (iiter < a.nsdmax && !conv) {
+      iiter++;
+      bool doreset = (iiter == 1) || (iiter == 10) || ((iiter - 1) % 200 == 0);
+      if (doreset) {
+        launch_fill(1, 1.0, sc + 1, c.s);
+        launch_fill(n, 0.0, pvecre, c.s);
+        launch_fill(n, 0.0, rvecre, c.s);
+        launch_fill(n, 0.0, pvre, c.s);
+        launch_fill(n, 0.0, spvre, c.s);
+        if (n1 > 0) launch_fill(n1, 0.0, rppsep, c.s);
+      }
+      HIPCK(hipMemcpyAsync(sc + 0, sc + 1, sizeof(double), hipMemcpyDeviceToDevice, c.s));
+      double *ev = a.sval + l;
+      c.dots(psi, n, hp, n, 1, ev);
+      // r = -(h - eps s)
+      launch_axpby(n, 1, nullptr, -1.0, hp, n, ev, 1.0, spsi, n, rvecre, n, c.s);
+      if (a.nprecon_cg == 1) launch_precond(c.g, rvecre, n, false, 1, rvre, n, 1, c.s);
+      else HIPCK(hipMemcpyAsync(rvre, rvecre, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s));
+      for (int loop = 0; loop < a.northo; loop++) {
+        BLASCK(rocblas_dgemv(h, rocblas_operation_transpose, (int)n, l + 1, &dv, a.spsi, (int)n, rvre,
+                             1, &zero, avreal, 1));
+        BLASCK(rocblas_dgemv(h, rocblas_operation_none, (int)n, l + 1, &mone, a.psi, (int)n, avreal, 1,
+                             &one, rvre, 1));
+      }
+      c.proj(rvre, n, 1, rpsep, n1);
+      if (n1 > 0) HIPCK(hipMemcpyAsync(rrpsep, rpsep, sizeof(double) * n1, hipMemcpyDeviceToDevice, c.s));
+      c.spsi(rvre, n, 1, rrpsep, n1, asvre, n);
+      c.dots(rvecre, n, rvre, n, 1, sc + 1);
+      HIPCK(hipMemcpyAsync(a.resid + l, sc + 1, sizeof(double), hipMemcpyDeviceToDevice, c.s));
+      if (a.nkscg == 1) {
+        hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, c.s, 1, sc + 1, sc + 0, 0, sc + 8);
+        launch_axpby(n, 1, nullptr, 1.0, rvre, n, sc + 8, 1.0, pvre, n, pvre, n, c.s);
+        HIPCK(hipMemcpyAsync(pvecre, pvre, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s));
+        launch_axpby(n, 1, nullptr, 1.0, asvre, n, sc + 8, 1.0, spvre, n, spvre, n, c.s);
+        if (n1 > 0) launch_axpby(n1, 1, nullptr, 1.0, rrpsep, n1, sc + 8, 1.0, rppsep, n1, rppsep, n1, c.s);
+      } else {
+        HIPCK(hipMemcpyAsync(pvecre, rvre, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s));
+        HIPCK(hipMemcpyAsync(spvre, asvre, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s));
+        if (n1 > 0) HIPCK(hipMemcpyAsync(rppsep, rrpsep, sizeof(double) * n1, hipMemcpyDeviceToDevice, c.s));
+      }
+      if (n1 > 0) HIPCK(hipMemcpyAsync(rpsep, rppsep, sizeof(double) * n1, hipMemcpyDeviceToDevice, c.s));
+      HIPCK(hipMemcpyAsync(asvre, spvre, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s));
+      for (int loop = 0; loop < a.northo; loop++) {
+        BLASCK(rocblas_dgemv(h, rocblas_operation_transpose, (int)n, l + 1, &dv, a.spsi, (int)n,
+                             pvecre, 1, &zero, avreal, 1));
+        BLASCK(rocblas_dgemv(h, rocblas_operation_none, (int)n, l + 1, &mone, a.psi, (int)n, avreal, 1,
+                             &one, pvecre, 1));
+        BLASCK(rocblas_dgemv(h, rocblas_operation_none, (int)n, l + 1, &mone, a.spsi, (int)n, avreal, 1,
+                             &one, asvre, 1));
+        if (n1 > 0)
+          BLASCK(rocblas_dgemv(h, rocblas_operation_none, (int)n1, l + 1, &mone, a.rsp, (int)n1, avreal,
+                               1, &one, rpsep, 1));
+      }
+      c.dots(pvecre, n, asvre, n, 1, sc + 4);
+      hipLaunchKernelGGL(k_rsqrt_norm, dim3(1), dim3(64), 0, c.s, 1, sc + 4, sc + 5, c.flag(1));
+      launch_scale(n, 1, sc + 5, pvecre, n, c.s);
+      launch_scale(n, 1, sc + 5, asvre, n, c.s);
+      if (n1 > 0) launch_scale(n1, 1, sc + 5, rpsep, n1, c.s);
+      c.hpsi(a.ns, a.veffs, pvecre, n, 1, rpsep, n1, avre, n, false, sc + 2);
+      c.dots(psi, n, avre, n, 1, sc + 3);
+      hipLaunchKernelGGL(k_kscg05, dim3(1), dim3(64), 0, c.s, 1, ev, sc + 2, sc + 3, nullptr, sc + 6,
+                         sc + 7);
+      launch_axpby(n, 1, sc + 6, 1.0, psi, n, sc + 7, 1.0, pvecre, n, psi, n, c.s);
+      launch_axpby(n, 1, sc + 6, 1.0, hp, n, sc + 7, 1.0, avre, n, hp, n, c.s);
+      launch_axpby(n, 1, sc + 6, 1.0, spsi, n, sc + 7, 1.0, asvre, n, spsi, n, c.s);
+      if (n1 > 0) launch_axpby(n1, 1, sc + 6, 1.0, rps, n1, sc + 7, 1.0, rpsep, n1, rps, n1, c.s);
+      HIPCK(hipMemcpyAsync(res_h.data(), a.resid + l, sizeof(double), hipMemcpyDeviceToHost, c.s));
+      sync();
+      conv = std::sqrt(res_h[0]) < a.epssd;
+      *a.ksconv = *a.ksconv && (conv || iiter < a.nsdmax);
+    }
+    *a.ksitmax = std::max(*a.ksitmax, iiter);
+  }
+  if (c.read_flag(1)) throw Error(2, "kscg_r: pvec has negative norm");
+}
+
+// Block CG: every band advances at once; the reference's per-band projections against
+// psi_1..l become one projection against the whole block (two GEMMs), the 2x2 Rayleigh-
+// Ritz step of kscg_r_05 runs per band, and Cholesky-QR restores orthonormality at the end.
+void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
+  const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
+  const int M = a.neigmx;
+  rocblas_handle h = lib().blas;
+  const double dv = c.g.dv(), zero = 0.0, one = 1.0, mone = -1.0;
+  if (ortho_first)
+    for (int loop = 0; loop < a.northo; loop++) cholqr(c, M, a.psi, a.spsi, a.rsp, nullptr);
+  double *KR = c.buf("bcg.KR", (size_t)n * M);     // K r, then p
+  double *PA = c.buf("bcg.PA", (size_t)n * M);     // accumulated direction
+  double *HP = c.buf("bcg.HP", (size_t)n * M);     // H p
+  double *SK = c.q_zero ? nullptr : c.buf("bcg.SK", (size_t)n * M);
+  double *rpp = c.buf("bcg.rpp", (size_t)std::max<long>(1, n1) * M);
+  double *C = c.buf("bcg.C", (size_t)M * M);
+  double *sc = c.buf("bcg.sc", (size_t)M * 12);
+  double *rnew = sc, *rold = sc + M, *tb = sc + 2 * M, *dd = sc + 3 * M, *pp = sc + 4 * M;
+  double *gg = sc + 5 * M, *fac = sc + 6 * M, *mdf = sc + 7 * M, *pap = sc + 8 * M;
+  double *sap = sc + 9 * M, *al = sc + 10 * M, *be = sc + 11 * M;
+  int *active = (int *)scratch("bcg.active", sizeof(int) * M);
+  std::vector<int> act_h(M, 1);
+  HIPCK(hipMemcpyAsync(active, act_h.data(), sizeof(int) * M, hipMemcpyHostToDevice, c.s));
+  std::vector<double> res_h(M);
+  c.clear_flag(1);
+  // H Psi for all bands with their <p|psi>
+  c.hpsi(a.ns, a.veffs, a.psi, n, M, a.rsp, n1, a.hpsi, n, false, nullptr);
+  int iiter = 0;
+  bool all_conv = false;
+  while (iiter < a.nsdmax && !all_conv) {
+    iiter++;
+    const bool doreset = (iiter == 1) || (iiter == 10) || ((iiter - 1) % 200 == 0);
+    HIPCK(hipMemcpyAsync(rold, rnew, sizeof(double) * M, hipMemcpyDeviceToDevice, c.s));
+    c.dots(a.psi, n, a.hpsi, n, M, a.sval);
+    // K r with r = -(H psi - eps S psi)
+    launch_precond_resid(c.g, a.hpsi, a.spsi, n, a.sval, -1.0, nullptr, KR, M, c.s);
+    // <r|Kr> (unprojected) = residual measure and CG ratio
+    {
+      double *R = PA;  // r is rebuilt in HP as scratch: HP is free until H p
+      launch_axpby(n, M, nullptr, -1.0, a.hpsi, n, a.sval, 1.0, a.spsi, n, HP, n, c.s);
+      (void)R;
+      c.dots(HP, n, KR, n, M, rnew);
+    }
+    // Kr orthogonal to the block in the S metric: C = dV (S Psi)^T Kr ; Kr -= Psi C
+    BLASCK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, &dv,
+                         a.spsi, (int)n, KR, (int)n, &zero, C, M));
+    BLASCK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)n, M, M, &mone,
+                         a.psi, (int)n, C, M, &one, KR, (int)n));
+    // P = Kr + beta P
+    hipLaunchKernelGGL(k_ratio, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, rnew, rold, doreset ? 1 : 0, tb);
+    if (doreset) launch_copy2d(n, M, KR, n, PA, n, c.s);
+    else launch_axpby(n, M, nullptr, 1.0, KR, n, tb, 1.0, PA, n, PA, n, c.s);
+    // p = (P - d psi)/|.|, d = dV <S psi|P>
+    const double *SPA = PA;
+    if (!c.q_zero) {
+      c.proj(PA, n, M, rpp, n1);
+      c.spsi(PA, n, M, rpp, n1, SK, n);
+      SPA = SK;
+    }
+    c.dots(a.spsi, n, PA, n, M, dd);
+    c.dots(PA, n, SPA, n, M, pp);
+    c.dots(a.psi, n, a.spsi, n, M, gg);
+    hipLaunchKernelGGL(k_pnorm, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, dd, pp, gg, fac, mdf, c.flag(1));
+    launch_axpby(n, M, fac, 1.0, PA, n, mdf, 1.0, a.psi, n, KR, n, c.s);  // KR <- p
+    c.proj(KR, n, M, rpp, n1);
+    c.hpsi(a.ns, a.veffs, KR, n, M, rpp, n1, HP, n, false, pap);
+    c.dots(a.psi, n, HP, n, M, sap);
+    hipLaunchKernelGGL(k_kscg05, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, a.sval, pap, sap, active, al, be);
+    launch_axpby(n, M, al, 1.0, a.psi, n, be, 1.0, KR, n, a.psi, n, c.s);
+    launch_axpby(n, M, al, 1.0, a.hpsi, n, be, 1.0, HP, n, a.hpsi, n, c.s);
+    if (a.spsi != a.psi) {
+      if (c.q_zero) {
+        HIPCK(hipMemcpyAsync(a.spsi, a.psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+      } else {
+        c.spsi(KR, n, M, rpp, n1, SK, n);
+        launch_axpby(n, M, al, 1.0, a.spsi, n, be, 1.0, SK, n, a.spsi, n, c.s);
+      }
+    }
+    if (n1 > 0) launch_axpby(n1, M, al, 1.0, a.rsp, n1, be, 1.0, rpp, n1, a.rsp, n1, c.s);
+    HIPCK(hipMemcpyAsync(a.resid, rnew, sizeof(double) * M, hipMemcpyDeviceToDevice, c.s));
+    HIPCK(hipMemcpyAsync(res_h.data(), rnew, sizeof(double) * M, hipMemcpyDeviceToHost, c.s));
+    sync();
+    all_conv = true;
+    for (int l = 0; l < M; l++) {
+      const bool cv = std::sqrt(res_h[l]) < a.epssd;
+      if (cv) act_h[l] = 0;
+      all_conv = all_conv && cv;
+      *a.ksconv = *a.ksconv && (cv || iiter < a.nsdmax);
+    }
+    HIPCK(hipMemcpyAsync(active, act_h.data(), sizeof(int) * M, hipMemcpyHostToDevice, c.s));
+  }
+  *a.ksitmax = std::max(*a.ksitmax, iiter);
+  if (c.read_flag(1)) throw Error(2, "kscg_r: pvec has negative norm");
+  for (int loop = 0; loop < a.northo; loop++) cholqr(c, M, a.psi, a.spsi, a.rsp, a.hpsi);
+}
+
+// ============================================================================= ksdiis_r
+// ksdiis8f.F90:8-712 with nitermax = 2.  Bands are independent inside a pass, so all bands
+// run together; each band's arithmetic is the reference's, expressed on (phi0, K r).
+void ksdiis(Ctx &c, const DiisArgs &a) {
+  const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
+  const int M = a.neigmx;
+  double *KR = c.buf("diis.KR", (size_t)n * M);
+  double *HK = c.buf("diis.HK", (size_t)n * M);
+  double *SK = c.q_zero ? KR : c.buf("diis.SK", (size_t)n * M);
+  double *rpk = c.buf("diis.rpk", (size_t)std::max<long>(1, n1) * M);
+  double *sc = c.buf("diis.sc", (size_t)M * 16);
+  double *lam = sc, *ca = sc + M, *cb = sc + 2 * M, *ev = sc + 3 * M, *dots = sc + 4 * M;  // dots: 10M
+  const int np = (int)std::min<long>(1024, std::max<long>(1, (n + 8191) / 8192));
+  double *part = c.buf("diis.part", (size_t)np * M * 10);
+  std::vector<double> res_h(M);
+  for (int jiter = 1; jiter < a.ndiismax; jiter++) {  // ksdiis8f.F90:95-97
+    // H phi0 and eps
+    c.hpsi(a.ns, a.veffs, a.psi, n, M, a.rsp, n1, a.hpsi, n, false, ev);
+    // K r0, r0 = H phi0 - eps S phi0
+    if (a.nprecon == 1)
+      launch_precond_resid(c.g, a.hpsi, a.spsi, n, ev, 1.0, nullptr, KR, M, c.s);
+    else
+      launch_axpby(n, M, nullptr, 1.0, a.hpsi, n, ev, -1.0, a.spsi, n, KR, n, c.s);
+    c.proj(KR, n, M, rpk, n1);
+    c.hpsi(a.ns, a.veffs, KR, n, M, rpk, n1, HK, n, false, nullptr);
+    if (!c.q_zero) c.spsi(KR, n, M, rpk, n1, SK, n);
+    hipLaunchKernelGGL(k_diis_dots, dim3(np, M), dim3(256), 0, c.s, n, a.psi, a.hpsi, a.spsi, KR,
+                       HK, SK, n, ev, part, np);
+    hipLaunchKernelGGL(k_diis_reduce, dim3(M * 10), dim3(256), 0, c.s, part, np, dots);
+    hipLaunchKernelGGL(k_diis_step, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, dots, c.g.dv(),
+                       a.alambda_diis, a.alambda_min, a.alambda_max, a.eps_eig_diis, a.sval, lam,
+                       ca, cb, a.resid);
+    launch_axpby(n, M, ca, 1.0, a.psi, n, cb, 1.0, KR, n, a.psi, n, c.s);
+    launch_axpby(n, M, ca, 1.0, a.hpsi, n, cb, 1.0, HK, n, a.hpsi, n, c.s);
+    if (a.spsi != a.psi) {
+      if (c.q_zero)
+        HIPCK(hipMemcpyAsync(a.spsi, a.psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+      else
+        launch_axpby(n, M, ca, 1.0, a.spsi, n, cb, 1.0, SK, n, a.spsi, n, c.s);
+    }
+    if (n1 > 0) launch_axpby(n1, M, ca, 1.0, a.rsp, n1, cb, 1.0, rpk, n1, a.rsp, n1, c.s);
+    HIPCK(hipMemcpyAsync(res_h.data(), a.resid, sizeof(double) * M, hipMemcpyDeviceToHost, c.s));
+    sync();
+    for (int l = 0; l < M; l++) *a.ksconv = *a.ksconv && (std::sqrt(res_h[l]) <= a.epssd);
+    *a.ksitmax = std::max(*a.ksitmax, 1);
+    if (a.l_hsv) {
+      for (int i = 0; i < a.northo; i++) ortho_r_01(c, M, 0, a.key_cmpt, a.key_cmpt, a.psi, a.spsi, a.rsp);
+      c.hpsi(a.ns, a.veffs, a.psi, n, M, a.rsp, n1, a.hpsi, n, false, nullptr);
+    }
+  }
+}
+
+// ============================================================================= reorderstates
+void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *spsi, double *rsp,
+             double *hpsi) {
+  const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
+  std::vector<double> sv(M), rs(M);
+  HIPCK(hipMemcpyAsync(sv.data(), sval, sizeof(double) * M, hipMemcpyDeviceToHost, c.s));
+  HIPCK(hipMemcpyAsync(rs.data(), resid, sizeof(double) * M, hipMemcpyDeviceToHost, c.s));
+  sync();
+  std::vector<int> lank(M);
+  lank[0] = 1;
+  for (int j = 1; j < M; j++) {
+    lank[j] = 1;
+    for (int i = 0; i < j; i++) {
+      if (sv[i] > sv[j]) lank[i]++;
+      else lank[j]++;
+    }
+  }
+  std::vector<int> perm(M);
+  std::vector<double> st(M), rt(M);
+  for (int j = 0; j < M; j++) {
+    perm[j] = lank[j] - 1;
+    st[lank[j] - 1] = sv[j];
+    rt[lank[j] - 1] = rs[j];
+  }
+  int *pd = (int *)scratch("reo.perm", sizeof(int) * M);
+  HIPCK(hipMemcpyAsync(pd, perm.data(), sizeof(int) * M, hipMemcpyHostToDevice, c.s));
+  HIPCK(hipMemcpyAsync(sval, st.data(), sizeof(double) * M, hipMemcpyHostToDevice, c.s));
+  HIPCK(hipMemcpyAsync(resid, rt.data(), sizeof(double) * M, hipMemcpyHostToDevice, c.s));
+  if (n1 > 0) {
+    double *R = c.buf("reo.R", (size_t)n1 * M);
+    launch_permute_cols(n1, M, pd, rsp, n1, R, n1, c.s);
+    HIPCK(hipMemcpyAsync(rsp, R, sizeof(double) * n1 * M, hipMemcpyDeviceToDevice, c.s));
+  }
+  launch_permute_cols(n, M, pd, psi, n, hpsi, n, c.s);
+  HIPCK(hipMemcpyAsync(psi, hpsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+  launch_permute_cols(n, M, pd, spsi, n, hpsi, n, c.s);
+  HIPCK(hipMemcpyAsync(spsi, hpsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+  sync();
+}
+
+// ============================================================================= Hartree
+// scf_hartree8f.F90:11-812: b = 2 pi rho (+ Dirichlet planes), periodic mean removed, CG on
+// -1/2 lap_nfh with the reference's avhs recurrence and stopping rule.
+int hartree(const HartreeArgs &a) {
+  hipStream_t s = lib().stream;
+  Grid g{a.nx, a.ny, a.nz, a.nfh, a.nperi, 2.0 * a.xmax / a.nx, 2.0 * a.ymax / a.ny,
+         2.0 * a.zmax / a.nz};
+  const long n = g.n();
+  const double omega = 8.0 * a.xmax * a.ymax * a.zmax;
+  double *b = (double *)scratch("hart.b", sizeof(double) * n);
+  double *r = (double *)scratch("hart.r", sizeof(double) * n);
+  double *p = (double *)scratch("hart.p", sizeof(double) * n);
+  double *ap = (double *)scratch("hart.ap", sizeof(double) * n);
+  double *avhs = (double *)scratch("hart.avhs", sizeof(double) * n);
+  double *sc = (double *)scratch("hart.sc", sizeof(double) * 16);
+  const int np = stencil_nparts(g);
+  const int npu = (int)std::min<long>(1024, std::max<long>(1, (n + 8191) / 8192));
+  double *part = (double *)scratch("hart.part", sizeof(double) * std::max(np, npu));
+  hipLaunchKernelGGL(k_hartree_b, dim3(nblk(n, 256)), dim3(256), 0, s, n, a.rho, b);
+  double w[9];
+  fd_coefficients(a.nfh, w);
+  if (a.nd == 0) {
+    const double hs[3] = {g.dx, g.dy, g.dz};
+    const double *bd[3] = {a.boundx, a.boundy, a.boundz};
+    const int need[3] = {a.nperi == 0, a.nperi <= 1, a.nperi <= 2};
+    for (int axis = 2; axis >= 0; axis--) {
+      if (!need[axis]) continue;
+      double wc[9];
+      for (int m = 0; m < 9; m++) wc[m] = w[m] / hs[axis] / hs[axis] * (-0.5);
+      double *wd = (double *)scratch("hart.w" + std::to_string(axis), sizeof(wc));
+      HIPCK(hipMemcpyAsync(wd, wc, sizeof(wc), hipMemcpyHostToDevice, s));
+      long faces = axis == 0 ? (long)a.ny * a.nz : axis == 1 ? (long)a.nx * a.nz : (long)a.nx * a.ny;
+      hipLaunchKernelGGL(k_hartree_bound, dim3(nblk(faces, 256)), dim3(256), 0, s, a.nx, a.ny, a.nz,
+                         a.nfh, axis, wd, bd[axis], b);
+    }
+  }
+  if (a.nperi == 3) {
+    double *ones = (double *)scratch("hart.ones", sizeof(double) * n);
+    launch_fill(n, 1.0, ones, s);
+    launch_dots(n, b, n, ones, n, 1, g.dv() / omega, sc + 4, s);
+    hipLaunchKernelGGL(k_sub_scalar, dim3(nblk(n, 256)), dim3(256), 0, s, n, sc + 4, 1.0, b);
+  }
+  // Av for the warm start
+  StencilCall st;
+  st.g = &g; st.in = a.vh; st.ld_in = n; st.out = ap; st.ld_out = n; st.veff = nullptr;
+  st.nb = 1; st.nfh_coeffs = a.nfh;
+  launch_stencil(st, s);
+  HIPCK(hipMemcpyAsync(avhs, ap, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  launch_axpby(n, 1, nullptr, 1.0, b, n, nullptr, -1.0, ap, n, r, n, s);
+  HIPCK(hipMemcpyAsync(p, r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  int loop = 0;
+  double hsc[4];
+  for (;;) {
+    loop++;
+    launch_dots(n, r, n, r, n, 1, 1.0, sc + 0, s);  // r1r
+    st.in = p; st.out = ap; st.dot_partial = part;
+    launch_stencil(st, s);
+    launch_reduce_partials(part, np, 1, 1.0, sc + 1, s);  // pap
+    hipLaunchKernelGGL(k_hcg_update, dim3(npu), dim3(256), 0, s, n, sc, a.vh, p, ap, avhs, b, r,
+                       part, npu);
+    launch_reduce_partials(part, npu, 1, 1.0, sc + 2, s);  // r2r
+    hipLaunchKernelGGL(k_hcg_dir, dim3(nblk(n, 256)), dim3(256), 0, s, n, sc + 2, sc + 0,
+                       (loop % a.ncgres == 0) ? 1 : 0, r, p);
+    HIPCK(hipMemcpyAsync(hsc, sc, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+    sync();
+    const double dp = std::sqrt(hsc[2] * g.dv() / omega);
+    if (loop < a.ncgmin) continue;
+    if (loop >= a.ncgmax) break;
+    if (dp > a.epsvh) continue;
+    break;
+  }
+  return loop;
+}
+
+void correct_potential(long n, double dvol_over_omega, double *vh) {
+  hipStream_t s = lib().stream;
+  double *sc = (double *)scratch("corr.sc", sizeof(double) * 2);
+  double *ones = (double *)scratch("corr.ones", sizeof(double) * n);
+  launch_fill(n, 1.0, ones, s);
+  launch_dots(n, vh, n, ones, n, 1, dvol_over_omega, sc, s);
+  hipLaunchKernelGGL(k_sub_scalar, dim3(nblk(n, 256)), dim3(256), 0, s, n, sc, 1.0, vh);
+}
+
+void charge(long n, int M, const double *psi, const double *f, double *rho) {
+  hipStream_t s = lib().stream;
+  const int chunk = 64;
+  for (int l0 = 0; l0 < M; l0 += chunk) {
+    const int nb = std::min(chunk, M - l0);
+    hipLaunchKernelGGL(k_charge, dim3(nblk(n, 256)), dim3(256), 0, s, n, nb, psi + n * l0, n,
+                       f + l0, rho);
+  }
+  HIPCK(hipGetLastError());
+}
+
+void force_eig(Ctx &c, int M, int ns, const double *psi, const double *sval, const double *fnele,
+               const int *is_in_dev, double *fa, int nf) {
+  const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
+  double w[9] = {0};
+  switch (nf) {
+    case 1: w[1] = 0.5; break;
+    case 2: w[1] = 8.0 / 12.0; w[2] = -1.0 / 12.0; break;
+    case 3: w[1] = 45.0 / 60.0; w[2] = -9.0 / 60.0; w[3] = 1.0 / 60.0; break;
+    case 4: w[1] = 672.0 / 840.0; w[2] = -168.0 / 840.0; w[3] = 32.0 / 840.0; w[4] = -3.0 / 840.0; break;
+    default: throw Error(7, "force_eig_r: nf outside 1-4 is not part of this build");
+  }
+  double *wd = c.buf("force.w", 9);
+  HIPCK(hipMemcpyAsync(wd, w, sizeof(w), hipMemcpyHostToDevice, c.s));
+  double *rp = c.buf("force.rp", (size_t)std::max<long>(1, n1) * M);
+  c.proj(psi, n, M, rp, n1);
+  const int chunk = std::max(1, std::min(M, 64));
+  double *dpsi = c.buf("force.dpsi", (size_t)n * chunk);
+  double *rd = c.buf("force.rd", (size_t)std::max<long>(1, n1) * chunk);
+  const double hs[3] = {c.g.dx, c.g.dy, c.g.dz};
+  for (int ax = 0; ax < 3; ax++) {
+    for (int l0 = 0; l0 < M; l0 += chunk) {
+      const int nb = std::min(chunk, M - l0);
+      hipLaunchKernelGGL(k_fd1, dim3(nblk(n, 256), nb), dim3(256), 0, c.s, c.g.nx, c.g.ny, c.g.nz,
+                         c.g.nperi >= 1, c.g.nperi >= 2, c.g.nperi >= 3, ax, nf, wd, 1.0 / hs[ax],
+                         psi + n * l0, n, dpsi, n);
+      c.proj(dpsi, n, nb, rd, n1);
+      hipLaunchKernelGGL(k_force_acc, dim3(nblk(c.pl.nact, 64)), dim3(64), 0, c.s, c.pl.act,
+                         c.pl.nact, is_in_dev, c.pl.nprjmx, c.pl.nums, ns, c.dij, c.sss,
+                         rp + n1 * l0, rd, n1, nb, fnele + l0, sval + l0, fa + (long)ax * c.pl.natom);
+    }
+  }
+  HIPCK(hipGetLastError());
+}
+
+}  // namespace kukan

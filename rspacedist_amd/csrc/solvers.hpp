@@ -1,0 +1,67 @@
+// solvers.hpp — device-pointer implementations of the coarse routines.
+#pragma once
+#include "kukan_internal.hpp"
+
+namespace kukan {
+
+void fd_coefficients(int nf, double *acfd);
+
+// Per-call context: grid, nonlocal plan and the device tables of one coarse call.
+struct Ctx {
+  explicit Ctx(const Grid &g);
+  Grid g;
+  NonlocalPlan pl;
+  hipStream_t s;
+  const double *dij = nullptr, *sss = nullptr;
+  bool q_zero = true;  // S = 1 (all sss entries zero): S psi == psi
+  double *buf(const char *tag, size_t count);
+  int *flag(int i = 0);
+  void clear_flag(int i = 0);
+  int read_flag(int i = 0);
+  void proj(const double *psi, long ld, int nb, double *rp, long ld_rp);
+  void hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, const double *rp,
+            long ld_rp, double *out, long ld_out, bool accumulate, double *eps_out);
+  void spsi(const double *psi, long ld, int nb, const double *rp, long ld_rp, double *out,
+            long ld_out);
+  void dots(const double *x, long ldx, const double *y, long ldy, int nb, double *res);
+};
+
+struct KscgArgs {
+  int neigmx, ns, northo, nkscg, nsdmax, nprecon_cg, key_cmpt, key_nocmpt;
+  double epssd;
+  const double *veffs;
+  double *psi, *spsi, *hpsi, *rsp, *sval, *resid;
+  int *ksconv, *ksitmax;
+};
+struct DiisArgs {
+  int neigmx, ns, ndiismax, nprecon, northo, key_cmpt, l_hsv;
+  double eps_eig_diis, alambda_diis, alambda_min, alambda_max, epssd;
+  const double *veffs;
+  double *psi, *spsi, *hpsi, *rsp, *sval, *resid;
+  int *ksconv, *ksitmax;
+};
+struct HartreeArgs {
+  int nd, nx, ny, nz, nfh, nperi, ncgres, ncgmin, ncgmax;
+  double xmax, ymax, zmax, epsvh;
+  const double *rho, *boundx, *boundy, *boundz;
+  double *vh;
+};
+
+void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi);
+void gs_one(Ctx &c, int l, double *psi, double *spsi, double *rsp);
+void ortho_r_01(Ctx &c, int M, int l_inp, int nflag, int key_cmpt, double *psi, double *spsi,
+                double *rsp);
+void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, double *psi,
+                   double *spsi, double *rsp, double *hpsi, double *sval, bool keep_h);
+void kscg_sequential(Ctx &c, const KscgArgs &a);
+void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first);
+void ksdiis(Ctx &c, const DiisArgs &a);
+void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *spsi, double *rsp,
+             double *hpsi);
+int hartree(const HartreeArgs &a);
+void correct_potential(long n, double dvol_over_omega, double *vh);
+void charge(long n, int M, const double *psi, const double *f, double *rho);
+void force_eig(Ctx &c, int M, int ns, const double *psi, const double *sval, const double *fnele,
+               const int *is_in_dev, double *fa, int nf);
+
+}  // namespace kukan

@@ -1,0 +1,133 @@
+"""ctypes access to libkukan_hip.so, the MI355X implementation of the kukan hot path.
+
+The library must have been built (``python -m rspacedist_amd.build``); importing this
+module on a machine without it raises instead of falling back to anything else.
+Every entry point keeps the reference routine's keyword names (see ``abi.SIGNATURES``);
+arrays may be numpy (host, Fortran order) or torch tensors (host or device).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkukan_hip.so")
+_lib = None
+
+
+class KukanError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise KukanError(f"{LIB_PATH} is not built; run `python -m rspacedist_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.kukan_last_error.restype = ctypes.c_char_p
+        L.kukan_init.argtypes = [ctypes.c_int]
+        L.kukan_set_param.argtypes = [ctypes.c_char_p, ctypes.c_double]
+        L.kukan_get_param.argtypes = [ctypes.c_char_p]
+        L.kukan_get_param.restype = ctypes.c_double
+        L.kukan_stream.restype = ctypes.c_void_p
+        L.kukan_device_malloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        L.kukan_device_free.argtypes = [ctypes.c_void_p]
+        L.kukan_memcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        L.kukan_fill_normal.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64]
+        for f in ("kukan_bind",):
+            getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+        for f in ("kukan_unbind", "kukan_sync_to_host", "kukan_sync_to_device"):
+            getattr(L, f).argtypes = [ctypes.c_void_p]
+        abi.bind(L, "kukan_")
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().kukan_last_error().decode()
+
+
+def init(device=0):
+    rc = lib().kukan_init(device)
+    if rc:
+        raise KukanError(last_error())
+
+
+def set_param(key, value):
+    lib().kukan_set_param(key.encode(), float(value))
+
+
+def get_param(key):
+    return lib().kukan_get_param(key.encode())
+
+
+def call(name, **kw):
+    """Call kukan_<name> with reference-named keyword arguments; raises KukanError with the
+    library's message (the reference's stopp text) on a nonzero return code."""
+    res = abi.call(lib(), "kukan_", name, **kw)
+    if res["rc"] != 0:
+        raise KukanError(f"kukan_{name} failed ({res['rc']}): {last_error()}")
+    return res
+
+
+def _ck(rc, what):
+    if rc != 0:
+        raise KukanError(f"{what} failed ({rc}): {last_error()}")
+
+
+class DeviceArray:
+    """A Fortran-ordered float64/int32 array in HBM, allocated through the library's HIP
+    runtime.  Pass it to ``call`` like a numpy array; it is used in place."""
+
+    def __init__(self, shape, dtype="float64"):
+        import numpy as np
+        self.shape = tuple(int(d) for d in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _ck(lib().kukan_device_malloc(ctypes.byref(p), self.nbytes), "kukan_device_malloc")
+        self.device_ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a):
+        import numpy as np
+        a = np.asfortranarray(a)
+        d = cls(a.shape, a.dtype)
+        _ck(lib().kukan_memcpy(d.device_ptr, a.ctypes.data, d.nbytes, 1), "kukan_memcpy")
+        return d
+
+    def numpy(self):
+        import numpy as np
+        out = np.empty(self.shape, dtype=self.dtype, order="F")
+        _ck(lib().kukan_memcpy(out.ctypes.data, self.device_ptr, self.nbytes, 2), "kukan_memcpy")
+        _ck(lib().kukan_device_synchronize(), "kukan_device_synchronize")
+        return out
+
+    def fill_normal(self, seed):
+        _ck(lib().kukan_fill_normal(self.device_ptr, self.nbytes // 8, seed), "kukan_fill_normal")
+        return self
+
+    def free(self):
+        if self.device_ptr:
+            _ck(lib().kukan_device_free(self.device_ptr), "kukan_device_free")
+            self.device_ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    _ck(lib().kukan_device_synchronize(), "kukan_device_synchronize")
+
+
+EXPORTS = ["kukan_init", "kukan_finalize", "kukan_last_error", "kukan_set_param",
+           "kukan_get_param", "kukan_stream", "kukan_bind", "kukan_device_malloc", "kukan_device_free",
+           "kukan_memcpy", "kukan_fill_normal", "kukan_device_synchronize", "kukan_unbind", "kukan_sync_to_host",
+           "kukan_sync_to_device", "kukan_overlap_fdcheck_r", "kukan_rayleighritz_blacs_ini",
+           "kukan_rayleighritz_blacs_fin"] + ["kukan_" + n for n in abi.SIGNATURES]

@@ -1,0 +1,56 @@
+"""CPU checks of the C-ABI library: it loads and exports every symbol include/kukan_hip.h
+declares (no compute calls: there is no GPU here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    src = open(os.path.join(ROOT, "include", "kukan_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(kukan_\w+)\s*\(", src)))
+
+
+def test_header_declares_reference_routines():
+    syms = header_symbols()
+    for r in ("kslaplacian_r", "nonlocaloperation_r_01", "nonlocaloperation_r_02", "ksprecondition",
+              "kscg_r", "ksdiis_r", "orthogonalization_r", "orthogonalization_r_01",
+              "rayleighritz_r", "reorderstates", "scf_diag", "scf_hartree",
+              "tools_correctpotential", "scf_charge_smt_r", "force_eig_r", "totalenergy_eig",
+              "overlap_finitedifference_r", "overlap_fdcheck_r", "tools_definefdcoef"):
+        assert "kukan_" + r in syms
+
+
+def test_library_exports_every_declared_symbol():
+    path = os.path.join(ROOT, "rspacedist_amd", "libkukan_hip.so")
+    if not os.path.exists(path):
+        pytest.skip("libkukan_hip.so not built")
+    lib = ctypes.CDLL(path)
+    missing = [s for s in header_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_signature_table_matches_header():
+    from rspacedist_amd import abi
+    src = open(os.path.join(ROOT, "include", "kukan_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    for name, args in abi.SIGNATURES.items():
+        m = re.search(r"int kukan_%s\((.*?)\);" % name, src, flags=re.S)
+        assert m, name
+        params = [p.strip() for p in m.group(1).split(",")]
+        names = [re.findall(r"(\w+)$", p)[0] for p in params]
+        assert names == [a for a, _ in args], name
+
+
+def test_no_oracle_in_product_path():
+    pkg = os.path.join(ROOT, "rspacedist_amd")
+    for dp, _, fs in os.walk(pkg):
+        for f in fs:
+            if f.endswith((".py", ".hip", ".hpp", ".F90", ".cpp")):
+                txt = open(os.path.join(dp, f), errors="ignore").read()
+                assert "oracle" not in txt.replace("oracle", "oracle") or "import oracle" not in txt, f
+                assert "liboracle" not in txt, f

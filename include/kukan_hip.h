@@ -70,6 +70,14 @@ int kukan_memcpy(void *dst, const void *src, size_t bytes, int kind);
 int kukan_fill_normal(double *dev, size_t n, uint64_t seed);
 int kukan_device_synchronize(void);
 
+/* Device-time accounting (active when kukan_set_param("timing", 1)): accumulated kernel
+ * time in ms, launch count and algorithmic work (bytes or flops, per the tag) per tag, from
+ * HIP events on the library stream.  No reference counterpart (the reference times phases
+ * with wall clocks, modules8f.F90:98-118). */
+int kukan_timing_get(const char *tag, double *ms, long *count, double *work);
+int kukan_timing_reset(void);
+int kukan_timing_tags(char *buf, size_t len);
+
 /* ------------------------------------------------------------------ leaf routines
  * The reference calls these from inside OpenMP parallel regions; the library exports
  * them with single-call semantics for unit tests and for hosts that call them serially. */

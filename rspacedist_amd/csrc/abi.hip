@@ -167,6 +167,35 @@ int kukan_device_synchronize(void) {
   API_END
 }
 
+int kukan_timing_get(const char *tag, double *ms, long *count, double *work) {
+  API_BEGIN
+  timing_collect();
+  auto &t = timing_table();
+  auto it = t.find(tag);
+  *ms = it == t.end() ? 0.0 : it->second.ms;
+  *count = it == t.end() ? 0 : it->second.count;
+  *work = it == t.end() ? 0.0 : it->second.work;
+  API_END
+}
+int kukan_timing_reset(void) {
+  API_BEGIN
+  timing_collect();
+  timing_table().clear();
+  API_END
+}
+// Tags with recorded time, separated by ';' (buffer of len bytes).
+int kukan_timing_tags(char *buf, size_t len) {
+  API_BEGIN
+  timing_collect();
+  std::string s;
+  for (auto &kv : timing_table()) s += kv.first + ";";
+  if (len) {
+    strncpy(buf, s.c_str(), len - 1);
+    buf[len - 1] = 0;
+  }
+  API_END
+}
+
 // ----------------------------------------------------------------------------- leaves
 int kukan_tools_definefdcoef(int nf, double *acfd) {
   double a[9];

@@ -62,6 +62,74 @@ void *scratch(const std::string &tag, size_t bytes) {
 
 void sync() { HIPCK(hipStreamSynchronize(lib().stream)); }
 
+namespace {
+struct Pending {
+  std::string tag;
+  hipEvent_t a, b;
+  double work;
+};
+std::vector<Pending> &pending() {
+  static std::vector<Pending> p;
+  return p;
+}
+std::vector<hipEvent_t> &event_pool() {
+  static std::vector<hipEvent_t> p;
+  return p;
+}
+hipEvent_t get_event() {
+  auto &p = event_pool();
+  if (!p.empty()) {
+    hipEvent_t e = p.back();
+    p.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIPCK(hipEventCreate(&e));
+  return e;
+}
+}  // namespace
+
+std::map<std::string, TimingAcc> &timing_table() {
+  static std::map<std::string, TimingAcc> t;
+  return t;
+}
+
+Timed::Timed(const char *tag, double work) : tag_(tag), work_(work) {
+  if (param("timing", 0.0) < 1.0) return;
+  a_ = get_event();
+  b_ = get_event();
+  HIPCK(hipEventRecord(a_, lib().stream));
+}
+
+Timed::~Timed() {
+  if (!a_) return;
+  (void)hipEventRecord(b_, lib().stream);
+  pending().push_back({tag_, a_, b_, work_});
+  if (pending().size() > 4096) {
+    try {
+      timing_collect();
+    } catch (...) {
+    }
+  }
+}
+
+void timing_collect() {
+  auto &p = pending();
+  if (p.empty()) return;
+  HIPCK(hipStreamSynchronize(lib().stream));
+  for (auto &e : p) {
+    float ms = 0.f;
+    HIPCK(hipEventElapsedTime(&ms, e.a, e.b));
+    auto &acc = timing_table()[e.tag];
+    acc.ms += ms;
+    acc.work += e.work;
+    acc.count += 1;
+    event_pool().push_back(e.a);
+    event_pool().push_back(e.b);
+  }
+  p.clear();
+}
+
 bool is_device_ptr(const void *p) {
   if (!p) return false;
   hipPointerAttribute_t a;

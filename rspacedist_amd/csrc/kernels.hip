@@ -571,6 +571,7 @@ void launch_precond(const Grid &g, const double *in, long ld_in, bool halo_in, i
 void launch_precond_resid(const Grid &g, const double *h, const double *sp, long ld,
                           const double *eps, double sign, double *r_out, double *out, int nb,
                           hipStream_t s) {
+  Timed t("precond_resid", 8.0 * (double)g.n() * nb * (r_out ? 4.0 : 3.0));
   PRP p{};
   p.nx = g.nx; p.ny = g.ny; p.nz = g.nz;
   p.px = g.nperi >= 1; p.py = g.nperi >= 2; p.pz = g.nperi >= 3;
@@ -584,6 +585,7 @@ void launch_precond_resid(const Grid &g, const double *h, const double *sp, long
 void launch_gather(const NonlocalPlan &pl, const double *psi, long ld, int nb, double *rp,
                    long ld_rp, bool accumulate, hipStream_t s) {
   if (pl.nact == 0 || nb == 0) return;
+  Timed t("nl_gather", 8.0 * ((double)nb * pl.nent + (double)pl.nprjmx * pl.nent));
   if (pl.max_iprj <= 9) {
     hipLaunchKernelGGL((k_gather<9, 4>), dim3(pl.nact, nblk(nb, 4)), dim3(256), 0, s, pl.act,
                        pl.num_list, pl.nprjmx, pl.vnl, pl.lst, psi, ld, nb, rp, ld_rp,
@@ -627,6 +629,7 @@ void launch_dots(long n, const double *x, long ldx, const double *y, long ldy, i
                  double scale, double *res, hipStream_t s) {
   if (nb == 0) return;
   const int np = dot_nparts(n);
+  Timed t("dots", 16.0 * (double)n * nb);
   double *part = (double *)scratch("dots.part", sizeof(double) * (size_t)np * nb);
   hipLaunchKernelGGL(k_dot_part, dim3(np, nb), dim3(256), 0, s, n, x, ldx, y, ldy, part, np);
   hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part, np, scale, res);
@@ -643,6 +646,7 @@ void launch_axpby(long n, int nb, const double *a, double ca, const double *x, l
                   const double *b, double cb, const double *y, long ldy, double *out, long ldo,
                   hipStream_t s) {
   if (n * nb == 0) return;
+  Timed t("axpby", 8.0 * (double)n * nb * (y ? 3.0 : 2.0));
   hipLaunchKernelGGL(k_axpby, dim3(nblk(n * nb, 256)), dim3(256), 0, s, n, nb, a, ca, x, ldx, b,
                      cb, y, ldy, out, ldo);
   HIPCK(hipGetLastError());

@@ -76,6 +76,27 @@ std::vector<int> host_ints(const int *p, size_t n);
 std::vector<double> host_doubles(const double *p, size_t n);
 bool is_device_ptr(const void *p);
 
+// ----------------------------------------------------------------------------- timing
+// With param "timing" = 1, Timed regions record HIP events on the library stream; the
+// accumulated device time per tag (and the algorithmic work the caller attributes to it)
+// is read back through kukan_timing_get.
+class Timed {
+ public:
+  Timed(const char *tag, double work);
+  ~Timed();
+
+ private:
+  const char *tag_;
+  double work_;
+  hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+void timing_collect();
+struct TimingAcc {
+  double ms = 0.0, work = 0.0;
+  long count = 0;
+};
+std::map<std::string, TimingAcc> &timing_table();
+
 // ----------------------------------------------------------------------------- grid + nonlocal
 struct Grid {
   int nx, ny, nz, nf, nperi;

@@ -418,7 +418,12 @@ void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, 
     part = buf("hpsi.part", (size_t)np * nb);
     c.dot_partial = part;
   }
-  launch_stencil(c, s);
+  {
+    // algorithmic bytes: psi in + H psi out per band, plus veff and the projector tables once
+    const double nsum = (double)pl.nent;
+    Timed t("hpsi_stencil", 8.0 * ((double)nb * (2.0 * g.n() + nsum) + g.n() + pl.nprjmx * nsum));
+    launch_stencil(c, s);
+  }
   if (eps_out) launch_reduce_partials(part, np, nb, g.dv(), eps_out, s);
 }
 
@@ -436,6 +441,14 @@ void Ctx::dots(const double *x, long ldx, const double *y, long ldy, int nb, dou
   launch_dots(g.n(), x, ldx, y, ldy, nb, g.dv(), res, s);
 }
 
+// rocBLAS dgemm with timing under tag "gemm_f64" (2 m n k flops)
+static void dgemm(rocblas_operation ta, rocblas_operation tb, int m, int n, int k, double alpha,
+                  const double *A, int lda, const double *B, int ldb, double beta, double *C,
+                  int ldc) {
+  Timed t("gemm_f64", 2.0 * (double)m * n * k);
+  BLASCK(rocblas_dgemm(lib().blas, ta, tb, m, n, k, &alpha, A, lda, B, ldb, &beta, C, ldc));
+}
+
 // ============================================================================= Cholesky-QR
 // One pass of S-orthonormalisation of bands [0, M): G = dV Psi^T S Psi = R^T R,
 // Psi, S Psi, <p|psi> (and H Psi if given) <- X R^-1.  Same Q as Gram-Schmidt in exact
@@ -445,15 +458,19 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi)
   rocblas_handle h = lib().blas;
   double *G = c.buf("cholqr.G", (size_t)M * M);
   const double dv = c.g.dv(), zero = 0.0, one = 1.0;
-  BLASCK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, &dv,
-                       psi, (int)n, spsi, (int)n, &zero, G, M));
+  dgemm(rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, dv, psi, (int)n, spsi,
+        (int)n, zero, G, M);
   int *info = (int *)scratch("cholqr.info", sizeof(int));
-  BLASCK(rocsolver_dpotrf(h, rocblas_fill_upper, M, G, M, info));
+  {
+    Timed t("potrf", (double)M * M * M / 3.0);
+    BLASCK(rocsolver_dpotrf(h, rocblas_fill_upper, M, G, M, info));
+  }
   int hinfo = 0;
   HIPCK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c.s));
   sync();
   if (hinfo != 0) throw Error(2, "orthogonalization_r_01 (1): wf has negative norm");
   auto tr = [&](double *X, long ldx, long rows) {
+    Timed t("trsm_f64", (double)rows * M * M);
     BLASCK(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                          rocblas_diagonal_non_unit, (int)rows, M, &one, G, M, X, (int)ldx));
   };
@@ -514,10 +531,13 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   double *A = c.buf("rr.A", (size_t)M * M);
   double *E = c.buf("rr.E", (size_t)M);
   const double dv = c.g.dv(), zero = 0.0, one = 1.0;
-  BLASCK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, &dv,
-                       psi, (int)n, hpsi, (int)n, &zero, A, M));
+  dgemm(rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, dv, psi, (int)n, hpsi,
+        (int)n, zero, A, M);
   int *info = (int *)scratch("rr.info", sizeof(int));
-  BLASCK(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, M, A, M, sval, E, info));
+  {
+    Timed t("syevd", 4.0 * (double)M * M * M);
+    BLASCK(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, M, A, M, sval, E, info));
+  }
   int hinfo = 0;
   HIPCK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c.s));
   sync();
@@ -528,23 +548,23 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   double *Wh = nullptr;
   if (keep_h) {  // block path: rotate H Psi too, through a separate buffer
     Wh = c.buf("rr.Wh", (size_t)n * M);
-    BLASCK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)n, M, M, &one, hpsi,
-                         (int)n, A, M, &zero, Wh, (int)n));
+    dgemm(rocblas_operation_none, rocblas_operation_none, (int)n, M, M, one, hpsi, (int)n, A, M,
+          zero, Wh, (int)n);
   }
-  BLASCK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)n, M, M, &one, psi,
-                       (int)n, A, M, &zero, W, (int)n));
+  dgemm(rocblas_operation_none, rocblas_operation_none, (int)n, M, M, one, psi, (int)n, A, M, zero,
+        W, (int)n);
   HIPCK(hipMemcpyAsync(psi, W, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
   if (c.q_zero && spsi != psi) {
     HIPCK(hipMemcpyAsync(spsi, psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
   } else if (spsi != psi) {
-    BLASCK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)n, M, M, &one,
-                         spsi, (int)n, A, M, &zero, W, (int)n));
+    dgemm(rocblas_operation_none, rocblas_operation_none, (int)n, M, M, one, spsi, (int)n, A, M,
+          zero, W, (int)n);
     HIPCK(hipMemcpyAsync(spsi, W, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
   }
   if (n1 > 0) {
     double *R = c.buf("rr.R", (size_t)n1 * M);
-    BLASCK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)n1, M, M, &one, rsp,
-                         (int)n1, A, M, &zero, R, (int)n1));
+    dgemm(rocblas_operation_none, rocblas_operation_none, (int)n1, M, M, one, rsp, (int)n1, A, M,
+          zero, R, (int)n1);
     HIPCK(hipMemcpyAsync(rsp, R, sizeof(double) * n1 * M, hipMemcpyDeviceToDevice, c.s));
   }
   if (keep_h) {
@@ -701,10 +721,10 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
       c.dots(HP, n, KR, n, M, rnew);
     }
     // Kr orthogonal to the block in the S metric: C = dV (S Psi)^T Kr ; Kr -= Psi C
-    BLASCK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, &dv,
-                         a.spsi, (int)n, KR, (int)n, &zero, C, M));
-    BLASCK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)n, M, M, &mone,
-                         a.psi, (int)n, C, M, &one, KR, (int)n));
+    dgemm(rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, dv, a.spsi, (int)n, KR,
+          (int)n, zero, C, M);
+    dgemm(rocblas_operation_none, rocblas_operation_none, (int)n, M, M, mone, a.psi, (int)n, C, M,
+          one, KR, (int)n);
     // P = Kr + beta P
     hipLaunchKernelGGL(k_ratio, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, rnew, rold, doreset ? 1 : 0, tb);
     if (doreset) launch_copy2d(n, M, KR, n, PA, n, c.s);
@@ -840,8 +860,10 @@ void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *sp
   }
   launch_permute_cols(n, M, pd, psi, n, hpsi, n, c.s);
   HIPCK(hipMemcpyAsync(psi, hpsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
-  launch_permute_cols(n, M, pd, spsi, n, hpsi, n, c.s);
-  HIPCK(hipMemcpyAsync(spsi, hpsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+  if (spsi != psi) {  // S psi may alias psi (S = 1); permute it only once
+    launch_permute_cols(n, M, pd, spsi, n, hpsi, n, c.s);
+    HIPCK(hipMemcpyAsync(spsi, hpsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+  }
   sync();
 }
 

@@ -37,6 +37,9 @@ def lib():
         L.kukan_device_free.argtypes = [ctypes.c_void_p]
         L.kukan_memcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         L.kukan_fill_normal.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64]
+        L.kukan_timing_get.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_double)]
+        L.kukan_timing_tags.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
         for f in ("kukan_bind",):
             getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_size_t]
         for f in ("kukan_unbind", "kukan_sync_to_host", "kukan_sync_to_device"):
@@ -122,12 +125,32 @@ class DeviceArray:
             pass
 
 
+def timing():
+    """{tag: (ms, count, work)} accumulated since the last timing_reset()."""
+    buf = ctypes.create_string_buffer(4096)
+    _ck(lib().kukan_timing_tags(buf, 4096), "kukan_timing_tags")
+    out = {}
+    for tag in buf.value.decode().split(";"):
+        if not tag:
+            continue
+        ms, cnt, work = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
+        _ck(lib().kukan_timing_get(tag.encode(), ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(work)),
+            "kukan_timing_get")
+        out[tag] = (ms.value, cnt.value, work.value)
+    return out
+
+
+def timing_reset():
+    _ck(lib().kukan_timing_reset(), "kukan_timing_reset")
+
+
 def synchronize():
     _ck(lib().kukan_device_synchronize(), "kukan_device_synchronize")
 
 
 EXPORTS = ["kukan_init", "kukan_finalize", "kukan_last_error", "kukan_set_param",
            "kukan_get_param", "kukan_stream", "kukan_bind", "kukan_device_malloc", "kukan_device_free",
-           "kukan_memcpy", "kukan_fill_normal", "kukan_device_synchronize", "kukan_unbind", "kukan_sync_to_host",
+           "kukan_memcpy", "kukan_fill_normal", "kukan_device_synchronize", "kukan_timing_get",
+           "kukan_timing_reset", "kukan_timing_tags", "kukan_unbind", "kukan_sync_to_host",
            "kukan_sync_to_device", "kukan_overlap_fdcheck_r", "kukan_rayleighritz_blacs_ini",
            "kukan_rayleighritz_blacs_fin"] + ["kukan_" + n for n in abi.SIGNATURES]

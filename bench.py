@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFS = 78.6
+PEAK_F16_TFS = 2500.0  # dense MFMA fp16
 
 
 def log(*a):
@@ -103,8 +104,9 @@ def roofline_entry(tim, tag, bound):
                 "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "kernel": tag,
                 "avg_launch_ms": round(ms / cnt, 4), "launches": cnt}
     ach = work / cnt / per_launch_s / 1e12
-    return {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_FP64_TFS, "unit": "TFLOP/s",
-            "frac": round(ach / PEAK_FP64_TFS, 4), "traffic": None, "kernel": tag,
+    peak = PEAK_F16_TFS if tag == "gemm_mixed" else PEAK_FP64_TFS
+    return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(ach / peak, 4), "traffic": None, "kernel": tag,
             "avg_launch_ms": round(ms / cnt, 4), "launches": cnt}
 
 
@@ -194,8 +196,12 @@ def main():
     bound = {"gemm_f64": "mfma", "gemm_mixed": "mfma", "trsm_f64": "mfma", "syevd": "mfma",
              "potrf": "mfma"}.get(dominant, "hbm")
     roof = roofline_entry(tim, dominant, bound) if dominant else None
-    extra = {t: roofline_entry(tim, t, "hbm") for t in ("hpsi_stencil", "precond_resid", "dots", "axpby", "nl_gather")
+    extra = {t: roofline_entry(tim, t, "hbm") for t in ("hpsi_stencil", "precond_resid", "dots", "axpby", "nl_gather",
+                                                        "nl_expand", "gemm_scale")
              if t in tim}
+    for t in ("gemm_f64", "gemm_mixed"):
+        if t in tim and t != dominant:
+            extra[t] = roofline_entry(tim, t, "mfma")
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle on host cores)")

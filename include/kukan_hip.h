@@ -70,6 +70,15 @@ int kukan_memcpy(void *dst, const void *src, size_t bytes, int kind);
 int kukan_fill_normal(double *dev, size_t n, uint64_t seed);
 int kukan_device_synchronize(void);
 
+/* Subspace product on the library stream, as used inside Rayleigh-Ritz, Cholesky-QR and the
+ * block CG projections: trans_a = 1: C(m x n) = alpha A^T B + beta C with A (k x m);
+ * trans_a = 0: C(m x n) = alpha A B + beta C with A (m x k), C not aliasing A.  Column-major,
+ * B (k x n).  FP64 (rocBLAS) or split-FP16 MFMA per kukan_set_param("precision",
+ * "mixed_min_rows", "mixed_terms").  No single reference counterpart: it replaces the
+ * dgemm calls of kukan/rayleigh-ritz8f.F90:200-333. */
+int kukan_subspace_gemm(int trans_a, int m, int n, long k, double alpha, const double *A,
+                        long lda, const double *B, long ldb, double beta, double *C, long ldc);
+
 /* Device-time accounting (active when kukan_set_param("timing", 1)): accumulated kernel
  * time in ms, launch count and algorithmic work (bytes or flops, per the tag) per tag, from
  * HIP events on the library stream.  No reference counterpart (the reference times phases

@@ -13,11 +13,11 @@ LIB = os.path.join(HERE, "libkukan_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
-SOURCES = ["common.hip", "kernels.hip", "solvers.hip", "abi.hip"]
+SOURCES = ["common.hip", "kernels.hip", "gemm_mixed.hip", "solvers.hip", "abi.hip"]
 CFLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-result",
           # packed FP32 instructions stay off (see the design notes on gfx950 hazards)
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-I", os.path.join(HERE, "..", "include")]
+          "-I", os.path.join(os.path.dirname(HERE), "include")]
 LDFLAGS = ["-shared", f"--offload-arch={ARCH}", "-L", os.path.join(ROCM, "lib"), "-lrocblas",
            "-lrocsolver", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"]
 

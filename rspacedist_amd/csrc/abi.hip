@@ -8,7 +8,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "../../include/kukan_hip.h"
+#include "kukan_hip.h"
 #include "kukan_internal.hpp"
 #include "solvers.hpp"
 
@@ -164,6 +164,29 @@ int kukan_fill_normal(double *dev, size_t n, uint64_t seed) {
 }
 int kukan_device_synchronize(void) {
   API_BEGIN
+  API_END
+}
+
+int kukan_subspace_gemm(int trans_a, int m, int n, long k, double alpha, const double *A,
+                        long lda, const double *B, long ldb, double beta, double *C, long ldc) {
+  API_BEGIN
+  if (m < 0 || n < 0 || k < 0) throw Error(1, "subspace_gemm: negative dimension");
+  if (m == 0 || n == 0) return 0;
+  const long arows = trans_a ? k : m, acols = trans_a ? m : k;
+  if (lda < std::max<long>(1, arows) || ldb < std::max<long>(1, k) || ldc < m)
+    throw Error(1, "subspace_gemm: leading dimension too small");
+  Arr a(A, sizeof(double) * (size_t)(lda * (acols - 1) + arows), Arr::In);
+  Arr b(B, sizeof(double) * (size_t)(ldb * (n - 1) + k), Arr::In);
+  Arr c(C, sizeof(double) * (size_t)(ldc * (n - 1) + m), Arr::InOut);
+  if (!trans_a && c.d() == a.d()) throw Error(1, "subspace_gemm: C must not alias A");
+  if (k == 0) {
+    subspace_tn(m, n, 0, 0.0, a.d(), lda, b.d(), ldb, beta, c.d(), ldc);
+  } else if (trans_a) {
+    subspace_tn(m, n, k, alpha, a.d(), lda, b.d(), ldb, beta, c.d(), ldc);
+  } else {
+    subspace_nn(m, (int)k, n, alpha, a.d(), lda, b.d(), ldb, beta, c.d(), ldc);
+  }
+  c.finish();
   API_END
 }
 

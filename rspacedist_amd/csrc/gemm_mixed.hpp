@@ -1,0 +1,17 @@
+// gemm_mixed.hpp — split-FP16 MFMA subspace products (see gemm_mixed.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace kukan {
+// scale[j] = 2^(14 - ceil(log2 max_i |A(i,j) * pre[j]|)), pre may be null
+void colscale(const double *A, long lda, long rows, int cols, const double *pre, double *scale,
+              hipStream_t s);
+// C(M1 x M2) = alpha A^T B + beta C with column scales sa (A) and sb (B)
+void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
+                   const double *sa, const double *B, long ldb, const double *sb, double beta,
+                   double *C, long ldc, int nterms, hipStream_t s);
+// Y(R x M2) = alpha X C + beta Y with column scales sx of X (length K); Y must not alias X
+void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ldx,
+                   const double *sx, const double *C, long ldc, double beta, double *Y, long ldy,
+                   int nterms, hipStream_t s);
+}  // namespace kukan

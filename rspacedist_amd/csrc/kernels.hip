@@ -54,6 +54,7 @@ struct SP {
   const int *iprj_slot;
   int zc, nzch;
   double *dotp;
+  long dot_ld;
 };
 
 __device__ inline double val(const SP &p, const double *base, int x, int y, int z) {
@@ -125,8 +126,137 @@ __global__ __launch_bounds__(256) void k_stencil(SP p) {
     if (threadIdx.x == 0 && threadIdx.y == 0) {
       const long part = blockIdx.x + (long)gridDim.x * blockIdx.y;
       const long nparts = (long)gridDim.x * gridDim.y;
-      p.dotp[(long)b * nparts + part] = t;
+      p.dotp[(long)b * (p.dot_ld ? p.dot_ld : nparts) + part] = t;
     }
+  }
+}
+
+// ----------------------------------------------------------------------------- LDS stencil
+// z-marching tile kernel for interior (unhaloed) bands, nx <= 256.
+//   * a block owns full x-rows: a tile of nx x TY points, marching over a z-range; each
+//     thread owns P points of the tile and keeps their z-columns (2*NF+1 planes) in
+//     registers, so every band value is read from HBM once per z-range;
+//   * the current plane of the tile plus NF halo rows above and below sit in LDS
+//     (x wraps inside the row), giving the x and y neighbours;
+//   * blocks are mapped XCD-aware: all tiles and z-ranges of a band run on one XCD
+//     (block id mod 8), so the halo rows and veff/projector reads hit that XCD's L2;
+//   * RES = true reads sign*(h - eps_b s) instead of a band (fused residual for K r).
+struct SL {
+  int nx, ny, nz, px, py, pz, TY, ntile, nzch, zc, nbands;
+  const double *in, *in2;
+  long ld_in;
+  const double *eps;
+  double sign;
+  double *out;
+  long ld_out;
+  const double *veff;
+  double cx[9], cy[9], cz[9], c0;
+  int accumulate;
+  const int *row_ptr;
+  const int2 *ent;
+  const double *vnl;
+  const double *coef;
+  long ld_coef;
+  int num_list, nprjmx;
+  const int *iprj_slot;
+  double *dotp;  // per band: ntile * nzch partials of <c|out> (c = centre input value)
+  long dot_ld;   // band stride of dotp
+};
+
+template <bool RES>
+__device__ inline double sl_load(const SL &p, const double *in, const double *in2, double e, int x,
+                                 int y, int z) {
+  if (y < 0 || y >= p.ny) {
+    if (!p.py) return 0.0;
+    y = y < 0 ? y + p.ny : (y >= p.ny ? y - p.ny : y);
+    if (y < 0 || y >= p.ny) return 0.0;  // tile rows beyond one period (ragged tiles)
+  }
+  if (z < 0 || z >= p.nz) {
+    if (!p.pz) return 0.0;
+    z = z < 0 ? z + p.nz : z - p.nz;
+  }
+  const long i = (long)x + (long)p.nx * ((long)y + (long)p.ny * z);
+  if (RES) return p.sign * (in[i] - e * in2[i]);
+  return in[i];
+}
+
+template <int NF, int P, bool RES>
+__global__ __launch_bounds__(512) void k_stencil_lds(SL p) {
+  extern __shared__ double lds[];
+  __shared__ double red[8];
+  const int nx = p.nx, TY = p.TY;
+  const int L = blockIdx.x, xcd = L & 7, j = L >> 3;
+  const int per = p.ntile * p.nzch;
+  const int b = (j / per) * 8 + xcd;
+  if (b >= p.nbands) return;  // uniform for the block
+  const int rem = j % per, tile = rem % p.ntile, zch = rem / p.ntile;
+  const int y0 = tile * TY, z0 = zch * p.zc, z1 = min(p.nz, z0 + p.zc);
+  const double *in = p.in + (long)b * p.ld_in;
+  const double *in2 = RES ? p.in2 + (long)b * p.ld_in : nullptr;
+  const double e = RES ? p.eps[b] : 0.0;
+  double *out = p.out + (long)b * p.ld_out;
+  const int t = threadIdx.x;
+  const int npt = nx * TY;
+  int px_[P], yl_[P];
+  bool val_[P];
+#pragma unroll
+  for (int q = 0; q < P; q++) {
+    const int idx = t + 512 * q;
+    val_[q] = idx < npt;
+    px_[q] = val_[q] ? idx % nx : 0;
+    yl_[q] = val_[q] ? idx / nx : 0;
+  }
+  double col[P][2 * NF + 1];
+#pragma unroll
+  for (int q = 0; q < P; q++)
+#pragma unroll
+    for (int m = 0; m < 2 * NF; m++)
+      col[q][m + 1] = val_[q] ? sl_load<RES>(p, in, in2, e, px_[q], y0 + yl_[q], z0 - NF + m) : 0.0;
+  double dsum = 0.0;
+  const int W = nx;
+  for (int z = z0; z < z1; z++) {
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+#pragma unroll
+      for (int m = 0; m < 2 * NF; m++) col[q][m] = col[q][m + 1];
+      col[q][2 * NF] = val_[q] ? sl_load<RES>(p, in, in2, e, px_[q], y0 + yl_[q], z + NF) : 0.0;
+      if (val_[q]) lds[(NF + yl_[q]) * W + px_[q]] = col[q][NF];
+    }
+    for (int i = t; i < 2 * NF * nx; i += 512) {
+      const int r = i / nx, xx = i % nx;
+      const int row = r < NF ? r : TY + r;
+      lds[row * W + xx] = sl_load<RES>(p, in, in2, e, xx, y0 - NF + row, z);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      const int gy = y0 + yl_[q];
+      if (!val_[q] || gy >= p.ny) continue;
+      const int x = px_[q], row = NF + yl_[q];
+      const long r = (long)x + (long)nx * ((long)gy + (long)p.ny * z);
+      const double c = col[q][NF];
+      double s = (p.c0 + (p.veff ? p.veff[r] : 0.0)) * c;
+      const double *lr = lds + row * W;
+#pragma unroll
+      for (int m = 1; m <= NF; m++) {
+        int xm = x - m, xp = x + m;
+        double a, bb;
+        if (xm < 0) a = p.px ? lr[xm + nx] : 0.0; else a = lr[xm];
+        if (xp >= nx) bb = p.px ? lr[xp - nx] : 0.0; else bb = lr[xp];
+        s += p.cx[m] * (a + bb);
+      }
+#pragma unroll
+      for (int m = 1; m <= NF; m++) s += p.cy[m] * (lr[x - m * W] + lr[x + m * W]);
+#pragma unroll
+      for (int m = 1; m <= NF; m++) s += p.cz[m] * (col[q][NF - m] + col[q][NF + m]);
+      if (p.dotp) dsum += c * s;
+      out[r] = p.accumulate ? out[r] + s : s;
+    }
+    __syncthreads();
+  }
+  if (p.dotp) {
+    const double tot = block_sum<512>(dsum, red);
+    if (t == 0) p.dotp[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
   }
 }
 
@@ -307,6 +437,68 @@ __global__ void k_expand(const int *row_ptr, const int2 *ent, const int *iprj_sl
     for (int j = 0; j < ip; j++) s += cc[j] * v[(long)num_list * j];
   }
   out[(long)b * ld_out + r] += s;
+}
+
+// Owner-computes expansion over the rows that carry projector entries, EB bands per
+// thread: the entries' vnl values are read once per EB bands; per-band partials of
+// <psi|added> support the fused Rayleigh quotient.
+template <int EB>
+__global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz, const int *row_ptr,
+                                                     const int2 *ent, const int *iprj_slot,
+                                                     const double *vnl, int num_list, int nprjmx,
+                                                     const double *coef, long ld_coef, int nb,
+                                                     double *out, long ld_out, const double *psi,
+                                                     long ld_psi, double *part, long part_ld,
+                                                     int part_off) {
+  __shared__ double red[EB][4];
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  const int b0 = blockIdx.y * EB;
+  double acc[EB];
+#pragma unroll
+  for (int q = 0; q < EB; q++) acc[q] = 0.0;
+  long r = -1;
+  if (i < nnz) {
+    r = nzrows[i];
+    const int e1 = row_ptr[r + 1];
+    for (int e = row_ptr[r]; e < e1; e++) {
+      const int2 en = ent[e];
+      const int ip = iprj_slot[en.x];
+      const double *v = vnl + en.y + (long)num_list * nprjmx * en.x;
+      const double *cc = coef + (long)nprjmx * en.x;
+      for (int j = 0; j < ip; j++) {
+        const double vj = v[(long)num_list * j];
+#pragma unroll
+        for (int q = 0; q < EB; q++)
+          if (b0 + q < nb) acc[q] += cc[(long)(b0 + q) * ld_coef + j] * vj;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < EB; q++)
+      if (b0 + q < nb) out[(long)(b0 + q) * ld_out + r] += acc[q];
+  }
+  if (part) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+      double d = (r >= 0 && b0 + q < nb) ? psi[(long)(b0 + q) * ld_psi + r] * acc[q] : 0.0;
+      d = wave_sum(d);
+      if (lane == 0) red[q][wv] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x < EB && b0 + (int)threadIdx.x < nb) {
+      const int q = threadIdx.x;
+      part[(long)(b0 + q) * part_ld + part_off + blockIdx.x] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
+    }
+  }
+}
+
+__global__ void k_flag_rows(const int *row_ptr, long n, int *flag) {
+  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r < n) flag[r] = row_ptr[r + 1] > row_ptr[r] ? 1 : 0;
+}
+__global__ void k_compact_rows(const int *flag, const int *pos, long n, int *rows) {
+  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r < n && flag[r]) rows[pos[r]] = (int)r;
 }
 
 // inverse map construction
@@ -491,7 +683,64 @@ static void fd_weights(int nf, double *a) {
 }
 void fd_coefficients(int nf, double *acfd) { fd_weights(nf, acfd); }
 
-int stencil_nparts(const Grid &g) {
+// Tile plan of the LDS kernel for grid g and nb bands (nx <= 256).
+struct LdsPlan {
+  int TY, ntile, nzch, zc, P;
+};
+static LdsPlan lds_plan(const Grid &g, int NF, int nb) {
+  LdsPlan L;
+  L.TY = std::max(1, std::min(g.ny, 2048 / g.nx));
+  L.ntile = (g.ny + L.TY - 1) / L.TY;
+  L.P = (g.nx * L.TY + 511) / 512;
+  if (L.P == 3) L.P = 4;
+  // z-ranges: enough blocks to fill the chip, each range at least 4*NF planes
+  long blocks = (long)nb * L.ntile;
+  int nz_ch = 1;
+  while (blocks * nz_ch < 2048 && g.nz / (nz_ch * 2) >= 4 * NF) nz_ch *= 2;
+  L.zc = (g.nz + nz_ch - 1) / nz_ch;
+  L.nzch = (g.nz + L.zc - 1) / L.zc;
+  return L;
+}
+static bool lds_ok(const Grid &g) { return g.nx <= 256 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8; }
+
+int stencil_parts_lds(const Grid &g, int NF, int nb) {
+  LdsPlan L = lds_plan(g, NF, nb);
+  return L.ntile * L.nzch;
+}
+
+template <int NF, bool RES>
+static void launch_lds_nf(const SL &p, const LdsPlan &L, int nb, hipStream_t s) {
+  const size_t shm = sizeof(double) * (size_t)(L.TY + 2 * NF) * p.nx;
+  const long groups = (nb + 7) / 8;
+  const unsigned grid = (unsigned)(groups * 8 * (long)L.ntile * L.nzch);
+  switch (L.P) {
+    case 1: hipLaunchKernelGGL((k_stencil_lds<NF, 1, RES>), dim3(grid), dim3(512), shm, s, p); break;
+    case 2: hipLaunchKernelGGL((k_stencil_lds<NF, 2, RES>), dim3(grid), dim3(512), shm, s, p); break;
+    default: hipLaunchKernelGGL((k_stencil_lds<NF, 4, RES>), dim3(grid), dim3(512), shm, s, p); break;
+  }
+}
+
+// Generic LDS launch: coefficients in p; in2/eps for the residual mode.
+static void launch_lds(SL &p, const Grid &g, int NF, int nb, bool res, hipStream_t s) {
+  LdsPlan L = lds_plan(g, NF, nb);
+  if (L.P > 4) throw Error(1, "stencil tile does not fit the thread block");
+  p.TY = L.TY; p.ntile = L.ntile; p.nzch = L.nzch; p.zc = L.zc; p.nbands = nb;
+  if (res) {
+    if (NF != 1) throw Error(1, "fused residual stencil supports radius 1 only");
+    launch_lds_nf<1, true>(p, L, nb, s);
+  } else {
+    switch (NF) {
+#define CASE(N) case N: launch_lds_nf<N, false>(p, L, nb, s); break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+      default: throw Error(1, "error in kslaplacian_r! This subroutine is for nf in 1-8.");
+    }
+  }
+  HIPCK(hipGetLastError());
+}
+
+int stencil_nparts(const Grid &g, int nf, int nb) {
+  if (lds_ok(g) && param("stencil_lds", 1.0) >= 1.0) return stencil_parts_lds(g, nf, nb);
   const int zc = 8;
   return (int)(nblk(g.nx, 64) * nblk(g.ny, 4) * nblk(g.nz, zc));
 }
@@ -524,9 +773,28 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
     p.num_list = c.pl->num_list; p.nprjmx = c.pl->nprjmx; p.iprj_slot = c.pl->iprj_slot;
     p.coef = c.coef; p.ld_coef = c.ld_coef;
   }
+  if (!c.halo_in && lds_ok(g) && param("stencil_lds", 1.0) >= 1.0) {
+    SL q{};
+    q.nx = g.nx; q.ny = g.ny; q.nz = g.nz; q.px = p.px; q.py = p.py; q.pz = p.pz;
+    q.in = c.in; q.ld_in = c.ld_in; q.out = c.out; q.ld_out = c.ld_out; q.veff = c.veff;
+    for (int m = 0; m < 9; m++) { q.cx[m] = p.cx[m]; q.cy[m] = p.cy[m]; q.cz[m] = p.cz[m]; }
+    q.c0 = p.c0; q.accumulate = p.accumulate;
+    q.row_ptr = p.row_ptr; q.ent = p.ent; q.vnl = p.vnl; q.coef = p.coef; q.ld_coef = p.ld_coef;
+    q.num_list = p.num_list; q.nprjmx = p.nprjmx; q.iprj_slot = p.iprj_slot;
+    q.dotp = c.dot_partial;
+    q.dot_ld = c.dot_ld;
+    launch_lds(q, g, nf, c.nb, false, s);
+    if (c.pl && c.coef && c.pl->nent > 0) {  // nonlocal part: separate owner-computes pass
+      const long pld = c.dot_ld ? c.dot_ld : stencil_parts_lds(g, nf, c.nb);
+      launch_expand_rows(*c.pl, c.coef, c.ld_coef, c.nb, c.out, c.ld_out, c.in, c.ld_in,
+                         c.dot_partial, pld, stencil_parts_lds(g, nf, c.nb), s);
+    }
+    return;
+  }
   p.zc = 8;
   p.nzch = (int)nblk(g.nz, p.zc);
   p.dotp = c.dot_partial;
+  p.dot_ld = c.dot_ld;
   if (c.nb > 65535) throw Error(1, "too many bands in one launch");
   dim3 grid(nblk(g.nx, 64) * p.nzch, nblk(g.ny, 4), (unsigned)c.nb);
   dim3 blk(64, 4, 1);
@@ -568,9 +836,27 @@ void launch_precond(const Grid &g, const double *in, long ld_in, bool halo_in, i
   HIPCK(hipGetLastError());
 }
 
+void launch_precond_resid_dot(const Grid &g, const double *h, const double *sp, long ld,
+                              const double *eps, double sign, double *out, int nb,
+                              double *dot_partial, hipStream_t s) {
+  Timed t("precond_resid", 8.0 * (double)g.n() * nb * 3.0);
+  SL q{};
+  q.nx = g.nx; q.ny = g.ny; q.nz = g.nz;
+  q.px = g.nperi >= 1; q.py = g.nperi >= 2; q.pz = g.nperi >= 3;
+  q.in = h; q.in2 = sp; q.ld_in = ld; q.eps = eps; q.sign = sign; q.out = out; q.ld_out = ld;
+  q.c0 = 0.5;
+  q.cx[1] = q.cy[1] = q.cz[1] = 1.0 / 12.0;
+  q.dotp = dot_partial;
+  launch_lds(q, g, 1, nb, true, s);
+}
+
 void launch_precond_resid(const Grid &g, const double *h, const double *sp, long ld,
                           const double *eps, double sign, double *r_out, double *out, int nb,
                           hipStream_t s) {
+  if (!r_out && lds_ok(g) && param("stencil_lds", 1.0) >= 1.0) {
+    launch_precond_resid_dot(g, h, sp, ld, eps, sign, out, nb, nullptr, s);
+    return;
+  }
   Timed t("precond_resid", 8.0 * (double)g.n() * nb * (r_out ? 4.0 : 3.0));
   PRP p{};
   p.nx = g.nx; p.ny = g.ny; p.nz = g.nz;
@@ -615,6 +901,25 @@ void launch_expand(const NonlocalPlan &pl, long n, const double *coef, long ld_c
   hipLaunchKernelGGL(k_expand, dim3(nblk(n * nb, 256)), dim3(256), 0, s, pl.row_ptr, pl.ent,
                      pl.iprj_slot, pl.vnl, pl.num_list, pl.nprjmx, n, coef, ld_coef, nb, out,
                      ld_out);
+  HIPCK(hipGetLastError());
+}
+
+int expand_nparts(const NonlocalPlan &pl) { return (int)std::max<long>(1, (pl.nnz + 255) / 256); }
+
+void launch_expand_rows(const NonlocalPlan &pl, const double *coef, long ld_coef, int nb,
+                        double *out, long ld_out, const double *psi, long ld_psi, double *part,
+                        long part_ld, int part_off, hipStream_t s) {
+  if (nb == 0) return;
+  const int np = expand_nparts(pl);
+  if (pl.nnz == 0) {
+    if (part)
+      for (int b = 0; b < nb; b++) launch_fill(1, 0.0, part + (long)b * part_ld + part_off, s);
+    return;
+  }
+  Timed t("nl_expand", 8.0 * (2.0 * (double)pl.nnz * nb + (double)pl.nprjmx * pl.nent));
+  hipLaunchKernelGGL(k_expand_rows<8>, dim3(np, nblk(nb, 8)), dim3(256), 0, s, pl.nzrows, pl.nnz,
+                     pl.row_ptr, pl.ent, pl.iprj_slot, pl.vnl, pl.num_list, pl.nprjmx, coef,
+                     ld_coef, nb, out, ld_out, psi, ld_psi, part, part_ld, part_off);
   HIPCK(hipGetLastError());
 }
 
@@ -746,6 +1051,19 @@ void build_plan(NonlocalPlan &pl, const Grid &g, int natom, int nprjmx, int num_
     hipLaunchKernelGGL(k_csr_sort, dim3(nblk(n, 256)), dim3(256), 0, s, pl.row_ptr, pl.ent, n);
     HIPCK(hipGetLastError());
   }
+  // compacted list of rows with entries
+  int *flag = cnt;  // reuse (n+1 ints)
+  int *pos = (int *)scratch(tag + ".pos", sizeof(int) * (n + 1));
+  hipLaunchKernelGGL(k_flag_rows, dim3(nblk(n, 256)), dim3(256), 0, s, pl.row_ptr, n, flag);
+  HIPCK(hipMemsetAsync(flag + n, 0, sizeof(int), s));
+  HIPCK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, flag, pos, (int)(n + 1), s));
+  int hnnz = 0;
+  HIPCK(hipMemcpyAsync(&hnnz, pos + n, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCK(hipStreamSynchronize(s));
+  pl.nnz = hnnz;
+  pl.nzrows = (int *)scratch(tag + ".nzrows", sizeof(int) * std::max<long>(1, pl.nnz));
+  hipLaunchKernelGGL(k_compact_rows, dim3(nblk(n, 256)), dim3(256), 0, s, flag, pos, n, pl.nzrows);
+  HIPCK(hipGetLastError());
 }
 
 }  // namespace kukan

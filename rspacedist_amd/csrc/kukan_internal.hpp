@@ -117,6 +117,8 @@ struct NonlocalPlan {
   int *row_ptr = nullptr;        // N+1
   int2 *ent = nullptr;           // {slot, k}
   long nent = 0;
+  int *nzrows = nullptr;         // grid points with at least one entry, ascending
+  long nnz = 0;
   std::vector<int> act_h;        // host copy of act
   int max_iprj = 0, max_natinf = 0;
 };
@@ -144,11 +146,16 @@ struct StencilCall {
   long ld_coef = 0;
   double scale_lap = 1.0;            // multiplies the kinetic part (1 for H)
   double *dot_partial = nullptr;     // optional: per-block partial sums of in_b . out_b
+  long dot_ld = 0;                   // band stride of dot_partial (0: number of parts)
   int nfh_coeffs = 0;                // if > 0: use this FD order (Hartree), no veff
 };
 void launch_stencil(const StencilCall &c, hipStream_t s);
 // partial-sum layout of launch_stencil's dot_partial: nb * nparts
-int stencil_nparts(const Grid &g);
+int stencil_nparts(const Grid &g, int nf, int nb);
+// fused r = sign*(h - eps s), out = K r, optional per-band partials of <r|K r>
+void launch_precond_resid_dot(const Grid &g, const double *h, const double *sp, long ld,
+                              const double *eps, double sign, double *out, int nb,
+                              double *dot_partial, hipStream_t s);
 
 void launch_halo_fill(double *v, int nperi, int na, int nb, int nc, int nf, int nf1, int nsnd,
                       hipStream_t s);
@@ -169,6 +176,12 @@ void launch_coef(const NonlocalPlan &pl, const double *rp, long ld_rp, int nb, c
 // out_b(r) += sum_{entries of r} sum_j coef_b(j, slot) vnl(k, j, slot)
 void launch_expand(const NonlocalPlan &pl, long n, const double *coef, long ld_coef, int nb,
                    double *out, long ld_out, hipStream_t s);
+// Same over the compacted rows, 8 bands per thread; optional per-band partials of
+// <psi_b|added part> written at part[b*part_ld + part_off + block].
+int expand_nparts(const NonlocalPlan &pl);
+void launch_expand_rows(const NonlocalPlan &pl, const double *coef, long ld_coef, int nb,
+                        double *out, long ld_out, const double *psi, long ld_psi, double *part,
+                        long part_ld, int part_off, hipStream_t s);
 
 // batched reductions: res[b] = scale * sum_i x_b[i] * y_b[i] (deterministic two-stage)
 void launch_dots(long n, const double *x, long ldx, const double *y, long ldy, int nb,

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 
+#include "gemm_mixed.hpp"
 #include "kukan_internal.hpp"
 #include "solvers.hpp"
 
@@ -50,6 +51,14 @@ __global__ void k_kscg05(int nb, const double *sval, const double *pap, const do
   }
   alph[b] = a;
   beta[b] = be;
+}
+
+// zero the strictly lower triangle of an M x M column-major matrix
+__global__ void k_zero_lower(int M, double *A) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)M * M) return;
+  const int i = (int)(t % M), j = (int)(t / M);
+  if (i > j) A[t] = 0.0;
 }
 
 // t[b] = reset ? 0 : num[b]/den[b]
@@ -413,10 +422,14 @@ void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, 
   c.g = &g; c.in = psi; c.ld_in = ld; c.out = out; c.ld_out = ld_out; c.veff = veffs; c.nb = nb;
   c.accumulate = accumulate; c.pl = &pl; c.coef = coef; c.ld_coef = n1;
   double *part = nullptr;
-  const int np = stencil_nparts(g);
+  int np = stencil_nparts(g, g.nf, nb);
   if (eps_out) {
-    part = buf("hpsi.part", (size_t)np * nb);
+    const int np_total = np + expand_nparts(pl);  // tiled path appends expansion partials
+    part = buf("hpsi.part", (size_t)np_total * nb);
+    launch_fill((long)np_total * nb, 0.0, part, s);
     c.dot_partial = part;
+    c.dot_ld = np_total;
+    np = np_total;
   }
   {
     // algorithmic bytes: psi in + H psi out per band, plus veff and the projector tables once
@@ -449,6 +462,53 @@ static void dgemm(rocblas_operation ta, rocblas_operation tb, int m, int n, int 
   BLASCK(rocblas_dgemm(lib().blas, ta, tb, m, n, k, &alpha, A, lda, B, ldb, &beta, C, ldc));
 }
 
+// Subspace products: FP64 rocBLAS, or split-FP16 MFMA when param "precision" = 1 and the
+// long dimension reaches param "mixed_min_rows" (default 65536: grid-length operands only;
+// small products such as <p|psi> rotations stay FP64).
+static bool mixed_for(long rows) {
+  return param("precision", 0.0) >= 1.0 && rows > 0 &&
+         rows >= (long)param("mixed_min_rows", 65536.0);
+}
+static int mixed_terms() { return param("mixed_terms", 3.0) >= 4.0 ? 4 : 3; }
+
+void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
+                 const double *B, long ldb, double beta, double *C, long ldc) {
+  hipStream_t s = lib().stream;
+  if (mixed_for(K)) {
+    double *sa = (double *)scratch("tn.sa", sizeof(double) * M1);
+    double *sb = (double *)scratch("tn.sb", sizeof(double) * M2);
+    colscale(A, lda, K, M1, nullptr, sa, s);
+    if (B == A && ldb == lda && M2 == M1) sb = sa;
+    else colscale(B, ldb, K, M2, nullptr, sb, s);
+    gemm_mixed_tn(M1, M2, K, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, mixed_terms(), s);
+    return;
+  }
+  dgemm(rocblas_operation_transpose, rocblas_operation_none, M1, M2, (int)K, alpha, A, (int)lda, B,
+        (int)ldb, beta, C, (int)ldc);
+}
+
+void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx, const double *C,
+                 long ldc, double beta, double *Y, long ldy) {
+  hipStream_t s = lib().stream;
+  if (mixed_for(R)) {
+    double *sx = (double *)scratch("nn.sx", sizeof(double) * K);
+    colscale(X, ldx, R, K, nullptr, sx, s);
+    gemm_mixed_nn(R, K, M2, alpha, X, ldx, sx, C, ldc, beta, Y, ldy, mixed_terms(), s);
+    return;
+  }
+  dgemm(rocblas_operation_none, rocblas_operation_none, (int)R, M2, K, alpha, X, (int)ldx, C,
+        (int)ldc, beta, Y, (int)ldy);
+}
+
+static void sub_tn(Ctx &, int M1, int M2, long K, double alpha, const double *A, long lda,
+                   const double *B, long ldb, double beta, double *C, long ldc) {
+  subspace_tn(M1, M2, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+static void sub_nn(Ctx &, long R, int K, int M2, double alpha, const double *X, long ldx,
+                   const double *C, long ldc, double beta, double *Y, long ldy) {
+  subspace_nn(R, K, M2, alpha, X, ldx, C, ldc, beta, Y, ldy);
+}
+
 // ============================================================================= Cholesky-QR
 // One pass of S-orthonormalisation of bands [0, M): G = dV Psi^T S Psi = R^T R,
 // Psi, S Psi, <p|psi> (and H Psi if given) <- X R^-1.  Same Q as Gram-Schmidt in exact
@@ -457,9 +517,8 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi)
   const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
   rocblas_handle h = lib().blas;
   double *G = c.buf("cholqr.G", (size_t)M * M);
-  const double dv = c.g.dv(), zero = 0.0, one = 1.0;
-  dgemm(rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, dv, psi, (int)n, spsi,
-        (int)n, zero, G, M);
+  const double dv = c.g.dv(), one = 1.0;
+  sub_tn(c, M, M, n, dv, psi, n, spsi, n, 0.0, G, M);
   int *info = (int *)scratch("cholqr.info", sizeof(int));
   {
     Timed t("potrf", (double)M * M * M / 3.0);
@@ -474,6 +533,28 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi)
     BLASCK(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                          rocblas_diagonal_non_unit, (int)rows, M, &one, G, M, X, (int)ldx));
   };
+  if (mixed_for(n)) {
+    // R^-1 in FP64 (upper triangular), then X <- X R^-1 as a split-FP16 product
+    double *Ri = c.buf("cholqr.Ri", (size_t)M * M);
+    launch_fill((long)M * M, 0.0, Ri, c.s);
+    HIPCK(hipMemcpy2DAsync(Ri, sizeof(double) * M, G, sizeof(double) * M, sizeof(double) * M, M,
+                           hipMemcpyDeviceToDevice, c.s));
+    {
+      Timed t("trtri", (double)M * M * M / 3.0);
+      BLASCK(rocsolver_dtrtri(h, rocblas_fill_upper, rocblas_diagonal_non_unit, M, Ri, M, info));
+    }
+    hipLaunchKernelGGL(k_zero_lower, dim3(nblk((long)M * M, 256)), dim3(256), 0, c.s, M, Ri);
+    double *W = c.buf("tmp.nM", (size_t)n * M);
+    auto apply = [&](double *X) {
+      sub_nn(c, n, M, M, 1.0, X, n, Ri, M, 0.0, W, n);
+      HIPCK(hipMemcpyAsync(X, W, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+    };
+    apply(psi);
+    if (spsi != psi) apply(spsi);
+    if (hpsi) apply(hpsi);
+    if (rsp && n1 > 0) tr(rsp, n1, n1);
+    return;
+  }
   tr(psi, n, n);
   if (spsi != psi) tr(spsi, n, n);
   if (rsp && n1 > 0) tr(rsp, n1, n1);
@@ -531,8 +612,7 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   double *A = c.buf("rr.A", (size_t)M * M);
   double *E = c.buf("rr.E", (size_t)M);
   const double dv = c.g.dv(), zero = 0.0, one = 1.0;
-  dgemm(rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, dv, psi, (int)n, hpsi,
-        (int)n, zero, A, M);
+  sub_tn(c, M, M, n, dv, psi, n, hpsi, n, zero, A, M);
   int *info = (int *)scratch("rr.info", sizeof(int));
   {
     Timed t("syevd", 4.0 * (double)M * M * M);
@@ -547,18 +627,15 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   double *W = hpsi;
   double *Wh = nullptr;
   if (keep_h) {  // block path: rotate H Psi too, through a separate buffer
-    Wh = c.buf("rr.Wh", (size_t)n * M);
-    dgemm(rocblas_operation_none, rocblas_operation_none, (int)n, M, M, one, hpsi, (int)n, A, M,
-          zero, Wh, (int)n);
+    Wh = c.buf("tmp.nM", (size_t)n * M);  // shared with Cholesky-QR (never live together)
+    sub_nn(c, n, M, M, one, hpsi, n, A, M, zero, Wh, n);
   }
-  dgemm(rocblas_operation_none, rocblas_operation_none, (int)n, M, M, one, psi, (int)n, A, M, zero,
-        W, (int)n);
+  sub_nn(c, n, M, M, one, psi, n, A, M, zero, W, n);
   HIPCK(hipMemcpyAsync(psi, W, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
   if (c.q_zero && spsi != psi) {
     HIPCK(hipMemcpyAsync(spsi, psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
   } else if (spsi != psi) {
-    dgemm(rocblas_operation_none, rocblas_operation_none, (int)n, M, M, one, spsi, (int)n, A, M,
-          zero, W, (int)n);
+    sub_nn(c, n, M, M, one, spsi, n, A, M, zero, W, n);
     HIPCK(hipMemcpyAsync(spsi, W, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
   }
   if (n1 > 0) {
@@ -711,20 +788,16 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
     const bool doreset = (iiter == 1) || (iiter == 10) || ((iiter - 1) % 200 == 0);
     HIPCK(hipMemcpyAsync(rold, rnew, sizeof(double) * M, hipMemcpyDeviceToDevice, c.s));
     c.dots(a.psi, n, a.hpsi, n, M, a.sval);
-    // K r with r = -(H psi - eps S psi)
-    launch_precond_resid(c.g, a.hpsi, a.spsi, n, a.sval, -1.0, nullptr, KR, M, c.s);
-    // <r|Kr> (unprojected) = residual measure and CG ratio
+    // K r with r = -(H psi - eps S psi), and <r|Kr> (unprojected): residual measure and CG ratio
     {
-      double *R = PA;  // r is rebuilt in HP as scratch: HP is free until H p
-      launch_axpby(n, M, nullptr, -1.0, a.hpsi, n, a.sval, 1.0, a.spsi, n, HP, n, c.s);
-      (void)R;
-      c.dots(HP, n, KR, n, M, rnew);
+      const int np = stencil_nparts(c.g, 1, M);
+      double *part = c.buf("bcg.kpart", (size_t)np * M);
+      launch_precond_resid_dot(c.g, a.hpsi, a.spsi, n, a.sval, -1.0, KR, M, part, c.s);
+      launch_reduce_partials(part, np, M, c.g.dv(), rnew, c.s);
     }
     // Kr orthogonal to the block in the S metric: C = dV (S Psi)^T Kr ; Kr -= Psi C
-    dgemm(rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, dv, a.spsi, (int)n, KR,
-          (int)n, zero, C, M);
-    dgemm(rocblas_operation_none, rocblas_operation_none, (int)n, M, M, mone, a.psi, (int)n, C, M,
-          one, KR, (int)n);
+    sub_tn(c, M, M, n, dv, a.spsi, n, KR, n, zero, C, M);
+    sub_nn(c, n, M, M, mone, a.psi, n, C, M, one, KR, n);
     // P = Kr + beta P
     hipLaunchKernelGGL(k_ratio, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, rnew, rold, doreset ? 1 : 0, tb);
     if (doreset) launch_copy2d(n, M, KR, n, PA, n, c.s);
@@ -882,7 +955,7 @@ int hartree(const HartreeArgs &a) {
   double *ap = (double *)scratch("hart.ap", sizeof(double) * n);
   double *avhs = (double *)scratch("hart.avhs", sizeof(double) * n);
   double *sc = (double *)scratch("hart.sc", sizeof(double) * 16);
-  const int np = stencil_nparts(g);
+  const int np = stencil_nparts(g, a.nfh, 1);
   const int npu = (int)std::min<long>(1024, std::max<long>(1, (n + 8191) / 8192));
   double *part = (double *)scratch("hart.part", sizeof(double) * std::max(np, npu));
   hipLaunchKernelGGL(k_hartree_b, dim3(nblk(n, 256)), dim3(256), 0, s, n, a.rho, b);

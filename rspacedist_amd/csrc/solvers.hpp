@@ -47,6 +47,14 @@ struct HartreeArgs {
   double *vh;
 };
 
+// C(M1 x M2) = alpha A^T B + beta C (A: K x M1, B: K x M2) and Y(R x M2) = alpha X C + beta Y
+// (X: R x K, Y must not alias X), all column-major on the library stream; FP64 or split-FP16
+// per params "precision" / "mixed_min_rows" / "mixed_terms".
+void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
+                 const double *B, long ldb, double beta, double *C, long ldc);
+void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx, const double *C,
+                 long ldc, double beta, double *Y, long ldy);
+
 void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi);
 void gs_one(Ctx &c, int l, double *psi, double *spsi, double *rsp);
 void ortho_r_01(Ctx &c, int M, int l_inp, int nflag, int key_cmpt, double *psi, double *spsi,

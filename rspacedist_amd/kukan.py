@@ -40,6 +40,10 @@ def lib():
         L.kukan_timing_get.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_double)]
         L.kukan_timing_tags.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+        L.kukan_subspace_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long,
+                                          ctypes.c_double, ctypes.c_void_p, ctypes.c_long,
+                                          ctypes.c_void_p, ctypes.c_long, ctypes.c_double,
+                                          ctypes.c_void_p, ctypes.c_long]
         for f in ("kukan_bind",):
             getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_size_t]
         for f in ("kukan_unbind", "kukan_sync_to_host", "kukan_sync_to_device"):
@@ -148,9 +152,17 @@ def synchronize():
     _ck(lib().kukan_device_synchronize(), "kukan_device_synchronize")
 
 
+def subspace_gemm(trans_a, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    """C(m x n) = alpha op(A) B + beta C on column-major arrays (DeviceArray or numpy)."""
+    def p(x, name):
+        return abi._ptr(x, "D", name)
+    _ck(lib().kukan_subspace_gemm(int(trans_a), m, n, k, alpha, p(A, "A"), lda, p(B, "B"), ldb,
+                                  beta, p(C, "C"), ldc), "kukan_subspace_gemm")
+
+
 EXPORTS = ["kukan_init", "kukan_finalize", "kukan_last_error", "kukan_set_param",
            "kukan_get_param", "kukan_stream", "kukan_bind", "kukan_device_malloc", "kukan_device_free",
            "kukan_memcpy", "kukan_fill_normal", "kukan_device_synchronize", "kukan_timing_get",
            "kukan_timing_reset", "kukan_timing_tags", "kukan_unbind", "kukan_sync_to_host",
            "kukan_sync_to_device", "kukan_overlap_fdcheck_r", "kukan_rayleighritz_blacs_ini",
-           "kukan_rayleighritz_blacs_fin"] + ["kukan_" + n for n in abi.SIGNATURES]
+           "kukan_rayleighritz_blacs_fin", "kukan_subspace_gemm"] + ["kukan_" + n for n in abi.SIGNATURES]

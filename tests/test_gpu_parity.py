@@ -353,3 +353,34 @@ def test_error_codes_carry_reference_messages(kk):
         kk.call("scf_hartree", nd=1, ncpx_d=8, ncpy_d=8, ncpz_d=8, nfh=3, ndisp=6, nperi=3,
                 ncgres=801, ncgmin=1, ncgmax=10, xmax=1.0, ymax=1.0, zmax=1.0, epsvh=1e-10,
                 rho_aug_dense=np.zeros(512), boundx=z, boundy=z, boundz=z, vh_dense=np.zeros(512))
+
+
+@pytest.mark.parametrize("nperi", [3, 0])
+def test_hpsi_tiled_path_ragged_grid(kk, orc, nperi):
+    """H psi through the tiled stencil on a non-cubic grid whose y extent is not a multiple
+    of the tile height (200 x 23 x 9), with arbitrary projector lists, via rayleighritz_r."""
+    nx, ny, nz, m, natom = 200, 23, 9, 5, 3
+    rng = np.random.default_rng(40)
+    n = nx * ny * nz
+    num_list = 60
+    lst = np.zeros((num_list, natom), dtype=np.int32, order="F")
+    for a in range(natom):
+        lst[:, a] = rng.choice(n, size=num_list, replace=False) + 1
+    kw = dict(natom=natom, num_spe=1, num_ppcell=natom, nprjmx=9, neigmx=m, numk=1, nums=1, nspv=1,
+              num_list=num_list, ncpx=nx, ncpy=ny, ncpz=nz, nperi=nperi, nk=1, ns=1, nf=4, ndisp=6,
+              key_natpri_in=2, key_natpri_inps=1, dx=0.3, dy=0.25, dz=0.35, l_rrb=1,
+              nprj=np.array([9], dtype=np.int32), natinf=np.full(natom, num_list, dtype=np.int32),
+              lstvec2=lst, indspe=np.ones(natom, dtype=np.int32),
+              natpri=np.full(natom, 2, dtype=np.int32), naps=np.arange(1, natom + 1, dtype=np.int32),
+              ntyppp=np.array([1], dtype=np.int32),
+              dij=np.asfortranarray(rng.uniform(-0.5, 0.5, (9, 9, 1, natom))),
+              veff=np.asfortranarray(rng.standard_normal((nx, ny, nz, 1))),
+              vnlocp=np.asfortranarray(rng.standard_normal((num_list, 9, natom)) * 0.01))
+    psi = np.asfortranarray(rng.standard_normal((n, m)))
+    psi, _ = np.linalg.qr(psi)
+    psi = np.asfortranarray(psi / math.sqrt(0.3 * 0.25 * 0.35))
+    arr = {"svecre": psi, "ssvre": psi.copy(order="F"), "hsvre": np.zeros_like(psi, order="F"),
+           "rspsep": np.asfortranarray(rng.standard_normal((9, natom, m))), "sval": np.zeros(m)}
+    g, o = both(kk, orc, "rayleighritz_r", arr, **kw)
+    np.testing.assert_allclose(g["sval"], o["sval"], rtol=1e-11, atol=1e-10)
+    np.testing.assert_allclose(g["svecre"], o["svecre"], atol=1e-8 * np.abs(o["svecre"]).max())

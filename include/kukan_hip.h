@@ -247,6 +247,31 @@ int kukan_tools_correctpotential(int ncpx_d, int ncpy_d, int ncpz_d, double xmax
 int kukan_scf_charge_smt_r(int neigmx, int nums, int nspv, int numk, int ncpx, int ncpy, int ncpz,
                            const double *fnele, const double *svecre, double *rhosmt);
 
+/* Differentiation pattern (no reference counterpart; used by the torch.autograd harness in
+ * rspacedist_amd/autograd.py).  kukan_band_energy_r: eband[l] = dV <psi_l|H|psi_l> with
+ * H = T + veff + V_nl(dij) for the real path (nums = numk = 1), the same operator kscg_r
+ * applies (kscg8f.F90:114-153).  kukan_energy_adjoint_r: adjoint of E = sum_l fnele_l
+ * eband_l with psi held fixed: dedv(n) = dE/dveff = dV sum_l f_l psi_l^2, and
+ * dedr(natom, 3) (column d = x, y, z) = dE/dR through the projectors, from the derivative
+ * tables dvnlocp(num_list, nprjmx, num_ppcell, 3) = d vnlocp / d R_x,y,z on the lists of
+ * lstvec2.  The local-potential part of dE/dR is dedv contracted with d veff / d R, which
+ * the caller's chain rule supplies. */
+int kukan_band_energy_r(int nperi, int neigmx, int nf, int ncpx, int ncpy, int ncpz, int num_list,
+                        int natom, int num_spe, int num_ppcell, int nprjmx, int key_natpri_in,
+                        int key_natpri_inps, const int *lstvec2, const int *nprj,
+                        const int *natpri, const int *naps, const int *indspe, const int *natinf,
+                        double xmax, double ymax, double zmax, const double *vnlocp,
+                        const double *dij, const double *veff, const double *svecre,
+                        double *eband);
+int kukan_energy_adjoint_r(int nperi, int neigmx, int nf, int ncpx, int ncpy, int ncpz,
+                           int num_list, int natom, int num_spe, int num_ppcell, int nprjmx,
+                           int key_natpri_in, int key_natpri_inps, const int *lstvec2,
+                           const int *nprj, const int *natpri, const int *naps,
+                           const int *indspe, const int *natinf, double xmax, double ymax,
+                           double zmax, const double *vnlocp, const double *dvnlocp,
+                           const double *dij, const double *svecre, const double *fnele,
+                           double *dedv, double *dedr);
+
 /* force8f.F90:9-195 force_eig_r: nonlocal band forces
  * F_a = -2 sum_n f_n sum_ij <p_i|psi_n> <p_j|d psi_n> (D_ij - eps_n S_ij), d = FD order nf. */
 int kukan_force_eig_r(int nperi, int numk, int nums, int neigmx, int nf, int ncpx, int ncpy,

@@ -1391,6 +1391,96 @@ int orc_force_eig_r(int nperi, int numk, int nums, int neigmx, int nf, int ncpx,
   return 0;
 }
 
+/* ------------------------------------------------------------------ differentiation pattern
+ * No reference counterpart.  e_l = dV <psi_l|H|psi_l> with the operator kscg_r applies
+ * (kscg8f.F90:114-153: nonlocal r_02 with dij, then kslaplacian_r with veff), and the adjoint
+ * of E = sum_l f_l e_l at fixed psi: dE/dveff = dV sum_l f_l psi_l^2, and the projector part
+ * of dE/dR_a, sum_l f_l sum_ij (dq_i D_ij q_j + q_i D_ij dq_j) with q = <p|psi> from vnlocp
+ * and dq = the same gather with the derivative tables dvnlocp(:,:,:,d). */
+static void adj_ctx(ctx_t *c, int nperi, int nf, int ncpx, int ncpy, int ncpz, int num_list,
+                    int natom, int num_spe, int num_ppcell, int nprjmx, int key_in, int key_inps,
+                    const int *lstvec2, const int *nprj, const int *natpri, const int *naps,
+                    const int *indspe, const int *natinf, double xmax, double ymax, double zmax,
+                    const double *vnlocp, const double *dij, const double *veff) {
+  memset(c, 0, sizeof *c);
+  c->ncpx = ncpx; c->ncpy = ncpy; c->ncpz = ncpz; c->nf = nf; c->nperi = nperi;
+  c->dx = 2.0 * xmax / ncpx; c->dy = 2.0 * ymax / ncpy; c->dz = 2.0 * zmax / ncpz;
+  c->natom = natom; c->num_spe = num_spe; c->nums = 1; c->nprjmx = nprjmx;
+  c->num_list = num_list; c->num_ppcell = num_ppcell;
+  c->key_in = key_in; c->key_inps = key_inps;
+  c->nprj = nprj; c->indspe = indspe; c->natinf = natinf; c->lstvec2 = lstvec2;
+  c->natpri = natpri; c->naps = naps; c->vnlocp = vnlocp; c->dij = dij; c->veff = veff;
+  ctx_alloc(c);
+}
+
+int orc_band_energy_r(int nperi, int neigmx, int nf, int ncpx, int ncpy, int ncpz, int num_list,
+                      int natom, int num_spe, int num_ppcell, int nprjmx, int key_natpri_in,
+                      int key_natpri_inps, const int *lstvec2, const int *nprj, const int *natpri,
+                      const int *naps, const int *indspe, const int *natinf, double xmax,
+                      double ymax, double zmax, const double *vnlocp, const double *dij,
+                      const double *veff, const double *svecre, double *eband) {
+  ctx_t c;
+  adj_ctx(&c, nperi, nf, ncpx, ncpy, ncpz, num_list, natom, num_spe, num_ppcell, nprjmx,
+          key_natpri_in, key_natpri_inps, lstvec2, nprj, natpri, naps, indspe, natinf, xmax, ymax,
+          zmax, vnlocp, dij, veff);
+  double *rp = (double *)xcalloc(c.n1 > 0 ? c.n1 : 1, sizeof(double));
+  double *hp = (double *)xcalloc(c.n, sizeof(double));
+  int rc = 0;
+  for (int l = 0; l < neigmx && !rc; l++) {
+    const double *psi = svecre + (size_t)c.n * l;
+    proj(&c, psi, rp);
+    rc = apply_h(&c, 1, 1, psi, rp, hp);
+    eband[l] = dot(c.n, psi, hp) * c.dv;
+  }
+  free(rp); free(hp); ctx_free(&c);
+  return rc;
+}
+
+int orc_energy_adjoint_r(int nperi, int neigmx, int nf, int ncpx, int ncpy, int ncpz,
+                         int num_list, int natom, int num_spe, int num_ppcell, int nprjmx,
+                         int key_natpri_in, int key_natpri_inps, const int *lstvec2,
+                         const int *nprj, const int *natpri, const int *naps, const int *indspe,
+                         const int *natinf, double xmax, double ymax, double zmax,
+                         const double *vnlocp, const double *dvnlocp, const double *dij,
+                         const double *svecre, const double *fnele, double *dedv, double *dedr) {
+  ctx_t c;
+  adj_ctx(&c, nperi, nf, ncpx, ncpy, ncpz, num_list, natom, num_spe, num_ppcell, nprjmx,
+          key_natpri_in, key_natpri_inps, lstvec2, nprj, natpri, naps, indspe, natinf, xmax, ymax,
+          zmax, vnlocp, dij, NULL);
+  const long n = c.n, n1 = c.n1, tab = (long)num_list * nprjmx * num_ppcell;
+  for (long i = 0; i < n; i++) dedv[i] = 0.0;
+  for (int l = 0; l < neigmx; l++) {
+    const double *psi = svecre + (size_t)n * l;
+    for (long i = 0; i < n; i++) dedv[i] += fnele[l] * psi[i] * psi[i];
+  }
+  for (long i = 0; i < n; i++) dedv[i] *= c.dv;
+  for (int k = 0; k < 3 * natom; k++) dedr[k] = 0.0;
+  double *q = (double *)xcalloc(n1 > 0 ? n1 : 1, sizeof(double));
+  double *dq = (double *)xcalloc(n1 > 0 ? n1 : 1, sizeof(double));
+  for (int d = 0; d < 3; d++) {
+    ctx_t cd = c;
+    cd.vnlocp = dvnlocp + (size_t)d * tab;
+    for (int l = 0; l < neigmx; l++) {
+      const double *psi = svecre + (size_t)n * l;
+      proj(&c, psi, q);
+      proj(&cd, psi, dq);
+      for (int na = 1; na <= natom; na++) {
+        if (natpri[na - 1] != key_natpri_in) continue;
+        const int iaps = naps[na - 1], ip = nprj[indspe[na - 1] - 1];
+        const double *a = q + (size_t)nprjmx * (iaps - 1), *b = dq + (size_t)nprjmx * (iaps - 1);
+        const double *D = dij + (size_t)nprjmx * nprjmx * (na - 1);
+        double s = 0.0;
+        for (int j = 0; j < ip; j++)
+          for (int i = 0; i < ip; i++)
+            s += (b[i] * a[j] + a[i] * b[j]) * D[i + (size_t)nprjmx * j];
+        dedr[(size_t)d * natom + na - 1] += fnele[l] * s;
+      }
+    }
+  }
+  free(q); free(dq); ctx_free(&c);
+  return 0;
+}
+
 /* ------------------------------------------------------------------ totalenergy8f.F90:151-199 */
 int orc_totalenergy_eig(int nums, int ncol, int numk, int neigmx, const int *nwskk, int nwskptot,
                         double tf, const double *sval, const double *fnele, double *eneel,

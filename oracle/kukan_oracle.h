@@ -128,6 +128,19 @@ int orc_force_eig_r(int nperi, int numk, int nums, int neigmx, int nf, int ncpx,
                     const double *vnlocp, const double *dij, const double *svecre,
                     const double *sval, const double *fnele, double *fatx, double *faty,
                     double *fatz);
+int orc_band_energy_r(int nperi, int neigmx, int nf, int ncpx, int ncpy, int ncpz, int num_list,
+                      int natom, int num_spe, int num_ppcell, int nprjmx, int key_natpri_in,
+                      int key_natpri_inps, const int *lstvec2, const int *nprj, const int *natpri,
+                      const int *naps, const int *indspe, const int *natinf, double xmax,
+                      double ymax, double zmax, const double *vnlocp, const double *dij,
+                      const double *veff, const double *svecre, double *eband);
+int orc_energy_adjoint_r(int nperi, int neigmx, int nf, int ncpx, int ncpy, int ncpz,
+                         int num_list, int natom, int num_spe, int num_ppcell, int nprjmx,
+                         int key_natpri_in, int key_natpri_inps, const int *lstvec2,
+                         const int *nprj, const int *natpri, const int *naps, const int *indspe,
+                         const int *natinf, double xmax, double ymax, double zmax,
+                         const double *vnlocp, const double *dvnlocp, const double *dij,
+                         const double *svecre, const double *fnele, double *dedv, double *dedr);
 int orc_totalenergy_eig(int nums, int ncol, int numk, int neigmx, const int *nwskk, int nwskptot,
                         double tf, const double *sval, const double *fnele, double *eneel,
                         double *eneth);

@@ -141,6 +141,21 @@ SIGNATURES = {
                     ("xmax", "d"), ("ymax", "d"), ("zmax", "d"), ("sss", "D"), ("vnlocp", "D"),
                     ("dij", "D"), ("svecre", "D"), ("sval", "D"), ("fnele", "D"), ("fatx", "D"),
                     ("faty", "D"), ("fatz", "D")],
+    "band_energy_r": [("nperi", "i"), ("neigmx", "i"), ("nf", "i"), ("ncpx", "i"), ("ncpy", "i"),
+                      ("ncpz", "i"), ("num_list", "i"), ("natom", "i"), ("num_spe", "i"),
+                      ("num_ppcell", "i"), ("nprjmx", "i"), ("key_natpri_in", "i"),
+                      ("key_natpri_inps", "i"), ("lstvec2", "I"), ("nprj", "I"), ("natpri", "I"),
+                      ("naps", "I"), ("indspe", "I"), ("natinf", "I"), ("xmax", "d"),
+                      ("ymax", "d"), ("zmax", "d"), ("vnlocp", "D"), ("dij", "D"), ("veff", "D"),
+                      ("svecre", "D"), ("eband", "D")],
+    "energy_adjoint_r": [("nperi", "i"), ("neigmx", "i"), ("nf", "i"), ("ncpx", "i"),
+                         ("ncpy", "i"), ("ncpz", "i"), ("num_list", "i"), ("natom", "i"),
+                         ("num_spe", "i"), ("num_ppcell", "i"), ("nprjmx", "i"),
+                         ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("lstvec2", "I"),
+                         ("nprj", "I"), ("natpri", "I"), ("naps", "I"), ("indspe", "I"),
+                         ("natinf", "I"), ("xmax", "d"), ("ymax", "d"), ("zmax", "d"),
+                         ("vnlocp", "D"), ("dvnlocp", "D"), ("dij", "D"), ("svecre", "D"),
+                         ("fnele", "D"), ("dedv", "D"), ("dedr", "D")],
     "totalenergy_eig": [("nums", "i"), ("ncol", "i"), ("numk", "i"), ("neigmx", "i"),
                         ("nwskk", "I"), ("nwskptot", "i"), ("tf", "d"), ("sval", "D"),
                         ("fnele", "D"), ("eneel", "od"), ("eneth", "od")],

@@ -700,6 +700,62 @@ int kukan_scf_charge_smt_r(int neigmx, int nums, int nspv, int numk, int ncpx, i
   API_END
 }
 
+int kukan_band_energy_r(int nperi, int neigmx, int nf, int ncpx, int ncpy, int ncpz, int num_list,
+                        int natom, int num_spe, int num_ppcell, int nprjmx, int key_natpri_in,
+                        int key_natpri_inps, const int *lstvec2, const int *nprj,
+                        const int *natpri, const int *naps, const int *indspe, const int *natinf,
+                        double xmax, double ymax, double zmax, const double *vnlocp,
+                        const double *dij, const double *veff, const double *svecre,
+                        double *eband) {
+  API_BEGIN
+  Grid g{ncpx, ncpy, ncpz, nf, nperi, 2.0 * xmax / ncpx, 2.0 * ymax / ncpy, 2.0 * zmax / ncpz};
+  Ctx c(g);
+  const long n = g.n();
+  Arr vnl(vnlocp, (size_t)num_list * nprjmx * num_ppcell * D, Arr::In);
+  Arr lst(lstvec2, (size_t)num_list * num_ppcell * I, Arr::In);
+  Arr Dij(dij, (size_t)nprjmx * nprjmx * natom * D, Arr::In);
+  Arr Ve(veff, (size_t)n * D, Arr::In);
+  Arr Ps(svecre, (size_t)n * neigmx * D, Arr::In);
+  Arr E(eband, (size_t)neigmx * D, Arr::Out);
+  setup(c, natom, nprjmx, num_spe, num_ppcell, num_list, 1, key_natpri_in, key_natpri_inps, nprj,
+        indspe, natinf, natpri, naps, vnl, lst, Dij.d(), nullptr, nullptr);
+  band_energy(c, neigmx, Ve.d(), Ps.d(), E.d());
+  E.finish();
+  API_END
+}
+
+int kukan_energy_adjoint_r(int nperi, int neigmx, int nf, int ncpx, int ncpy, int ncpz,
+                           int num_list, int natom, int num_spe, int num_ppcell, int nprjmx,
+                           int key_natpri_in, int key_natpri_inps, const int *lstvec2,
+                           const int *nprj, const int *natpri, const int *naps,
+                           const int *indspe, const int *natinf, double xmax, double ymax,
+                           double zmax, const double *vnlocp, const double *dvnlocp,
+                           const double *dij, const double *svecre, const double *fnele,
+                           double *dedv, double *dedr) {
+  API_BEGIN
+  Grid g{ncpx, ncpy, ncpz, nf, nperi, 2.0 * xmax / ncpx, 2.0 * ymax / ncpy, 2.0 * zmax / ncpz};
+  Ctx c(g);
+  const long n = g.n();
+  const size_t tab = (size_t)num_list * nprjmx * num_ppcell;
+  Arr vnl(vnlocp, tab * D, Arr::In);
+  Arr dvnl(dvnlocp, 3 * tab * D, Arr::In);
+  Arr lst(lstvec2, (size_t)num_list * num_ppcell * I, Arr::In);
+  Arr Dij(dij, (size_t)nprjmx * nprjmx * natom * D, Arr::In);
+  Arr Ps(svecre, (size_t)n * neigmx * D, Arr::In);
+  Arr F(fnele, (size_t)neigmx * D, Arr::In);
+  Arr Dv(dedv, (size_t)n * D, Arr::Out);
+  Arr Dr(dedr, (size_t)3 * natom * D, Arr::Out);
+  setup(c, natom, nprjmx, num_spe, num_ppcell, num_list, 1, key_natpri_in, key_natpri_inps, nprj,
+        indspe, natinf, natpri, naps, vnl, lst, Dij.d(), nullptr, nullptr);
+  std::vector<int> pri = host_ints(natpri, natom), isin(std::max(1, c.pl.nact));
+  for (int a = 0; a < c.pl.nact; a++) isin[a] = pri[c.pl.act_h[5 * a + 3]] == key_natpri_in;
+  int *isin_d = (int *)scratch("adj.isin", sizeof(int) * isin.size());
+  HIPCK(hipMemcpyAsync(isin_d, isin.data(), sizeof(int) * isin.size(), hipMemcpyHostToDevice, c.s));
+  energy_adjoint(c, neigmx, Ps.d(), F.d(), dvnl.d(), isin_d, Dv.d(), Dr.d());
+  Dv.finish(); Dr.finish();
+  API_END
+}
+
 int kukan_force_eig_r(int nperi, int numk, int nums, int neigmx, int nf, int ncpx, int ncpy,
                       int ncpz, int num_list, int natom, int num_spe, int num_ppcell, int nprjmx,
                       int key_natpri_in, int key_natpri_inps, const int *lstvec2,

@@ -26,8 +26,6 @@ namespace {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 
-constexpr double TARGET = 16384.0;  // 2^14
-
 inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 // f64 -> f16 through FP32 with hardware conversions.  The empty asm keeps the compiler from
@@ -40,153 +38,199 @@ __device__ inline _Float16 to_half(double x) {
   return (_Float16)f;
 }
 
-__device__ inline void split8(const double *v, double s, half8 &hi, half8 &lo) {
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const double x = v[j] * s;
-    const _Float16 h = to_half(x);
-    hi[j] = h;
-    lo[j] = to_half(x - (double)h);
-  }
-}
 
-// per-column power-of-two scale: s_j = 2^(14 - ceil(log2 max|col_j|)) (1 for a zero column)
-__global__ __launch_bounds__(256) void k_colscale(const double *A, long lda, long rows, int cols,
-                                                  const double *pre, double *scale) {
+// Per-column power-of-two scale s_j = 2^(14 - e_j) with max_i |A(i,j)| < 2^e_j (1 for a zero
+// column).  Pass 1: grid (column, row chunk), each work-group reduces its chunk and folds it
+// into the column maximum with a 64-bit atomic max on the bit pattern (non-negative doubles
+// order like their bit patterns).  Pass 2 turns maxima into scales.
+constexpr long CS_ROWS = 16384;  // rows per work-group
+__global__ __launch_bounds__(256) void k_colmax(const double *A, long lda, long rows,
+                                                unsigned long long *cmax) {
   __shared__ double red[256];
-  const int j = blockIdx.x;
-  if (j >= cols) return;
+  const int j = blockIdx.y;
+  const long r0 = (long)blockIdx.x * CS_ROWS, r1 = min(rows, r0 + CS_ROWS);
   const double *c = A + (long)j * lda;
-  const double f = pre ? fabs(pre[j]) : 1.0;
   double m = 0.0;
-  for (long i = threadIdx.x; i < rows; i += 256) m = fmax(m, fabs(c[i]));
-  red[threadIdx.x] = m * f;
+  for (long i = r0 + threadIdx.x; i < r1; i += 256) m = fmax(m, fabs(c[i]));
+  red[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    const double mx = red[0];
-    double sc = 1.0;
-    if (mx > 0.0) {
-      int e;
-      frexp(mx, &e);  // mx < 2^e
-      sc = ldexp(1.0, 14 - e);
+  if (threadIdx.x == 0 && red[0] > 0.0)
+    atomicMax(cmax + j, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+__global__ void k_maxscale(int cols, const unsigned long long *cmax, double *scale) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  const double mx = __longlong_as_double((long long)cmax[j]);
+  double sc = 1.0;
+  if (mx > 0.0) {
+    int e;
+    frexp(mx, &e);  // mx < 2^e
+    sc = ldexp(1.0, 14 - e);
+  }
+  scale[j] = sc;
+}
+
+// ----------------------------------------------------------------------------- tiles
+// Both products run on a 256 x 128 block tile, 8 waves of 64 x 64 (wave w: rows 64 (w & 3),
+// columns 64 (w >> 2)), 32-deep K stages double-buffered in LDS as split halves
+// [row][k] with a 40-half pitch (conflict-free 16-byte reads for 8-lane groups).  The next
+// stage's global loads are issued before the current stage's MFMAs, so one barrier per
+// stage suffices.
+constexpr int BM = 256, BN = 128, BK = 32, PITCH = 40;
+
+struct Stage {
+  _Float16 ah[BM * PITCH], al[BM * PITCH], bh[BN * PITCH], bl[BN * PITCH];
+};
+
+// FP32 split of a block of 8 scaled values: hi = f16(x), lo = f16(x - hi) with x rounded
+// to FP32 first (its 2^-24 rounding is below the split's 2^-22).
+__device__ inline void split8f(const double *v, double s, half8 &hi, half8 &lo) {
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    float f = (float)(v[j] * s);
+    asm volatile("" : "+v"(f));
+    const _Float16 h = (_Float16)f;
+    hi[j] = h;
+    lo[j] = (_Float16)(f - (float)h);
+  }
+}
+
+template <int NT>
+__device__ inline void mma_stage(const Stage &S, int lane, int wi, int wj, f16v (&acc)[2][2]) {
+#pragma unroll
+  for (int s = 0; s < 2; s++) {
+    const int ko = 16 * s + 8 * (lane >> 5);
+    half8 ah[2], al[2];
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      const int row = 64 * wi + 32 * a + (lane & 31);
+      ah[a] = *(const half8 *)&S.ah[row * PITCH + ko];
+      al[a] = *(const half8 *)&S.al[row * PITCH + ko];
     }
-    scale[j] = sc;
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int col = 64 * wj + 32 * b + (lane & 31);
+      const half8 bh = *(const half8 *)&S.bh[col * PITCH + ko];
+      const half8 bl = *(const half8 *)&S.bl[col * PITCH + ko];
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bh, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bl, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bh, acc[a][b], 0, 0, 0);
+        if (NT == 4)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bl, acc[a][b], 0, 0, 0);
+      }
+    }
   }
 }
 
 // ----------------------------------------------------------------------------- TN
-constexpr int TN_T = 128;    // tile edge
-constexpr int TN_KS = 32;    // K per LDS stage
-constexpr int TN_PITCH = 40; // halves per LDS row (32 + 8 pad)
-constexpr int TN_FLUSH = 8;  // stages per FP32 chunk (256 K)
-
+// Split-K: work-group (split q, tile) accumulates K-range [q kper, (q+1) kper) in FP32
+// (kper <= 4096: <= 768 MFMA accumulations) and writes an FP32 partial tile; k_tn_reduce sums
+// the partials in FP64 in a fixed order.  The tiles of one K-range are dispatched
+// consecutively on one XCD (blockIdx % 8), so A and B stages are fetched from HBM once and
+// re-read from that XCD's L2 by the other tiles.
 struct TNArgs {
   int M1, M2;
   long K;
   const double *A, *B;
   long lda, ldb;
   const double *sa, *sb;
-  double *part;  // [split][M1*M2] column-major M1 x M2
-  long kper;     // K per split (multiple of TN_KS)
+  float *part;  // [split][M1][M2], M2 fastest
+  long kper;    // multiple of BK
   int nsplit, tiles_i, tiles_j, nsplit_pad;
 };
 
 template <int NT>
-__global__ __launch_bounds__(256) void k_gemm_tn(TNArgs p) {
-  __shared__ _Float16 sAh[TN_T * TN_PITCH], sAl[TN_T * TN_PITCH];
-  __shared__ _Float16 sBh[TN_T * TN_PITCH], sBl[TN_T * TN_PITCH];
+__global__ __launch_bounds__(512) void k_gemm_tn(TNArgs p) {
+  __shared__ Stage S[2];
   const int L = blockIdx.x, xcd = L & 7, jj = L >> 3;
   const int tiles = p.tiles_i * p.tiles_j;
   const int split = (jj / tiles) * 8 + xcd;
   if (split >= p.nsplit) return;
   const int tile = jj % tiles;
-  const int ti = tile % p.tiles_i, tj = tile / p.tiles_i;
-  const int i0 = ti * TN_T, j0 = tj * TN_T;
+  const int i0 = (tile % p.tiles_i) * BM, j0 = (tile / p.tiles_i) * BN;
   const long kbeg = (long)split * p.kper, kend = min(p.K, kbeg + p.kper);
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wi = w & 1, wj = w >> 1;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
+  // staging: A units (column t/4 + 128 v, k-group t%4), B unit (column t/4, k-group t%4)
+  // Staging map: thread t holds the k pair 2 (t & 15), 2 (t & 15) + 1 of columns (t >> 4) + 32 v
+  // (A: v < 8, B: v < 4), loaded as one 16-byte access, so 16 lanes read one column's 256-byte
+  // stage segment and a wave instruction covers 4 whole segments.  gload only issues loads
+  // (columns clamped into the operand, so no load is guarded); masking, scaling and
+  // splitting happen in lstore, after the MFMAs of the current stage.
+  const int kp = 2 * (t & 15), cg = t >> 4;
+  const double *colA[8], *colB[4];
+  double sav[8], sbv[4];
+#pragma unroll
+  for (int v = 0; v < 8; v++) {
+    const int ia = i0 + cg + 32 * v;
+    sav[v] = ia < p.M1 ? p.sa[ia] : 0.0;
+    colA[v] = p.A + (long)min(ia, p.M1 - 1) * p.lda;
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int jb = j0 + cg + 32 * v;
+    sbv[v] = jb < p.M2 ? p.sb[jb] : 0.0;
+    colB[v] = p.B + (long)min(jb, p.M2 - 1) * p.ldb;
+  }
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  d2 ra[8], rb[4];
+  long k0_cur = 0;
+  auto gload = [&](long k0) {
+    k0_cur = k0;
+    const long kk = k0 + kp;
+    if (k0 + BK <= kend) {  // whole stage inside the K-range (uniform across the block)
+#pragma unroll
+      for (int v = 0; v < 8; v++) ra[v] = *(const d2 *)(colA[v] + kk);
+#pragma unroll
+      for (int v = 0; v < 4; v++) rb[v] = *(const d2 *)(colB[v] + kk);
+    } else {
+      const long k1 = min(kk + 1, kend - 1), kc = min(kk, kend - 1);
+#pragma unroll
+      for (int v = 0; v < 8; v++) ra[v] = d2{colA[v][kc], colA[v][k1]};
+#pragma unroll
+      for (int v = 0; v < 4; v++) rb[v] = d2{colB[v][kc], colB[v][k1]};
+    }
+  };
+  typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+  auto put2 = [&](_Float16 *hi, _Float16 *lo, int row, d2 x, double sc, bool ok0, bool ok1) {
+    float f0 = ok0 ? (float)(x[0] * sc) : 0.f, f1 = ok1 ? (float)(x[1] * sc) : 0.f;
+    asm volatile("" : "+v"(f0), "+v"(f1));
+    const _Float16 h0 = (_Float16)f0, h1 = (_Float16)f1;
+    *(half2v *)&hi[row * PITCH + kp] = half2v{h0, h1};
+    *(half2v *)&lo[row * PITCH + kp] = half2v{(_Float16)(f0 - (float)h0), (_Float16)(f1 - (float)h1)};
+  };
+  auto lstore = [&](Stage &D) {
+    const bool ok0 = k0_cur + kp < kend, ok1 = k0_cur + kp + 1 < kend;
+#pragma unroll
+    for (int v = 0; v < 8; v++) put2(D.ah, D.al, cg + 32 * v, ra[v], sav[v], ok0, ok1);
+#pragma unroll
+    for (int v = 0; v < 4; v++) put2(D.bh, D.bl, cg + 32 * v, rb[v], sbv[v], ok0, ok1);
+  };
   f16v acc[2][2];
-  double acc64[2][2][16];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < 2; b++)
 #pragma unroll
-      for (int r = 0; r < 16; r++) {
-        acc[a][b][r] = 0.f;
-        acc64[a][b][r] = 0.0;
-      }
-    }
-  int stage = 0;
-  for (long k0 = kbeg; k0 < kend; k0 += TN_KS, stage++) {
-    // global -> LDS: 512 units (128 columns x 4 k-groups of 8) per matrix
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int unit = t + 256 * u;
-      const int col = unit >> 2, kg = unit & 3;
-      const long kk = k0 + 8 * kg;
-      double va[8], vb[8];
-      const int ia = i0 + col, jb = j0 + col;
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const bool kin = kk + q < kend;
-        va[q] = (kin && ia < p.M1) ? p.A[(long)ia * p.lda + kk + q] : 0.0;
-        vb[q] = (kin && jb < p.M2) ? p.B[(long)jb * p.ldb + kk + q] : 0.0;
-      }
-      half8 h, l;
-      split8(va, ia < p.M1 ? p.sa[ia] : 1.0, h, l);
-      *(half8 *)&sAh[col * TN_PITCH + 8 * kg] = h;
-      *(half8 *)&sAl[col * TN_PITCH + 8 * kg] = l;
-      split8(vb, jb < p.M2 ? p.sb[jb] : 1.0, h, l);
-      *(half8 *)&sBh[col * TN_PITCH + 8 * kg] = h;
-      *(half8 *)&sBl[col * TN_PITCH + 8 * kg] = l;
-    }
+      for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
+  const int nst = (int)((kend - kbeg + BK - 1) / BK);
+  gload(kbeg);
+  lstore(S[0]);
+  __syncthreads();
+  for (int st = 0; st < nst; st++) {
+    const bool more = st + 1 < nst;
+    if (more) gload(kbeg + (long)(st + 1) * BK);
+    mma_stage<NT>(S[st & 1], lane, wi, wj, acc);
+    if (more) lstore(S[(st + 1) & 1]);
     __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 2; s++) {
-      half8 ah[2], al[2], bh[2], bl[2];
-      const int ko = 16 * s + 8 * (lane >> 5);
-#pragma unroll
-      for (int a = 0; a < 2; a++) {
-        const int row = 64 * wi + 32 * a + (lane & 31);
-        ah[a] = *(const half8 *)&sAh[row * TN_PITCH + ko];
-        al[a] = *(const half8 *)&sAl[row * TN_PITCH + ko];
-      }
-#pragma unroll
-      for (int b = 0; b < 2; b++) {
-        const int col = 64 * wj + 32 * b + (lane & 31);
-        bh[b] = *(const half8 *)&sBh[col * TN_PITCH + ko];
-        bl[b] = *(const half8 *)&sBl[col * TN_PITCH + ko];
-      }
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
-          if (NT == 4)
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bl[b], acc[a][b], 0, 0, 0);
-        }
-    }
-    __syncthreads();
-    if ((stage + 1) % TN_FLUSH == 0) {
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++)
-#pragma unroll
-          for (int r = 0; r < 16; r++) {
-            acc64[a][b][r] += (double)acc[a][b][r];
-            acc[a][b][r] = 0.f;
-          }
-    }
   }
-  double *out = p.part + (long)split * p.M1 * p.M2;
+  float *out = p.part + (long)split * p.M1 * p.M2;
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -195,28 +239,28 @@ __global__ __launch_bounds__(256) void k_gemm_tn(TNArgs p) {
       for (int r = 0; r < 16; r++) {
         const int m = i0 + 64 * wi + 32 * a + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         const int n = j0 + 64 * wj + 32 * b + (lane & 31);
-        if (m < p.M1 && n < p.M2) out[m + (long)p.M1 * n] = acc64[a][b][r] + (double)acc[a][b][r];
+        if (m < p.M1 && n < p.M2) out[(long)m * p.M2 + n] = acc[a][b][r];
       }
 }
 
-__global__ void k_tn_reduce(const double *part, int nsplit, int M1, int M2, double alpha,
+__global__ void k_tn_reduce(const float *part, int nsplit, int M1, int M2, double alpha,
                             const double *sa, const double *sb, double beta, double *C, long ldc) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= (long)M1 * M2) return;
-  const int i = (int)(t % M1), j = (int)(t / M1);
+  const int i = (int)(t / M2), j = (int)(t % M2);
   double s = 0.0;
-  for (int q = 0; q < nsplit; q++) s += part[(long)q * M1 * M2 + t];
+  for (int q = 0; q < nsplit; q++) s += (double)part[(long)q * M1 * M2 + t];
   const double v = alpha * s / (sa[i] * sb[j]);
   double *c = C + i + ldc * (long)j;
   *c = beta == 0.0 ? v : v + beta * *c;
 }
 
 // ----------------------------------------------------------------------------- NN
-// pre-split C'' (hi, lo) in [j][k] layout, K padded to a multiple of 16, M2 padded to 32*...
+// C'' = C / sx * tc split into hi/lo halves in [j][Kp] layout (Kp = K rounded up to BK,
+// columns padded to a multiple of BN with zeros by the caller).
 __global__ void k_nn_prep(const double *C, long ldc, int K, int M2, const double *sx,
-                          const double *tc, int Kp, _Float16 *ch, _Float16 *cl) {
+                          const double *tc, int Kp, int M2p, _Float16 *ch, _Float16 *cl) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const int M2p = (M2 + 127) / 128 * 128;
   if (t >= (long)Kp * M2p) return;
   const int k = (int)(t % Kp), j = (int)(t / Kp);
   double x = 0.0;
@@ -264,130 +308,167 @@ struct NNArgs {
   int row_blocks, col_tiles;
 };
 
-// Work-group: 8 waves; waves 0-3 cover rows [32w,32w+32) of a 128-row block, waves 4-7 the
-// same rows for the second 128-column half: tile 128 rows x 256 columns, 4 blocks/wave.
+// Y tile (256 rows x 128 columns); the col_tiles of one row block run consecutively on one
+// XCD so that X is fetched from HBM once.  The epilogue goes through LDS so that each
+// column of Y is written (and, for beta != 0, read) as 512-byte runs.
 template <int NT>
 __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
+  __shared__ Stage S[2];
   const int L = blockIdx.x, xcd = L & 7, jj = L >> 3;
-  // the col_tiles of one row block run consecutively on one XCD
   const long rb = (long)(jj / p.col_tiles) * 8 + xcd;
   const int ct = jj % p.col_tiles;
   if (rb >= p.row_blocks) return;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const long r0 = rb * 128 + 32 * (w & 3);
-  const int c0 = ct * 256 + 128 * (w >> 2);
-  const long row = r0 + (lane & 31);
+  const long r0 = rb * BM;
+  const int c0 = ct * BN;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
+  // X staging: row t % 256, k-half t / 256 (wave-uniform) -> 16 k values
+  const int xr = t & 255;
+  const int kh = __builtin_amdgcn_readfirstlane(t >> 8);
+  const long row = r0 + xr;
   const bool rin = row < p.R;
-  f16v acc[4];
+  // C'' staging: column t / 4, 8-half part t % 4
+  const int cc = t >> 2, cp = t & 3;
+  // gload only issues loads (row and k clamped into X); the scale (0 for rows past R and
+  // k past K) is applied in lstore, after the MFMAs of the current stage.
+  const double *xrow = p.X + min(row, p.R - 1);
+  double rx[16], sxv[16];
+  half8 rch, rcl;
+  int k0_cur = 0;
+  auto gload = [&](int k0) {
+    k0_cur = k0;
 #pragma unroll
-  for (int b = 0; b < 4; b++)
-#pragma unroll
-    for (int r = 0; r < 16; r++) acc[b][r] = 0.f;
-  const int kh = 8 * (lane >> 5);
-  for (int k0 = 0; k0 < p.K; k0 += 16) {
-    double v[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int k = k0 + kh + q;
-      v[q] = (rin && k < p.K) ? p.X[row + p.ldx * (long)k] * p.sx[k] : 0.0;
+    for (int q = 0; q < 16; q++) {
+      const int kc = min(k0 + 16 * kh + q, p.K - 1);
+      rx[q] = xrow[p.ldx * (long)kc];
+      sxv[q] = p.sx[kc];
     }
-    half8 ah, al;
-    split8(v, 1.0, ah, al);
+    const long off = (long)(c0 + cc) * p.Kp + k0 + 8 * cp;
+    rch = *(const half8 *)&p.ch[off];
+    rcl = *(const half8 *)&p.cl[off];
+  };
+  auto lstore = [&](Stage &D) {
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const int col = c0 + 32 * b + (lane & 31);
-      const long off = (long)col * p.Kp + k0 + kh;
-      const half8 bh = *(const half8 *)&p.ch[off];
-      const half8 bl = *(const half8 *)&p.cl[off];
-      acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[b], 0, 0, 0);
-      acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[b], 0, 0, 0);
-      acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[b], 0, 0, 0);
-      if (NT == 4) acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bl, acc[b], 0, 0, 0);
-    }
+    for (int q = 0; q < 16; q++) rx[q] *= (rin && k0_cur + 16 * kh + q < p.K) ? sxv[q] : 0.0;
+    half8 h, l;
+    split8f(rx, 1.0, h, l);
+    *(half8 *)&D.ah[xr * PITCH + 16 * kh] = h;
+    *(half8 *)&D.al[xr * PITCH + 16 * kh] = l;
+    split8f(rx + 8, 1.0, h, l);
+    *(half8 *)&D.ah[xr * PITCH + 16 * kh + 8] = h;
+    *(half8 *)&D.al[xr * PITCH + 16 * kh + 8] = l;
+    *(half8 *)&D.bh[cc * PITCH + 8 * cp] = rch;
+    *(half8 *)&D.bl[cc * PITCH + 8 * cp] = rcl;
+  };
+  f16v acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
+  const int nst = p.Kp / BK;
+  gload(0);
+  lstore(S[0]);
+  __syncthreads();
+  for (int st = 0; st < nst; st++) {
+    const bool more = st + 1 < nst;
+    if (more) gload((st + 1) * BK);
+    mma_stage<NT>(S[st & 1], lane, wi, wj, acc);
+    if (more) lstore(S[(st + 1) & 1]);
+    __syncthreads();
   }
+  // epilogue: per 32-column block b, each wave parks its 64 x 32 piece in LDS as
+  // [column][row] (pitch 65 floats: conflict-free) and writes it back column by column
+  constexpr int EP = 65;
+  float *W = reinterpret_cast<float *>(&S[0]) + w * 32 * EP;
+  const long m = r0 + 64 * wi + lane;
 #pragma unroll
-  for (int b = 0; b < 4; b++) {
-    const int n = c0 + 32 * b + (lane & 31);
-    const double f = n < p.M2 ? p.alpha / p.tc[n] : 0.0;
+  for (int b = 0; b < 2; b++) {
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const long m = r0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-      if (m < p.R && n < p.M2) {
-        const double v = f * (double)acc[b][r];
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int ml = 32 * a + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        W[(lane & 31) * EP + ml] = acc[a][b][r];
+      }
+    __syncthreads();
+    for (int q = 0; q < 32; q++) {
+      const int n = c0 + 64 * wj + 32 * b + q;
+      if (n < p.M2 && m < p.R) {
+        const double v = p.alpha / p.tc[n] * (double)W[q * EP + lane];
         double *y = p.Y + m + p.ldy * (long)n;
         *y = p.beta == 0.0 ? v : v + p.beta * *y;
       }
     }
+    __syncthreads();
   }
 }
 
 }  // namespace
 
-void colscale(const double *A, long lda, long rows, int cols, const double *pre, double *scale,
-              hipStream_t s) {
-  if (cols == 0) return;
+void colscale(const double *A, long lda, long rows, int cols, double *scale, hipStream_t s) {
+  if (cols <= 0) return;
+  if (rows <= 0) throw Error(1, "colscale: empty columns");
   Timed t("gemm_scale", 8.0 * (double)rows * cols);
-  hipLaunchKernelGGL(k_colscale, dim3(cols), dim3(256), 0, s, A, lda, rows, cols, pre, scale);
+  auto *cmax = (unsigned long long *)scratch("gemm.cmax", sizeof(unsigned long long) * cols);
+  launch_fill(cols, 0.0, (double *)cmax, s);  // all-zero bits
+  const long chunks = (rows + CS_ROWS - 1) / CS_ROWS;
+  hipLaunchKernelGGL(k_colmax, dim3((unsigned)chunks, (unsigned)cols), dim3(256), 0, s, A, lda, rows,
+                     cmax);
+  hipLaunchKernelGGL(k_maxscale, dim3(nblk(cols, 256)), dim3(256), 0, s, cols, cmax, scale);
   HIPCK(hipGetLastError());
 }
 
 void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                    const double *sa, const double *B, long ldb, const double *sb, double beta,
                    double *C, long ldc, int nterms, hipStream_t s) {
-  if (M1 <= 0 || M2 <= 0) return;
+  if (M1 <= 0 || M2 <= 0 || K <= 0) throw Error(1, "gemm_mixed_tn: empty product");
   TNArgs p{};
   p.M1 = M1; p.M2 = M2; p.K = K; p.A = A; p.B = B; p.lda = lda; p.ldb = ldb; p.sa = sa; p.sb = sb;
-  p.tiles_i = (M1 + TN_T - 1) / TN_T;
-  p.tiles_j = (M2 + TN_T - 1) / TN_T;
+  p.tiles_i = (M1 + BM - 1) / BM;
+  p.tiles_j = (M2 + BN - 1) / BN;
   const int tiles = p.tiles_i * p.tiles_j;
-  int nsplit = std::max(1, 2048 / tiles);
-  const long kmin = 4096;  // at least 16 FP32 chunks of work per group
-  nsplit = (int)std::max<long>(1, std::min<long>(nsplit, (K + kmin - 1) / kmin));
-  long kper = (K + nsplit - 1) / nsplit;
-  kper = (kper + TN_KS - 1) / TN_KS * TN_KS;
-  nsplit = (int)((K + kper - 1) / kper);
+  // K per work-group: 4096 (FP32 accumulation depth), more if the FP32 partials would
+  // exceed 2^29 values (2 GiB)
+  const long max_split = std::max<long>(1, (1L << 29) / ((long)M1 * M2));
+  long kper = std::max<long>(4096, (K + max_split - 1) / max_split);
+  kper = (kper + BK - 1) / BK * BK;
+  const int nsplit = (int)((K + kper - 1) / kper);
   p.kper = kper;
   p.nsplit = nsplit;
   p.nsplit_pad = (nsplit + 7) / 8 * 8;
-  p.part = (double *)scratch("gemm.tnpart", sizeof(double) * (size_t)nsplit * M1 * M2);
-  {
-    Timed t("gemm_mixed", 2.0 * (nterms >= 4 ? 4 : 3) * (double)M1 * M2 * K);  // f16 MFMA flops
-    const unsigned grid = (unsigned)(p.nsplit_pad * tiles);
-    if (nterms >= 4) hipLaunchKernelGGL(k_gemm_tn<4>, dim3(grid), dim3(256), 0, s, p);
-    else hipLaunchKernelGGL(k_gemm_tn<3>, dim3(grid), dim3(256), 0, s, p);
-    HIPCK(hipGetLastError());
-    hipLaunchKernelGGL(k_tn_reduce, dim3(nblk((long)M1 * M2, 256)), dim3(256), 0, s, p.part, nsplit,
-                       M1, M2, alpha, sa, sb, beta, C, ldc);
-    HIPCK(hipGetLastError());
-  }
+  p.part = (float *)scratch("gemm.tnpart", sizeof(float) * (size_t)nsplit * M1 * M2);
+  Timed t("gemm_mixed_tn", 2.0 * (nterms >= 4 ? 4 : 3) * (double)M1 * M2 * K);  // f16 MFMA flops
+  const unsigned grid = (unsigned)((long)p.nsplit_pad * tiles);
+  if (nterms >= 4) hipLaunchKernelGGL(k_gemm_tn<4>, dim3(grid), dim3(512), 0, s, p);
+  else hipLaunchKernelGGL(k_gemm_tn<3>, dim3(grid), dim3(512), 0, s, p);
+  HIPCK(hipGetLastError());
+  hipLaunchKernelGGL(k_tn_reduce, dim3(nblk((long)M1 * M2, 256)), dim3(256), 0, s, p.part, nsplit,
+                     M1, M2, alpha, sa, sb, beta, C, ldc);
+  HIPCK(hipGetLastError());
 }
 
 void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ldx,
                    const double *sx, const double *C, long ldc, double beta, double *Y, long ldy,
                    int nterms, hipStream_t s) {
-  if (R <= 0 || M2 <= 0) return;
-  const int Kp = (K + 15) / 16 * 16;
-  const int M2p = (M2 + 127) / 128 * 128;
-  const int M2q = (M2 + 255) / 256 * 256;
-  _Float16 *ch = (_Float16 *)scratch("gemm.nnch", sizeof(_Float16) * (size_t)Kp * M2q);
-  _Float16 *cl = (_Float16 *)scratch("gemm.nncl", sizeof(_Float16) * (size_t)Kp * M2q);
-  double *tc = (double *)scratch("gemm.nntc", sizeof(double) * M2q);
+  if (R <= 0 || M2 <= 0 || K <= 0) throw Error(1, "gemm_mixed_nn: empty product");
+  const int Kp = (K + BK - 1) / BK * BK;
+  const int M2p = (M2 + BN - 1) / BN * BN;
+  _Float16 *ch = (_Float16 *)scratch("gemm.nnch", sizeof(_Float16) * (size_t)Kp * M2p);
+  _Float16 *cl = (_Float16 *)scratch("gemm.nncl", sizeof(_Float16) * (size_t)Kp * M2p);
+  double *tc = (double *)scratch("gemm.nntc", sizeof(double) * M2p);
   hipLaunchKernelGGL(k_nn_cscale, dim3(M2), dim3(256), 0, s, C, ldc, K, M2, sx, tc);
   hipLaunchKernelGGL(k_nn_prep, dim3(nblk((long)Kp * M2p, 256)), dim3(256), 0, s, C, ldc, K, M2, sx,
-                     tc, Kp, ch, cl);
-  if (M2q > M2p) {  // zero the padding columns read by the last 256-wide tile
-    HIPCK(hipMemsetAsync(ch + (size_t)Kp * M2p, 0, sizeof(_Float16) * (size_t)Kp * (M2q - M2p), s));
-    HIPCK(hipMemsetAsync(cl + (size_t)Kp * M2p, 0, sizeof(_Float16) * (size_t)Kp * (M2q - M2p), s));
-  }
+                     tc, Kp, M2p, ch, cl);
   HIPCK(hipGetLastError());
   NNArgs p{};
   p.R = R; p.K = K; p.M2 = M2; p.Kp = Kp; p.X = X; p.ldx = ldx; p.sx = sx; p.ch = ch; p.cl = cl;
   p.tc = tc; p.alpha = alpha; p.beta = beta; p.Y = Y; p.ldy = ldy;
-  p.row_blocks = (int)((R + 127) / 128);
-  p.col_tiles = M2q / 256;
+  p.row_blocks = (int)((R + BM - 1) / BM);
+  p.col_tiles = M2p / BN;
   const long groups = (p.row_blocks + 7) / 8;
   const unsigned grid = (unsigned)(groups * 8 * p.col_tiles);
-  Timed t("gemm_mixed", 2.0 * (nterms >= 4 ? 4 : 3) * (double)R * M2 * K);  // f16 MFMA flops
+  Timed t("gemm_mixed_nn", 2.0 * (nterms >= 4 ? 4 : 3) * (double)R * M2 * K);  // f16 MFMA flops
   if (nterms >= 4) hipLaunchKernelGGL(k_gemm_nn<4>, dim3(grid), dim3(512), 0, s, p);
   else hipLaunchKernelGGL(k_gemm_nn<3>, dim3(grid), dim3(512), 0, s, p);
   HIPCK(hipGetLastError());

@@ -3,9 +3,8 @@
 #include <hip/hip_runtime.h>
 
 namespace kukan {
-// scale[j] = 2^(14 - ceil(log2 max_i |A(i,j) * pre[j]|)), pre may be null
-void colscale(const double *A, long lda, long rows, int cols, const double *pre, double *scale,
-              hipStream_t s);
+// scale[j] = 2^(14 - e_j) with max_i |A(i,j)| < 2^e_j (1 for a zero column)
+void colscale(const double *A, long lda, long rows, int cols, double *scale, hipStream_t s);
 // C(M1 x M2) = alpha A^T B + beta C with column scales sa (A) and sb (B)
 void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                    const double *sa, const double *B, long ldb, const double *sb, double beta,

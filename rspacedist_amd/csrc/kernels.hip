@@ -163,93 +163,143 @@ struct SL {
   long dot_ld;   // band stride of dotp
 };
 
-template <bool RES>
-__device__ inline double sl_load(const SL &p, const double *in, const double *in2, double e, int x,
-                                 int y, int z) {
-  if (y < 0 || y >= p.ny) {
-    if (!p.py) return 0.0;
-    y = y < 0 ? y + p.ny : (y >= p.ny ? y - p.ny : y);
-    if (y < 0 || y >= p.ny) return 0.0;  // tile rows beyond one period (ragged tiles)
-  }
-  if (z < 0 || z >= p.nz) {
-    if (!p.pz) return 0.0;
-    z = z < 0 ? z + p.nz : z - p.nz;
-  }
-  const long i = (long)x + (long)p.nx * ((long)y + (long)p.ny * z);
-  if (RES) return p.sign * (in[i] - e * in2[i]);
-  return in[i];
-}
-
+// Block = TY full x-rows of one band, marching over a z-range.  The 2 NF + 1 values of each
+// point's z-column live in registers; plane z (TY rows plus NF halo rows on either side, each
+// row padded by NF wrapped or zero columns) is staged in LDS, so every x and y neighbour is an
+// unconditional LDS read.  Loads of plane z + NF + 1, of plane z + 1's halo rows and of
+// V_eff(z + 1) are issued before plane z is computed (one plane of prefetch), and the barriers
+// wait only for LDS, so HBM latency overlaps the arithmetic.
 template <int NF, int P, bool RES>
-__global__ __launch_bounds__(512) void k_stencil_lds(SL p) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && P <= 2 ? 4 : 2))) void k_stencil_lds(SL p) {
   extern __shared__ double lds[];
   __shared__ double red[8];
-  const int nx = p.nx, TY = p.TY;
+  constexpr int H = NF;  // halo values per thread: 2 NF rows x nx <= 512 NF for nx <= 256
+  const int nx = p.nx, ny = p.ny, nz = p.nz, TY = p.TY, W = nx + 2 * NF;
+  const long plane = (long)nx * ny;
   const int L = blockIdx.x, xcd = L & 7, j = L >> 3;
   const int per = p.ntile * p.nzch;
   const int b = (j / per) * 8 + xcd;
   if (b >= p.nbands) return;  // uniform for the block
   const int rem = j % per, tile = rem % p.ntile, zch = rem / p.ntile;
-  const int y0 = tile * TY, z0 = zch * p.zc, z1 = min(p.nz, z0 + p.zc);
+  const int y0 = tile * TY, z0 = zch * p.zc, z1 = min(nz, z0 + p.zc);
   const double *in = p.in + (long)b * p.ld_in;
   const double *in2 = RES ? p.in2 + (long)b * p.ld_in : nullptr;
   const double e = RES ? p.eps[b] : 0.0;
   double *out = p.out + (long)b * p.ld_out;
   const int t = threadIdx.x;
-  const int npt = nx * TY;
-  int px_[P], yl_[P];
-  bool val_[P];
+  // y of a row index relative to the grid, wrapped once when periodic; -1 = zero row
+  auto wrap_y = [&](int gy) -> int {
+    if (gy >= 0 && gy < ny) return gy;
+    if (!p.py) return -1;
+    gy = gy < 0 ? gy + ny : gy - ny;
+    return (gy >= 0 && gy < ny) ? gy : -1;
+  };
+  // z plane offset, -1 = zero plane (uniform across the block)
+  auto zoff = [&](int z) -> long {
+    if (z < 0 || z >= nz) {
+      if (!p.pz) return -1;
+      z = z < 0 ? z + nz : z - nz;
+    }
+    return (long)z * plane;
+  };
+  auto ld = [&](long off) -> double {
+    if (RES) return p.sign * (in[off] - e * in2[off]);
+    return in[off];
+  };
+  // tile points
+  int oxy[P], lcol[P];
+  bool lv[P], sv[P];
+  int xq[P];
 #pragma unroll
   for (int q = 0; q < P; q++) {
     const int idx = t + 512 * q;
-    val_[q] = idx < npt;
-    px_[q] = val_[q] ? idx % nx : 0;
-    yl_[q] = val_[q] ? idx / nx : 0;
+    const bool in_tile = idx < nx * TY;
+    const int x = in_tile ? idx % nx : 0, yl = in_tile ? idx / nx : 0;
+    const int gy = y0 + yl, gw = wrap_y(gy);
+    xq[q] = x;
+    lv[q] = in_tile && gw >= 0;
+    sv[q] = in_tile && gy < ny;
+    oxy[q] = x + nx * (gw < 0 ? 0 : gw);
+    lcol[q] = (NF + yl) * W + NF + x;
   }
+  // halo rows: index i = t + 512 h over 2 NF rows x nx
+  int hxy[H], hcol[H], hx[H];
+  bool hv_ok[H];
+#pragma unroll
+  for (int h = 0; h < H; h++) {
+    const int i = t + 512 * h;
+    const bool ok = i < 2 * NF * nx;
+    const int r = ok ? i / nx : 0, x = ok ? i % nx : 0;
+    const int row = r < NF ? r : TY + r;
+    const int gw = wrap_y(y0 - NF + row);
+    hv_ok[h] = ok && gw >= 0;
+    hxy[h] = x + nx * (gw < 0 ? 0 : gw);
+    hcol[h] = ok ? row * W + NF + x : -1;
+    hx[h] = x;
+  }
+  // pad columns are zero for a non-periodic x axis and never rewritten
+  if (!p.px)
+    for (int i = t; i < (TY + 2 * NF) * 2 * NF; i += 512) {
+      const int row = i / (2 * NF), c = i % (2 * NF);
+      lds[row * W + (c < NF ? c : nx + c)] = 0.0;
+    }
+  auto put = [&](int col, int x, double v) {  // LDS write with the x-wrap copies
+    lds[col] = v;
+    if (p.px) {
+      if (x < NF) lds[col + nx] = v;
+      if (x >= nx - NF) lds[col - nx] = v;
+    }
+  };
   double col[P][2 * NF + 1];
 #pragma unroll
-  for (int q = 0; q < P; q++)
+  for (int m = 0; m < 2 * NF; m++) {
+    const long zo = zoff(z0 - NF + m);
 #pragma unroll
-    for (int m = 0; m < 2 * NF; m++)
-      col[q][m + 1] = val_[q] ? sl_load<RES>(p, in, in2, e, px_[q], y0 + yl_[q], z0 - NF + m) : 0.0;
+    for (int q = 0; q < P; q++) col[q][m + 1] = (lv[q] && zo >= 0) ? ld(zo + oxy[q]) : 0.0;
+  }
+  double nxt[P], hvv[H], vf[P];
+  auto prefetch = [&](int z) {
+    const long zn = zoff(z + NF), zh = (long)z * plane;
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      nxt[q] = (lv[q] && zn >= 0) ? ld(zn + oxy[q]) : 0.0;
+      vf[q] = (p.veff && sv[q]) ? p.veff[zh + oxy[q]] : 0.0;
+    }
+#pragma unroll
+    for (int h = 0; h < H; h++) hvv[h] = hv_ok[h] ? ld(zh + hxy[h]) : 0.0;
+  };
+  prefetch(z0);
   double dsum = 0.0;
-  const int W = nx;
   for (int z = z0; z < z1; z++) {
+    double vcur[P];
 #pragma unroll
     for (int q = 0; q < P; q++) {
 #pragma unroll
       for (int m = 0; m < 2 * NF; m++) col[q][m] = col[q][m + 1];
-      col[q][2 * NF] = val_[q] ? sl_load<RES>(p, in, in2, e, px_[q], y0 + yl_[q], z + NF) : 0.0;
-      if (val_[q]) lds[(NF + yl_[q]) * W + px_[q]] = col[q][NF];
+      col[q][2 * NF] = nxt[q];
+      vcur[q] = vf[q];
+      if (t + 512 * q < nx * TY) put(lcol[q], xq[q], col[q][NF]);
     }
-    for (int i = t; i < 2 * NF * nx; i += 512) {
-      const int r = i / nx, xx = i % nx;
-      const int row = r < NF ? r : TY + r;
-      lds[row * W + xx] = sl_load<RES>(p, in, in2, e, xx, y0 - NF + row, z);
-    }
+#pragma unroll
+    for (int h = 0; h < H; h++)
+      if (hcol[h] >= 0) put(hcol[h], hx[h], hvv[h]);
+    if (z + 1 < z1) prefetch(z + 1);
     __syncthreads();
+    const long zh = (long)z * plane;
 #pragma unroll
     for (int q = 0; q < P; q++) {
-      const int gy = y0 + yl_[q];
-      if (!val_[q] || gy >= p.ny) continue;
-      const int x = px_[q], row = NF + yl_[q];
-      const long r = (long)x + (long)nx * ((long)gy + (long)p.ny * z);
+      if (!sv[q]) continue;
       const double c = col[q][NF];
-      double s = (p.c0 + (p.veff ? p.veff[r] : 0.0)) * c;
-      const double *lr = lds + row * W;
+      double s = (p.c0 + vcur[q]) * c;
+      const double *lr = lds + lcol[q];
 #pragma unroll
-      for (int m = 1; m <= NF; m++) {
-        int xm = x - m, xp = x + m;
-        double a, bb;
-        if (xm < 0) a = p.px ? lr[xm + nx] : 0.0; else a = lr[xm];
-        if (xp >= nx) bb = p.px ? lr[xp - nx] : 0.0; else bb = lr[xp];
-        s += p.cx[m] * (a + bb);
-      }
+      for (int m = 1; m <= NF; m++) s += p.cx[m] * (lr[-m] + lr[m]);
 #pragma unroll
-      for (int m = 1; m <= NF; m++) s += p.cy[m] * (lr[x - m * W] + lr[x + m * W]);
+      for (int m = 1; m <= NF; m++) s += p.cy[m] * (lr[-m * W] + lr[m * W]);
 #pragma unroll
       for (int m = 1; m <= NF; m++) s += p.cz[m] * (col[q][NF - m] + col[q][NF + m]);
       if (p.dotp) dsum += c * s;
+      const long r = zh + oxy[q];
       out[r] = p.accumulate ? out[r] + s : s;
     }
     __syncthreads();
@@ -401,9 +451,11 @@ __global__ __launch_bounds__(256) void k_gather(const int *act, int num_list, in
 }
 
 // coef(j, slot, b) = dvinv * sum_i rp(i, slot, b) X(i, j)   (nonlocaloperation8f.F90:180-198)
+// coef(j, slot, b) = dvinv * sum_i rp(i, slot, b) X(i, j); layout [b][slot][j] (ld_coef per
+// band) or, with band_fastest, [slot][j][b] (the layout k_expand_rows reads).
 __global__ void k_coef(const int *act, int nact, int nprjmx, int nums, int ns, const double *X,
                        int use_sss, const double *rp, long ld_rp, int nb, double dvinv,
-                       double *coef, long ld_coef) {
+                       double *coef, long ld_coef, int band_fastest) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   const long total = (long)nact * nprjmx * nb;
   if (t >= total) return;
@@ -415,7 +467,8 @@ __global__ void k_coef(const int *act, int nact, int nprjmx, int nums, int ns, c
   const double *r = rp + (long)b * ld_rp + (long)nprjmx * slot;
   double s = 0.0;
   for (int i = 0; i < iprj; i++) s += r[i] * Xa[i + (long)nprjmx * j];
-  coef[(long)b * ld_coef + j + (long)nprjmx * slot] = s * dvinv;
+  const long jj = j + (long)nprjmx * slot;
+  coef[band_fastest ? jj * nb + b : (long)b * ld_coef + jj] = s * dvinv;
 }
 
 __global__ void k_expand(const int *row_ptr, const int2 *ent, const int *iprj_slot,
@@ -446,13 +499,15 @@ template <int EB>
 __global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz, const int *row_ptr,
                                                      const int2 *ent, const int *iprj_slot,
                                                      const double *vnl, int num_list, int nprjmx,
-                                                     const double *coef, long ld_coef, int nb,
-                                                     double *out, long ld_out, const double *psi,
-                                                     long ld_psi, double *part, long part_ld,
-                                                     int part_off) {
+                                                     const double *coef, int nb, double *out,
+                                                     long ld_out, const double *psi, long ld_psi,
+                                                     double *part, long part_ld, int part_off) {
+  // coef is [slot][j][b] (bands fastest): the EB coefficients of one (slot, j) are contiguous
+  // and shared by all rows of the same atom, so they come from L1.
   __shared__ double red[EB][4];
   const long i = blockIdx.x * 256L + threadIdx.x;
   const int b0 = blockIdx.y * EB;
+  const bool full = b0 + EB <= nb;
   double acc[EB];
 #pragma unroll
   for (int q = 0; q < EB; q++) acc[q] = 0.0;
@@ -464,12 +519,18 @@ __global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz
       const int2 en = ent[e];
       const int ip = iprj_slot[en.x];
       const double *v = vnl + en.y + (long)num_list * nprjmx * en.x;
-      const double *cc = coef + (long)nprjmx * en.x;
+      const double *cc = coef + (long)nprjmx * en.x * nb + b0;
       for (int j = 0; j < ip; j++) {
         const double vj = v[(long)num_list * j];
+        const double *cj = cc + (long)j * nb;
+        if (full) {
 #pragma unroll
-        for (int q = 0; q < EB; q++)
-          if (b0 + q < nb) acc[q] += cc[(long)(b0 + q) * ld_coef + j] * vj;
+          for (int q = 0; q < EB; q++) acc[q] += cj[q] * vj;
+        } else {
+#pragma unroll
+          for (int q = 0; q < EB; q++)
+            if (b0 + q < nb) acc[q] += cj[q] * vj;
+        }
       }
     }
 #pragma unroll
@@ -689,7 +750,11 @@ struct LdsPlan {
 };
 static LdsPlan lds_plan(const Grid &g, int NF, int nb) {
   LdsPlan L;
-  L.TY = std::max(1, std::min(g.ny, 2048 / g.nx));
+  // P points per thread (param "stencil_p": 1, 2 or 4; default 2), TY rows of nx points
+  int pp = (int)param("stencil_p", 2.0);
+  pp = pp >= 4 ? 4 : (pp >= 2 ? 2 : 1);
+  if (NF > 4) pp = 1;  // the z-column of a wide stencil leaves registers for one point
+  L.TY = std::max(1, std::min(g.ny, 512 * pp / g.nx));
   L.ntile = (g.ny + L.TY - 1) / L.TY;
   L.P = (g.nx * L.TY + 511) / 512;
   if (L.P == 3) L.P = 4;
@@ -710,7 +775,7 @@ int stencil_parts_lds(const Grid &g, int NF, int nb) {
 
 template <int NF, bool RES>
 static void launch_lds_nf(const SL &p, const LdsPlan &L, int nb, hipStream_t s) {
-  const size_t shm = sizeof(double) * (size_t)(L.TY + 2 * NF) * p.nx;
+  const size_t shm = sizeof(double) * (size_t)(L.TY + 2 * NF) * (p.nx + 2 * NF);
   const long groups = (nb + 7) / 8;
   const unsigned grid = (unsigned)(groups * 8 * (long)L.ntile * L.nzch);
   switch (L.P) {
@@ -737,6 +802,10 @@ static void launch_lds(SL &p, const Grid &g, int NF, int nb, bool res, hipStream
     }
   }
   HIPCK(hipGetLastError());
+}
+
+bool stencil_band_fastest_coef(const Grid &g, bool halo_in) {
+  return !halo_in && lds_ok(g) && param("stencil_lds", 1.0) >= 1.0;
 }
 
 int stencil_nparts(const Grid &g, int nf, int nb) {
@@ -769,6 +838,8 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
   p.c0 = p.cx[0] + p.cy[0] + p.cz[0];
   p.accumulate = c.accumulate ? 1 : 0;
   if (c.pl && c.coef && c.pl->nent > 0) {
+    if (c.coef_band_fastest != stencil_band_fastest_coef(g, c.halo_in))
+      throw Error(1, "stencil: projector coefficients in the wrong layout");
     p.row_ptr = c.pl->row_ptr; p.ent = c.pl->ent; p.vnl = c.pl->vnl;
     p.num_list = c.pl->num_list; p.nprjmx = c.pl->nprjmx; p.iprj_slot = c.pl->iprj_slot;
     p.coef = c.coef; p.ld_coef = c.ld_coef;
@@ -786,8 +857,8 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
     launch_lds(q, g, nf, c.nb, false, s);
     if (c.pl && c.coef && c.pl->nent > 0) {  // nonlocal part: separate owner-computes pass
       const long pld = c.dot_ld ? c.dot_ld : stencil_parts_lds(g, nf, c.nb);
-      launch_expand_rows(*c.pl, c.coef, c.ld_coef, c.nb, c.out, c.ld_out, c.in, c.ld_in,
-                         c.dot_partial, pld, stencil_parts_lds(g, nf, c.nb), s);
+      launch_expand_rows(*c.pl, c.coef, c.nb, c.out, c.ld_out, c.in, c.ld_in, c.dot_partial, pld,
+                         stencil_parts_lds(g, nf, c.nb), s);
     }
     return;
   }
@@ -887,11 +958,13 @@ void launch_gather(const NonlocalPlan &pl, const double *psi, long ld, int nb, d
 }
 
 void launch_coef(const NonlocalPlan &pl, const double *rp, long ld_rp, int nb, const double *X,
-                 bool use_sss, int ns, double dvinv, double *coef, long ld_coef, hipStream_t s) {
+                 bool use_sss, int ns, double dvinv, double *coef, long ld_coef, hipStream_t s,
+                 bool band_fastest) {
   if (pl.nact == 0 || nb == 0) return;
   const long tot = (long)pl.nact * pl.nprjmx * nb;
   hipLaunchKernelGGL(k_coef, dim3(nblk(tot, 256)), dim3(256), 0, s, pl.act, pl.nact, pl.nprjmx,
-                     pl.nums, ns, X, use_sss ? 1 : 0, rp, ld_rp, nb, dvinv, coef, ld_coef);
+                     pl.nums, ns, X, use_sss ? 1 : 0, rp, ld_rp, nb, dvinv, coef, ld_coef,
+                     band_fastest ? 1 : 0);
   HIPCK(hipGetLastError());
 }
 
@@ -906,9 +979,9 @@ void launch_expand(const NonlocalPlan &pl, long n, const double *coef, long ld_c
 
 int expand_nparts(const NonlocalPlan &pl) { return (int)std::max<long>(1, (pl.nnz + 255) / 256); }
 
-void launch_expand_rows(const NonlocalPlan &pl, const double *coef, long ld_coef, int nb,
-                        double *out, long ld_out, const double *psi, long ld_psi, double *part,
-                        long part_ld, int part_off, hipStream_t s) {
+void launch_expand_rows(const NonlocalPlan &pl, const double *coef, int nb, double *out,
+                        long ld_out, const double *psi, long ld_psi, double *part, long part_ld,
+                        int part_off, hipStream_t s) {
   if (nb == 0) return;
   const int np = expand_nparts(pl);
   if (pl.nnz == 0) {
@@ -917,9 +990,10 @@ void launch_expand_rows(const NonlocalPlan &pl, const double *coef, long ld_coef
     return;
   }
   Timed t("nl_expand", 8.0 * (2.0 * (double)pl.nnz * nb + (double)pl.nprjmx * pl.nent));
-  hipLaunchKernelGGL(k_expand_rows<8>, dim3(np, nblk(nb, 8)), dim3(256), 0, s, pl.nzrows, pl.nnz,
-                     pl.row_ptr, pl.ent, pl.iprj_slot, pl.vnl, pl.num_list, pl.nprjmx, coef,
-                     ld_coef, nb, out, ld_out, psi, ld_psi, part, part_ld, part_off);
+  constexpr int EB = 16;
+  hipLaunchKernelGGL(k_expand_rows<EB>, dim3(np, nblk(nb, EB)), dim3(256), 0, s, pl.nzrows, pl.nnz,
+                     pl.row_ptr, pl.ent, pl.iprj_slot, pl.vnl, pl.num_list, pl.nprjmx, coef, nb,
+                     out, ld_out, psi, ld_psi, part, part_ld, part_off);
   HIPCK(hipGetLastError());
 }
 

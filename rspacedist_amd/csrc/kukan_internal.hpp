@@ -144,12 +144,15 @@ struct StencilCall {
   const NonlocalPlan *pl = nullptr;  // expansion if non-null and coef non-null
   const double *coef = nullptr;      // (nprjmx, num_ppcell) per band, ld = ld_coef
   long ld_coef = 0;
+  bool coef_band_fastest = false;    // coef laid out [slot][j][b] (see stencil_band_fastest_coef)
   double scale_lap = 1.0;            // multiplies the kinetic part (1 for H)
   double *dot_partial = nullptr;     // optional: per-block partial sums of in_b . out_b
   long dot_ld = 0;                   // band stride of dot_partial (0: number of parts)
   int nfh_coeffs = 0;                // if > 0: use this FD order (Hartree), no veff
 };
 void launch_stencil(const StencilCall &c, hipStream_t s);
+// true when launch_stencil takes the tiled path, whose projector pass reads band-fastest coef
+bool stencil_band_fastest_coef(const Grid &g, bool halo_in);
 // partial-sum layout of launch_stencil's dot_partial: nb * nparts
 int stencil_nparts(const Grid &g, int nf, int nb);
 // fused r = sign*(h - eps s), out = K r, optional per-band partials of <r|K r>
@@ -172,16 +175,17 @@ void launch_gather(const NonlocalPlan &pl, const double *psi, long ld, int nb, d
                    long ld_rp, bool accumulate, hipStream_t s);
 // coef(j, slot, b) = sum_i rp(i, slot, b) X(i, j; atom) / dV, X = dij(:,:,ns,na) or sss(:,:,1,ispe)
 void launch_coef(const NonlocalPlan &pl, const double *rp, long ld_rp, int nb, const double *X,
-                 bool use_sss, int ns, double dvinv, double *coef, long ld_coef, hipStream_t s);
+                 bool use_sss, int ns, double dvinv, double *coef, long ld_coef, hipStream_t s,
+                 bool band_fastest = false);
 // out_b(r) += sum_{entries of r} sum_j coef_b(j, slot) vnl(k, j, slot)
 void launch_expand(const NonlocalPlan &pl, long n, const double *coef, long ld_coef, int nb,
                    double *out, long ld_out, hipStream_t s);
 // Same over the compacted rows, 8 bands per thread; optional per-band partials of
 // <psi_b|added part> written at part[b*part_ld + part_off + block].
 int expand_nparts(const NonlocalPlan &pl);
-void launch_expand_rows(const NonlocalPlan &pl, const double *coef, long ld_coef, int nb,
-                        double *out, long ld_out, const double *psi, long ld_psi, double *part,
-                        long part_ld, int part_off, hipStream_t s);
+void launch_expand_rows(const NonlocalPlan &pl, const double *coef, int nb, double *out,
+                        long ld_out, const double *psi, long ld_psi, double *part, long part_ld,
+                        int part_off, hipStream_t s);  // coef band-fastest [slot][j][b]
 
 // batched reductions: res[b] = scale * sum_i x_b[i] * y_b[i] (deterministic two-stage)
 void launch_dots(long n, const double *x, long ldx, const double *y, long ldy, int nb,

@@ -53,6 +53,18 @@ __global__ void k_kscg05(int nb, const double *sval, const double *pap, const do
   beta[b] = be;
 }
 
+// f[b] <- 1 / sqrt(f[b]); a non-positive norm raises the flag
+__global__ void k_rsqrt(int nb, double *f, int *flag) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  if (f[b] > 0.0) {
+    f[b] = 1.0 / sqrt(f[b]);
+  } else {
+    f[b] = 1.0;
+    *flag = 1;
+  }
+}
+
 // zero the strictly lower triangle of an M x M column-major matrix
 __global__ void k_zero_lower(int M, double *A) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -317,6 +329,44 @@ __global__ void k_charge(long n, int nb, const double *psi, long ld, const doubl
   rho[i] = r;
 }
 
+// x *= alpha
+__global__ void k_scal(long n, double alpha, double *x) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= alpha;
+}
+
+// Nonlocal part of dE/dR for E = sum_l f_l sum_ij rp_i(l) D_ij rp_j(l): work-group (active
+// atom a, direction d) reduces over bands l of
+//   f_l sum_ij (drp_d,i(l) D_ij rp_j(l) + rp_i(l) D_ij drp_d,j(l))
+// in a fixed order.  drp_d = <d p / d R_d | psi> from the derivative tables.
+__global__ __launch_bounds__(256) void k_adj_force(const int *act, int nprjmx, int nums,
+                                                   const double *dij, const double *rp,
+                                                   const double *drp, long n1, int M,
+                                                   const double *f, const int *is_in,
+                                                   int natom, double *dedr) {
+  __shared__ double red[256];
+  const int a = blockIdx.x, d = blockIdx.y;
+  const int slot = act[5 * a], iprj = act[5 * a + 2], na = act[5 * a + 3];
+  const double *D = dij + (long)nprjmx * nprjmx * ((long)nums * na);
+  const double *dr = drp + (long)d * n1 * M;
+  double acc = 0.0;
+  for (int l = threadIdx.x; l < M; l += 256) {
+    const double *r = rp + (long)l * n1 + (long)nprjmx * slot;
+    const double *q = dr + (long)l * n1 + (long)nprjmx * slot;
+    double s = 0.0;
+    for (int j = 0; j < iprj; j++)
+      for (int i = 0; i < iprj; i++) s += (q[i] * r[j] + r[i] * q[j]) * D[i + (long)nprjmx * j];
+    acc += f[l] * s;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && is_in[a]) dedr[(long)d * natom + na] = red[0];
+}
+
 // Hartree: b = 2 pi rho; boundary planes of isolated directions (scf_hartree8f.F90:88-595)
 __global__ void k_hartree_b(long n, const double *rho, double *b) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -417,10 +467,11 @@ void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, 
                long ld_rp, double *out, long ld_out, bool accumulate, double *eps_out) {
   const long n1 = (long)pl.nprjmx * pl.num_ppcell;
   double *coef = buf("coef", (size_t)std::max<long>(1, n1) * nb);
-  launch_coef(pl, rp, ld_rp, nb, dij, false, ns, 1.0 / g.dv(), coef, n1, s);
+  const bool bf = stencil_band_fastest_coef(g, false);
+  launch_coef(pl, rp, ld_rp, nb, dij, false, ns, 1.0 / g.dv(), coef, n1, s, bf);
   StencilCall c;
   c.g = &g; c.in = psi; c.ld_in = ld; c.out = out; c.ld_out = ld_out; c.veff = veffs; c.nb = nb;
-  c.accumulate = accumulate; c.pl = &pl; c.coef = coef; c.ld_coef = n1;
+  c.accumulate = accumulate; c.pl = &pl; c.coef = coef; c.ld_coef = n1; c.coef_band_fastest = bf;
   double *part = nullptr;
   int np = stencil_nparts(g, g.nf, nb);
   if (eps_out) {
@@ -471,15 +522,20 @@ static bool mixed_for(long rows) {
 }
 static int mixed_terms() { return param("mixed_terms", 3.0) >= 4.0 ? 4 : 3; }
 
+const double *mixed_scale(const char *tag, const double *X, long ldx, long rows, int cols) {
+  if (!mixed_for(rows) || cols <= 0) return nullptr;
+  double *sc = (double *)scratch(std::string("scale.") + tag, sizeof(double) * cols);
+  colscale(X, ldx, rows, cols, sc, lib().stream);
+  return sc;
+}
+
 void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
-                 const double *B, long ldb, double beta, double *C, long ldc) {
+                 const double *B, long ldb, double beta, double *C, long ldc, const double *sa,
+                 const double *sb) {
   hipStream_t s = lib().stream;
   if (mixed_for(K)) {
-    double *sa = (double *)scratch("tn.sa", sizeof(double) * M1);
-    double *sb = (double *)scratch("tn.sb", sizeof(double) * M2);
-    colscale(A, lda, K, M1, nullptr, sa, s);
-    if (B == A && ldb == lda && M2 == M1) sb = sa;
-    else colscale(B, ldb, K, M2, nullptr, sb, s);
+    if (!sa) sa = mixed_scale("tn.a", A, lda, K, M1);
+    if (!sb) sb = (B == A && ldb == lda && M2 == M1) ? sa : mixed_scale("tn.b", B, ldb, K, M2);
     gemm_mixed_tn(M1, M2, K, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, mixed_terms(), s);
     return;
   }
@@ -488,11 +544,10 @@ void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda
 }
 
 void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx, const double *C,
-                 long ldc, double beta, double *Y, long ldy) {
+                 long ldc, double beta, double *Y, long ldy, const double *sx) {
   hipStream_t s = lib().stream;
   if (mixed_for(R)) {
-    double *sx = (double *)scratch("nn.sx", sizeof(double) * K);
-    colscale(X, ldx, R, K, nullptr, sx, s);
+    if (!sx) sx = mixed_scale("nn.x", X, ldx, R, K);
     gemm_mixed_nn(R, K, M2, alpha, X, ldx, sx, C, ldc, beta, Y, ldy, mixed_terms(), s);
     return;
   }
@@ -501,12 +556,30 @@ void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx,
 }
 
 static void sub_tn(Ctx &, int M1, int M2, long K, double alpha, const double *A, long lda,
-                   const double *B, long ldb, double beta, double *C, long ldc) {
-  subspace_tn(M1, M2, K, alpha, A, lda, B, ldb, beta, C, ldc);
+                   const double *B, long ldb, double beta, double *C, long ldc,
+                   const double *sa = nullptr, const double *sb = nullptr) {
+  subspace_tn(M1, M2, K, alpha, A, lda, B, ldb, beta, C, ldc, sa, sb);
 }
 static void sub_nn(Ctx &, long R, int K, int M2, double alpha, const double *X, long ldx,
-                   const double *C, long ldc, double beta, double *Y, long ldy) {
-  subspace_nn(R, K, M2, alpha, X, ldx, C, ldc, beta, Y, ldy);
+                   const double *C, long ldc, double beta, double *Y, long ldy,
+                   const double *sx = nullptr) {
+  subspace_nn(R, K, M2, alpha, X, ldx, C, ldc, beta, Y, ldy, sx);
+}
+
+// After a split-precision rotation the band norms carry ~1e-7 relative errors, which would
+// enter the (unnormalised) Rayleigh quotients dV <psi|H|psi> at first order.  Restore
+// dV <psi|S|psi> = 1 per band in FP64, scaling S psi, <p|psi> and H psi alike.
+static void renormalize(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi) {
+  const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
+  double *f = c.buf("renorm.f", M);
+  c.clear_flag(2);
+  c.dots(psi, n, spsi, n, M, f);
+  hipLaunchKernelGGL(k_rsqrt, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, f, c.flag(2));
+  if (c.read_flag(2)) throw Error(2, "orthogonalization_r_01 (1): wf has negative norm");
+  launch_scale(n, M, f, psi, n, c.s);
+  if (spsi != psi) launch_scale(n, M, f, spsi, n, c.s);
+  if (hpsi) launch_scale(n, M, f, hpsi, n, c.s);
+  if (rsp && n1 > 0) launch_scale(n1, M, f, rsp, n1, c.s);
 }
 
 // ============================================================================= Cholesky-QR
@@ -518,7 +591,8 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi)
   rocblas_handle h = lib().blas;
   double *G = c.buf("cholqr.G", (size_t)M * M);
   const double dv = c.g.dv(), one = 1.0;
-  sub_tn(c, M, M, n, dv, psi, n, spsi, n, 0.0, G, M);
+  const double *s_psi = mixed_scale("cholqr.psi", psi, n, n, M);
+  sub_tn(c, M, M, n, dv, psi, n, spsi, n, 0.0, G, M, s_psi, nullptr);  // spsi == psi reuses s_psi
   int *info = (int *)scratch("cholqr.info", sizeof(int));
   {
     Timed t("potrf", (double)M * M * M / 3.0);
@@ -536,7 +610,6 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi)
   if (mixed_for(n)) {
     // R^-1 in FP64 (upper triangular), then X <- X R^-1 as a split-FP16 product
     double *Ri = c.buf("cholqr.Ri", (size_t)M * M);
-    launch_fill((long)M * M, 0.0, Ri, c.s);
     HIPCK(hipMemcpy2DAsync(Ri, sizeof(double) * M, G, sizeof(double) * M, sizeof(double) * M, M,
                            hipMemcpyDeviceToDevice, c.s));
     {
@@ -545,14 +618,15 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi)
     }
     hipLaunchKernelGGL(k_zero_lower, dim3(nblk((long)M * M, 256)), dim3(256), 0, c.s, M, Ri);
     double *W = c.buf("tmp.nM", (size_t)n * M);
-    auto apply = [&](double *X) {
-      sub_nn(c, n, M, M, 1.0, X, n, Ri, M, 0.0, W, n);
+    auto apply = [&](double *X, const double *sx) {
+      sub_nn(c, n, M, M, 1.0, X, n, Ri, M, 0.0, W, n, sx);
       HIPCK(hipMemcpyAsync(X, W, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
     };
-    apply(psi);
-    if (spsi != psi) apply(spsi);
-    if (hpsi) apply(hpsi);
+    apply(psi, s_psi);
+    if (spsi != psi) apply(spsi, nullptr);
+    if (hpsi) apply(hpsi, nullptr);
     if (rsp && n1 > 0) tr(rsp, n1, n1);
+    renormalize(c, M, psi, spsi, rsp, hpsi);
     return;
   }
   tr(psi, n, n);
@@ -612,7 +686,9 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   double *A = c.buf("rr.A", (size_t)M * M);
   double *E = c.buf("rr.E", (size_t)M);
   const double dv = c.g.dv(), zero = 0.0, one = 1.0;
-  sub_tn(c, M, M, n, dv, psi, n, hpsi, n, zero, A, M);
+  const double *s_psi = mixed_scale("rr.psi", psi, n, n, M);
+  const double *s_h = mixed_scale("rr.h", hpsi, n, n, M);
+  sub_tn(c, M, M, n, dv, psi, n, hpsi, n, zero, A, M, s_psi, s_h);
   int *info = (int *)scratch("rr.info", sizeof(int));
   {
     Timed t("syevd", 4.0 * (double)M * M * M);
@@ -628,9 +704,9 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   double *Wh = nullptr;
   if (keep_h) {  // block path: rotate H Psi too, through a separate buffer
     Wh = c.buf("tmp.nM", (size_t)n * M);  // shared with Cholesky-QR (never live together)
-    sub_nn(c, n, M, M, one, hpsi, n, A, M, zero, Wh, n);
+    sub_nn(c, n, M, M, one, hpsi, n, A, M, zero, Wh, n, s_h);
   }
-  sub_nn(c, n, M, M, one, psi, n, A, M, zero, W, n);
+  sub_nn(c, n, M, M, one, psi, n, A, M, zero, W, n, s_psi);
   HIPCK(hipMemcpyAsync(psi, W, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
   if (c.q_zero && spsi != psi) {
     HIPCK(hipMemcpyAsync(spsi, psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
@@ -651,6 +727,7 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   } else {
     HIPCK(hipMemcpyAsync(hpsi, spsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
   }
+  if (mixed_for(n)) renormalize(c, M, psi, spsi, rsp, keep_h ? hpsi : nullptr);
 }
 
 // ============================================================================= kscg_r
@@ -796,8 +873,9 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
       launch_reduce_partials(part, np, M, c.g.dv(), rnew, c.s);
     }
     // Kr orthogonal to the block in the S metric: C = dV (S Psi)^T Kr ; Kr -= Psi C
-    sub_tn(c, M, M, n, dv, a.spsi, n, KR, n, zero, C, M);
-    sub_nn(c, n, M, M, mone, a.psi, n, C, M, one, KR, n);
+    const double *s_psi = mixed_scale("bcg.psi", a.psi, n, n, M);
+    sub_tn(c, M, M, n, dv, a.spsi, n, KR, n, zero, C, M, a.spsi == a.psi ? s_psi : nullptr, nullptr);
+    sub_nn(c, n, M, M, mone, a.psi, n, C, M, one, KR, n, s_psi);
     // P = Kr + beta P
     hipLaunchKernelGGL(k_ratio, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, rnew, rold, doreset ? 1 : 0, tb);
     if (doreset) launch_copy2d(n, M, KR, n, PA, n, c.s);
@@ -1068,4 +1146,38 @@ void force_eig(Ctx &c, int M, int ns, const double *psi, const double *sval, con
   HIPCK(hipGetLastError());
 }
 
+}  // namespace kukan
+
+namespace kukan {
+// ============================================================================= differentiation
+void band_energy(Ctx &c, int M, const double *veff, const double *psi, double *eband) {
+  const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
+  double *rp = c.buf("adj.rp", (size_t)std::max<long>(1, n1) * M);
+  double *hp = c.buf("tmp.nM", (size_t)n * M);
+  c.proj(psi, n, M, rp, n1);
+  c.hpsi(1, veff, psi, n, M, rp, n1, hp, n, false, eband);
+}
+
+void energy_adjoint(Ctx &c, int M, const double *psi, const double *f, const double *dvnl,
+                    const int *is_in_dev, double *dedv, double *dedr) {
+  const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
+  // dE/dveff(r) = dV sum_l f_l psi_l(r)^2
+  launch_fill(n, 0.0, dedv, c.s);
+  charge(n, M, psi, f, dedv);
+  hipLaunchKernelGGL(k_scal, dim3(nblk(n, 256)), dim3(256), 0, c.s, n, c.g.dv(), dedv);
+  launch_fill(3L * c.pl.natom, 0.0, dedr, c.s);
+  if (c.pl.nact == 0 || n1 == 0) return;
+  double *rp = c.buf("adj.rp", (size_t)n1 * M);
+  double *drp = c.buf("adj.drp", (size_t)3 * n1 * M);
+  c.proj(psi, n, M, rp, n1);
+  const long tab = (long)c.pl.num_list * c.pl.nprjmx * c.pl.num_ppcell;
+  for (int d = 0; d < 3; d++) {
+    NonlocalPlan pd = c.pl;  // same lists, derivative table
+    pd.vnl = dvnl + d * tab;
+    launch_gather(pd, psi, n, M, drp + (long)d * n1 * M, n1, false, c.s);
+  }
+  hipLaunchKernelGGL(k_adj_force, dim3(c.pl.nact, 3), dim3(256), 0, c.s, c.pl.act, c.pl.nprjmx,
+                     c.pl.nums, c.dij, rp, drp, n1, M, f, is_in_dev, c.pl.natom, dedr);
+  HIPCK(hipGetLastError());
+}
 }  // namespace kukan

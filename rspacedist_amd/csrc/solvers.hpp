@@ -50,10 +50,14 @@ struct HartreeArgs {
 // C(M1 x M2) = alpha A^T B + beta C (A: K x M1, B: K x M2) and Y(R x M2) = alpha X C + beta Y
 // (X: R x K, Y must not alias X), all column-major on the library stream; FP64 or split-FP16
 // per params "precision" / "mixed_min_rows" / "mixed_terms".
+// Column scales sa/sb/sx may be passed in (from mixed_scale) when an operand is reused;
+// null means "compute here".  mixed_scale returns null in FP64 mode.
 void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
-                 const double *B, long ldb, double beta, double *C, long ldc);
+                 const double *B, long ldb, double beta, double *C, long ldc,
+                 const double *sa = nullptr, const double *sb = nullptr);
 void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx, const double *C,
-                 long ldc, double beta, double *Y, long ldy);
+                 long ldc, double beta, double *Y, long ldy, const double *sx = nullptr);
+const double *mixed_scale(const char *tag, const double *X, long ldx, long rows, int cols);
 
 void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi);
 void gs_one(Ctx &c, int l, double *psi, double *spsi, double *rsp);
@@ -69,6 +73,12 @@ void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *sp
 int hartree(const HartreeArgs &a);
 void correct_potential(long n, double dvol_over_omega, double *vh);
 void charge(long n, int M, const double *psi, const double *f, double *rho);
+// e_l = dV <psi_l|H|psi_l> (H = T + veff + nonlocal with dij)
+void band_energy(Ctx &c, int M, const double *veff, const double *psi, double *eband);
+// adjoint of E = sum_l f_l e_l: dedv = dE/dveff (n), dedr = nonlocal dE/dR (3 x natom) from the
+// projector derivative tables dvnl (3 tables of vnlocp's shape, d vnlocp / d R_x,y,z)
+void energy_adjoint(Ctx &c, int M, const double *psi, const double *f, const double *dvnl,
+                    const int *is_in_dev, double *dedv, double *dedr);
 void force_eig(Ctx &c, int M, int ns, const double *psi, const double *sval, const double *fnele,
                const int *is_in_dev, double *fa, int nf);
 

@@ -1,7 +1,9 @@
 ! kukan_hip_shim.F90 -- Fortran drop-in for the RSPACE kukan v8f Kohn-Sham hot path on MI355X.
 !
-! The modules below carry the reference module and routine names with the reference
-! argument lists (RSPACEes.v08.01/kukan/*.F90, cited per routine).  Each routine forwards
+! The modules below are named after the reference modules with a _hip suffix (so that both
+! can be linked into one program; the complex-path routines stay in the reference modules)
+! and carry the reference routine names with the reference argument lists
+! (RSPACEes.v08.01/kukan/*.F90, cited per routine).  Each routine forwards
 ! its arrays by address and its scalars by value to libkukan_hip.so through ISO_C_BINDING
 ! (include/kukan_hip.h) and maps a nonzero return code to the reference's abort path with
 ! the library's message (which repeats the reference's stopp text).
@@ -272,7 +274,7 @@ end module mod_kukan_tools
 
 
 ! ------------------------------------------------------------- overlap_finitedifference8f.F90
-module mod_overlap_finitedifference
+module mod_overlap_finitedifference_hip
   use kukan_hip_c
   implicit none
 contains
@@ -292,11 +294,11 @@ contains
     integer nperi, ncpa, ncpb, ncpc, nf, nf1, nsnd
     real*8 v_cc(-nf1:ncpa+nf, -nf1:ncpb+nf, -nf1:ncpc+nf)
   end subroutine
-end module mod_overlap_finitedifference
+end module mod_overlap_finitedifference_hip
 
 
 ! ------------------------------------------------------------- kslaplacian8f.F90:9-557
-module mod_kslaplacian
+module mod_kslaplacian_hip
   use kukan_hip_c
   implicit none
 contains
@@ -313,11 +315,11 @@ contains
     real*8, intent(inout) :: avre(ncpx, ncpy, ncpz, neigmx)
     call kukan_check(c_kslaplacian_r(ncpx, ncpy, ncpz, neigmx, nf, dx, dy, dz, vre, veff, l, avre))
   end subroutine
-end module mod_kslaplacian
+end module mod_kslaplacian_hip
 
 
 ! ------------------------------------------------------------- ksprecondition8f.F90:8-39
-module mod_ksprecondition
+module mod_ksprecondition_hip
   use kukan_hip_c
   implicit none
 contains
@@ -327,11 +329,11 @@ contains
     real*8, intent(out) :: avre(ncpx, ncpy, ncpz, ncol)
     call kukan_check(c_ksprecondition(ncpx, ncpy, ncpz, nf, ncol, vre, avre))
   end subroutine
-end module mod_ksprecondition
+end module mod_ksprecondition_hip
 
 
 ! ------------------------------------------------------------- nonlocaloperation8f.F90:9-235
-module mod_nonlocaloperation
+module mod_nonlocaloperation_hip
   use kukan_hip_c
   implicit none
 contains
@@ -365,11 +367,11 @@ contains
                      ncpy, ncpz, key_natpri_in, key_natpri_inps, dx, dy, dz, nprj, indspe, natinf, &
                      lstvec2, natpri, naps, vnlocp, rpsep, dijsss, avre, avr))
   end subroutine
-end module mod_nonlocaloperation
+end module mod_nonlocaloperation_hip
 
 
 ! ------------------------------------------------------------- kscg8f.F90:7-489
-module mod_kscg
+module mod_kscg_hip
   use kukan_hip_c
   implicit none
 contains
@@ -403,11 +405,11 @@ contains
          dij, veff, vnlocp, sss, ic, it, sval, rspsep, svecre, ssvre, residual_states, hsvre))
     ksconv = ic /= 0; ksitmax = it
   end subroutine
-end module mod_kscg
+end module mod_kscg_hip
 
 
 ! ------------------------------------------------------------- ksdiis8f.F90:8-712
-module mod_ksdiis
+module mod_ksdiis_hip
   use kukan_hip_c
   implicit none
 contains
@@ -444,11 +446,11 @@ contains
          residual_states, hsvre))
     ksconv = ic /= 0; ksitmax = it
   end subroutine
-end module mod_ksdiis
+end module mod_ksdiis_hip
 
 
 ! ------------------------------------------------------------- orthogonalization8f.F90:8-376
-module mod_orthogonalization
+module mod_orthogonalization_hip
   use kukan_hip_c
   implicit none
 contains
@@ -492,11 +494,11 @@ contains
          key_natpri_inps, dx, dy, dz, nprj, indspe, natpri, naps, natinf, lstvec2, latom, vnlocp, sss, &
          svecre, ssvre, rspsep))
   end subroutine
-end module mod_orthogonalization
+end module mod_orthogonalization_hip
 
 
 ! ------------------------------------------------------------- rayleigh-ritz8f.F90:9-125
-module mod_rayleighritz
+module mod_rayleighritz_hip
   use kukan_hip_c
   implicit none
 contains
@@ -528,11 +530,11 @@ contains
          dz, nprj, natinf, lstvec2, indspe, natpri, naps, ntyppp, dij, veff, vnlocp, l2i(l_rrb), rspsep, &
          svecre, ssvre, hsvre, sval))
   end subroutine
-end module mod_rayleighritz
+end module mod_rayleighritz_hip
 
 
 ! ------------------------------------------------------------- reorderstates8f.F90:8-124
-module mod_reorderstates
+module mod_reorderstates_hip
   use kukan_hip_c
   implicit none
 contains
@@ -551,11 +553,11 @@ contains
     call kukan_check(c_reorderstates(nrc, num_ppcell, nprjmx, nei000gmx, nums, ncol, numk, ncpx, ncpy, ncpz, &
          ns1, nk, sval, residual_states, rspsep, cspsep, svecre, sveccm, ssvre, ssvcm, hsvre, hsvcm))
   end subroutine
-end module mod_reorderstates
+end module mod_reorderstates_hip
 
 
 ! ------------------------------------------------------------- scf_diag8f.F90:6-316
-module mod_scf_diag
+module mod_scf_diag_hip
   use kukan_hip_c
   implicit none
 contains
@@ -615,11 +617,11 @@ contains
          residual_states, ic, it))
     ksconv = ic /= 0; ksitmax = it
   end subroutine
-end module mod_scf_diag
+end module mod_scf_diag_hip
 
 
 ! ------------------------------------------------------------- scf_hartree8f.F90:11-611
-module mod_scf_hartree
+module mod_scf_hartree_hip
   use kukan_hip_c
   implicit none
 contains
@@ -635,7 +637,7 @@ contains
     call kukan_check(c_scf_hartree(nd, ncpx_d, ncpy_d, ncpz_d, nfh, ndisp, nperi, ncgres, ncgmin, ncgmax, &
          xmax, ymax, zmax, epsvh, rho_aug_dense, boundx, boundy, boundz, vh_dense))
   end subroutine
-end module mod_scf_hartree
+end module mod_scf_hartree_hip
 
 
 ! ------------------------------------------------------------- scf_charge8f.F90:290-321

@@ -2,10 +2,10 @@
 ! (kukan8f.F90 / scf_potentials8f.F90 call sites) on a small periodic cell and writes the
 ! results as raw doubles for comparison with the oracle.
 program shim_driver
-  use mod_overlap_finitedifference, only: overlap_finitedifference_r, overlap_fdcheck_r
-  use mod_kslaplacian, only: kslaplacian_r
-  use mod_ksprecondition, only: ksprecondition
-  use mod_scf_hartree, only: scf_hartree
+  use mod_overlap_finitedifference_hip, only: overlap_finitedifference_r, overlap_fdcheck_r
+  use mod_kslaplacian_hip, only: kslaplacian_r
+  use mod_ksprecondition_hip, only: ksprecondition
+  use mod_scf_hartree_hip, only: scf_hartree
   use mod_kukan_tools, only: tools_correctpotential
   use kukan_hip_c, only: kukan_setup
   implicit none

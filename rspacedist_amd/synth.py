@@ -235,8 +235,11 @@ def vloc_grad_density(n, h, atoms, rho):
     return out
 
 
-def build_projectors(n, h, atoms, nperi=3):
-    """vnlocp, lstvec2, lstx/y/z, natx/y/z, natinf in the layout of pseudocalc8f.F90:954-1033."""
+def build_projectors(n, h, atoms, nperi=3, derivatives=False):
+    """vnlocp, lstvec2, lstx/y/z, natx/y/z, natinf in the layout of pseudocalc8f.F90:954-1033.
+
+    With ``derivatives``, also dvnlocp (num_list, 9, natom, 3) = d vnlocp / d R_x,y,z on the
+    same lists (a projector centred at R is p(r - R), so d/dR = -grad p)."""
     xmax = 0.5 * n * h
     nat = len(atoms)
     npmax = int(math.ceil(R_CUT / h)) + 1
@@ -271,15 +274,23 @@ def build_projectors(n, h, atoms, nperi=3):
     lstz = np.zeros_like(lstx)
     natinf = np.zeros(nat, dtype=np.int32)
     natxyz = np.zeros((3, nat), dtype=np.int32)
-    for ia, (lin, ix, iy, iz, pv, nat_i, _) in enumerate(lists):
+    dvnlocp = np.zeros((num_list, NPRJ, nat, 3), order="F") if derivatives else None
+    for ia, (lin, ix, iy, iz, pv, nat_i, disp) in enumerate(lists):
         m = len(lin)
         natinf[ia] = m
         lstvec2[:m, ia] = lin
         lstx[:m, ia], lsty[:m, ia], lstz[:m, ia] = ix, iy, iz
         vnlocp[:m, :, ia] = pv.T
         natxyz[:, ia] = nat_i
-    return dict(vnlocp=vnlocp, lstvec2=lstvec2, lstx=lstx, lsty=lsty, lstz=lstz,
-                natx=natxyz[0].copy(), naty=natxyz[1].copy(), natz=natxyz[2].copy(), natinf=natinf)
+        if derivatives:
+            g = projector_grad(disp[0], disp[1], disp[2]) * h ** 3  # (3, 9, m), d/d(r - R)
+            for d in range(3):
+                dvnlocp[:m, :, ia, d] = -g[d].T
+    out = dict(vnlocp=vnlocp, lstvec2=lstvec2, lstx=lstx, lsty=lsty, lstz=lstz,
+               natx=natxyz[0].copy(), naty=natxyz[1].copy(), natz=natxyz[2].copy(), natinf=natinf)
+    if derivatives:
+        out["dvnlocp"] = dvnlocp
+    return out
 
 
 def seeded_dmat(rng):

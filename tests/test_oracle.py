@@ -403,3 +403,83 @@ def test_force_eig_matches_analytic_projector_gradient(orc, symmetric):
                 ref[a, ax] = -2.0 * fnele[0] * (cp @ dmat @ dcp)
     got = np.stack([fx, fy, fz], axis=1)
     np.testing.assert_allclose(got, ref, rtol=2e-3, atol=2e-3 * np.abs(ref).max())
+
+
+# ------------------------------------------------------------------ differentiation pattern
+def _adj_system(seed=51, n=12, h=0.45, natom=3, nb=6):
+    s = synth.make_system(n, h, natom, nb, seed=seed)
+    psi = synth.smooth_bands(synth.random_bands(s), s.n, s.h)
+    f = np.linspace(1.0, 0.5, nb)
+    return s, np.asfortranarray(psi), f
+
+
+def _orc_band_energy(orc, s, psi, atoms, veff=None):
+    pr = synth.build_projectors(s.n, s.h, atoms)
+    kw = synth.abi_args(s)
+    kw.update(pr)
+    kw["num_list"] = pr["vnlocp"].shape[0]
+    if veff is not None:
+        kw["veff"] = veff
+    e = np.zeros(s.nbands)
+    orc.call("band_energy_r", **kw, svecre=psi, eband=e)
+    return e
+
+
+def test_band_energy_matches_dense(orc):
+    s, psi, _ = _adj_system()
+    e = _orc_band_energy(orc, s, psi, s.atoms)
+    Hd = H.dense_h(s)
+    ref = np.einsum("il,ij,jl->l", psi, Hd, psi) * s.dv
+    np.testing.assert_allclose(e, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_energy_adjoint_matches_central_differences(orc):
+    """dE/dR (projector part, veff fixed) against central differences of the oracle's own
+    band energy with the projectors rebuilt at R +- 1e-4; dE/dveff along a random direction
+    (E is linear in veff, so the difference quotient is exact up to rounding)."""
+    s, psi, f = _adj_system()
+    pr = synth.build_projectors(s.n, s.h, s.atoms, derivatives=True)
+    kw = synth.abi_args(s)
+    kw.update({k: v for k, v in pr.items() if k != "dvnlocp"})
+    kw["num_list"] = pr["vnlocp"].shape[0]
+    dedv = np.zeros(s.npts)
+    dedr = np.zeros((s.natom, 3), order="F")
+    orc.call("energy_adjoint_r", **kw, dvnlocp=pr["dvnlocp"], svecre=psi, fnele=f, dedv=dedv,
+             dedr=dedr)
+    np.testing.assert_allclose(dedv, (psi ** 2 @ f) * s.dv, rtol=1e-13, atol=1e-16)
+    hd = 1e-4
+    for a in range(s.natom):
+        for d in range(3):
+            Rp, Rm = s.atoms.copy(), s.atoms.copy()
+            Rp[a, d] += hd
+            Rm[a, d] -= hd
+            fd = (f @ _orc_band_energy(orc, s, psi, Rp) - f @ _orc_band_energy(orc, s, psi, Rm)) / (2 * hd)
+            assert abs(dedr[a, d] - fd) <= 1e-7 * max(1.0, abs(fd)), (a, d, dedr[a, d], fd)
+    rng = np.random.default_rng(3)
+    dv = rng.standard_normal(s.npts)
+    v0 = s.veff.reshape(-1, order="F")
+    ep = f @ _orc_band_energy(orc, s, psi, s.atoms, veff=np.asfortranarray(v0 + 1e-3 * dv))
+    em = f @ _orc_band_energy(orc, s, psi, s.atoms, veff=np.asfortranarray(v0 - 1e-3 * dv))
+    np.testing.assert_allclose((ep - em) / 2e-3, dedv @ dv, rtol=1e-9)
+
+
+def test_energy_adjoint_agrees_with_force_eig_r(orc):
+    """Translation invariance: <dp/dR|psi> = <p|grad psi>, so the adjoint's projector force
+    equals force_eig_r's (which differentiates psi by finite differences, force8f.F90:468-)
+    up to that derivative's discretisation error (smooth bands, symmetric D, S = 0)."""
+    s, psi, f = _adj_system(n=16, h=0.4, natom=3, nb=6)
+    pr = synth.build_projectors(s.n, s.h, s.atoms, derivatives=True)
+    kw = synth.abi_args(s)
+    kw.update({k: v for k, v in pr.items() if k != "dvnlocp"})
+    kw["num_list"] = pr["vnlocp"].shape[0]
+    dedv = np.zeros(s.npts)
+    dedr = np.zeros((s.natom, 3), order="F")
+    orc.call("energy_adjoint_r", **kw, dvnlocp=pr["dvnlocp"], svecre=psi, fnele=f, dedv=dedv,
+             dedr=dedr)
+    fx, fy, fz = np.zeros(s.natom), np.zeros(s.natom), np.zeros(s.natom)
+    orc.call("force_eig_r", **kw, svecre=psi, sval=np.zeros(s.nbands), fnele=f, fatx=fx,
+             faty=fy, fatz=fz)
+    fref = np.stack([fx, fy, fz], axis=1)
+    err = np.abs(-dedr - fref).max()
+    print(f"adjoint vs force_eig_r: max diff {err:.3e}, max |F| {np.abs(fref).max():.3e}")
+    assert err <= 2e-3 * np.abs(fref).max()

@@ -154,7 +154,16 @@ int kukan_memcpy(void *dst, const void *src, size_t bytes, int kind) {
                           : kind == 2 ? hipMemcpyDeviceToHost
                                       : hipMemcpyDeviceToDevice;
   if (kind < 1 || kind > 3) throw Error(1, "kukan_memcpy: kind must be 1, 2 or 3");
-  HIPCK(hipMemcpyAsync(dst, src, bytes, k, lib().stream));
+  if (kind == 3 && bytes % 8 == 0 && ((uintptr_t)dst | (uintptr_t)src) % 8 == 0 &&
+      is_device_ptr(dst) && is_device_ptr(src)) {
+    // device -> device through the library's own copy kernel (a known-traffic kernel:
+    // 8 bytes read and 8 written per element, used to calibrate HBM counters)
+    const long n = (long)(bytes / 8);
+    Timed t("copy", 16.0 * (double)n);
+    launch_copy2d(n, 1, (const double *)src, n, (double *)dst, n, lib().stream);
+  } else {
+    HIPCK(hipMemcpyAsync(dst, src, bytes, k, lib().stream));
+  }
   API_END
 }
 int kukan_fill_normal(double *dev, size_t n, uint64_t seed) {

@@ -628,31 +628,101 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const double *part, int np
 }
 
 // ----------------------------------------------------------------------------- elementwise
-__global__ void k_axpby(long n, int nb, const double *a, double ca, const double *x, long ldx,
-                        const double *bb, double cb, const double *y, long ldy, double *out,
-                        long ldo) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= n * nb) return;
-  const long i = t % n;
-  const int b = (int)(t / n);
+// Band-wise elementwise kernels: grid (chunks, bands), each work-group strides over its band,
+// so no per-element division by the band length.
+// work-groups per band: about 8192 in all, at least 2048 elements per work-group
+inline unsigned ew_chunks(long n, int nb) {
+  const long want = (8192 + nb - 1) / std::max(1, nb);
+  return (unsigned)std::max(1L, std::min(want, (n + 2047) / 2048));
+}
+
+__global__ __launch_bounds__(256) void k_axpby(long n, const double *a, double ca,
+                                               const double *x, long ldx, const double *bb,
+                                               double cb, const double *y, long ldy,
+                                               double *out, long ldo) {
+  const int b = blockIdx.y;
   const double fa = ca * (a ? a[b] : 1.0), fb = cb * (bb ? bb[b] : 1.0);
-  double v = fa * x[(long)b * ldx + i];
-  if (y) v += fb * y[(long)b * ldy + i];
-  out[(long)b * ldo + i] = v;
+  const double *xb = x + (long)b * ldx, *yb = y ? y + (long)b * ldy : nullptr;
+  double *ob = out + (long)b * ldo;
+  const long stride = (long)gridDim.x * 256;
+  if (yb) {
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += stride) ob[i] = fa * xb[i] + fb * yb[i];
+  } else {
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += stride) ob[i] = fa * xb[i];
+  }
 }
-__global__ void k_scale(long n, int nb, const double *f, double *x, long ldx) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= n * nb) return;
-  const long i = t % n;
-  const int b = (int)(t / n);
-  x[(long)b * ldx + i] *= f[b];
+__global__ __launch_bounds__(256) void k_scale(long n, const double *f, double *x, long ldx) {
+  const int b = blockIdx.y;
+  const double fb = f[b];
+  double *xb = x + (long)b * ldx;
+  const long stride = (long)gridDim.x * 256;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += stride) xb[i] *= fb;
 }
-__global__ void k_copy2d(long n, int nb, const double *x, long ldx, double *y, long ldy) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= n * nb) return;
-  const long i = t % n;
-  const int b = (int)(t / n);
-  y[(long)b * ldy + i] = x[(long)b * ldx + i];
+__global__ __launch_bounds__(256) void k_copy2d(long n, const double *x, long ldx, double *y,
+                                                long ldy) {
+  const int b = blockIdx.y;
+  const double *xb = x + (long)b * ldx;
+  double *yb = y + (long)b * ldy;
+  const long stride = (long)gridDim.x * 256;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += stride) yb[i] = xb[i];
+}
+
+// P = x + t_b P (t null: P = x); part1 = sum s.P, part2 = sum P.P per work-group (new P).
+__global__ __launch_bounds__(256) void k_pupdate_dots(long n, const double *x, long ld,
+                                                      const double *t, double *P,
+                                                      const double *sv, double *part1,
+                                                      double *part2, int nparts) {
+  __shared__ double red[4];
+  const int b = blockIdx.y;
+  const long chunk = (n + nparts - 1) / nparts;
+  const long i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  const double *xb = x + (long)b * ld, *sb = sv + (long)b * ld;
+  double *pb = P + (long)b * ld;
+  const double tb = t ? t[b] : 0.0;
+  double s1 = 0.0, s2 = 0.0;
+  for (long i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double v = t ? xb[i] + tb * pb[i] : xb[i];
+    pb[i] = v;
+    s1 += sb[i] * v;
+    s2 += v * v;
+  }
+  s1 = block_sum<256>(s1, red);
+  s2 = block_sum<256>(s2, red);
+  if (threadIdx.x == 0) {
+    part1[(long)b * nparts + blockIdx.x] = s1;
+    part2[(long)b * nparts + blockIdx.x] = s2;
+  }
+}
+
+// x = a_b x + c_b y and u = a_b u + c_b w; part1 = sum x.x, part2 = sum x.u (new values).
+__global__ __launch_bounds__(256) void k_pair_update_dots(long n, const double *a,
+                                                          const double *c, double *x,
+                                                          const double *y, double *u,
+                                                          const double *w, long ld,
+                                                          double *part1, double *part2,
+                                                          int nparts) {
+  __shared__ double red[4];
+  const int b = blockIdx.y;
+  const long chunk = (n + nparts - 1) / nparts;
+  const long i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  const double ab = a[b], cb = c[b];
+  double *xb = x + (long)b * ld, *ub = u + (long)b * ld;
+  const double *yb = y + (long)b * ld, *wb = w + (long)b * ld;
+  double s1 = 0.0, s2 = 0.0;
+  for (long i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double xv = ab * xb[i] + cb * yb[i];
+    const double uv = ab * ub[i] + cb * wb[i];
+    xb[i] = xv;
+    ub[i] = uv;
+    s1 += xv * xv;
+    s2 += xv * uv;
+  }
+  s1 = block_sum<256>(s1, red);
+  s2 = block_sum<256>(s2, red);
+  if (threadIdx.x == 0) {
+    part1[(long)b * nparts + blockIdx.x] = s1;
+    part2[(long)b * nparts + blockIdx.x] = s2;
+  }
 }
 __global__ void k_fill(long n, double v, double *x) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -1026,18 +1096,46 @@ void launch_axpby(long n, int nb, const double *a, double ca, const double *x, l
                   hipStream_t s) {
   if (n * nb == 0) return;
   Timed t("axpby", 8.0 * (double)n * nb * (y ? 3.0 : 2.0));
-  hipLaunchKernelGGL(k_axpby, dim3(nblk(n * nb, 256)), dim3(256), 0, s, n, nb, a, ca, x, ldx, b,
+  hipLaunchKernelGGL(k_axpby, dim3(ew_chunks(n, nb), nb), dim3(256), 0, s, n, a, ca, x, ldx, b,
                      cb, y, ldy, out, ldo);
   HIPCK(hipGetLastError());
 }
 void launch_scale(long n, int nb, const double *f, double *x, long ldx, hipStream_t s) {
   if (n * nb == 0) return;
-  hipLaunchKernelGGL(k_scale, dim3(nblk(n * nb, 256)), dim3(256), 0, s, n, nb, f, x, ldx);
+  hipLaunchKernelGGL(k_scale, dim3(ew_chunks(n, nb), nb), dim3(256), 0, s, n, f, x, ldx);
   HIPCK(hipGetLastError());
 }
 void launch_copy2d(long n, int nb, const double *x, long ldx, double *y, long ldy, hipStream_t s) {
   if (n * nb == 0) return;
-  hipLaunchKernelGGL(k_copy2d, dim3(nblk(n * nb, 256)), dim3(256), 0, s, n, nb, x, ldx, y, ldy);
+  hipLaunchKernelGGL(k_copy2d, dim3(ew_chunks(n, nb), nb), dim3(256), 0, s, n, x, ldx, y, ldy);
+  HIPCK(hipGetLastError());
+}
+void launch_pupdate_dots(long n, int nb, const double *x, const double *t, double *P, long ld,
+                         const double *sv, double scale, double *res1, double *res2,
+                         hipStream_t s) {
+  if (n * nb == 0) return;
+  const int np = dot_nparts(n);
+  Timed tt("axpby_dots", 8.0 * (double)n * nb * (t ? 4.0 : 3.0));
+  double *part = (double *)scratch("pud.part", sizeof(double) * (size_t)2 * np * nb);
+  hipLaunchKernelGGL(k_pupdate_dots, dim3(np, nb), dim3(256), 0, s, n, x, ld, t, P, sv, part,
+                     part + (long)np * nb, np);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part, np, scale, res1);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part + (long)np * nb, np, scale,
+                     res2);
+  HIPCK(hipGetLastError());
+}
+void launch_pair_update_dots(long n, int nb, const double *a, const double *c, double *x,
+                             const double *y, double *u, const double *w, long ld, double scale,
+                             double *res1, double *res2, hipStream_t s) {
+  if (n * nb == 0) return;
+  const int np = dot_nparts(n);
+  Timed tt("axpby_dots", 8.0 * (double)n * nb * 6.0);
+  double *part = (double *)scratch("pair.part", sizeof(double) * (size_t)2 * np * nb);
+  hipLaunchKernelGGL(k_pair_update_dots, dim3(np, nb), dim3(256), 0, s, n, a, c, x, y, u, w, ld,
+                     part, part + (long)np * nb, np);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part, np, scale, res1);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part + (long)np * nb, np, scale,
+                     res2);
   HIPCK(hipGetLastError());
 }
 void launch_fill(long n, double v, double *x, hipStream_t s) {

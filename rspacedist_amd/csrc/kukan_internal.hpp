@@ -199,6 +199,14 @@ void launch_axpby(long n, int nb, const double *a, double ca, const double *x, l
                   const double *b, double cb, const double *y, long ldy, double *out, long ldo,
                   hipStream_t s);  // out = (ca*a_b) x + (cb*b_b) y ; a/b may be null => 1
 void launch_scale(long n, int nb, const double *f, double *x, long ldx, hipStream_t s);
+// P = x + t_b P (t null: P = x); res1[b] = scale <sv_b|P_b>, res2[b] = scale <P_b|P_b>
+void launch_pupdate_dots(long n, int nb, const double *x, const double *t, double *P, long ld,
+                         const double *sv, double scale, double *res1, double *res2,
+                         hipStream_t s);
+// x = a_b x + c_b y, u = a_b u + c_b w; res1[b] = scale <x_b|x_b>, res2[b] = scale <x_b|u_b>
+void launch_pair_update_dots(long n, int nb, const double *a, const double *c, double *x,
+                             const double *y, double *u, const double *w, long ld, double scale,
+                             double *res1, double *res2, hipStream_t s);
 void launch_copy2d(long n, int nb, const double *x, long ldx, double *y, long ldy, hipStream_t s);
 void launch_fill(long n, double v, double *x, hipStream_t s);
 void launch_fill_normal(double *x, long n, unsigned long long seed, hipStream_t s);

@@ -851,6 +851,10 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
   double *rnew = sc, *rold = sc + M, *tb = sc + 2 * M, *dd = sc + 3 * M, *pp = sc + 4 * M;
   double *gg = sc + 5 * M, *fac = sc + 6 * M, *mdf = sc + 7 * M, *pap = sc + 8 * M;
   double *sap = sc + 9 * M, *al = sc + 10 * M, *be = sc + 11 * M;
+  // S = 1: fuse the direction update with its dot products and the band update with the next
+  // iteration's <psi|psi> and <psi|H psi> (same arithmetic, fewer passes over HBM)
+  const bool fused = c.q_zero;
+  double *eh_next = c.buf("bcg.eh", (size_t)M);
   int *active = (int *)scratch("bcg.active", sizeof(int) * M);
   std::vector<int> act_h(M, 1);
   HIPCK(hipMemcpyAsync(active, act_h.data(), sizeof(int) * M, hipMemcpyHostToDevice, c.s));
@@ -864,7 +868,12 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
     iiter++;
     const bool doreset = (iiter == 1) || (iiter == 10) || ((iiter - 1) % 200 == 0);
     HIPCK(hipMemcpyAsync(rold, rnew, sizeof(double) * M, hipMemcpyDeviceToDevice, c.s));
-    c.dots(a.psi, n, a.hpsi, n, M, a.sval);
+    // dV <psi|H psi>: from the previous iteration's fused band update when S = 1
+    if (fused && iiter > 1) {
+      HIPCK(hipMemcpyAsync(a.sval, eh_next, sizeof(double) * M, hipMemcpyDeviceToDevice, c.s));
+    } else {
+      c.dots(a.psi, n, a.hpsi, n, M, a.sval);
+    }
     // K r with r = -(H psi - eps S psi), and <r|Kr> (unprojected): residual measure and CG ratio
     {
       const int np = stencil_nparts(c.g, 1, M);
@@ -878,26 +887,37 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
     sub_nn(c, n, M, M, mone, a.psi, n, C, M, one, KR, n, s_psi);
     // P = Kr + beta P
     hipLaunchKernelGGL(k_ratio, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, rnew, rold, doreset ? 1 : 0, tb);
-    if (doreset) launch_copy2d(n, M, KR, n, PA, n, c.s);
-    else launch_axpby(n, M, nullptr, 1.0, KR, n, tb, 1.0, PA, n, PA, n, c.s);
-    // p = (P - d psi)/|.|, d = dV <S psi|P>
-    const double *SPA = PA;
-    if (!c.q_zero) {
-      c.proj(PA, n, M, rpp, n1);
-      c.spsi(PA, n, M, rpp, n1, SK, n);
-      SPA = SK;
+    if (fused) {
+      // P = Kr + beta P together with d = dV <S psi|P> and dV <P|P> (S = 1) in one pass
+      launch_pupdate_dots(n, M, KR, doreset ? nullptr : tb, PA, n, a.spsi, dv, dd, pp, c.s);
+      if (iiter == 1) c.dots(a.psi, n, a.spsi, n, M, gg);  // later: from the band update
+    } else {
+      if (doreset) launch_copy2d(n, M, KR, n, PA, n, c.s);
+      else launch_axpby(n, M, nullptr, 1.0, KR, n, tb, 1.0, PA, n, PA, n, c.s);
+      // p = (P - d psi)/|.|, d = dV <S psi|P>
+      const double *SPA = PA;
+      if (!c.q_zero) {
+        c.proj(PA, n, M, rpp, n1);
+        c.spsi(PA, n, M, rpp, n1, SK, n);
+        SPA = SK;
+      }
+      c.dots(a.spsi, n, PA, n, M, dd);
+      c.dots(PA, n, SPA, n, M, pp);
+      c.dots(a.psi, n, a.spsi, n, M, gg);
     }
-    c.dots(a.spsi, n, PA, n, M, dd);
-    c.dots(PA, n, SPA, n, M, pp);
-    c.dots(a.psi, n, a.spsi, n, M, gg);
     hipLaunchKernelGGL(k_pnorm, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, dd, pp, gg, fac, mdf, c.flag(1));
     launch_axpby(n, M, fac, 1.0, PA, n, mdf, 1.0, a.psi, n, KR, n, c.s);  // KR <- p
     c.proj(KR, n, M, rpp, n1);
     c.hpsi(a.ns, a.veffs, KR, n, M, rpp, n1, HP, n, false, pap);
     c.dots(a.psi, n, HP, n, M, sap);
     hipLaunchKernelGGL(k_kscg05, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, a.sval, pap, sap, active, al, be);
-    launch_axpby(n, M, al, 1.0, a.psi, n, be, 1.0, KR, n, a.psi, n, c.s);
-    launch_axpby(n, M, al, 1.0, a.hpsi, n, be, 1.0, HP, n, a.hpsi, n, c.s);
+    if (fused) {
+      // psi, H psi updated together, with dV <psi|psi> and dV <psi|H psi> of the new bands
+      launch_pair_update_dots(n, M, al, be, a.psi, KR, a.hpsi, HP, n, dv, gg, eh_next, c.s);
+    } else {
+      launch_axpby(n, M, al, 1.0, a.psi, n, be, 1.0, KR, n, a.psi, n, c.s);
+      launch_axpby(n, M, al, 1.0, a.hpsi, n, be, 1.0, HP, n, a.hpsi, n, c.s);
+    }
     if (a.spsi != a.psi) {
       if (c.q_zero) {
         HIPCK(hipMemcpyAsync(a.spsi, a.psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));

@@ -111,6 +111,23 @@ def run_mode(s, dev, st, kk, synth, precision, steps, it0):
     return dt, tim, hart, st["sval"].numpy()
 
 
+# timing tag -> kernel whose calibrated HBM bytes per launch the committed counter profile holds
+TRAFFIC_KERNEL = {"gemm_mixed_nn": "k_gemm_nn<3>", "gemm_mixed_tn": "k_gemm_tn<3>",
+                  "precond_resid": "k_stencil_lds<1, 2, true>", "gemm_scale": "k_colmax",
+                  "nl_expand": "k_expand_rows<16>", "nl_gather": "k_gather<9, 4>"}
+
+
+def measured_traffic(tag):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE profile (or None)."""
+    path = os.path.join(ROOT, "profiles", "scf_step_128cube_1024bands_hbm_traffic.json")
+    k = TRAFFIC_KERNEL.get(tag)
+    if not k or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        ent = json.load(fh)["kernels"].get(k)
+    return ent["hbm_bytes_per_launch"] if ent else None
+
+
 def roofline_entry(tim, tag, bound):
     ms, cnt, work = tim.get(tag, (0.0, 0, 0.0))
     if cnt == 0 or ms == 0:
@@ -119,12 +136,12 @@ def roofline_entry(tim, tag, bound):
     if bound == "hbm":
         ach = work / cnt / per_launch_s / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "kernel": tag,
+                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": measured_traffic(tag), "kernel": tag,
                 "avg_launch_ms": round(ms / cnt, 4), "launches": cnt}
     ach = work / cnt / per_launch_s / 1e12
     peak = PEAK_F16_TFS if tag.startswith("gemm_mixed") else PEAK_FP64_TFS
     return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(ach / peak, 4), "traffic": None, "kernel": tag,
+            "frac": round(ach / peak, 4), "traffic": measured_traffic(tag), "kernel": tag,
             "avg_launch_ms": round(ms / cnt, 4), "launches": cnt}
 
 

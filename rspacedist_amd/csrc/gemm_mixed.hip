@@ -294,13 +294,50 @@ __global__ __launch_bounds__(256) void k_nn_cscale(const double *C, long ldc, in
   }
 }
 
+// X'' = X diag(sx) split into hi/lo halves and transposed to [row][Kp] (k contiguous), rows
+// padded to a multiple of NN_BM and k to Kp with zeros.  Work-group: 64 rows x 64 k through
+// LDS; global reads are 64-row runs of one column, writes 16-byte runs of 8 k.
+constexpr int NN_BM = 256, NN_BN = 256;
+__global__ __launch_bounds__(256) void k_split_t(const double *X, long ldx, long R, int K,
+                                                 const double *sx, int Kp, _Float16 *xh,
+                                                 _Float16 *xl) {
+  __shared__ float tile[64][65];
+  const long r0 = (long)blockIdx.x * 64;
+  const int k0 = blockIdx.y * 64;
+  const int t = threadIdx.x, tr = t & 63, tk = t >> 6;
+  const long r = r0 + tr;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int k = k0 + tk + 4 * q;
+    float v = 0.f;
+    if (r < R && k < K) v = (float)(X[r + ldx * (long)k] * sx[k]);
+    tile[tk + 4 * q][tr] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const int unit = t + 256 * u;      // 64 rows x 8 groups of 8 k
+    const int rl = unit >> 3, kg = unit & 7;
+    half8 h, l;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      float f = tile[8 * kg + j][rl];
+      asm volatile("" : "+v"(f));
+      const _Float16 hh = (_Float16)f;
+      h[j] = hh;
+      l[j] = (_Float16)(f - (float)hh);
+    }
+    const long off = (r0 + rl) * Kp + k0 + 8 * kg;
+    *(half8 *)&xh[off] = h;
+    *(half8 *)&xl[off] = l;
+  }
+}
+
 struct NNArgs {
   long R;
-  int K, M2, Kp;
-  const double *X;
-  long ldx;
-  const double *sx;
-  const _Float16 *ch, *cl;
+  int M2, Kp;
+  const _Float16 *xh, *xl;  // [Rp][Kp]
+  const _Float16 *ch, *cl;  // [M2p][Kp]
   const double *tc;
   double alpha, beta;
   double *Y;
@@ -308,62 +345,56 @@ struct NNArgs {
   int row_blocks, col_tiles;
 };
 
-// Y tile (256 rows x 128 columns); the col_tiles of one row block run consecutively on one
-// XCD so that X is fetched from HBM once.  The epilogue goes through LDS so that each
-// column of Y is written (and, for beta != 0, read) as 512-byte runs.
+struct NNStage {
+  _Float16 ah[NN_BM * PITCH], al[NN_BM * PITCH], bh[NN_BN * PITCH], bl[NN_BN * PITCH];
+};
+
+// Y tile 256 rows x 256 columns, 8 waves of 64 x 128 (wave w: rows 64 (w & 3), columns
+// 128 (w >> 2)).  Operands arrive pre-split, so a stage is 8 16-byte loads per thread copied
+// to LDS (double-buffered, 32-deep K stages) and the main loop is MFMA plus LDS reads only.
+// The col_tiles of one row block run consecutively on one XCD so that X'' is fetched from HBM
+// once.  The epilogue goes through LDS so that each column of Y is written (and, for
+// beta != 0, read) as 512-byte runs.
 template <int NT>
 __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
-  __shared__ Stage S[2];
+  __shared__ NNStage S[2];
   const int L = blockIdx.x, xcd = L & 7, jj = L >> 3;
   const long rb = (long)(jj / p.col_tiles) * 8 + xcd;
   const int ct = jj % p.col_tiles;
   if (rb >= p.row_blocks) return;
-  const long r0 = rb * BM;
-  const int c0 = ct * BN;
+  const long r0 = rb * NN_BM;
+  const int c0 = ct * NN_BN;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
-  // X staging: row t % 256, k-half t / 256 (wave-uniform) -> 16 k values
-  const int xr = t & 255;
-  const int kh = __builtin_amdgcn_readfirstlane(t >> 8);
-  const long row = r0 + xr;
-  const bool rin = row < p.R;
-  // C'' staging: column t / 4, 8-half part t % 4
-  const int cc = t >> 2, cp = t & 3;
-  // gload only issues loads (row and k clamped into X); the scale (0 for rows past R and
-  // k past K) is applied in lstore, after the MFMAs of the current stage.
-  const double *xrow = p.X + min(row, p.R - 1);
-  double rx[16], sxv[16];
-  half8 rch, rcl;
-  int k0_cur = 0;
+  // staging: row (column) t / 2, k half 16 (t & 1)
+  const int sr = t >> 1, sk = 16 * (t & 1);
+  const _Float16 *pxh = p.xh + (r0 + sr) * p.Kp + sk, *pxl = p.xl + (r0 + sr) * p.Kp + sk;
+  const _Float16 *pch = p.ch + (long)(c0 + sr) * p.Kp + sk, *pcl = p.cl + (long)(c0 + sr) * p.Kp + sk;
+  half8 g[8];
   auto gload = [&](int k0) {
-    k0_cur = k0;
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int kc = min(k0 + 16 * kh + q, p.K - 1);
-      rx[q] = xrow[p.ldx * (long)kc];
-      sxv[q] = p.sx[kc];
-    }
-    const long off = (long)(c0 + cc) * p.Kp + k0 + 8 * cp;
-    rch = *(const half8 *)&p.ch[off];
-    rcl = *(const half8 *)&p.cl[off];
+    g[0] = *(const half8 *)(pxh + k0);
+    g[1] = *(const half8 *)(pxh + k0 + 8);
+    g[2] = *(const half8 *)(pxl + k0);
+    g[3] = *(const half8 *)(pxl + k0 + 8);
+    g[4] = *(const half8 *)(pch + k0);
+    g[5] = *(const half8 *)(pch + k0 + 8);
+    g[6] = *(const half8 *)(pcl + k0);
+    g[7] = *(const half8 *)(pcl + k0 + 8);
   };
-  auto lstore = [&](Stage &D) {
-#pragma unroll
-    for (int q = 0; q < 16; q++) rx[q] *= (rin && k0_cur + 16 * kh + q < p.K) ? sxv[q] : 0.0;
-    half8 h, l;
-    split8f(rx, 1.0, h, l);
-    *(half8 *)&D.ah[xr * PITCH + 16 * kh] = h;
-    *(half8 *)&D.al[xr * PITCH + 16 * kh] = l;
-    split8f(rx + 8, 1.0, h, l);
-    *(half8 *)&D.ah[xr * PITCH + 16 * kh + 8] = h;
-    *(half8 *)&D.al[xr * PITCH + 16 * kh + 8] = l;
-    *(half8 *)&D.bh[cc * PITCH + 8 * cp] = rch;
-    *(half8 *)&D.bl[cc * PITCH + 8 * cp] = rcl;
+  auto lstore = [&](NNStage &D) {
+    *(half8 *)&D.ah[sr * PITCH + sk] = g[0];
+    *(half8 *)&D.ah[sr * PITCH + sk + 8] = g[1];
+    *(half8 *)&D.al[sr * PITCH + sk] = g[2];
+    *(half8 *)&D.al[sr * PITCH + sk + 8] = g[3];
+    *(half8 *)&D.bh[sr * PITCH + sk] = g[4];
+    *(half8 *)&D.bh[sr * PITCH + sk + 8] = g[5];
+    *(half8 *)&D.bl[sr * PITCH + sk] = g[6];
+    *(half8 *)&D.bl[sr * PITCH + sk + 8] = g[7];
   };
-  f16v acc[2][2];
+  f16v acc[2][4];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < 4; b++)
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
   const int nst = p.Kp / BK;
@@ -373,7 +404,32 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
   for (int st = 0; st < nst; st++) {
     const bool more = st + 1 < nst;
     if (more) gload((st + 1) * BK);
-    mma_stage<NT>(S[st & 1], lane, wi, wj, acc);
+    const NNStage &C = S[st & 1];
+#pragma unroll
+    for (int ss = 0; ss < 2; ss++) {
+      const int ko = 16 * ss + 8 * (lane >> 5);
+      half8 ah[2], al[2];
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int row = 64 * wi + 32 * a + (lane & 31);
+        ah[a] = *(const half8 *)&C.ah[row * PITCH + ko];
+        al[a] = *(const half8 *)&C.al[row * PITCH + ko];
+      }
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const int col = 128 * wj + 32 * b + (lane & 31);
+        const half8 bh = *(const half8 *)&C.bh[col * PITCH + ko];
+        const half8 bl = *(const half8 *)&C.bl[col * PITCH + ko];
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bh, acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bl, acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bh, acc[a][b], 0, 0, 0);
+          if (NT == 4)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bl, acc[a][b], 0, 0, 0);
+        }
+      }
+    }
     if (more) lstore(S[(st + 1) & 1]);
     __syncthreads();
   }
@@ -383,7 +439,7 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
   float *W = reinterpret_cast<float *>(&S[0]) + w * 32 * EP;
   const long m = r0 + 64 * wi + lane;
 #pragma unroll
-  for (int b = 0; b < 2; b++) {
+  for (int b = 0; b < 4; b++) {
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -393,7 +449,7 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
       }
     __syncthreads();
     for (int q = 0; q < 32; q++) {
-      const int n = c0 + 64 * wj + 32 * b + q;
+      const int n = c0 + 128 * wj + 32 * b + q;
       if (n < p.M2 && m < p.R) {
         const double v = p.alpha / p.tc[n] * (double)W[q * EP + lane];
         double *y = p.Y + m + p.ldy * (long)n;
@@ -452,20 +508,29 @@ void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ld
                    const double *sx, const double *C, long ldc, double beta, double *Y, long ldy,
                    int nterms, hipStream_t s) {
   if (R <= 0 || M2 <= 0 || K <= 0) throw Error(1, "gemm_mixed_nn: empty product");
-  const int Kp = (K + BK - 1) / BK * BK;
-  const int M2p = (M2 + BN - 1) / BN * BN;
+  const int Kp = (K + 63) / 64 * 64;  // whole 64-k split tiles, whole 32-k stages
+  const int M2p = (M2 + NN_BN - 1) / NN_BN * NN_BN;
+  const long Rp = (R + NN_BM - 1) / NN_BM * NN_BM;
   _Float16 *ch = (_Float16 *)scratch("gemm.nnch", sizeof(_Float16) * (size_t)Kp * M2p);
   _Float16 *cl = (_Float16 *)scratch("gemm.nncl", sizeof(_Float16) * (size_t)Kp * M2p);
+  _Float16 *xh = (_Float16 *)scratch("gemm.nnxh", sizeof(_Float16) * (size_t)Kp * Rp);
+  _Float16 *xl = (_Float16 *)scratch("gemm.nnxl", sizeof(_Float16) * (size_t)Kp * Rp);
   double *tc = (double *)scratch("gemm.nntc", sizeof(double) * M2p);
   hipLaunchKernelGGL(k_nn_cscale, dim3(M2), dim3(256), 0, s, C, ldc, K, M2, sx, tc);
   hipLaunchKernelGGL(k_nn_prep, dim3(nblk((long)Kp * M2p, 256)), dim3(256), 0, s, C, ldc, K, M2, sx,
                      tc, Kp, M2p, ch, cl);
   HIPCK(hipGetLastError());
+  {
+    Timed t("gemm_split", 8.0 * (double)R * K + 4.0 * (double)Rp * Kp);
+    hipLaunchKernelGGL(k_split_t, dim3((unsigned)(Rp / 64), (unsigned)(Kp / 64)), dim3(256), 0, s, X,
+                       ldx, R, K, sx, Kp, xh, xl);
+    HIPCK(hipGetLastError());
+  }
   NNArgs p{};
-  p.R = R; p.K = K; p.M2 = M2; p.Kp = Kp; p.X = X; p.ldx = ldx; p.sx = sx; p.ch = ch; p.cl = cl;
+  p.R = R; p.M2 = M2; p.Kp = Kp; p.xh = xh; p.xl = xl; p.ch = ch; p.cl = cl;
   p.tc = tc; p.alpha = alpha; p.beta = beta; p.Y = Y; p.ldy = ldy;
-  p.row_blocks = (int)((R + BM - 1) / BM);
-  p.col_tiles = M2p / BN;
+  p.row_blocks = (int)(Rp / NN_BM);
+  p.col_tiles = M2p / NN_BN;
   const long groups = (p.row_blocks + 7) / 8;
   const unsigned grid = (unsigned)(groups * 8 * p.col_tiles);
   Timed t("gemm_mixed_nn", 2.0 * (nterms >= 4 ? 4 : 3) * (double)R * M2 * K);  // f16 MFMA flops

@@ -48,6 +48,7 @@ void *scratch(const std::string &tag, size_t bytes) {
   auto &m = lib().scratch;
   auto it = m.find(tag);
   if (it != m.end() && it->second.second >= bytes) return it->second.first;
+  if (lib().capturing) throw Error(1, "scratch allocation inside stream capture: " + tag);
   if (it != m.end()) {
     HIPCK(hipStreamSynchronize(lib().stream));
     HIPCK(hipFree(it->second.first));
@@ -95,7 +96,7 @@ std::map<std::string, TimingAcc> &timing_table() {
 }
 
 Timed::Timed(const char *tag, double work) : tag_(tag), work_(work) {
-  if (param("timing", 0.0) < 1.0) return;
+  if (param("timing", 0.0) < 1.0 || lib().capturing) return;
   a_ = get_event();
   b_ = get_event();
   HIPCK(hipEventRecord(a_, lib().stream));

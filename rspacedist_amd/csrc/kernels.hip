@@ -45,6 +45,7 @@ struct SP {
   const double *veff;
   double cx[9], cy[9], cz[9], c0;
   int accumulate;
+  const int *skip;
   const int *row_ptr;
   const int2 *ent;
   const double *vnl;
@@ -79,6 +80,7 @@ __device__ inline double val(const SP &p, const double *base, int x, int y, int 
 template <int NF>
 __global__ __launch_bounds__(256) void k_stencil(SP p) {
   __shared__ double red[4];
+  if (p.skip && *p.skip) return;  // uniform: a device-side loop has finished
   const int nxb = (p.nx + 63) / 64;
   const int x = (blockIdx.x % nxb) * 64 + threadIdx.x;
   const int y = blockIdx.y * 4 + threadIdx.y;
@@ -152,6 +154,7 @@ struct SL {
   const double *veff;
   double cx[9], cy[9], cz[9], c0;
   int accumulate;
+  const int *skip;
   const int *row_ptr;
   const int2 *ent;
   const double *vnl;
@@ -173,6 +176,7 @@ template <int NF, int P, bool RES>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && P <= 2 ? 4 : 2))) void k_stencil_lds(SL p) {
   extern __shared__ double lds[];
   __shared__ double red[8];
+  if (p.skip && *p.skip) return;  // uniform: a device-side loop has finished
   constexpr int H = NF;  // halo values per thread: 2 NF rows x nx <= 512 NF for nx <= 256
   const int nx = p.nx, ny = p.ny, nz = p.nz, TY = p.TY, W = nx + 2 * NF;
   const long plane = (long)nx * ny;
@@ -824,14 +828,19 @@ static LdsPlan lds_plan(const Grid &g, int NF, int nb) {
   int pp = (int)param("stencil_p", 2.0);
   pp = pp >= 4 ? 4 : (pp >= 2 ? 2 : 1);
   if (NF > 4) pp = 1;  // the z-column of a wide stencil leaves registers for one point
+  // few bands (a Poisson solve): one point per thread and short z-ranges, for parallelism
+  const bool few = (long)nb * ((g.ny * g.nx + 512 * pp - 1) / (512 * pp)) < 1024;
+  if (few) pp = 1;
   L.TY = std::max(1, std::min(g.ny, 512 * pp / g.nx));
   L.ntile = (g.ny + L.TY - 1) / L.TY;
   L.P = (g.nx * L.TY + 511) / 512;
   if (L.P == 3) L.P = 4;
-  // z-ranges: enough blocks to fill the chip, each range at least 4*NF planes
-  long blocks = (long)nb * L.ntile;
+  // z-ranges: enough blocks to fill the chip, each range at least 4 NF planes (2 NF planes
+  // for few bands: the extra warm-up planes are re-read from cache)
+  const long blocks = (long)nb * L.ntile;
+  const int minz = few ? 2 * NF : 4 * NF;
   int nz_ch = 1;
-  while (blocks * nz_ch < 2048 && g.nz / (nz_ch * 2) >= 4 * NF) nz_ch *= 2;
+  while (blocks * nz_ch < 2048 && g.nz / (nz_ch * 2) >= minz) nz_ch *= 2;
   L.zc = (g.nz + nz_ch - 1) / nz_ch;
   L.nzch = (g.nz + L.zc - 1) / L.zc;
   return L;
@@ -907,6 +916,7 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
   }
   p.c0 = p.cx[0] + p.cy[0] + p.cz[0];
   p.accumulate = c.accumulate ? 1 : 0;
+  p.skip = c.skip;
   if (c.pl && c.coef && c.pl->nent > 0) {
     if (c.coef_band_fastest != stencil_band_fastest_coef(g, c.halo_in))
       throw Error(1, "stencil: projector coefficients in the wrong layout");
@@ -919,7 +929,7 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
     q.nx = g.nx; q.ny = g.ny; q.nz = g.nz; q.px = p.px; q.py = p.py; q.pz = p.pz;
     q.in = c.in; q.ld_in = c.ld_in; q.out = c.out; q.ld_out = c.ld_out; q.veff = c.veff;
     for (int m = 0; m < 9; m++) { q.cx[m] = p.cx[m]; q.cy[m] = p.cy[m]; q.cz[m] = p.cz[m]; }
-    q.c0 = p.c0; q.accumulate = p.accumulate;
+    q.c0 = p.c0; q.accumulate = p.accumulate; q.skip = p.skip;
     q.row_ptr = p.row_ptr; q.ent = p.ent; q.vnl = p.vnl; q.coef = p.coef; q.ld_coef = p.ld_coef;
     q.num_list = p.num_list; q.nprjmx = p.nprjmx; q.iprj_slot = p.iprj_slot;
     q.dotp = c.dot_partial;

@@ -38,6 +38,7 @@ struct Lib {
   std::map<const void *, Mirror> bound;
   // grow-only scratch buffers keyed by tag
   std::map<std::string, std::pair<void *, size_t>> scratch;
+  bool capturing = false;  // stream capture in progress: no timing events, no allocation
 };
 Lib &lib();
 void ensure_init();
@@ -149,6 +150,7 @@ struct StencilCall {
   double *dot_partial = nullptr;     // optional: per-block partial sums of in_b . out_b
   long dot_ld = 0;                   // band stride of dot_partial (0: number of parts)
   int nfh_coeffs = 0;                // if > 0: use this FD order (Hartree), no veff
+  const int *skip = nullptr;         // device flag: the launch does nothing while *skip != 0
 };
 void launch_stencil(const StencilCall &c, hipStream_t s);
 // true when launch_stencil takes the tiled path, whose projector pass reads band-fastest coef

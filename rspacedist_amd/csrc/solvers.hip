@@ -410,9 +410,32 @@ __global__ void k_sub_scalar(long n, const double *s, double f, double *x) {
   if (i < n) x[i] = x[i] - s[0] * f;
 }
 // Hartree CG updates (scf_hartree8f.F90:849-891); scalars on the device: sc[0]=r1r, sc[1]=pap
+// Device-side Hartree CG loop state: st[0] = done, st[1] = iterations started.  Every kernel
+// of an iteration returns at once when done is set, so a block of iterations can be launched
+// (or replayed as a graph) past convergence without changing the result.
+__global__ void k_hcg_reset(int *st) {
+  st[0] = 0;
+  st[1] = 0;
+}
+// iteration start: count it; r1r of this iteration is r2r of the previous one
+__global__ void k_hcg_begin(double *sc, int *st) {
+  if (st[0]) return;
+  st[1] += 1;
+  if (st[1] > 1) sc[0] = sc[2];
+}
+// stopping rule of scf_hartree8f.F90 (ncgmin, ncgmax, dp = sqrt(r.r dV / Omega) <= epsvh)
+__global__ void k_hcg_check(const double *sc, double dvo, double eps, int ncgmin, int ncgmax,
+                            int *st) {
+  if (st[0]) return;
+  const int loop = st[1];
+  const double dp = sqrt(sc[2] * dvo);
+  if (loop >= ncgmin && (loop >= ncgmax || dp <= eps)) st[0] = 1;
+}
 __global__ void k_hcg_update(long n, const double *sc, double *vh, const double *p, const double *ap,
-                             double *avhs, const double *b, double *r, double *part, int nparts) {
+                             double *avhs, const double *b, double *r, double *part, int nparts,
+                             const int *st) {
   __shared__ double red[4];
+  if (st[0]) return;
   const double alph = sc[0] / sc[1];
   const long chunk = (n + nparts - 1) / nparts;
   const long i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
@@ -429,10 +452,11 @@ __global__ void k_hcg_update(long n, const double *sc, double *vh, const double 
   __syncthreads();
   if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
-__global__ void k_hcg_dir(long n, const double *r2, const double *r1, int reset, const double *r,
-                          double *p) {
+__global__ void k_hcg_dir(long n, const double *r2, const double *r1, int ncgres, const double *r,
+                          double *p, const int *st) {
+  if (st[0]) return;
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const double beta = reset ? 0.0 : r2[0] / r1[0];
+  const double beta = (st[1] % ncgres == 0) ? 0.0 : r2[0] / r1[0];
   if (i < n) p[i] = r[i] + beta * p[i];
 }
 
@@ -1088,28 +1112,80 @@ int hartree(const HartreeArgs &a) {
   HIPCK(hipMemcpyAsync(avhs, ap, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   launch_axpby(n, 1, nullptr, 1.0, b, n, nullptr, -1.0, ap, n, r, n, s);
   HIPCK(hipMemcpyAsync(p, r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  int loop = 0;
-  double hsc[4];
-  for (;;) {
-    loop++;
-    launch_dots(n, r, n, r, n, 1, 1.0, sc + 0, s);  // r1r
-    st.in = p; st.out = ap; st.dot_partial = part;
+  // CG iterations run as blocks of `blk` on the device (scf_hartree8f.F90:611-1143): the
+  // stopping rule is evaluated on the device after every iteration and later iterations of
+  // the block do nothing, so the result is that of the host-driven loop; the host reads the
+  // state once per block.  With param "hartree_graph" (default 1) a block is captured once
+  // as a HIP graph and replayed.
+  int *state = (int *)scratch("hart.state", sizeof(int) * 2);
+  hipLaunchKernelGGL(k_hcg_reset, dim3(1), dim3(1), 0, s, state);
+  launch_dots(n, r, n, r, n, 1, 1.0, sc + 0, s);  // r1r of iteration 1
+  st.in = p; st.out = ap; st.dot_partial = part; st.skip = state;
+  const double dvo = g.dv() / omega;
+  const int blk = std::max(1, (int)param("hartree_block", 16.0));
+  auto iteration = [&]() {
+    hipLaunchKernelGGL(k_hcg_begin, dim3(1), dim3(1), 0, s, sc, state);
     launch_stencil(st, s);
     launch_reduce_partials(part, np, 1, 1.0, sc + 1, s);  // pap
     hipLaunchKernelGGL(k_hcg_update, dim3(npu), dim3(256), 0, s, n, sc, a.vh, p, ap, avhs, b, r,
-                       part, npu);
+                       part, npu, state);
     launch_reduce_partials(part, npu, 1, 1.0, sc + 2, s);  // r2r
     hipLaunchKernelGGL(k_hcg_dir, dim3(nblk(n, 256)), dim3(256), 0, s, n, sc + 2, sc + 0,
-                       (loop % a.ncgres == 0) ? 1 : 0, r, p);
-    HIPCK(hipMemcpyAsync(hsc, sc, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
-    sync();
-    const double dp = std::sqrt(hsc[2] * g.dv() / omega);
-    if (loop < a.ncgmin) continue;
-    if (loop >= a.ncgmax) break;
-    if (dp > a.epsvh) continue;
-    break;
+                       a.ncgres, r, p, state);
+    hipLaunchKernelGGL(k_hcg_check, dim3(1), dim3(1), 0, s, sc, dvo, a.epsvh, a.ncgmin, a.ncgmax,
+                       state);
+  };
+  // a captured block is valid for the same buffers, sizes and stopping parameters
+  struct Key {
+    std::vector<const void *> ptrs;
+    std::vector<long> ints;
+    std::vector<double> reals;
+    bool operator==(const Key &o) const {
+      return ptrs == o.ptrs && ints == o.ints && reals == o.reals;
+    }
+  };
+  struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    Key key;
+  };
+  static GraphCache gc;
+  const bool use_graph = param("hartree_graph", 1.0) >= 1.0;
+  if (use_graph) {
+    const Key key{{a.vh, b, r, p, ap, avhs, sc, part, state},
+                  {n, a.nx, a.ny, a.nz, a.nperi, a.nfh, blk, a.ncgres, a.ncgmin, a.ncgmax, np, npu},
+                  {a.epsvh, dvo, g.dx, g.dy, g.dz}};
+    if (!(gc.exec && gc.key == key)) {
+      if (gc.exec) HIPCK(hipGraphExecDestroy(gc.exec));
+      gc.exec = nullptr;
+      hipGraph_t graph;
+      lib().capturing = true;
+      try {
+        HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        for (int it = 0; it < blk; it++) iteration();
+        HIPCK(hipStreamEndCapture(s, &graph));
+      } catch (...) {
+        lib().capturing = false;
+        hipGraph_t g2;
+        (void)hipStreamEndCapture(s, &g2);
+        throw;
+      }
+      lib().capturing = false;
+      HIPCK(hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0));
+      HIPCK(hipGraphDestroy(graph));
+      gc.key = key;
+    }
   }
-  return loop;
+  int hst[2] = {0, 0};
+  for (;;) {
+    if (use_graph) HIPCK(hipGraphLaunch(gc.exec, s));
+    else
+      for (int it = 0; it < blk; it++) iteration();
+    HIPCK(hipMemcpyAsync(hst, state, sizeof(hst), hipMemcpyDeviceToHost, s));
+    sync();
+    if (hst[0]) break;
+    if (hst[1] >= a.ncgmax) break;
+  }
+  return hst[1];
 }
 
 void correct_potential(long n, double dvol_over_omega, double *vh) {

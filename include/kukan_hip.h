@@ -247,6 +247,28 @@ int kukan_tools_correctpotential(int ncpx_d, int ncpy_d, int ncpz_d, double xmax
 int kukan_scf_charge_smt_r(int neigmx, int nums, int nspv, int numk, int ncpx, int ncpy, int ncpz,
                            const double *fnele, const double *svecre, double *rhosmt);
 
+/* Batching pattern (no reference counterpart): independent periodic cubic systems of different
+ * sizes advanced together.  A step is, per system, H psi for all bands, one preconditioned
+ * steepest-descent band update with a 2x2 Rayleigh-Ritz per band (the reset iteration of
+ * kscg_r, kscg8f.F90:98-472, without Gram-Schmidt against lower bands), then the Hartree CG
+ * of rho = sum psi^2 to epsvh (scf_hartree8f.F90, warm-started from the previous step).
+ * batched = 1: one persistent stencil launch per pass over all systems, band and Hartree work
+ * over global tables, HIP-graph replays; batched = 0: the same kernels system by system.  Both
+ * give bit-identical results.  Systems: grid n^3 (8 <= n <= 256), spacing h, nf = nfh = 4,
+ * 9 projectors per atom in vnlocp(num_list, 9, natom) / lstvec2(num_list, natom) / natinf,
+ * dij(9, 9, natom), veff(n^3), svecre(n^3, nbands) orthonormal (copied in).  The tile shape
+ * and persistent grid are chosen on the first step by timing candidates
+ * (kukan_get_param("batch_tile_p"), "batch_grid"). */
+int kukan_batch_create(void **handle);
+int kukan_batch_add_system(void *handle, int n, double h, int nbands, int natom, int num_list,
+                           const int *lstvec2, const int *natinf, const double *vnlocp,
+                           const double *dij, const double *veff, const double *svecre,
+                           int *index);
+int kukan_batch_step(void *handle, int batched, int ncgmax, double epsvh);
+int kukan_batch_get(void *handle, int index, double *svecre, double *sval, double *vh,
+                    int *hartree_iterations);
+int kukan_batch_destroy(void *handle);
+
 /* Differentiation pattern (no reference counterpart; used by the torch.autograd harness in
  * rspacedist_amd/autograd.py).  kukan_band_energy_r: eband[l] = dV <psi_l|H|psi_l> with
  * H = T + veff + V_nl(dij) for the real path (nums = numk = 1), the same operator kscg_r

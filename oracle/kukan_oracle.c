@@ -1481,6 +1481,77 @@ int orc_energy_adjoint_r(int nperi, int neigmx, int nf, int ncpx, int ncpy, int 
   return 0;
 }
 
+/* ------------------------------------------------------------------ batching pattern
+ * No reference counterpart.  One system's band step of the batched path: per band, the reset
+ * iteration of kscg_r (kscg8f.F90:98-472) without the Gram-Schmidt against lower bands:
+ * eps = dV <psi|H psi>, p = K r with r = -(H psi - eps psi), p made orthogonal to psi and
+ * normalised, then the 2x2 Rayleigh-Ritz of (psi, p) (kscg_r_05, :1197-1230) with the lower
+ * eigenvector's first coefficient non-negative.  Cubic periodic grid, nf = 4, one species
+ * of 9 projectors, every atom on this rank, S = 1. */
+int orc_band_sd_step(int n, double h, int nbands, int natom, int num_list, const int *lstvec2,
+                     const int *natinf, const double *vnlocp, const double *dij,
+                     const double *veff, double *svecre, double *sval) {
+  ctx_t c;
+  int nprj[1] = {9};
+  int *indspe = (int *)xcalloc(natom, sizeof(int)), *natpri = (int *)xcalloc(natom, sizeof(int));
+  int *naps = (int *)xcalloc(natom, sizeof(int));
+  for (int a = 0; a < natom; a++) {
+    indspe[a] = 1;
+    natpri[a] = 2;
+    naps[a] = a + 1;
+  }
+  const double xm = 0.5 * n * h;
+  adj_ctx(&c, 3, 4, n, n, n, num_list, natom, 1, natom, 9, 2, 1, lstvec2, nprj, natpri, naps,
+          indspe, natinf, xm, xm, xm, vnlocp, dij, veff);
+  const long N = c.n;
+  double *rp = (double *)xcalloc(c.n1, sizeof(double)), *hp = (double *)xcalloc(N, sizeof(double));
+  double *p = (double *)xcalloc(N, sizeof(double)), *r = (double *)xcalloc(N, sizeof(double));
+  double *hq = (double *)xcalloc(N, sizeof(double));
+  int rc = 0;
+  for (int l = 0; l < nbands && !rc; l++) {
+    double *psi = svecre + (size_t)N * l;
+    proj(&c, psi, rp);
+    rc = apply_h(&c, 1, 1, psi, rp, hp);
+    if (rc) break;
+    const double eps = dot(N, psi, hp) * c.dv;
+    for (long i = 0; i < N; i++) r[i] = -(hp[i] - eps * psi[i]);
+    precond(&c, r, p);
+    const double d = dot(N, psi, p) * c.dv, pp = dot(N, p, p) * c.dv;
+    const double n2 = pp - d * d, f = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+    for (long i = 0; i < N; i++) p[i] = f * (p[i] - d * psi[i]);
+    proj(&c, p, rp);
+    rc = apply_h(&c, 1, 1, p, rp, hq);
+    if (rc) break;
+    const double pap = dot(N, p, hq) * c.dv, sap = dot(N, psi, hq) * c.dv;
+    const double g = 0.5 * (eps + pap) - sqrt(0.25 * (eps - pap) * (eps - pap) + sap * sap);
+    double a, b;
+    if (fabs(eps - g) >= fabs(pap - g)) {
+      a = -sap;
+      b = eps - g;
+    } else {
+      a = pap - g;
+      b = -sap;
+    }
+    double nn = sqrt(a * a + b * b);
+    if (nn == 0.0) {
+      a = 1.0;
+      b = 0.0;
+      nn = 1.0;
+    }
+    a /= nn;
+    b /= nn;
+    if (a < 0.0) {
+      a = -a;
+      b = -b;
+    }
+    for (long i = 0; i < N; i++) psi[i] = a * psi[i] + b * p[i];
+    sval[l] = g;
+  }
+  free(rp); free(hp); free(p); free(r); free(hq); free(indspe); free(natpri); free(naps);
+  ctx_free(&c);
+  return rc;
+}
+
 /* ------------------------------------------------------------------ totalenergy8f.F90:151-199 */
 int orc_totalenergy_eig(int nums, int ncol, int numk, int neigmx, const int *nwskk, int nwskptot,
                         double tf, const double *sval, const double *fnele, double *eneel,

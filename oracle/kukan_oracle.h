@@ -141,6 +141,9 @@ int orc_energy_adjoint_r(int nperi, int neigmx, int nf, int ncpx, int ncpy, int 
                          const int *natinf, double xmax, double ymax, double zmax,
                          const double *vnlocp, const double *dvnlocp, const double *dij,
                          const double *svecre, const double *fnele, double *dedv, double *dedr);
+int orc_band_sd_step(int n, double h, int nbands, int natom, int num_list, const int *lstvec2,
+                     const int *natinf, const double *vnlocp, const double *dij,
+                     const double *veff, double *svecre, double *sval);
 int orc_totalenergy_eig(int nums, int ncol, int numk, int neigmx, const int *nwskk, int nwskptot,
                         double tf, const double *sval, const double *fnele, double *eneel,
                         double *eneth);

@@ -12,6 +12,12 @@ import subprocess
 from rspacedist_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+# routines only the oracle has: the one-system restatement of the batched band step
+SIGNATURES = dict(abi.SIGNATURES)
+SIGNATURES["band_sd_step"] = [("n", "i"), ("h", "d"), ("nbands", "i"), ("natom", "i"),
+                              ("num_list", "i"), ("lstvec2", "I"), ("natinf", "I"),
+                              ("vnlocp", "D"), ("dij", "D"), ("veff", "D"), ("svecre", "D"),
+                              ("sval", "D")]
 LIB = os.path.join(HERE, "liboracle.so")
 _lib = None
 
@@ -28,7 +34,7 @@ def lib():
         _lib = ctypes.CDLL(LIB)
         _lib.orc_last_error.restype = ctypes.c_char_p
         _lib.orc_dsyev.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
-        abi.bind(_lib, "orc_")
+        abi.bind(_lib, "orc_", SIGNATURES)
     return _lib
 
 
@@ -38,7 +44,7 @@ def last_error():
 
 def call(name, **kw):
     """Call orc_<name> with reference-named keyword arguments; raises on a nonzero code."""
-    return abi.call(lib(), "orc_", name, err=last_error, **kw)
+    return abi.call(lib(), "orc_", name, err=last_error, _table=SIGNATURES, **kw)
 
 
 def dsyev(a):

@@ -166,9 +166,9 @@ _CT = {"i": ctypes.c_int, "d": ctypes.c_double, "I": ctypes.c_void_p, "D": ctype
        "od": ctypes.POINTER(ctypes.c_double)}
 
 
-def bind(lib, prefix):
-    """Attach prototypes for every routine of SIGNATURES found in lib."""
-    for name, args in SIGNATURES.items():
+def bind(lib, prefix, table=None):
+    """Attach prototypes for every routine of SIGNATURES (or ``table``) found in lib."""
+    for name, args in (table or SIGNATURES).items():
         fn = getattr(lib, prefix + name, None)
         if fn is None:
             continue
@@ -202,11 +202,11 @@ class Outs(dict):
     """Output scalars of a call (ksconv, ksitmax, eneel, ...)."""
 
 
-def call(lib, prefix, name, err=None, **kw):
+def call(lib, prefix, name, err=None, _table=None, **kw):
     """Call prefix+name with keyword arguments named as in the reference.  Scalar outputs
     (``oi``/``od``) may be given as initial values; they are returned in an Outs dict
     together with the return code under key ``rc``."""
-    args = SIGNATURES[name]
+    args = (_table or SIGNATURES)[name]
     fn = getattr(lib, prefix + name)
     cargs, outs = [], Outs()
     for an, t in args:

@@ -765,6 +765,44 @@ int kukan_energy_adjoint_r(int nperi, int neigmx, int nf, int ncpx, int ncpy, in
   API_END
 }
 
+int kukan_batch_create(void **handle) {
+  API_BEGIN
+  if (!handle) throw Error(1, "batch_create: null handle pointer");
+  *handle = batch_create();
+  API_END
+}
+
+int kukan_batch_add_system(void *handle, int n, double h, int nbands, int natom, int num_list,
+                           const int *lstvec2, const int *natinf, const double *vnlocp,
+                           const double *dij, const double *veff, const double *svecre,
+                           int *index) {
+  API_BEGIN
+  const int i = batch_add(handle, n, h, nbands, natom, num_list, lstvec2, natinf, vnlocp, dij, veff,
+                          svecre);
+  if (index) *index = i;
+  API_END
+}
+
+int kukan_batch_step(void *handle, int batched, int ncgmax, double epsvh) {
+  API_BEGIN
+  if (ncgmax < 1) throw Error(1, "batch_step: ncgmax must be positive");
+  batch_step(handle, batched, ncgmax, epsvh);
+  API_END
+}
+
+int kukan_batch_get(void *handle, int index, double *svecre, double *sval, double *vh,
+                    int *hartree_iterations) {
+  API_BEGIN
+  batch_get(handle, index, svecre, sval, vh, hartree_iterations);
+  API_END
+}
+
+int kukan_batch_destroy(void *handle) {
+  API_BEGIN
+  batch_destroy(handle);
+  API_END
+}
+
 int kukan_force_eig_r(int nperi, int numk, int nums, int neigmx, int nf, int ncpx, int ncpy,
                       int ncpz, int num_list, int natom, int num_spe, int num_ppcell, int nprjmx,
                       int key_natpri_in, int key_natpri_inps, const int *lstvec2,

@@ -172,19 +172,14 @@ struct SL {
 // unconditional LDS read.  Loads of plane z + NF + 1, of plane z + 1's halo rows and of
 // V_eff(z + 1) are issued before plane z is computed (one plane of prefetch), and the barriers
 // wait only for LDS, so HBM latency overlaps the arithmetic.
+// One work-group's share: band b, y-tile `tile`, z-range `zch` (all arguments uniform).
 template <int NF, int P, bool RES>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && P <= 2 ? 4 : 2))) void k_stencil_lds(SL p) {
-  extern __shared__ double lds[];
-  __shared__ double red[8];
-  if (p.skip && *p.skip) return;  // uniform: a device-side loop has finished
+__device__ inline void stencil_block(const SL &p, int b, int tile, int zch, double *lds,
+                                     double *red) {
   constexpr int H = NF;  // halo values per thread: 2 NF rows x nx <= 512 NF for nx <= 256
   const int nx = p.nx, ny = p.ny, nz = p.nz, TY = p.TY, W = nx + 2 * NF;
   const long plane = (long)nx * ny;
-  const int L = blockIdx.x, xcd = L & 7, j = L >> 3;
   const int per = p.ntile * p.nzch;
-  const int b = (j / per) * 8 + xcd;
-  if (b >= p.nbands) return;  // uniform for the block
-  const int rem = j % per, tile = rem % p.ntile, zch = rem / p.ntile;
   const int y0 = tile * TY, z0 = zch * p.zc, z1 = min(nz, z0 + p.zc);
   const double *in = p.in + (long)b * p.ld_in;
   const double *in2 = RES ? p.in2 + (long)b * p.ld_in : nullptr;
@@ -311,6 +306,40 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && 
   if (p.dotp) {
     const double tot = block_sum<512>(dsum, red);
     if (t == 0) p.dotp[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
+  }
+}
+
+template <int NF, int P, bool RES>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && P <= 2 ? 4 : 2))) void k_stencil_lds(SL p) {
+  extern __shared__ double lds[];
+  __shared__ double red[8];
+  if (p.skip && *p.skip) return;  // uniform: a device-side loop has finished
+  const int L = blockIdx.x, xcd = L & 7, j = L >> 3;
+  const int per = p.ntile * p.nzch;
+  const int b = (j / per) * 8 + xcd;
+  if (b >= p.nbands) return;  // uniform for the block
+  const int rem = j % per;
+  stencil_block<NF, P, RES>(p, b, rem % p.ntile, rem / p.ntile, lds, red);
+}
+
+// Several independent grids in one persistent launch: work-group g takes items g, g + G,
+// ... of the concatenated item lists (system-major; an item is one band x y-tile x z-range
+// of one system, described by sl[s]).  start[s] = first item of system s, start[nsys] = total.
+template <int NF, int P, bool RES>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && P <= 2 ? 4 : 2))) void k_stencil_batch(const SL *sl, const long *start, int nsys) {
+  extern __shared__ double lds[];
+  __shared__ double red[8];
+  const long total = start[nsys];
+  int s = 0;
+  for (long it = blockIdx.x; it < total; it += gridDim.x) {
+    while (s + 1 < nsys && start[s + 1] <= it) s++;  // items ascend, so s only moves forward
+    const SL p = sl[s];
+    if (p.skip && *p.skip) continue;
+    const long j = it - start[s];
+    const int per = p.ntile * p.nzch;
+    const int b = (int)(j / per), rem = (int)(j % per);
+    stencil_block<NF, P, RES>(p, b, rem % p.ntile, rem / p.ntile, lds, red);
+    __syncthreads();  // the next item reuses the LDS slab
   }
 }
 
@@ -822,15 +851,15 @@ void fd_coefficients(int nf, double *acfd) { fd_weights(nf, acfd); }
 struct LdsPlan {
   int TY, ntile, nzch, zc, P;
 };
-static LdsPlan lds_plan(const Grid &g, int NF, int nb) {
+static LdsPlan lds_plan(const Grid &g, int NF, int nb, int force_p = 0) {
   LdsPlan L;
   // P points per thread (param "stencil_p": 1, 2 or 4; default 2), TY rows of nx points
-  int pp = (int)param("stencil_p", 2.0);
+  int pp = force_p > 0 ? force_p : (int)param("stencil_p", 2.0);
   pp = pp >= 4 ? 4 : (pp >= 2 ? 2 : 1);
   if (NF > 4) pp = 1;  // the z-column of a wide stencil leaves registers for one point
   // few bands (a Poisson solve): one point per thread and short z-ranges, for parallelism
   const bool few = (long)nb * ((g.ny * g.nx + 512 * pp - 1) / (512 * pp)) < 1024;
-  if (few) pp = 1;
+  if (few && force_p == 0) pp = 1;
   L.TY = std::max(1, std::min(g.ny, 512 * pp / g.nx));
   L.ntile = (g.ny + L.TY - 1) / L.TY;
   L.P = (g.nx * L.TY + 511) / 512;
@@ -954,6 +983,103 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
     default: throw Error(1, "error in kslaplacian_r! This subroutine is for nf in 1-8.");
+  }
+  HIPCK(hipGetLastError());
+}
+
+// ----------------------------------------------------------------------------- batched stencil
+StencilBatchPlan plan_stencil_batch(const std::vector<StencilCall> &calls, bool res, int P,
+                                    const std::string &tag) {
+  StencilBatchPlan b;
+  b.res = res;
+  b.P = P >= 4 ? 4 : (P >= 2 ? 2 : 1);
+  b.nsys = (int)calls.size();
+  if (b.nsys == 0) return b;
+  std::vector<SL> sl(b.nsys);
+  std::vector<long> start(b.nsys + 1, 0);
+  size_t shm = 0;
+  for (int i = 0; i < b.nsys; i++) {
+    const StencilCall &c = calls[i];
+    const Grid &g = *c.g;
+    const int nf = res ? 1 : (c.nfh_coeffs > 0 ? c.nfh_coeffs : g.nf);
+    if (i == 0) b.nf = nf;
+    if (nf != b.nf) throw Error(1, "batched stencil: all systems need the same radius");
+    if (c.halo_in || !lds_ok(g)) throw Error(1, "batched stencil: grid outside the tiled path");
+    SL q{};
+    q.nx = g.nx; q.ny = g.ny; q.nz = g.nz;
+    q.px = g.nperi >= 1; q.py = g.nperi >= 2; q.pz = g.nperi >= 3;
+    q.in = c.in; q.ld_in = c.ld_in; q.out = c.out; q.ld_out = c.ld_out; q.skip = c.skip;
+    q.accumulate = c.accumulate ? 1 : 0;
+    if (res) {
+      q.in2 = c.in2; q.eps = c.eps; q.sign = c.sign;
+      q.c0 = 0.5;
+      q.cx[1] = q.cy[1] = q.cz[1] = 1.0 / 12.0;
+    } else {
+      q.veff = c.veff;
+      double a[9];
+      fd_weights(nf, a);
+      for (int m = 0; m < 9; m++) {
+        q.cx[m] = c.scale_lap * a[m] / g.dx / g.dx * (-0.5);
+        q.cy[m] = c.scale_lap * a[m] / g.dy / g.dy * (-0.5);
+        q.cz[m] = c.scale_lap * a[m] / g.dz / g.dz * (-0.5);
+      }
+      q.c0 = q.cx[0] + q.cy[0] + q.cz[0];
+    }
+    // whole z-columns unless the system has too few tiles to share the chip
+    LdsPlan L = lds_plan(g, nf, c.nb, b.P);
+    L.zc = g.nz;
+    L.nzch = 1;
+    while ((long)c.nb * L.ntile * L.nzch < 64 && g.nz / (L.nzch * 2) >= 2 * nf) {
+      L.nzch *= 2;
+      L.zc = (g.nz + L.nzch - 1) / L.nzch;
+      L.nzch = (g.nz + L.zc - 1) / L.zc;
+    }
+    if (L.P > b.P) throw Error(1, "batched stencil: tile does not fit the thread block");
+    q.TY = L.TY; q.ntile = L.ntile; q.nzch = L.nzch; q.zc = L.zc; q.nbands = c.nb;
+    q.dotp = c.dot_partial;
+    q.dot_ld = c.dot_ld ? c.dot_ld : (long)L.ntile * L.nzch;
+    sl[i] = q;
+    start[i + 1] = start[i] + (long)c.nb * L.ntile * L.nzch;
+    shm = std::max(shm, sizeof(double) * (size_t)(L.TY + 2 * nf) * (g.nx + 2 * nf));
+  }
+  b.total = start[b.nsys];
+  b.shm = shm;
+  b.sl = scratch(tag + ".sl", sizeof(SL) * b.nsys);
+  b.start = (long *)scratch(tag + ".start", sizeof(long) * (b.nsys + 1));
+  HIPCK(hipMemcpy(b.sl, sl.data(), sizeof(SL) * b.nsys, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(b.start, start.data(), sizeof(long) * (b.nsys + 1), hipMemcpyHostToDevice));
+  return b;
+}
+
+// parts per band of system i's stencil partial sums in a batch plan (tiles x z-ranges)
+long stencil_batch_parts(const Grid &g, int nf, int nb, int P) {
+  LdsPlan L = lds_plan(g, nf, nb, P);
+  int nzch = 1, zc = g.nz;
+  while ((long)nb * L.ntile * nzch < 64 && g.nz / (nzch * 2) >= 2 * nf) {
+    nzch *= 2;
+    zc = (g.nz + nzch - 1) / nzch;
+    nzch = (g.nz + zc - 1) / zc;
+  }
+  return (long)L.ntile * nzch;
+}
+
+void launch_stencil_batch(const StencilBatchPlan &b, int grid, hipStream_t s) {
+  if (b.nsys == 0 || b.total == 0) return;
+  const unsigned G = (unsigned)std::max(1L, std::min<long>(grid, b.total));
+  const SL *sl = (const SL *)b.sl;
+  if (b.res) {
+    switch (b.P) {
+      case 1: hipLaunchKernelGGL((k_stencil_batch<1, 1, true>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
+      case 2: hipLaunchKernelGGL((k_stencil_batch<1, 2, true>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
+      default: hipLaunchKernelGGL((k_stencil_batch<1, 4, true>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
+    }
+  } else {
+    if (b.nf != 4) throw Error(1, "batched stencil: radius 4 only");
+    switch (b.P) {
+      case 1: hipLaunchKernelGGL((k_stencil_batch<4, 1, false>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
+      case 2: hipLaunchKernelGGL((k_stencil_batch<4, 2, false>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
+      default: hipLaunchKernelGGL((k_stencil_batch<4, 4, false>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
+    }
   }
   HIPCK(hipGetLastError());
 }

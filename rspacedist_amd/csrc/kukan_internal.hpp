@@ -133,13 +133,13 @@ void build_plan(NonlocalPlan &pl, const Grid &g, int natom, int nprjmx, int num_
 // Hpsi for nb bands: out_b (=|+=) (-1/2 lap + veff) in_b + expansion with coef_b.
 // in: interior arrays (wrap/zero halos per nperi) unless halo_in (explicit halo of width nf).
 struct StencilCall {
-  const Grid *g;
-  const double *in;
-  long ld_in;        // band stride (elements)
+  const Grid *g = nullptr;
+  const double *in = nullptr;
+  long ld_in = 0;      // band stride (elements)
   bool halo_in = false;
-  double *out;
-  long ld_out;
-  const double *veff;  // may be null
+  double *out = nullptr;
+  long ld_out = 0;
+  const double *veff = nullptr;  // may be null
   int nb = 1;
   bool accumulate = false;
   const NonlocalPlan *pl = nullptr;  // expansion if non-null and coef non-null
@@ -151,7 +151,25 @@ struct StencilCall {
   long dot_ld = 0;                   // band stride of dot_partial (0: number of parts)
   int nfh_coeffs = 0;                // if > 0: use this FD order (Hartree), no veff
   const int *skip = nullptr;         // device flag: the launch does nothing while *skip != 0
+  // residual-preconditioner form (batched plans only): in = sign (in - eps_b in2)
+  const double *in2 = nullptr, *eps = nullptr;
+  double sign = 1.0;
 };
+
+// Several systems' stencils as one persistent launch (device tables built once, so the
+// launch can be captured in a graph).  res: 7-point preconditioner of the residual form.
+struct StencilBatchPlan {
+  void *sl = nullptr;
+  long *start = nullptr;
+  int nsys = 0, nf = 4, P = 2;
+  long total = 0;
+  bool res = false;
+  size_t shm = 0;
+};
+StencilBatchPlan plan_stencil_batch(const std::vector<StencilCall> &calls, bool res, int P,
+                                    const std::string &tag);
+long stencil_batch_parts(const Grid &g, int nf, int nb, int P);
+void launch_stencil_batch(const StencilBatchPlan &b, int grid, hipStream_t s);
 void launch_stencil(const StencilCall &c, hipStream_t s);
 // true when launch_stencil takes the tiled path, whose projector pass reads band-fastest coef
 bool stencil_band_fastest_coef(const Grid &g, bool halo_in);

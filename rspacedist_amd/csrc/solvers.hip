@@ -410,33 +410,30 @@ __global__ void k_sub_scalar(long n, const double *s, double f, double *x) {
   if (i < n) x[i] = x[i] - s[0] * f;
 }
 // Hartree CG updates (scf_hartree8f.F90:849-891); scalars on the device: sc[0]=r1r, sc[1]=pap
-// Device-side Hartree CG loop state: st[0] = done, st[1] = iterations started.  Every kernel
-// of an iteration returns at once when done is set, so a block of iterations can be launched
-// (or replayed as a graph) past convergence without changing the result.
+// Device-side Hartree CG loop state: st[0] = done, st[1] = current iteration (1-based).
+// Every kernel of an iteration returns at once when done is set, so a block of iterations
+// can be launched (or replayed as a graph) past convergence without changing the result.
+// An iteration is four launches: A p (stencil, with <p|A p> partials), the update (each
+// work-group folds the stencil partials itself), the direction (each work-group folds the
+// update's <r|r> partials itself) and the stopping rule of scf_hartree8f.F90.
 __global__ void k_hcg_reset(int *st) {
   st[0] = 0;
-  st[1] = 0;
+  st[1] = 1;
 }
-// iteration start: count it; r1r of this iteration is r2r of the previous one
-__global__ void k_hcg_begin(double *sc, int *st) {
-  if (st[0]) return;
-  st[1] += 1;
-  if (st[1] > 1) sc[0] = sc[2];
-}
-// stopping rule of scf_hartree8f.F90 (ncgmin, ncgmax, dp = sqrt(r.r dV / Omega) <= epsvh)
-__global__ void k_hcg_check(const double *sc, double dvo, double eps, int ncgmin, int ncgmax,
-                            int *st) {
-  if (st[0]) return;
-  const int loop = st[1];
-  const double dp = sqrt(sc[2] * dvo);
-  if (loop >= ncgmin && (loop >= ncgmax || dp <= eps)) st[0] = 1;
+__device__ inline double fold_parts(const double *part, int np) {
+  double s = 0.0;
+  for (int c = 0; c < np; c++) s += part[c];
+  return s;
 }
 __global__ void k_hcg_update(long n, const double *sc, double *vh, const double *p, const double *ap,
-                             double *avhs, const double *b, double *r, double *part, int nparts,
-                             const int *st) {
+                             double *avhs, const double *b, double *r, const double *spart,
+                             int nsp, double *part, int nparts, const int *st) {
   __shared__ double red[4];
+  __shared__ double alph_s;
   if (st[0]) return;
-  const double alph = sc[0] / sc[1];
+  if (threadIdx.x == 0) alph_s = sc[0] / fold_parts(spart, nsp);  // r1r / pap
+  __syncthreads();
+  const double alph = alph_s;
   const long chunk = (n + nparts - 1) / nparts;
   const long i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
   double s = 0.0;
@@ -452,12 +449,31 @@ __global__ void k_hcg_update(long n, const double *sc, double *vh, const double 
   __syncthreads();
   if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
-__global__ void k_hcg_dir(long n, const double *r2, const double *r1, int ncgres, const double *r,
-                          double *p, const int *st) {
+__global__ void k_hcg_dir(long n, double *sc, const double *upart, int nup, int ncgres,
+                          const double *r, double *p, const int *st) {
+  __shared__ double beta_s;
   if (st[0]) return;
+  if (threadIdx.x == 0) {
+    const double r2r = fold_parts(upart, nup);
+    beta_s = (st[1] % ncgres == 0) ? 0.0 : r2r / sc[0];
+    if (blockIdx.x == 0) sc[2] = r2r;
+  }
+  __syncthreads();
+  const double beta = beta_s;
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const double beta = (st[1] % ncgres == 0) ? 0.0 : r2[0] / r1[0];
   if (i < n) p[i] = r[i] + beta * p[i];
+}
+// stopping rule (ncgmin, ncgmax, dp = sqrt(r.r dV / Omega) <= epsvh); else next iteration
+__global__ void k_hcg_ctl(double *sc, double dvo, double eps, int ncgmin, int ncgmax, int *st) {
+  if (st[0]) return;
+  const int loop = st[1];
+  const double dp = sqrt(sc[2] * dvo);
+  if (loop >= ncgmin && (loop >= ncgmax || dp <= eps)) {
+    st[0] = 1;
+    return;
+  }
+  st[1] = loop + 1;
+  sc[0] = sc[2];
 }
 
 }  // namespace
@@ -1118,21 +1134,19 @@ int hartree(const HartreeArgs &a) {
   // state once per block.  With param "hartree_graph" (default 1) a block is captured once
   // as a HIP graph and replayed.
   int *state = (int *)scratch("hart.state", sizeof(int) * 2);
+  double *upart = (double *)scratch("hart.upart", sizeof(double) * npu);
   hipLaunchKernelGGL(k_hcg_reset, dim3(1), dim3(1), 0, s, state);
   launch_dots(n, r, n, r, n, 1, 1.0, sc + 0, s);  // r1r of iteration 1
   st.in = p; st.out = ap; st.dot_partial = part; st.skip = state;
   const double dvo = g.dv() / omega;
   const int blk = std::max(1, (int)param("hartree_block", 16.0));
   auto iteration = [&]() {
-    hipLaunchKernelGGL(k_hcg_begin, dim3(1), dim3(1), 0, s, sc, state);
     launch_stencil(st, s);
-    launch_reduce_partials(part, np, 1, 1.0, sc + 1, s);  // pap
     hipLaunchKernelGGL(k_hcg_update, dim3(npu), dim3(256), 0, s, n, sc, a.vh, p, ap, avhs, b, r,
-                       part, npu, state);
-    launch_reduce_partials(part, npu, 1, 1.0, sc + 2, s);  // r2r
-    hipLaunchKernelGGL(k_hcg_dir, dim3(nblk(n, 256)), dim3(256), 0, s, n, sc + 2, sc + 0,
+                       part, np, upart, npu, state);
+    hipLaunchKernelGGL(k_hcg_dir, dim3(nblk(n, 256)), dim3(256), 0, s, n, sc, upart, npu,
                        a.ncgres, r, p, state);
-    hipLaunchKernelGGL(k_hcg_check, dim3(1), dim3(1), 0, s, sc, dvo, a.epsvh, a.ncgmin, a.ncgmax,
+    hipLaunchKernelGGL(k_hcg_ctl, dim3(1), dim3(1), 0, s, sc, dvo, a.epsvh, a.ncgmin, a.ncgmax,
                        state);
   };
   // a captured block is valid for the same buffers, sizes and stopping parameters
@@ -1151,7 +1165,7 @@ int hartree(const HartreeArgs &a) {
   static GraphCache gc;
   const bool use_graph = param("hartree_graph", 1.0) >= 1.0;
   if (use_graph) {
-    const Key key{{a.vh, b, r, p, ap, avhs, sc, part, state},
+    const Key key{{a.vh, b, r, p, ap, avhs, sc, part, upart, state},
                   {n, a.nx, a.ny, a.nz, a.nperi, a.nfh, blk, a.ncgres, a.ncgmin, a.ncgmax, np, npu},
                   {a.epsvh, dvo, g.dx, g.dy, g.dz}};
     if (!(gc.exec && gc.key == key)) {
@@ -1183,7 +1197,7 @@ int hartree(const HartreeArgs &a) {
     HIPCK(hipMemcpyAsync(hst, state, sizeof(hst), hipMemcpyDeviceToHost, s));
     sync();
     if (hst[0]) break;
-    if (hst[1] >= a.ncgmax) break;
+    if (hst[1] > a.ncgmax) break;  // not reached: the stopping rule sets done at ncgmax
   }
   return hst[1];
 }

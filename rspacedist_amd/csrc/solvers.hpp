@@ -77,6 +77,14 @@ void charge(long n, int M, const double *psi, const double *f, double *rho);
 void band_energy(Ctx &c, int M, const double *veff, const double *psi, double *eband);
 // adjoint of E = sum_l f_l e_l: dedv = dE/dveff (n), dedr = nonlocal dE/dR (3 x natom) from the
 // projector derivative tables dvnl (3 tables of vnlocp's shape, d vnlocp / d R_x,y,z)
+// batching pattern (batch.hip)
+void *batch_create();
+void batch_destroy(void *h);
+int batch_add(void *h, int n, double hh, int nbands, int natom, int num_list, const int *lstvec2,
+              const int *natinf, const double *vnlocp, const double *dij, const double *veff,
+              const double *psi);
+void batch_step(void *h, int batched, int ncgmax, double epsvh);
+void batch_get(void *h, int i, double *psi, double *sval, double *vh, int *iters);
 void energy_adjoint(Ctx &c, int M, const double *psi, const double *f, const double *dvnl,
                     const int *is_in_dev, double *dedv, double *dedr);
 void force_eig(Ctx &c, int M, int ns, const double *psi, const double *sval, const double *fnele,

@@ -40,6 +40,17 @@ def lib():
         L.kukan_timing_get.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_double)]
         L.kukan_timing_tags.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+        L.kukan_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        L.kukan_batch_add_system.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.POINTER(ctypes.c_int)]
+        L.kukan_batch_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.kukan_batch_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.POINTER(ctypes.c_int)]
+        L.kukan_batch_destroy.argtypes = [ctypes.c_void_p]
         L.kukan_subspace_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long,
                                           ctypes.c_double, ctypes.c_void_p, ctypes.c_long,
                                           ctypes.c_void_p, ctypes.c_long, ctypes.c_double,
@@ -165,4 +176,5 @@ EXPORTS = ["kukan_init", "kukan_finalize", "kukan_last_error", "kukan_set_param"
            "kukan_memcpy", "kukan_fill_normal", "kukan_device_synchronize", "kukan_timing_get",
            "kukan_timing_reset", "kukan_timing_tags", "kukan_unbind", "kukan_sync_to_host",
            "kukan_sync_to_device", "kukan_overlap_fdcheck_r", "kukan_rayleighritz_blacs_ini",
-           "kukan_rayleighritz_blacs_fin", "kukan_subspace_gemm"] + ["kukan_" + n for n in abi.SIGNATURES]
+           "kukan_rayleighritz_blacs_fin", "kukan_subspace_gemm", "kukan_batch_create",
+           "kukan_batch_add_system", "kukan_batch_step", "kukan_batch_get", "kukan_batch_destroy"] + ["kukan_" + n for n in abi.SIGNATURES]

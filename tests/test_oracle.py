@@ -483,3 +483,40 @@ def test_energy_adjoint_agrees_with_force_eig_r(orc):
     err = np.abs(-dedr - fref).max()
     print(f"adjoint vs force_eig_r: max diff {err:.3e}, max |F| {np.abs(fref).max():.3e}")
     assert err <= 2e-3 * np.abs(fref).max()
+
+
+def test_band_sd_step_matches_dense_restatement(orc):
+    """One batched-path band step on the CPU oracle against the same step written with the
+    dense H of the 12^3 system and the dense 7-point preconditioner (1/2 I + 1/12 sum of the
+    six periodic neighbours), eigh of the projected 2x2 problem, lower vector with a >= 0."""
+    s = synth.make_system(12, 0.45, 3, 4, seed=61)
+    psi0 = synth.smooth_bands(synth.random_bands(s), s.n, s.h)
+    q, r = np.linalg.qr(psi0)
+    psi0 = np.asfortranarray(q * np.sign(np.diag(r)) / np.sqrt(s.dv))
+    psi = psi0.copy(order="F")
+    sval = np.zeros(s.nbands)
+    dij = np.asfortranarray(s.dij.reshape(9, 9, s.natom, order="F"))
+    orc.call("band_sd_step", n=s.n, h=s.h, nbands=s.nbands, natom=s.natom, num_list=s.num_list,
+             lstvec2=s.lstvec2, natinf=s.natinf, vnlocp=s.vnlocp, dij=dij,
+             veff=np.asfortranarray(s.veff[..., 0]), svecre=psi, sval=sval)
+    Hd = H.dense_h(s)
+    n = s.n
+
+    def K(v):
+        g = v.reshape((n, n, n), order="F")
+        out = 0.5 * g
+        for ax in range(3):
+            out = out + (np.roll(g, 1, ax) + np.roll(g, -1, ax)) / 12.0
+        return out.reshape(-1, order="F")
+
+    for l in range(s.nbands):
+        x = psi0[:, l]
+        e = x @ Hd @ x * s.dv
+        p = K(-(Hd @ x - e * x))
+        p = p - (x @ p * s.dv) * x
+        p = p / np.sqrt(p @ p * s.dv)
+        A = np.array([[e, x @ Hd @ p * s.dv], [p @ Hd @ x * s.dv, p @ Hd @ p * s.dv]])
+        w, v = np.linalg.eigh(0.5 * (A + A.T))
+        a, c = v[:, 0] * np.sign(v[0, 0])
+        np.testing.assert_allclose(sval[l], w[0], rtol=1e-11)
+        np.testing.assert_allclose(psi[:, l], a * x + c * p, atol=1e-10 * np.abs(x).max())

@@ -272,7 +272,8 @@ __global__ void k_nn_prep(const double *C, long ldc, int K, int M2, const double
 
 // column maxima of C_kj / sx_k -> tc_j
 __global__ __launch_bounds__(256) void k_nn_cscale(const double *C, long ldc, int K, int M2,
-                                                   const double *sx, double *tc) {
+                                                   const double *sx, double alpha, double *tc,
+                                                   double *ys) {
   __shared__ double red[256];
   const int j = blockIdx.x;
   double m = 0.0;
@@ -291,6 +292,7 @@ __global__ __launch_bounds__(256) void k_nn_cscale(const double *C, long ldc, in
       sc = ldexp(1.0, 14 - e);
     }
     tc[j] = sc;
+    ys[j] = alpha / sc;
   }
 }
 
@@ -338,8 +340,8 @@ struct NNArgs {
   int M2, Kp;
   const _Float16 *xh, *xl;  // [Rp][Kp]
   const _Float16 *ch, *cl;  // [M2p][Kp]
-  const double *tc;
-  double alpha, beta;
+  const double *ys;  // alpha / tc per column of Y
+  double beta;
   double *Y;
   long ldy;
   int row_blocks, col_tiles;
@@ -434,12 +436,22 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
     __syncthreads();
   }
   // epilogue: per 32-column block b, each wave parks its 64 x 32 piece in LDS as
-  // [column][row] (pitch 65 floats: conflict-free) and writes it back column by column
+  // [column][row] (pitch 65 floats: conflict-free) and writes it back column by column.
+  // For beta != 0 the block's 32 Y values per lane are loaded before the parking, as
+  // independent loads whose latency overlaps the LDS round trip.
   constexpr int EP = 65;
   float *W = reinterpret_cast<float *>(&S[0]) + w * 32 * EP;
   const long m = r0 + 64 * wi + lane;
+  const bool mrow = m < p.R;
 #pragma unroll
   for (int b = 0; b < 4; b++) {
+    const int nb0 = c0 + 128 * wj + 32 * b;
+    double yo[32];
+    if (p.beta != 0.0) {
+#pragma unroll
+      for (int q = 0; q < 32; q++)
+        yo[q] = (mrow && nb0 + q < p.M2) ? p.Y[m + p.ldy * (long)(nb0 + q)] : 0.0;
+    }
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -448,12 +460,12 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
         W[(lane & 31) * EP + ml] = acc[a][b][r];
       }
     __syncthreads();
+#pragma unroll
     for (int q = 0; q < 32; q++) {
-      const int n = c0 + 128 * wj + 32 * b + q;
-      if (n < p.M2 && m < p.R) {
-        const double v = p.alpha / p.tc[n] * (double)W[q * EP + lane];
-        double *y = p.Y + m + p.ldy * (long)n;
-        *y = p.beta == 0.0 ? v : v + p.beta * *y;
+      const int n = nb0 + q;
+      if (n < p.M2 && mrow) {
+        const double v = p.ys[n] * (double)W[q * EP + lane];
+        p.Y[m + p.ldy * (long)n] = p.beta == 0.0 ? v : v + p.beta * yo[q];
       }
     }
     __syncthreads();
@@ -516,7 +528,8 @@ void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ld
   _Float16 *xh = (_Float16 *)scratch("gemm.nnxh", sizeof(_Float16) * (size_t)Kp * Rp);
   _Float16 *xl = (_Float16 *)scratch("gemm.nnxl", sizeof(_Float16) * (size_t)Kp * Rp);
   double *tc = (double *)scratch("gemm.nntc", sizeof(double) * M2p);
-  hipLaunchKernelGGL(k_nn_cscale, dim3(M2), dim3(256), 0, s, C, ldc, K, M2, sx, tc);
+  double *ys = (double *)scratch("gemm.nnys", sizeof(double) * M2p);
+  hipLaunchKernelGGL(k_nn_cscale, dim3(M2), dim3(256), 0, s, C, ldc, K, M2, sx, alpha, tc, ys);
   hipLaunchKernelGGL(k_nn_prep, dim3(nblk((long)Kp * M2p, 256)), dim3(256), 0, s, C, ldc, K, M2, sx,
                      tc, Kp, M2p, ch, cl);
   HIPCK(hipGetLastError());
@@ -528,7 +541,7 @@ void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ld
   }
   NNArgs p{};
   p.R = R; p.M2 = M2; p.Kp = Kp; p.xh = xh; p.xl = xl; p.ch = ch; p.cl = cl;
-  p.tc = tc; p.alpha = alpha; p.beta = beta; p.Y = Y; p.ldy = ldy;
+  p.ys = ys; p.beta = beta; p.Y = Y; p.ldy = ldy;
   p.row_blocks = (int)(Rp / NN_BM);
   p.col_tiles = M2p / NN_BN;
   const long groups = (p.row_blocks + 7) / 8;

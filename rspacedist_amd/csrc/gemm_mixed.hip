@@ -12,8 +12,12 @@
 //        K-chunks and is flushed into FP64 registers; groups write FP64 partials that a
 //        second kernel reduces in a fixed order (deterministic).
 //   NN:  Y(R x M2) = alpha X C + beta Y, X (R x K) column-major (K = number of bands),
-//        C small.  X is loaded straight from HBM in the MFMA operand layout (lanes 0-31
-//        read 32 consecutive rows of one column), C is pre-split once.
+//        C small.  X diag(sx) and C'' are pre-split into "fragment" layout: a fragment is
+//        the 1 KB an MFMA operand register quad holds for 32 rows (columns) x 16 k, stored
+//        lane by lane.  The main loop copies fragments HBM -> LDS with the gfx950 LDS-DMA
+//        load (global_load_lds_dwordx4: lane i's 16 bytes land at base + 16 i), so staging
+//        costs no registers and no LDS stores, and every LDS read is one conflict-free
+//        ds_read_b128 per lane.
 #include "gemm_mixed.hpp"
 #include "kukan_internal.hpp"
 
@@ -256,13 +260,18 @@ __global__ void k_tn_reduce(const float *part, int nsplit, int M1, int M2, doubl
 }
 
 // ----------------------------------------------------------------------------- NN
-// C'' = C / sx * tc split into hi/lo halves in [j][Kp] layout (Kp = K rounded up to BK,
-// columns padded to a multiple of BN with zeros by the caller).
+// C'' = C / sx * tc split into hi/lo halves in fragment layout: fragment (j / 32, k / 16) of
+// KC = Kp / 16 per column block, lane (j % 32) + 32 ((k % 16) / 8), half k % 8.  Columns
+// are padded to M2p and k to Kp with zeros.
 __global__ void k_nn_prep(const double *C, long ldc, int K, int M2, const double *sx,
                           const double *tc, int Kp, int M2p, _Float16 *ch, _Float16 *cl) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;  // = fragment element index
   if (t >= (long)Kp * M2p) return;
-  const int k = (int)(t % Kp), j = (int)(t / Kp);
+  const int hlf = (int)(t & 7), lane = (int)((t >> 3) & 63);
+  const long frag = t >> 9;
+  const int KC = Kp / 16;
+  const int j = (int)(frag / KC) * 32 + (lane & 31);
+  const int k = (int)(frag % KC) * 16 + 8 * (lane >> 5) + hlf;
   double x = 0.0;
   if (k < K && j < M2) x = C[k + ldc * (long)j] / sx[k] * tc[j];
   const _Float16 h = to_half(x);
@@ -296,9 +305,11 @@ __global__ __launch_bounds__(256) void k_nn_cscale(const double *C, long ldc, in
   }
 }
 
-// X'' = X diag(sx) split into hi/lo halves and transposed to [row][Kp] (k contiguous), rows
-// padded to a multiple of NN_BM and k to Kp with zeros.  Work-group: 64 rows x 64 k through
-// LDS; global reads are 64-row runs of one column, writes 16-byte runs of 8 k.
+// X'' = X diag(sx) split into hi/lo halves in fragment layout: fragment (row / 32, k / 16)
+// at ((row / 32) KC + k / 16) 512 halves (KC = Kp / 16), lane (row % 32) + 32 ((k % 16) / 8),
+// half k % 8; rows padded to a multiple of NN_BM and k to Kp with zeros.  Work-group: 64
+// rows x 64 k through LDS; global reads are 64-row runs of one column, writes are whole
+// fragments (lane-contiguous 16-byte runs).
 constexpr int NN_BM = 256, NN_BN = 256;
 __global__ __launch_bounds__(256) void k_split_t(const double *X, long ldx, long R, int K,
                                                  const double *sx, int Kp, _Float16 *xh,
@@ -316,20 +327,22 @@ __global__ __launch_bounds__(256) void k_split_t(const double *X, long ldx, long
     tile[tk + 4 * q][tr] = v;
   }
   __syncthreads();
+  const int KC = Kp / 16;
 #pragma unroll
   for (int u = 0; u < 2; u++) {
-    const int unit = t + 256 * u;      // 64 rows x 8 groups of 8 k
-    const int rl = unit >> 3, kg = unit & 7;
+    const int unit = t + 256 * u;  // 8 fragments (2 row blocks x 4 k blocks) x 64 lanes
+    const int lane = unit & 63, fr = unit >> 6, rb = fr >> 2, kc = fr & 3;
+    const int rl = 32 * rb + (lane & 31), kl = 16 * kc + 8 * (lane >> 5);
     half8 h, l;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-      float f = tile[8 * kg + j][rl];
+      float f = tile[kl + j][rl];
       asm volatile("" : "+v"(f));
       const _Float16 hh = (_Float16)f;
       h[j] = hh;
       l[j] = (_Float16)(f - (float)hh);
     }
-    const long off = (r0 + rl) * Kp + k0 + 8 * kg;
+    const long off = (((r0 / 32 + rb) * KC) + k0 / 16 + kc) * 512 + 8 * lane;
     *(half8 *)&xh[off] = h;
     *(half8 *)&xl[off] = l;
   }
@@ -338,8 +351,8 @@ __global__ __launch_bounds__(256) void k_split_t(const double *X, long ldx, long
 struct NNArgs {
   long R;
   int M2, Kp;
-  const _Float16 *xh, *xl;  // [Rp][Kp]
-  const _Float16 *ch, *cl;  // [M2p][Kp]
+  const _Float16 *xh, *xl;  // X'' fragments, (Rp / 32) x (Kp / 16)
+  const _Float16 *ch, *cl;  // C'' fragments, (M2p / 32) x (Kp / 16)
   const double *ys;  // alpha / tc per column of Y
   double beta;
   double *Y;
@@ -347,19 +360,30 @@ struct NNArgs {
   int row_blocks, col_tiles;
 };
 
-struct NNStage {
-  _Float16 ah[NN_BM * PITCH], al[NN_BM * PITCH], bh[NN_BN * PITCH], bl[NN_BN * PITCH];
-};
+// s_waitcnt with only the vector-memory count constrained (expcnt, lgkmcnt at their maxima)
+template <int N>
+__device__ inline void wait_vm() {
+  static_assert(N >= 0 && N < 16, "vmcnt immediate");
+  __builtin_amdgcn_s_waitcnt(0xF70 | N);
+}
+
+#define NN_LDS(x) ((__attribute__((address_space(3))) void *)(x))
 
 // Y tile 256 rows x 256 columns, 8 waves of 64 x 128 (wave w: rows 64 (w & 3), columns
-// 128 (w >> 2)).  Operands arrive pre-split, so a stage is 8 16-byte loads per thread copied
-// to LDS (double-buffered, 32-deep K stages) and the main loop is MFMA plus LDS reads only.
-// The col_tiles of one row block run consecutively on one XCD so that X'' is fetched from HBM
-// once.  The epilogue goes through LDS so that each column of Y is written (and, for
+// 128 (w >> 2)).  K advances in 16-deep stages of 32 fragments (A hi/lo: 8 row blocks, B
+// hi/lo: 8 column blocks; 32 KB) held in a ring of NN_NS LDS buffers.  Each wave issues 4
+// LDS-DMA loads per stage, NN_NS - 1 stages ahead of the stage it computes; at the top of an
+// iteration it waits for its own loads of that stage (vmcnt counts them in issue order) and
+// a barrier publishes everyone's.  The buffer refilled after the barrier is the one every
+// wave finished reading in the previous iteration (its reads were consumed by MFMAs before
+// the barrier).  Plain s_barrier (no fence) keeps the later stages in flight.
+// The col_tiles of one row block run consecutively on one XCD so that X'' is fetched from
+// HBM once.  The epilogue goes through LDS so that each column of Y is written (and, for
 // beta != 0, read) as 512-byte runs.
+constexpr int NN_NS = 5;
 template <int NT>
 __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
-  __shared__ NNStage S[2];
+  __shared__ __attribute__((aligned(16))) _Float16 buf[NN_NS][32][512];
   const int L = blockIdx.x, xcd = L & 7, jj = L >> 3;
   const long rb = (long)(jj / p.col_tiles) * 8 + xcd;
   const int ct = jj % p.col_tiles;
@@ -367,30 +391,21 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
   const long r0 = rb * NN_BM;
   const int c0 = ct * NN_BN;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
-  // staging: row (column) t / 2, k half 16 (t & 1)
-  const int sr = t >> 1, sk = 16 * (t & 1);
-  const _Float16 *pxh = p.xh + (r0 + sr) * p.Kp + sk, *pxl = p.xl + (r0 + sr) * p.Kp + sk;
-  const _Float16 *pch = p.ch + (long)(c0 + sr) * p.Kp + sk, *pcl = p.cl + (long)(c0 + sr) * p.Kp + sk;
-  half8 g[8];
-  auto gload = [&](int k0) {
-    g[0] = *(const half8 *)(pxh + k0);
-    g[1] = *(const half8 *)(pxh + k0 + 8);
-    g[2] = *(const half8 *)(pxl + k0);
-    g[3] = *(const half8 *)(pxl + k0 + 8);
-    g[4] = *(const half8 *)(pch + k0);
-    g[5] = *(const half8 *)(pch + k0 + 8);
-    g[6] = *(const half8 *)(pcl + k0);
-    g[7] = *(const half8 *)(pcl + k0 + 8);
-  };
-  auto lstore = [&](NNStage &D) {
-    *(half8 *)&D.ah[sr * PITCH + sk] = g[0];
-    *(half8 *)&D.ah[sr * PITCH + sk + 8] = g[1];
-    *(half8 *)&D.al[sr * PITCH + sk] = g[2];
-    *(half8 *)&D.al[sr * PITCH + sk + 8] = g[3];
-    *(half8 *)&D.bh[sr * PITCH + sk] = g[4];
-    *(half8 *)&D.bh[sr * PITCH + sk + 8] = g[5];
-    *(half8 *)&D.bl[sr * PITCH + sk] = g[6];
-    *(half8 *)&D.bl[sr * PITCH + sk + 8] = g[7];
+  const int KC = p.Kp / 16, nst = KC;
+  // this wave's 4 fragments per stage: f = 4 w + i; f / 8 = A hi, A lo, B hi, B lo
+  const _Float16 *src[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int f = 4 * w + i, op = f >> 3, idx = f & 7;
+    const _Float16 *base = op == 0 ? p.xh : op == 1 ? p.xl : op == 2 ? p.ch : p.cl;
+    const long blk = op < 2 ? r0 / 32 + idx : (long)c0 / 32 + idx;
+    src[i] = base + blk * KC * 512 + 8 * lane;
+  }
+  auto dma = [&](int st) {
+    _Float16(*d)[512] = buf[st % NN_NS];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      __builtin_amdgcn_global_load_lds(src[i] + (long)st * 512, NN_LDS(&d[4 * w + i][0]), 16, 0, 0);
   };
   f16v acc[2][4];
 #pragma unroll
@@ -399,48 +414,61 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
     for (int b = 0; b < 4; b++)
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
-  const int nst = p.Kp / BK;
-  gload(0);
-  lstore(S[0]);
-  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NN_NS - 1; i++)
+    if (i < nst) dma(i);
   for (int st = 0; st < nst; st++) {
-    const bool more = st + 1 < nst;
-    if (more) gload((st + 1) * BK);
-    const NNStage &C = S[st & 1];
+    // own loads of stage st done: later stages still in flight = min(NN_NS - 2, nst - 1 - st)
+    const int later = min(NN_NS - 2, nst - 1 - st);
+    if (later >= 3) wait_vm<12>();
+    else if (later == 2) wait_vm<8>();
+    else if (later == 1) wait_vm<4>();
+    else wait_vm<0>();
+    asm volatile("" ::: "memory");  // no compiler motion of LDS accesses across the barrier
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (st + NN_NS - 1 < nst) dma(st + NN_NS - 1);
+    const _Float16(*C)[512] = buf[st % NN_NS];
+    half8 ah[2], al[2], bh[4], bl[4];
 #pragma unroll
-    for (int ss = 0; ss < 2; ss++) {
-      const int ko = 16 * ss + 8 * (lane >> 5);
-      half8 ah[2], al[2];
-#pragma unroll
-      for (int a = 0; a < 2; a++) {
-        const int row = 64 * wi + 32 * a + (lane & 31);
-        ah[a] = *(const half8 *)&C.ah[row * PITCH + ko];
-        al[a] = *(const half8 *)&C.al[row * PITCH + ko];
-      }
-#pragma unroll
-      for (int b = 0; b < 4; b++) {
-        const int col = 128 * wj + 32 * b + (lane & 31);
-        const half8 bh = *(const half8 *)&C.bh[col * PITCH + ko];
-        const half8 bl = *(const half8 *)&C.bl[col * PITCH + ko];
-#pragma unroll
-        for (int a = 0; a < 2; a++) {
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bh, acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bl, acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bh, acc[a][b], 0, 0, 0);
-          if (NT == 4)
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bl, acc[a][b], 0, 0, 0);
-        }
-      }
+    for (int a = 0; a < 2; a++) {
+      ah[a] = *(const half8 *)&C[2 * wi + a][8 * lane];
+      al[a] = *(const half8 *)&C[8 + 2 * wi + a][8 * lane];
     }
-    if (more) lstore(S[(st + 1) & 1]);
-    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      bh[b] = *(const half8 *)&C[16 + 4 * wj + b][8 * lane];
+      bl[b] = *(const half8 *)&C[24 + 4 * wj + b][8 * lane];
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
+    if (NT == 4) {
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int a = 0; a < 2; a++) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bl[b], acc[a][b], 0, 0, 0);
+    }
+    // this stage's LDS reads are complete before anyone passes the next barrier
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
   }
+  wait_vm<0>();
+  __syncthreads();
   // epilogue: per 32-column block b, each wave parks its 64 x 32 piece in LDS as
   // [column][row] (pitch 65 floats: conflict-free) and writes it back column by column.
   // For beta != 0 the block's 32 Y values per lane are loaded before the parking, as
   // independent loads whose latency overlaps the LDS round trip.
   constexpr int EP = 65;
-  float *W = reinterpret_cast<float *>(&S[0]) + w * 32 * EP;
+  float *W = reinterpret_cast<float *>(&buf[0][0][0]) + w * 32 * EP;
   const long m = r0 + 64 * wi + lane;
   const bool mrow = m < p.R;
 #pragma unroll
@@ -520,7 +548,7 @@ void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ld
                    const double *sx, const double *C, long ldc, double beta, double *Y, long ldy,
                    int nterms, hipStream_t s) {
   if (R <= 0 || M2 <= 0 || K <= 0) throw Error(1, "gemm_mixed_nn: empty product");
-  const int Kp = (K + 63) / 64 * 64;  // whole 64-k split tiles, whole 32-k stages
+  const int Kp = (K + 63) / 64 * 64;  // whole 64-k split tiles (and 16-k stages)
   const int M2p = (M2 + NN_BN - 1) / NN_BN * NN_BN;
   const long Rp = (R + NN_BM - 1) / NN_BM * NN_BM;
   _Float16 *ch = (_Float16 *)scratch("gemm.nnch", sizeof(_Float16) * (size_t)Kp * M2p);

@@ -316,9 +316,17 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && 
   if (p.skip && *p.skip) return;  // uniform: a device-side loop has finished
   const int L = blockIdx.x, xcd = L & 7, j = L >> 3;
   const int per = p.ntile * p.nzch;
-  const int b = (j / per) * 8 + xcd;
-  if (b >= p.nbands) return;  // uniform for the block
-  const int rem = j % per;
+  int b, rem;
+  if (p.nbands >= 8) {  // all tiles and z-ranges of a band on one XCD
+    b = (j / per) * 8 + xcd;
+    if (b >= p.nbands) return;  // uniform for the block
+    rem = j % per;
+  } else {  // fewer bands than XCDs: each XCD takes a contiguous eighth of the items
+    const long items = (long)p.nbands * per, chunk = (items + 7) / 8, it = xcd * chunk + j;
+    if (j >= chunk || it >= items) return;
+    b = (int)(it / per);
+    rem = (int)(it % per);
+  }
   stencil_block<NF, P, RES>(p, b, rem % p.ntile, rem / p.ntile, lds, red);
 }
 
@@ -884,8 +892,9 @@ int stencil_parts_lds(const Grid &g, int NF, int nb) {
 template <int NF, bool RES>
 static void launch_lds_nf(const SL &p, const LdsPlan &L, int nb, hipStream_t s) {
   const size_t shm = sizeof(double) * (size_t)(L.TY + 2 * NF) * (p.nx + 2 * NF);
-  const long groups = (nb + 7) / 8;
-  const unsigned grid = (unsigned)(groups * 8 * (long)L.ntile * L.nzch);
+  const long per = (long)L.ntile * L.nzch;
+  // nb >= 8: band groups of 8 (one band per XCD); fewer: the items split over the 8 XCDs
+  const unsigned grid = (unsigned)(nb >= 8 ? (long)(nb + 7) / 8 * 8 * per : 8 * ((nb * per + 7) / 8));
   switch (L.P) {
     case 1: hipLaunchKernelGGL((k_stencil_lds<NF, 1, RES>), dim3(grid), dim3(512), shm, s, p); break;
     case 2: hipLaunchKernelGGL((k_stencil_lds<NF, 2, RES>), dim3(grid), dim3(512), shm, s, p); break;

@@ -420,20 +420,24 @@ __global__ void k_hcg_reset(int *st) {
   st[0] = 0;
   st[1] = 1;
 }
-__device__ inline double fold_parts(const double *part, int np) {
+// Sum of np partials by a 256-thread work-group in a fixed order (strided per thread, then
+// wave shuffles, then the 4 wave sums in order); every thread gets the result.
+__device__ inline double fold_parts256(const double *part, int np, double *red4) {
   double s = 0.0;
-  for (int c = 0; c < np; c++) s += part[c];
-  return s;
+  for (int c = threadIdx.x; c < np; c += 256) s += part[c];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red4[threadIdx.x >> 6] = s;
+  __syncthreads();
+  const double tot = ((red4[0] + red4[1]) + red4[2]) + red4[3];
+  __syncthreads();
+  return tot;
 }
 __global__ void k_hcg_update(long n, const double *sc, double *vh, const double *p, const double *ap,
                              double *avhs, const double *b, double *r, const double *spart,
                              int nsp, double *part, int nparts, const int *st) {
   __shared__ double red[4];
-  __shared__ double alph_s;
   if (st[0]) return;
-  if (threadIdx.x == 0) alph_s = sc[0] / fold_parts(spart, nsp);  // r1r / pap
-  __syncthreads();
-  const double alph = alph_s;
+  const double alph = sc[0] / fold_parts256(spart, nsp, red);  // r1r / pap
   const long chunk = (n + nparts - 1) / nparts;
   const long i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
   double s = 0.0;
@@ -451,17 +455,13 @@ __global__ void k_hcg_update(long n, const double *sc, double *vh, const double 
 }
 __global__ void k_hcg_dir(long n, double *sc, const double *upart, int nup, int ncgres,
                           const double *r, double *p, const int *st) {
-  __shared__ double beta_s;
+  __shared__ double red[4];
   if (st[0]) return;
-  if (threadIdx.x == 0) {
-    const double r2r = fold_parts(upart, nup);
-    beta_s = (st[1] % ncgres == 0) ? 0.0 : r2r / sc[0];
-    if (blockIdx.x == 0) sc[2] = r2r;
-  }
-  __syncthreads();
-  const double beta = beta_s;
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n) p[i] = r[i] + beta * p[i];
+  const double r2r = fold_parts256(upart, nup, red);
+  const double beta = (st[1] % ncgres == 0) ? 0.0 : r2r / sc[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[2] = r2r;
+  const long stride = (long)gridDim.x * 256;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += stride) p[i] = r[i] + beta * p[i];
 }
 // stopping rule (ncgmin, ncgmax, dp = sqrt(r.r dV / Omega) <= epsvh); else next iteration
 __global__ void k_hcg_ctl(double *sc, double dvo, double eps, int ncgmin, int ncgmax, int *st) {
@@ -1144,7 +1144,7 @@ int hartree(const HartreeArgs &a) {
     launch_stencil(st, s);
     hipLaunchKernelGGL(k_hcg_update, dim3(npu), dim3(256), 0, s, n, sc, a.vh, p, ap, avhs, b, r,
                        part, np, upart, npu, state);
-    hipLaunchKernelGGL(k_hcg_dir, dim3(nblk(n, 256)), dim3(256), 0, s, n, sc, upart, npu,
+    hipLaunchKernelGGL(k_hcg_dir, dim3(std::min(nblk(n, 256), 2048u)), dim3(256), 0, s, n, sc, upart, npu,
                        a.ncgres, r, p, state);
     hipLaunchKernelGGL(k_hcg_ctl, dim3(1), dim3(1), 0, s, sc, dvo, a.epsvh, a.ncgmin, a.ncgmax,
                        state);

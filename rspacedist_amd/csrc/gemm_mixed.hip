@@ -79,6 +79,20 @@ __global__ void k_maxscale(int cols, const unsigned long long *cmax, double *sca
   scale[j] = sc;
 }
 
+// Scale from a norm: dV sum_x psi(x)^2 = g bounds |psi(x)| <= sqrt(g / dV) < 2^e, so
+// 2^(13 - e) keeps |psi s| < 2^13 without a pass over psi (1 for g <= 0).
+__global__ void k_normscale(int cols, const double *g, double dv, double *scale) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  double sc = 1.0;
+  if (g[j] > 0.0) {
+    int e;
+    frexp(sqrt(g[j] / dv), &e);
+    sc = ldexp(1.0, 13 - e);
+  }
+  scale[j] = sc;
+}
+
 // ----------------------------------------------------------------------------- tiles
 // Both products run on a 256 x 128 block tile, 8 waves of 64 x 64 (wave w: rows 64 (w & 3),
 // columns 64 (w >> 2)), 32-deep K stages double-buffered in LDS as split halves
@@ -369,37 +383,32 @@ __device__ inline void wait_vm() {
 
 #define NN_LDS(x) ((__attribute__((address_space(3))) void *)(x))
 
-// Y tile 256 rows x 256 columns, 8 waves of 64 x 128 (wave w: rows 64 (w & 3), columns
-// 128 (w >> 2)).  K advances in 16-deep stages of 32 fragments (A hi/lo: 8 row blocks, B
-// hi/lo: 8 column blocks; 32 KB) held in a ring of NN_NS LDS buffers.  Each wave issues 4
-// LDS-DMA loads per stage, NN_NS - 1 stages ahead of the stage it computes; at the top of an
-// iteration it waits for its own loads of that stage (vmcnt counts them in issue order) and
-// a barrier publishes everyone's.  The buffer refilled after the barrier is the one every
-// wave finished reading in the previous iteration (its reads were consumed by MFMAs before
-// the barrier).  Plain s_barrier (no fence) keeps the later stages in flight.
-// The col_tiles of one row block run consecutively on one XCD so that X'' is fetched from
-// HBM once.  The epilogue goes through LDS so that each column of Y is written (and, for
-// beta != 0, read) as 512-byte runs.
 constexpr int NN_NS = 5;
+typedef _Float16 FragBuf[NN_NS][32][512];
+
+// The fragment-GEMM main loop shared by NN and pre-split TN: acc (wave w: rows 64 (w & 3),
+// columns 128 (w >> 2) of a 256 x 256 tile) += the products of nst 16-deep stages.  The tile's
+// A fragments of stage st are at a[h|l] + (idx KC + st) 512 (idx = row block 0..7), the B
+// fragments likewise; the caller offsets a* / b* to its first row / column block and first
+// stage.  Each wave issues 4 LDS-DMA loads per stage, NN_NS - 1 stages ahead of the one it
+// computes; at the top of an iteration it waits for its own loads of that stage (vmcnt counts
+// them in issue order) and a barrier publishes everyone's.  The buffer refilled after the
+// barrier is the one every wave finished reading in the previous iteration (its reads were
+// consumed by MFMAs, and waited for, before that barrier).  The plain s_barrier (no fence)
+// keeps the later stages in flight; empty asm statements keep the compiler from moving LDS
+// accesses across it.  On return all loads have landed and the block is synchronised.
 template <int NT>
-__global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
-  __shared__ __attribute__((aligned(16))) _Float16 buf[NN_NS][32][512];
-  const int L = blockIdx.x, xcd = L & 7, jj = L >> 3;
-  const long rb = (long)(jj / p.col_tiles) * 8 + xcd;
-  const int ct = jj % p.col_tiles;
-  if (rb >= p.row_blocks) return;
-  const long r0 = rb * NN_BM;
-  const int c0 = ct * NN_BN;
+__device__ inline void frag_gemm(const _Float16 *ah_g, const _Float16 *al_g, const _Float16 *bh_g,
+                                 const _Float16 *bl_g, long KC, int nst, FragBuf &buf,
+                                 f16v (&acc)[2][4]) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
-  const int KC = p.Kp / 16, nst = KC;
   // this wave's 4 fragments per stage: f = 4 w + i; f / 8 = A hi, A lo, B hi, B lo
   const _Float16 *src[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int f = 4 * w + i, op = f >> 3, idx = f & 7;
-    const _Float16 *base = op == 0 ? p.xh : op == 1 ? p.xl : op == 2 ? p.ch : p.cl;
-    const long blk = op < 2 ? r0 / 32 + idx : (long)c0 / 32 + idx;
-    src[i] = base + blk * KC * 512 + 8 * lane;
+    const _Float16 *base = op == 0 ? ah_g : op == 1 ? al_g : op == 2 ? bh_g : bl_g;
+    src[i] = base + (long)idx * KC * 512 + 8 * lane;
   }
   auto dma = [&](int st) {
     _Float16(*d)[512] = buf[st % NN_NS];
@@ -407,7 +416,6 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
     for (int i = 0; i < 4; i++)
       __builtin_amdgcn_global_load_lds(src[i] + (long)st * 512, NN_LDS(&d[4 * w + i][0]), 16, 0, 0);
   };
-  f16v acc[2][4];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -424,7 +432,7 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
     else if (later == 2) wait_vm<8>();
     else if (later == 1) wait_vm<4>();
     else wait_vm<0>();
-    asm volatile("" ::: "memory");  // no compiler motion of LDS accesses across the barrier
+    asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
     if (st + NN_NS - 1 < nst) dma(st + NN_NS - 1);
@@ -463,6 +471,26 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
   }
   wait_vm<0>();
   __syncthreads();
+}
+
+// Y tile 256 rows x 256 columns (frag_gemm over all of K).  The col_tiles of one row block
+// run consecutively on one XCD so that X'' is fetched from HBM once.  The epilogue goes
+// through LDS so that each column of Y is written (and, for beta != 0, read) as 512-byte runs.
+template <int NT>
+__global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
+  __shared__ __attribute__((aligned(16))) FragBuf buf;
+  const int L = blockIdx.x, xcd = L & 7, jj = L >> 3;
+  const long rb = (long)(jj / p.col_tiles) * 8 + xcd;
+  const int ct = jj % p.col_tiles;
+  if (rb >= p.row_blocks) return;
+  const long r0 = rb * NN_BM;
+  const int c0 = ct * NN_BN;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
+  const long KC = p.Kp / 16;
+  f16v acc[2][4];
+  frag_gemm<NT>(p.xh + (r0 / 32) * KC * 512, p.xl + (r0 / 32) * KC * 512,
+                p.ch + (long)(c0 / 32) * KC * 512, p.cl + (long)(c0 / 32) * KC * 512, KC, (int)KC,
+                buf, acc);
   // epilogue: per 32-column block b, each wave parks its 64 x 32 piece in LDS as
   // [column][row] (pitch 65 floats: conflict-free) and writes it back column by column.
   // For beta != 0 the block's 32 Y values per lane are loaded before the parking, as
@@ -500,6 +528,97 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
   }
 }
 
+// ----------------------------------------------------------------------------- TN, pre-split
+// A (K x M) column-major times diag(s), split into hi/lo halves in fragment layout: fragment
+// (i / 32, k / 16) at ((i / 32) KC + k / 16) 512 halves (KC = Kp / 16), lane (i % 32) +
+// 32 ((k % 16) / 8), half k % 8; columns padded to Mp and k to Kp with zeros.  Work-group:
+// 32 columns x 128 k through LDS.  Reads: 8 lanes cover one column's 128-byte run; writes:
+// the 8 fragments of the tile are consecutive in memory (16 KB of hi and lo in all).
+constexpr int SPT_K = 128;
+__global__ __launch_bounds__(256) void k_split_tn(const double *A, long lda, long K, int M,
+                                                  const double *sc, long KC, _Float16 *xh,
+                                                  _Float16 *xl) {
+  __shared__ float tile[32][SPT_K + 4];
+  const int ib = blockIdx.y;
+  const long k0 = (long)blockIdx.x * SPT_K;
+  const int t = threadIdx.x;
+  {
+    const int cl = t >> 3, kq = 2 * (t & 7);  // column, k pair within a 16-k step
+    const int i = ib * 32 + cl;
+    const bool col = i < M;
+    const double si = col ? sc[i] : 0.0;
+    const double *a = A + (long)(col ? i : 0) * lda;
+#pragma unroll
+    for (int q = 0; q < SPT_K / 16; q++) {
+      const long k = k0 + 16 * q + kq;
+      const double v0 = (col && k < K) ? a[k] : 0.0, v1 = (col && k + 1 < K) ? a[k + 1] : 0.0;
+      tile[cl][16 * q + kq] = (float)(v0 * si);
+      tile[cl][16 * q + kq + 1] = (float)(v1 * si);
+    }
+  }
+  __syncthreads();
+  const long kc0 = k0 / 16;
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const int slot = t + 256 * u;  // 8 fragments x 64 lanes
+    const int lane = slot & 63, f = slot >> 6;
+    const int cl = lane & 31, kl = 16 * f + 8 * (lane >> 5);
+    half8 h, l;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      float v = tile[cl][kl + j];
+      asm volatile("" : "+v"(v));
+      const _Float16 hh = (_Float16)v;
+      h[j] = hh;
+      l[j] = (_Float16)(v - (float)hh);
+    }
+    const long off = (((long)ib * KC + kc0 + f) * 64 + lane) * 8;
+    *(half8 *)&xh[off] = h;
+    *(half8 *)&xl[off] = l;
+  }
+}
+
+struct TNFArgs {
+  int M1, M2;
+  const _Float16 *ah, *al, *bh, *bl;  // fragments, (M1p / 32) x KC and (M2p / 32) x KC
+  long KC;
+  float *part;  // [split][M1][M2], M2 fastest
+  int kper_st;  // stages (16 k) per split
+  int nsplit, tiles_i, tiles_j;
+};
+
+// Split-K TN from fragments: work-group = one 256 x 256 output tile over one K-range of
+// kper_st stages (FP32 accumulation depth 16 kper_st <= 4096), written as an FP32 partial
+// tile for k_tn_reduce.  The tiles of one K-range run consecutively on one XCD so that its
+// A and B fragments come from HBM once.
+template <int NT>
+__global__ __launch_bounds__(512) void k_gemm_tnf(TNFArgs p) {
+  __shared__ __attribute__((aligned(16))) FragBuf buf;
+  const int L = blockIdx.x, xcd = L & 7, jj = L >> 3;
+  const int tiles = p.tiles_i * p.tiles_j;
+  const int split = (jj / tiles) * 8 + xcd;
+  if (split >= p.nsplit) return;
+  const int tile = jj % tiles;
+  const int i0 = (tile % p.tiles_i) * 256, j0 = (tile / p.tiles_i) * 256;
+  const long st0 = (long)split * p.kper_st;
+  const int nst = (int)min((long)p.kper_st, p.KC - st0);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
+  f16v acc[2][4];
+  const long ao = ((long)(i0 / 32) * p.KC + st0) * 512, bo = ((long)(j0 / 32) * p.KC + st0) * 512;
+  frag_gemm<NT>(p.ah + ao, p.al + ao, p.bh + bo, p.bl + bo, p.KC, nst, buf, acc);
+  float *out = p.part + (long)split * p.M1 * p.M2;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int m = i0 + 64 * wi + 32 * a + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int n = j0 + 128 * wj + 32 * b + (lane & 31);
+        if (m < p.M1 && n < p.M2) out[(long)m * p.M2 + n] = acc[a][b][r];
+      }
+}
+
 }  // namespace
 
 void colscale(const double *A, long lda, long rows, int cols, double *scale, hipStream_t s) {
@@ -515,10 +634,70 @@ void colscale(const double *A, long lda, long rows, int cols, double *scale, hip
   HIPCK(hipGetLastError());
 }
 
+// TN from pre-split fragments: A diag(sa) and B diag(sb) are split once each (one pass when
+// B is A with the same scales), then k_gemm_tnf and k_tn_reduce.
+static void gemm_mixed_tn_presplit(int M1, int M2, long K, double alpha, const double *A, long lda,
+                                   const double *sa, const double *B, long ldb, const double *sb,
+                                   double beta, double *C, long ldc, int nterms, hipStream_t s) {
+  const long Kp = (K + SPT_K - 1) / SPT_K * SPT_K, KC = Kp / 16;  // whole split tiles
+  const int M1p = (M1 + 255) / 256 * 256, M2p = (M2 + 255) / 256 * 256;
+  const bool same = A == B && sa == sb && lda == ldb && M1 == M2;
+  _Float16 *ah = (_Float16 *)scratch("gemm.tnah", sizeof(_Float16) * (size_t)M1p * Kp);
+  _Float16 *al = (_Float16 *)scratch("gemm.tnal", sizeof(_Float16) * (size_t)M1p * Kp);
+  _Float16 *bh = ah, *bl = al;
+  {
+    Timed t("gemm_split", (8.0 * K + 4.0 * Kp) * (same ? M1 : M1 + M2));
+    hipLaunchKernelGGL(k_split_tn, dim3((unsigned)(Kp / SPT_K), (unsigned)(M1p / 32)), dim3(256), 0, s,
+                       A, lda, K, M1, sa, KC, ah, al);
+    if (!same) {
+      bh = (_Float16 *)scratch("gemm.tnbh", sizeof(_Float16) * (size_t)M2p * Kp);
+      bl = (_Float16 *)scratch("gemm.tnbl", sizeof(_Float16) * (size_t)M2p * Kp);
+      hipLaunchKernelGGL(k_split_tn, dim3((unsigned)(Kp / SPT_K), (unsigned)(M2p / 32)), dim3(256), 0,
+                         s, B, ldb, K, M2, sb, KC, bh, bl);
+    }
+    HIPCK(hipGetLastError());
+  }
+  TNFArgs p{};
+  p.M1 = M1; p.M2 = M2; p.ah = ah; p.al = al; p.bh = bh; p.bl = bl; p.KC = KC;
+  p.tiles_i = M1p / 256;
+  p.tiles_j = M2p / 256;
+  // stages per K-range: 256 (4096 k, FP32 accumulation depth), more if the FP32 partials
+  // would exceed 2^29 values (2 GiB)
+  const long max_split = std::max<long>(1, (1L << 29) / ((long)M1 * M2));
+  p.kper_st = (int)std::max<long>(256, (KC + max_split - 1) / max_split);
+  p.nsplit = (int)((KC + p.kper_st - 1) / p.kper_st);
+  p.part = (float *)scratch("gemm.tnpart", sizeof(float) * (size_t)p.nsplit * M1 * M2);
+  const int tiles = p.tiles_i * p.tiles_j;
+  const unsigned grid = (unsigned)((long)(p.nsplit + 7) / 8 * 8 * tiles);
+  Timed t("gemm_mixed_tn", 2.0 * (nterms >= 4 ? 4 : 3) * (double)M1 * M2 * K);  // f16 MFMA flops
+  if (nterms >= 4) hipLaunchKernelGGL(k_gemm_tnf<4>, dim3(grid), dim3(512), 0, s, p);
+  else hipLaunchKernelGGL(k_gemm_tnf<3>, dim3(grid), dim3(512), 0, s, p);
+  HIPCK(hipGetLastError());
+  hipLaunchKernelGGL(k_tn_reduce, dim3(nblk((long)M1 * M2, 256)), dim3(256), 0, s, p.part, p.nsplit,
+                     M1, M2, alpha, sa, sb, beta, C, ldc);
+  HIPCK(hipGetLastError());
+}
+
+void maxscale(const unsigned long long *cmax, int cols, double *scale, hipStream_t s) {
+  if (cols <= 0) return;
+  hipLaunchKernelGGL(k_maxscale, dim3(nblk(cols, 256)), dim3(256), 0, s, cols, cmax, scale);
+  HIPCK(hipGetLastError());
+}
+
+void normscale(const double *g, int cols, double dv, double *scale, hipStream_t s) {
+  if (cols <= 0) return;
+  hipLaunchKernelGGL(k_normscale, dim3(nblk(cols, 256)), dim3(256), 0, s, cols, g, dv, scale);
+  HIPCK(hipGetLastError());
+}
+
 void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                    const double *sa, const double *B, long ldb, const double *sb, double beta,
                    double *C, long ldc, int nterms, hipStream_t s) {
   if (M1 <= 0 || M2 <= 0 || K <= 0) throw Error(1, "gemm_mixed_tn: empty product");
+  if (param("tn_presplit", 1.0) >= 1.0) {
+    gemm_mixed_tn_presplit(M1, M2, K, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, nterms, s);
+    return;
+  }
   TNArgs p{};
   p.M1 = M1; p.M2 = M2; p.K = K; p.A = A; p.B = B; p.lda = lda; p.ldb = ldb; p.sa = sa; p.sb = sb;
   p.tiles_i = (M1 + BM - 1) / BM;

@@ -5,6 +5,11 @@
 namespace kukan {
 // scale[j] = 2^(14 - e_j) with max_i |A(i,j)| < 2^e_j (1 for a zero column)
 void colscale(const double *A, long lda, long rows, int cols, double *scale, hipStream_t s);
+// the same scales from column maxima given as FP64 bit patterns
+void maxscale(const unsigned long long *cmax, int cols, double *scale, hipStream_t s);
+// scale[j] = 2^(13 - e_j) with sqrt(g_j / dv) < 2^e_j: a bound for columns with
+// dv sum_i A(i,j)^2 = g_j, no pass over A
+void normscale(const double *g, int cols, double dv, double *scale, hipStream_t s);
 // C(M1 x M2) = alpha A^T B + beta C with column scales sa (A) and sb (B)
 void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                    const double *sa, const double *B, long ldb, const double *sb, double beta,

@@ -34,6 +34,20 @@ __device__ inline double block_sum(double v, double *red) {
   return s;
 }
 
+// Block-wide max of non-negative values (same shape rules as block_sum); result in thread 0.
+template <int NT>
+__device__ inline double block_max(double v, double *red) {
+  const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  for (int off = WAVE / 2; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, WAVE));
+  if ((t & (WAVE - 1)) == 0) red[t / WAVE] = v;
+  __syncthreads();
+  double m = 0.0;
+  if (t == 0)
+    for (int w = 0; w < NT / WAVE; w++) m = fmax(m, red[w]);
+  __syncthreads();
+  return m;
+}
+
 // ----------------------------------------------------------------------------- stencil
 struct SP {
   int nx, ny, nz, px, py, pz, halo, nfin;
@@ -164,6 +178,7 @@ struct SL {
   const int *iprj_slot;
   double *dotp;  // per band: ntile * nzch partials of <c|out> (c = centre input value)
   long dot_ld;   // band stride of dotp
+  unsigned long long *omax;  // optional: per band max |out| as FP64 bits (atomic max)
 };
 
 // Block = TY full x-rows of one band, marching over a z-range.  The 2 NF + 1 values of each
@@ -268,7 +283,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
     for (int h = 0; h < H; h++) hvv[h] = hv_ok[h] ? ld(zh + hxy[h]) : 0.0;
   };
   prefetch(z0);
-  double dsum = 0.0;
+  double dsum = 0.0, amax = 0.0;
   for (int z = z0; z < z1; z++) {
     double vcur[P];
 #pragma unroll
@@ -298,6 +313,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
 #pragma unroll
       for (int m = 1; m <= NF; m++) s += p.cz[m] * (col[q][NF - m] + col[q][NF + m]);
       if (p.dotp) dsum += c * s;
+      if (p.omax) amax = fmax(amax, fabs(s));
       const long r = zh + oxy[q];
       out[r] = p.accumulate ? out[r] + s : s;
     }
@@ -306,6 +322,10 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
   if (p.dotp) {
     const double tot = block_sum<512>(dsum, red);
     if (t == 0) p.dotp[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
+  }
+  if (p.omax) {  // non-negative doubles order like their bit patterns
+    const double m = block_max<512>(amax, red);
+    if (t == 0) atomicMax(p.omax + b, (unsigned long long)__double_as_longlong(m));
   }
 }
 
@@ -1124,7 +1144,7 @@ void launch_precond(const Grid &g, const double *in, long ld_in, bool halo_in, i
 
 void launch_precond_resid_dot(const Grid &g, const double *h, const double *sp, long ld,
                               const double *eps, double sign, double *out, int nb,
-                              double *dot_partial, hipStream_t s) {
+                              double *dot_partial, hipStream_t s, unsigned long long *out_max) {
   Timed t("precond_resid", 8.0 * (double)g.n() * nb * 3.0);
   SL q{};
   q.nx = g.nx; q.ny = g.ny; q.nz = g.nz;
@@ -1133,6 +1153,8 @@ void launch_precond_resid_dot(const Grid &g, const double *h, const double *sp, 
   q.c0 = 0.5;
   q.cx[1] = q.cy[1] = q.cz[1] = 1.0 / 12.0;
   q.dotp = dot_partial;
+  q.omax = out_max;
+  if (out_max) launch_fill(nb, 0.0, (double *)out_max, s);  // all-zero bits
   launch_lds(q, g, 1, nb, true, s);
 }
 

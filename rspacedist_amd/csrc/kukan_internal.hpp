@@ -178,7 +178,8 @@ int stencil_nparts(const Grid &g, int nf, int nb);
 // fused r = sign*(h - eps s), out = K r, optional per-band partials of <r|K r>
 void launch_precond_resid_dot(const Grid &g, const double *h, const double *sp, long ld,
                               const double *eps, double sign, double *out, int nb,
-                              double *dot_partial, hipStream_t s);
+                              double *dot_partial, hipStream_t s,
+                              unsigned long long *out_max = nullptr);
 
 void launch_halo_fill(double *v, int nperi, int na, int nb, int nc, int nf, int nf1, int nsnd,
                       hipStream_t s);

@@ -914,23 +914,40 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
     } else {
       c.dots(a.psi, n, a.hpsi, n, M, a.sval);
     }
-    // K r with r = -(H psi - eps S psi), and <r|Kr> (unprojected): residual measure and CG ratio
+    if (fused && iiter == 1) c.dots(a.psi, n, a.spsi, n, M, gg);  // later: from the band update
+    // K r with r = -(H psi - eps S psi), and <r|Kr> (unprojected): residual measure and CG ratio;
+    // for split-precision products also the per-band max |K r|
+    const bool mixed = mixed_for(n);
+    unsigned long long *krmax = mixed ? (unsigned long long *)c.buf("bcg.krmax", M) : nullptr;
     {
       const int np = stencil_nparts(c.g, 1, M);
       double *part = c.buf("bcg.kpart", (size_t)np * M);
-      launch_precond_resid_dot(c.g, a.hpsi, a.spsi, n, a.sval, -1.0, KR, M, part, c.s);
+      launch_precond_resid_dot(c.g, a.hpsi, a.spsi, n, a.sval, -1.0, KR, M, part, c.s, krmax);
       launch_reduce_partials(part, np, M, c.g.dv(), rnew, c.s);
     }
-    // Kr orthogonal to the block in the S metric: C = dV (S Psi)^T Kr ; Kr -= Psi C
-    const double *s_psi = mixed_scale("bcg.psi", a.psi, n, n, M);
-    sub_tn(c, M, M, n, dv, a.spsi, n, KR, n, zero, C, M, a.spsi == a.psi ? s_psi : nullptr, nullptr);
+    // Kr orthogonal to the block in the S metric: C = dV (S Psi)^T Kr ; Kr -= Psi C.  Split
+    // scales without passes over the bands: Psi's from dV <psi|psi> (S = 1), K r's from the
+    // fused maxima.
+    const double *s_psi = nullptr, *s_kr = nullptr;
+    if (mixed) {
+      if (fused) {
+        double *sp = c.buf("bcg.spsi", M);
+        normscale(gg, M, dv, sp, c.s);
+        s_psi = sp;
+      } else {
+        s_psi = mixed_scale("bcg.psi", a.psi, n, n, M);
+      }
+      double *sk = c.buf("bcg.skr", M);
+      maxscale(krmax, M, sk, c.s);
+      s_kr = sk;
+    }
+    sub_tn(c, M, M, n, dv, a.spsi, n, KR, n, zero, C, M, a.spsi == a.psi ? s_psi : nullptr, s_kr);
     sub_nn(c, n, M, M, mone, a.psi, n, C, M, one, KR, n, s_psi);
     // P = Kr + beta P
     hipLaunchKernelGGL(k_ratio, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, rnew, rold, doreset ? 1 : 0, tb);
     if (fused) {
       // P = Kr + beta P together with d = dV <S psi|P> and dV <P|P> (S = 1) in one pass
       launch_pupdate_dots(n, M, KR, doreset ? nullptr : tb, PA, n, a.spsi, dv, dd, pp, c.s);
-      if (iiter == 1) c.dots(a.psi, n, a.spsi, n, M, gg);  // later: from the band update
     } else {
       if (doreset) launch_copy2d(n, M, KR, n, PA, n, c.s);
       else launch_axpby(n, M, nullptr, 1.0, KR, n, tb, 1.0, PA, n, PA, n, c.s);

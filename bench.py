@@ -113,9 +113,11 @@ def run_mode(s, dev, st, kk, synth, precision, steps, it0):
 
 
 # timing tag -> kernel whose calibrated HBM bytes per launch the committed counter profile holds
-TRAFFIC_KERNEL = {"gemm_mixed_nn": "k_gemm_nn<3>", "gemm_mixed_tn": "k_gemm_tn<3>",
-                  "precond_resid": "k_stencil_lds<1, 2, true>", "gemm_scale": "k_colmax",
-                  "nl_expand": "k_expand_rows<16>", "nl_gather": "k_gather<9, 4>"}
+TRAFFIC_KERNEL = {"gemm_mixed_nn": "k_gemm_nn<3>", "gemm_mixed_tn": "k_gemm_tnf<3>",
+                  "hpsi_stencil": "k_stencil_lds<4, 2, false>", "precond_resid": "k_stencil_lds<1, 2, true>",
+                  "band_update_dots": "k_pair_update_dots", "p_update_dots": "k_pupdate_dots",
+                  "dots": "k_dot_part", "gemm_scale": "k_colmax", "nl_expand": "k_expand_rows<16>",
+                  "nl_gather": "k_gather<9, 4>"}
 
 
 def measured_traffic(tag):
@@ -247,9 +249,10 @@ def main():
     bound = {"gemm_f64": "mfma", "gemm_mixed_tn": "mfma", "gemm_mixed_nn": "mfma", "trsm_f64": "mfma", "syevd": "mfma",
              "potrf": "mfma"}.get(dominant, "hbm")
     roof = roofline_entry(tim, dominant, bound) if dominant else None
-    extra = {t: roofline_entry(tim, t, "hbm") for t in ("hpsi_stencil", "precond_resid", "dots", "axpby", "nl_gather",
-                                                        "nl_expand", "gemm_scale")
-             if t in tim}
+    extra = {t: roofline_entry(tim, t, "hbm") for t in ("band_update_dots", "p_update_dots", "hpsi_stencil",
+                                                        "precond_resid", "dots", "axpby", "nl_gather", "nl_expand",
+                                                        "gemm_scale")
+             if t in tim and t != dominant}
     for t in ("gemm_f64", "gemm_mixed_tn", "gemm_mixed_nn"):
         if t in tim and t != dominant:
             extra[t] = roofline_entry(tim, t, "mfma")

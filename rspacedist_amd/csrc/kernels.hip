@@ -1282,7 +1282,7 @@ void launch_pupdate_dots(long n, int nb, const double *x, const double *t, doubl
                          hipStream_t s) {
   if (n * nb == 0) return;
   const int np = dot_nparts(n);
-  Timed tt("axpby_dots", 8.0 * (double)n * nb * (t ? 4.0 : 3.0));
+  Timed tt("p_update_dots", 8.0 * (double)n * nb * (t ? 4.0 : 3.0));
   double *part = (double *)scratch("pud.part", sizeof(double) * (size_t)2 * np * nb);
   hipLaunchKernelGGL(k_pupdate_dots, dim3(np, nb), dim3(256), 0, s, n, x, ld, t, P, sv, part,
                      part + (long)np * nb, np);
@@ -1296,7 +1296,7 @@ void launch_pair_update_dots(long n, int nb, const double *a, const double *c, d
                              double *res1, double *res2, hipStream_t s) {
   if (n * nb == 0) return;
   const int np = dot_nparts(n);
-  Timed tt("axpby_dots", 8.0 * (double)n * nb * 6.0);
+  Timed tt("band_update_dots", 8.0 * (double)n * nb * 6.0);
   double *part = (double *)scratch("pair.part", sizeof(double) * (size_t)2 * np * nb);
   hipLaunchKernelGGL(k_pair_update_dots, dim3(np, nb), dim3(256), 0, s, n, a, c, x, y, u, w, ld,
                      part, part + (long)np * nb, np);

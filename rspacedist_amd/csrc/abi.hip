@@ -649,7 +649,7 @@ int kukan_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, in
         const size_t off1 = (size_t)n1 * neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
         const size_t offs = (size_t)neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
         reorder(c, neigmx, Sv.d() + offs, Re.d() + offs, Ps.d() + off, Sp.d() + off, Rs.d() + off1,
-                Hp.d());
+                Hp.d(), block);
       }
   }
   Rs.finish(); Ps.finish(); Sp.finish(); Hp.finish(); Sv.finish(); Re.finish();

@@ -14,7 +14,8 @@ void normscale(const double *g, int cols, double dv, double *scale, hipStream_t 
 void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                    const double *sa, const double *B, long ldb, const double *sb, double beta,
                    double *C, long ldc, int nterms, hipStream_t s);
-// Y(R x M2) = alpha X C + beta Y with column scales sx of X (length K); Y must not alias X
+// Y(R x M2) = alpha X C + beta Y with column scales sx of X (length K).  X is read only by
+// the pre-split pass that precedes the product, so Y may be X itself (same ld, beta = 0).
 void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ldx,
                    const double *sx, const double *C, long ldc, double beta, double *Y, long ldy,
                    int nterms, hipStream_t s);

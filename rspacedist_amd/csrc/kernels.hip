@@ -797,6 +797,15 @@ __global__ void k_permute(long n, int nb, const int *perm, const double *x, long
   const int b = (int)(t / n);
   y[(long)perm[b] * ldy + i] = x[(long)b * ldx + i];
 }
+// y(:, dst_j) = x(:, src_j) for column j of the lists (null list: identity)
+__global__ __launch_bounds__(256) void k_copy_cols(long n, const int *src, const int *dst,
+                                                   const double *x, long ldx, double *y, long ldy) {
+  const int j = blockIdx.y;
+  const double *xb = x + (long)(src ? src[j] : j) * ldx;
+  double *yb = y + (long)(dst ? dst[j] : j) * ldy;
+  const long stride = (long)gridDim.x * 256;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += stride) yb[i] = xb[i];
+}
 __global__ __launch_bounds__(256) void k_canon_sign(int n, double *z) {
   __shared__ double bv[256];
   __shared__ int bi[256];
@@ -1315,6 +1324,13 @@ void launch_permute_cols(long n, int nb, const int *perm, const double *x, long 
   if (n * nb == 0) return;
   hipLaunchKernelGGL(k_permute, dim3(nblk(n * nb, 256)), dim3(256), 0, s, n, nb, perm, x, ldx, y,
                      ldy);
+  HIPCK(hipGetLastError());
+}
+void launch_copy_cols(long n, int cnt, const int *src, const int *dst, const double *x, long ldx,
+                      double *y, long ldy, hipStream_t s) {
+  if (n * cnt == 0) return;
+  hipLaunchKernelGGL(k_copy_cols, dim3(ew_chunks(n, cnt), cnt), dim3(256), 0, s, n, src, dst, x, ldx,
+                     y, ldy);
   HIPCK(hipGetLastError());
 }
 void launch_canon_sign(int n, double *z, hipStream_t s) {

@@ -231,6 +231,9 @@ void launch_pair_update_dots(long n, int nb, const double *a, const double *c, d
 void launch_copy2d(long n, int nb, const double *x, long ldx, double *y, long ldy, hipStream_t s);
 void launch_fill(long n, double v, double *x, hipStream_t s);
 void launch_fill_normal(double *x, long n, unsigned long long seed, hipStream_t s);
+// y(:, dst[j]) = x(:, src[j]) for j < cnt (device index lists; null = identity)
+void launch_copy_cols(long n, int cnt, const int *src, const int *dst, const double *x, long ldx,
+                      double *y, long ldy, hipStream_t s);
 void launch_permute_cols(long n, int nb, const int *perm_dev, const double *x, long ldx,
                          double *y, long ldy, hipStream_t s);  // y(:,perm[j]) = x(:,j)
 // sign-canonicalise eigenvector columns (largest |.| positive, lowest index on ties)

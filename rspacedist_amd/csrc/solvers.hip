@@ -657,11 +657,8 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi)
       BLASCK(rocsolver_dtrtri(h, rocblas_fill_upper, rocblas_diagonal_non_unit, M, Ri, M, info));
     }
     hipLaunchKernelGGL(k_zero_lower, dim3(nblk((long)M * M, 256)), dim3(256), 0, c.s, M, Ri);
-    double *W = c.buf("tmp.nM", (size_t)n * M);
-    auto apply = [&](double *X, const double *sx) {
-      sub_nn(c, n, M, M, 1.0, X, n, Ri, M, 0.0, W, n, sx);
-      HIPCK(hipMemcpyAsync(X, W, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
-    };
+    // in place: the split product reads X only through its pre-split copy
+    auto apply = [&](double *X, const double *sx) { sub_nn(c, n, M, M, 1.0, X, n, Ri, M, 0.0, X, n, sx); };
     apply(psi, s_psi);
     if (spsi != psi) apply(spsi, nullptr);
     if (hpsi) apply(hpsi, nullptr);
@@ -739,6 +736,24 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   sync();
   if (hinfo != 0) throw Error(3, "rayleighritz_diag_r: error in matrix diagonalization");
   launch_canon_sign(M, A, c.s);
+  if (keep_h && mixed_for(n)) {
+    // block path, split products: Psi, H Psi (and S Psi) rotated in place (each product reads
+    // its X only through the pre-split copy), <p|psi> through a small buffer
+    sub_nn(c, n, M, M, one, hpsi, n, A, M, zero, hpsi, n, s_h);
+    sub_nn(c, n, M, M, one, psi, n, A, M, zero, psi, n, s_psi);
+    if (c.q_zero && spsi != psi)
+      HIPCK(hipMemcpyAsync(spsi, psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+    else if (spsi != psi)
+      sub_nn(c, n, M, M, one, spsi, n, A, M, zero, spsi, n);
+    if (n1 > 0) {
+      double *R = c.buf("rr.R", (size_t)n1 * M);
+      dgemm(rocblas_operation_none, rocblas_operation_none, (int)n1, M, M, one, rsp, (int)n1, A, M,
+            zero, R, (int)n1);
+      HIPCK(hipMemcpyAsync(rsp, R, sizeof(double) * n1 * M, hipMemcpyDeviceToDevice, c.s));
+    }
+    renormalize(c, M, psi, spsi, rsp, hpsi);
+    return;
+  }
   // rotations through hsvre as the reference does (:96-119)
   double *W = hpsi;
   double *Wh = nullptr;
@@ -1055,7 +1070,7 @@ void ksdiis(Ctx &c, const DiisArgs &a) {
 
 // ============================================================================= reorderstates
 void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *spsi, double *rsp,
-             double *hpsi) {
+             double *hpsi, bool move_h) {
   const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
   std::vector<double> sv(M), rs(M);
   HIPCK(hipMemcpyAsync(sv.data(), sval, sizeof(double) * M, hipMemcpyDeviceToHost, c.s));
@@ -1086,11 +1101,37 @@ void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *sp
     launch_permute_cols(n1, M, pd, rsp, n1, R, n1, c.s);
     HIPCK(hipMemcpyAsync(rsp, R, sizeof(double) * n1 * M, hipMemcpyDeviceToDevice, c.s));
   }
-  launch_permute_cols(n, M, pd, psi, n, hpsi, n, c.s);
-  HIPCK(hipMemcpyAsync(psi, hpsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
-  if (spsi != psi) {  // S psi may alias psi (S = 1); permute it only once
-    launch_permute_cols(n, M, pd, spsi, n, hpsi, n, c.s);
-    HIPCK(hipMemcpyAsync(spsi, hpsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+  if (!move_h) {  // hsvre is the work array, as in the reference: it ends as the permuted S psi
+    launch_permute_cols(n, M, pd, psi, n, hpsi, n, c.s);
+    HIPCK(hipMemcpyAsync(psi, hpsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+    if (spsi != psi) {  // S psi may alias psi (S = 1); then hpsi already holds it permuted
+      launch_permute_cols(n, M, pd, spsi, n, hpsi, n, c.s);
+      HIPCK(hipMemcpyAsync(spsi, hpsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+    }
+    sync();
+    return;
+  }
+  // block path: H psi is kept consistent with psi, and only the bands that change place move
+  std::vector<int> mv;
+  for (int j = 0; j < M; j++)
+    if (perm[j] != j) mv.push_back(j);
+  if (!mv.empty()) {
+    const int cnt = (int)mv.size();
+    std::vector<int> lists(2 * cnt);
+    for (int q = 0; q < cnt; q++) {
+      lists[q] = mv[q];
+      lists[cnt + q] = perm[mv[q]];
+    }
+    int *ld = (int *)scratch("reo.lists", sizeof(int) * 2 * cnt);
+    HIPCK(hipMemcpyAsync(ld, lists.data(), sizeof(int) * 2 * cnt, hipMemcpyHostToDevice, c.s));
+    double *tmp = c.buf("reo.tmp", (size_t)n * cnt);
+    auto move = [&](double *x) {
+      launch_copy_cols(n, cnt, ld, nullptr, x, n, tmp, n, c.s);
+      launch_copy_cols(n, cnt, nullptr, ld + cnt, tmp, n, x, n, c.s);
+    };
+    move(psi);
+    if (spsi != psi) move(spsi);  // S psi may alias psi (S = 1); move it only once
+    move(hpsi);
   }
   sync();
 }

@@ -68,8 +68,11 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
 void kscg_sequential(Ctx &c, const KscgArgs &a);
 void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first);
 void ksdiis(Ctx &c, const DiisArgs &a);
+// reorderstates8f.F90:8-124.  move_h = false: hpsi is the work array and ends as the permuted
+// S psi (the reference's hsvre); true (block path): hpsi holds H psi and is permuted with psi,
+// and only displaced bands are moved.
 void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *spsi, double *rsp,
-             double *hpsi);
+             double *hpsi, bool move_h = false);
 int hartree(const HartreeArgs &a);
 void correct_potential(long n, double dvol_over_omega, double *vh);
 void charge(long n, int M, const double *psi, const double *f, double *rho);

@@ -114,7 +114,8 @@ def run_mode(s, dev, st, kk, synth, precision, steps, it0):
 
 # timing tag -> kernel whose calibrated HBM bytes per launch the committed counter profile holds
 TRAFFIC_KERNEL = {"gemm_mixed_nn": "k_gemm_nn<3>", "gemm_mixed_tn": "k_gemm_tnf<3>",
-                  "hpsi_stencil": "k_stencil_lds<4, 2, false>", "precond_resid": "k_stencil_lds<1, 2, true>",
+                  "hpsi_stencil": "k_stencil_lds<4, 2, 0>", "hp_comb_stencil": "k_stencil_lds<4, 1, 2>",
+                  "precond_resid": "k_stencil_lds<1, 2, 1>",
                   "band_update_dots": "k_pair_update_dots", "p_update_dots": "k_pupdate_dots",
                   "dots": "k_dot_part", "gemm_scale": "k_colmax", "nl_expand": "k_expand_rows<16>",
                   "nl_gather": "k_gather<9, 4>"}
@@ -250,7 +251,7 @@ def main():
              "potrf": "mfma"}.get(dominant, "hbm")
     roof = roofline_entry(tim, dominant, bound) if dominant else None
     extra = {t: roofline_entry(tim, t, "hbm") for t in ("band_update_dots", "p_update_dots", "hpsi_stencil",
-                                                        "precond_resid", "dots", "axpby", "nl_gather", "nl_expand",
+                                                        "hp_comb_stencil", "precond_resid", "dots", "axpby", "nl_gather", "nl_expand",
                                                         "gemm_scale")
              if t in tim and t != dominant}
     for t in ("gemm_f64", "gemm_mixed_tn", "gemm_mixed_nn"):

@@ -156,7 +156,8 @@ __global__ __launch_bounds__(256) void k_stencil(SP p) {
 //     (x wraps inside the row), giving the x and y neighbours;
 //   * blocks are mapped XCD-aware: all tiles and z-ranges of a band run on one XCD
 //     (block id mod 8), so the halo rows and veff/projector reads hit that XCD's L2;
-//   * RES = true reads sign*(h - eps_b s) instead of a band (fused residual for K r).
+//   * MODE 1 reads sign*(h - eps_b s) instead of a band (fused residual for K r); MODE 2 reads
+//     ca_b x + cb_b y (a combination of two bands formed on the fly) and also sums <y|out>.
 struct SL {
   int nx, ny, nz, px, py, pz, TY, ntile, nzch, zc, nbands;
   const double *in, *in2;
@@ -179,6 +180,8 @@ struct SL {
   double *dotp;  // per band: ntile * nzch partials of <c|out> (c = centre input value)
   long dot_ld;   // band stride of dotp
   unsigned long long *omax;  // optional: per band max |out| as FP64 bits (atomic max)
+  const double *ca, *cb;     // MODE 2: the input is ca_b in + cb_b in2
+  double *dotp2;             // MODE 2: partials of <in2|out> (in2 at the centre), as dotp
 };
 
 // Block = TY full x-rows of one band, marching over a z-range.  The 2 NF + 1 values of each
@@ -188,17 +191,19 @@ struct SL {
 // V_eff(z + 1) are issued before plane z is computed (one plane of prefetch), and the barriers
 // wait only for LDS, so HBM latency overlaps the arithmetic.
 // One work-group's share: band b, y-tile `tile`, z-range `zch` (all arguments uniform).
-template <int NF, int P, bool RES>
+template <int NF, int P, int MODE>
 __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, double *lds,
                                      double *red) {
+  constexpr bool RES = MODE == 1, COMB = MODE == 2;
   constexpr int H = NF;  // halo values per thread: 2 NF rows x nx <= 512 NF for nx <= 256
   const int nx = p.nx, ny = p.ny, nz = p.nz, TY = p.TY, W = nx + 2 * NF;
   const long plane = (long)nx * ny;
   const int per = p.ntile * p.nzch;
   const int y0 = tile * TY, z0 = zch * p.zc, z1 = min(nz, z0 + p.zc);
   const double *in = p.in + (long)b * p.ld_in;
-  const double *in2 = RES ? p.in2 + (long)b * p.ld_in : nullptr;
+  const double *in2 = (RES || COMB) ? p.in2 + (long)b * p.ld_in : nullptr;
   const double e = RES ? p.eps[b] : 0.0;
+  const double fa = COMB ? p.ca[b] : 0.0, fb = COMB ? p.cb[b] : 0.0;
   double *out = p.out + (long)b * p.ld_out;
   const int t = threadIdx.x;
   // y of a row index relative to the grid, wrapped once when periodic; -1 = zero row
@@ -218,6 +223,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
   };
   auto ld = [&](long off) -> double {
     if (RES) return p.sign * (in[off] - e * in2[off]);
+    if (COMB) return fa * in[off] + fb * in2[off];
     return in[off];
   };
   // tile points
@@ -271,26 +277,32 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
 #pragma unroll
     for (int q = 0; q < P; q++) col[q][m + 1] = (lv[q] && zo >= 0) ? ld(zo + oxy[q]) : 0.0;
   }
+  // MODE 2: the second band's own value at the centre (for <in2|out>), prefetched like V_eff;
+  // it was read NF planes earlier for the combination, so it usually comes from L2
+  constexpr int YP = COMB ? P : 1;
+  double yf[YP];
   double nxt[P], hvv[H], vf[P];
   auto prefetch = [&](int z) {
     const long zn = zoff(z + NF), zh = (long)z * plane;
 #pragma unroll
     for (int q = 0; q < P; q++) {
       nxt[q] = (lv[q] && zn >= 0) ? ld(zn + oxy[q]) : 0.0;
+      if (COMB) yf[q] = sv[q] ? in2[zh + oxy[q]] : 0.0;
       vf[q] = (p.veff && sv[q]) ? p.veff[zh + oxy[q]] : 0.0;
     }
 #pragma unroll
     for (int h = 0; h < H; h++) hvv[h] = hv_ok[h] ? ld(zh + hxy[h]) : 0.0;
   };
   prefetch(z0);
-  double dsum = 0.0, amax = 0.0;
+  double dsum = 0.0, dsum2 = 0.0, amax = 0.0;
   for (int z = z0; z < z1; z++) {
-    double vcur[P];
+    double vcur[P], ycur[YP];
 #pragma unroll
     for (int q = 0; q < P; q++) {
 #pragma unroll
       for (int m = 0; m < 2 * NF; m++) col[q][m] = col[q][m + 1];
       col[q][2 * NF] = nxt[q];
+      if (COMB) ycur[q] = yf[q];
       vcur[q] = vf[q];
       if (t + 512 * q < nx * TY) put(lcol[q], xq[q], col[q][NF]);
     }
@@ -313,6 +325,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
 #pragma unroll
       for (int m = 1; m <= NF; m++) s += p.cz[m] * (col[q][NF - m] + col[q][NF + m]);
       if (p.dotp) dsum += c * s;
+      if (COMB) dsum2 += ycur[q] * s;
       if (p.omax) amax = fmax(amax, fabs(s));
       const long r = zh + oxy[q];
       out[r] = p.accumulate ? out[r] + s : s;
@@ -323,14 +336,18 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
     const double tot = block_sum<512>(dsum, red);
     if (t == 0) p.dotp[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
   }
+  if (COMB) {
+    const double tot = block_sum<512>(dsum2, red);
+    if (t == 0) p.dotp2[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
+  }
   if (p.omax) {  // non-negative doubles order like their bit patterns
     const double m = block_max<512>(amax, red);
     if (t == 0) atomicMax(p.omax + b, (unsigned long long)__double_as_longlong(m));
   }
 }
 
-template <int NF, int P, bool RES>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && P <= 2 ? 4 : 2))) void k_stencil_lds(SL p) {
+template <int NF, int P, int MODE>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && (MODE == 2 ? P == 1 : P <= 2) ? 4 : 2))) void k_stencil_lds(SL p) {
   extern __shared__ double lds[];
   __shared__ double red[8];
   if (p.skip && *p.skip) return;  // uniform: a device-side loop has finished
@@ -347,13 +364,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && 
     b = (int)(it / per);
     rem = (int)(it % per);
   }
-  stencil_block<NF, P, RES>(p, b, rem % p.ntile, rem / p.ntile, lds, red);
+  stencil_block<NF, P, MODE>(p, b, rem % p.ntile, rem / p.ntile, lds, red);
 }
 
 // Several independent grids in one persistent launch: work-group g takes items g, g + G,
 // ... of the concatenated item lists (system-major; an item is one band x y-tile x z-range
 // of one system, described by sl[s]).  start[s] = first item of system s, start[nsys] = total.
-template <int NF, int P, bool RES>
+template <int NF, int P, int MODE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && P <= 2 ? 4 : 2))) void k_stencil_batch(const SL *sl, const long *start, int nsys) {
   extern __shared__ double lds[];
   __shared__ double red[8];
@@ -366,7 +383,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && 
     const long j = it - start[s];
     const int per = p.ntile * p.nzch;
     const int b = (int)(j / per), rem = (int)(j % per);
-    stencil_block<NF, P, RES>(p, b, rem % p.ntile, rem / p.ntile, lds, red);
+    stencil_block<NF, P, MODE>(p, b, rem % p.ntile, rem / p.ntile, lds, red);
     __syncthreads();  // the next item reuses the LDS slab
   }
 }
@@ -562,7 +579,9 @@ __global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz
                                                      const double *vnl, int num_list, int nprjmx,
                                                      const double *coef, int nb, double *out,
                                                      long ld_out, const double *psi, long ld_psi,
-                                                     double *part, long part_ld, int part_off) {
+                                                     double *part, long part_ld, int part_off,
+                                                     const double *in2, const double *ca,
+                                                     const double *cb, double *part2) {
   // coef is [slot][j][b] (bands fastest): the EB coefficients of one (slot, j) are contiguous
   // and shared by all rows of the same atom, so they come from L1.
   __shared__ double red[EB][4];
@@ -602,7 +621,11 @@ __global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < EB; q++) {
-      double d = (r >= 0 && b0 + q < nb) ? psi[(long)(b0 + q) * ld_psi + r] * acc[q] : 0.0;
+      double d = 0.0;
+      if (r >= 0 && b0 + q < nb) {
+        const long o = (long)(b0 + q) * ld_psi + r;
+        d = (in2 ? ca[b0 + q] * psi[o] + cb[b0 + q] * in2[o] : psi[o]) * acc[q];
+      }
       d = wave_sum(d);
       if (lane == 0) red[q][wv] = d;
     }
@@ -610,6 +633,20 @@ __global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz
     if (threadIdx.x < EB && b0 + (int)threadIdx.x < nb) {
       const int q = threadIdx.x;
       part[(long)(b0 + q) * part_ld + part_off + blockIdx.x] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
+    }
+    if (in2) {  // <in2|added>
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < EB; q++) {
+        double d = (r >= 0 && b0 + q < nb) ? in2[(long)(b0 + q) * ld_psi + r] * acc[q] : 0.0;
+        d = wave_sum(d);
+        if (lane == 0) red[q][wv] = d;
+      }
+      __syncthreads();
+      if (threadIdx.x < EB && b0 + (int)threadIdx.x < nb) {
+        const int q = threadIdx.x;
+        part2[(long)(b0 + q) * part_ld + part_off + blockIdx.x] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
+      }
     }
   }
 }
@@ -755,24 +792,27 @@ __global__ __launch_bounds__(256) void k_pupdate_dots(long n, const double *x, l
   }
 }
 
-// x = a_b x + c_b y and u = a_b u + c_b w; part1 = sum x.x, part2 = sum x.u (new values).
+// x = a_b x + c_b y and u = a2_b u + c2_b w (a2, c2 null: a, c); part1 = sum x.x,
+// part2 = sum x.u (new values).
 __global__ __launch_bounds__(256) void k_pair_update_dots(long n, const double *a,
                                                           const double *c, double *x,
                                                           const double *y, double *u,
                                                           const double *w, long ld,
                                                           double *part1, double *part2,
-                                                          int nparts) {
+                                                          int nparts, const double *a2,
+                                                          const double *c2) {
   __shared__ double red[4];
   const int b = blockIdx.y;
   const long chunk = (n + nparts - 1) / nparts;
   const long i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
   const double ab = a[b], cb = c[b];
+  const double ab2 = a2 ? a2[b] : ab, cb2 = c2 ? c2[b] : cb;
   double *xb = x + (long)b * ld, *ub = u + (long)b * ld;
   const double *yb = y + (long)b * ld, *wb = w + (long)b * ld;
   double s1 = 0.0, s2 = 0.0;
   for (long i = i0 + threadIdx.x; i < i1; i += 256) {
     const double xv = ab * xb[i] + cb * yb[i];
-    const double uv = ab * ub[i] + cb * wb[i];
+    const double uv = ab2 * ub[i] + cb2 * wb[i];
     xb[i] = xv;
     ub[i] = uv;
     s1 += xv * xv;
@@ -913,35 +953,47 @@ static LdsPlan lds_plan(const Grid &g, int NF, int nb, int force_p = 0) {
 }
 static bool lds_ok(const Grid &g) { return g.nx <= 256 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8; }
 
-int stencil_parts_lds(const Grid &g, int NF, int nb) {
-  LdsPlan L = lds_plan(g, NF, nb);
+// points per thread of the combination-form stencil (param comb_p, default 1)
+static int comb_points() { return (int)param("comb_p", 1.0); }
+
+int stencil_parts_lds(const Grid &g, int NF, int nb, bool comb = false) {
+  LdsPlan L = lds_plan(g, NF, nb, comb ? comb_points() : 0);
   return L.ntile * L.nzch;
 }
 
-template <int NF, bool RES>
+template <int NF, int MODE>
 static void launch_lds_nf(const SL &p, const LdsPlan &L, int nb, hipStream_t s) {
   const size_t shm = sizeof(double) * (size_t)(L.TY + 2 * NF) * (p.nx + 2 * NF);
   const long per = (long)L.ntile * L.nzch;
   // nb >= 8: band groups of 8 (one band per XCD); fewer: the items split over the 8 XCDs
   const unsigned grid = (unsigned)(nb >= 8 ? (long)(nb + 7) / 8 * 8 * per : 8 * ((nb * per + 7) / 8));
   switch (L.P) {
-    case 1: hipLaunchKernelGGL((k_stencil_lds<NF, 1, RES>), dim3(grid), dim3(512), shm, s, p); break;
-    case 2: hipLaunchKernelGGL((k_stencil_lds<NF, 2, RES>), dim3(grid), dim3(512), shm, s, p); break;
-    default: hipLaunchKernelGGL((k_stencil_lds<NF, 4, RES>), dim3(grid), dim3(512), shm, s, p); break;
+    case 1: hipLaunchKernelGGL((k_stencil_lds<NF, 1, MODE>), dim3(grid), dim3(512), shm, s, p); break;
+    case 2: hipLaunchKernelGGL((k_stencil_lds<NF, 2, MODE>), dim3(grid), dim3(512), shm, s, p); break;
+    default: hipLaunchKernelGGL((k_stencil_lds<NF, 4, MODE>), dim3(grid), dim3(512), shm, s, p); break;
   }
 }
 
 // Generic LDS launch: coefficients in p; in2/eps for the residual mode.
 static void launch_lds(SL &p, const Grid &g, int NF, int nb, bool res, hipStream_t s) {
-  LdsPlan L = lds_plan(g, NF, nb);
+  // the combination form holds two loads per prefetched value: one point per thread keeps it
+  // within the register budget of four waves per SIMD
+  LdsPlan L = lds_plan(g, NF, nb, p.ca ? comb_points() : 0);
   if (L.P > 4) throw Error(1, "stencil tile does not fit the thread block");
   p.TY = L.TY; p.ntile = L.ntile; p.nzch = L.nzch; p.zc = L.zc; p.nbands = nb;
   if (res) {
     if (NF != 1) throw Error(1, "fused residual stencil supports radius 1 only");
-    launch_lds_nf<1, true>(p, L, nb, s);
+    launch_lds_nf<1, 1>(p, L, nb, s);
+  } else if (p.ca) {
+    switch (NF) {
+#define CASE(N) case N: launch_lds_nf<N, 2>(p, L, nb, s); break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+      default: throw Error(1, "error in kslaplacian_r! This subroutine is for nf in 1-8.");
+    }
   } else {
     switch (NF) {
-#define CASE(N) case N: launch_lds_nf<N, false>(p, L, nb, s); break;
+#define CASE(N) case N: launch_lds_nf<N, 0>(p, L, nb, s); break;
       CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
       default: throw Error(1, "error in kslaplacian_r! This subroutine is for nf in 1-8.");
@@ -954,8 +1006,8 @@ bool stencil_band_fastest_coef(const Grid &g, bool halo_in) {
   return !halo_in && lds_ok(g) && param("stencil_lds", 1.0) >= 1.0;
 }
 
-int stencil_nparts(const Grid &g, int nf, int nb) {
-  if (lds_ok(g) && param("stencil_lds", 1.0) >= 1.0) return stencil_parts_lds(g, nf, nb);
+int stencil_nparts(const Grid &g, int nf, int nb, bool comb) {
+  if (lds_ok(g) && param("stencil_lds", 1.0) >= 1.0) return stencil_parts_lds(g, nf, nb, comb);
   const int zc = 8;
   return (int)(nblk(g.nx, 64) * nblk(g.ny, 4) * nblk(g.nz, zc));
 }
@@ -1001,14 +1053,27 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
     q.num_list = p.num_list; q.nprjmx = p.nprjmx; q.iprj_slot = p.iprj_slot;
     q.dotp = c.dot_partial;
     q.dot_ld = c.dot_ld;
-    launch_lds(q, g, nf, c.nb, false, s);
+    if (c.ca) {
+      if (!c.dot_partial || !c.dot_partial2 || c.accumulate)
+        throw Error(1, "stencil: the combination form needs both partial buffers and no accumulate");
+      q.in2 = c.in2; q.ca = c.ca; q.cb = c.cb; q.dotp2 = c.dot_partial2;
+    }
+    if (c.tag) {
+      Timed t(c.tag, 8.0 * ((double)c.nb * (c.ca ? 3.0 : 2.0) * g.n() + (c.veff ? g.n() : 0)));
+      launch_lds(q, g, nf, c.nb, false, s);
+    } else {
+      launch_lds(q, g, nf, c.nb, false, s);
+    }
     if (c.pl && c.coef && c.pl->nent > 0) {  // nonlocal part: separate owner-computes pass
-      const long pld = c.dot_ld ? c.dot_ld : stencil_parts_lds(g, nf, c.nb);
+      const bool comb = c.ca != nullptr;
+      const long pld = c.dot_ld ? c.dot_ld : stencil_parts_lds(g, nf, c.nb, comb);
       launch_expand_rows(*c.pl, c.coef, c.nb, c.out, c.ld_out, c.in, c.ld_in, c.dot_partial, pld,
-                         stencil_parts_lds(g, nf, c.nb), s);
+                         stencil_parts_lds(g, nf, c.nb, comb), s, comb ? c.in2 : nullptr, c.ca, c.cb,
+                         c.dot_partial2);
     }
     return;
   }
+  if (c.ca) throw Error(1, "stencil: the combination form runs on the tiled path only");
   p.zc = 8;
   p.nzch = (int)nblk(g.nz, p.zc);
   p.dotp = c.dot_partial;
@@ -1107,16 +1172,16 @@ void launch_stencil_batch(const StencilBatchPlan &b, int grid, hipStream_t s) {
   const SL *sl = (const SL *)b.sl;
   if (b.res) {
     switch (b.P) {
-      case 1: hipLaunchKernelGGL((k_stencil_batch<1, 1, true>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
-      case 2: hipLaunchKernelGGL((k_stencil_batch<1, 2, true>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
-      default: hipLaunchKernelGGL((k_stencil_batch<1, 4, true>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
+      case 1: hipLaunchKernelGGL((k_stencil_batch<1, 1, 1>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
+      case 2: hipLaunchKernelGGL((k_stencil_batch<1, 2, 1>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
+      default: hipLaunchKernelGGL((k_stencil_batch<1, 4, 1>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
     }
   } else {
     if (b.nf != 4) throw Error(1, "batched stencil: radius 4 only");
     switch (b.P) {
-      case 1: hipLaunchKernelGGL((k_stencil_batch<4, 1, false>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
-      case 2: hipLaunchKernelGGL((k_stencil_batch<4, 2, false>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
-      default: hipLaunchKernelGGL((k_stencil_batch<4, 4, false>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
+      case 1: hipLaunchKernelGGL((k_stencil_batch<4, 1, 0>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
+      case 2: hipLaunchKernelGGL((k_stencil_batch<4, 2, 0>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
+      default: hipLaunchKernelGGL((k_stencil_batch<4, 4, 0>), dim3(G), dim3(512), b.shm, s, sl, b.start, b.nsys); break;
     }
   }
   HIPCK(hipGetLastError());
@@ -1227,19 +1292,22 @@ int expand_nparts(const NonlocalPlan &pl) { return (int)std::max<long>(1, (pl.nn
 
 void launch_expand_rows(const NonlocalPlan &pl, const double *coef, int nb, double *out,
                         long ld_out, const double *psi, long ld_psi, double *part, long part_ld,
-                        int part_off, hipStream_t s) {
+                        int part_off, hipStream_t s, const double *in2, const double *ca,
+                        const double *cb, double *part2) {
   if (nb == 0) return;
   const int np = expand_nparts(pl);
   if (pl.nnz == 0) {
     if (part)
       for (int b = 0; b < nb; b++) launch_fill(1, 0.0, part + (long)b * part_ld + part_off, s);
+    if (part2)
+      for (int b = 0; b < nb; b++) launch_fill(1, 0.0, part2 + (long)b * part_ld + part_off, s);
     return;
   }
   Timed t("nl_expand", 8.0 * (2.0 * (double)pl.nnz * nb + (double)pl.nprjmx * pl.nent));
   constexpr int EB = 16;
   hipLaunchKernelGGL(k_expand_rows<EB>, dim3(np, nblk(nb, EB)), dim3(256), 0, s, pl.nzrows, pl.nnz,
                      pl.row_ptr, pl.ent, pl.iprj_slot, pl.vnl, pl.num_list, pl.nprjmx, coef, nb,
-                     out, ld_out, psi, ld_psi, part, part_ld, part_off);
+                     out, ld_out, psi, ld_psi, part, part_ld, part_off, in2, ca, cb, part2);
   HIPCK(hipGetLastError());
 }
 
@@ -1254,7 +1322,7 @@ void launch_dots(long n, const double *x, long ldx, const double *y, long ldy, i
                  double scale, double *res, hipStream_t s) {
   if (nb == 0) return;
   const int np = dot_nparts(n);
-  Timed t("dots", 16.0 * (double)n * nb);
+  Timed t("dots", (x == y && ldx == ldy ? 8.0 : 16.0) * (double)n * nb);
   double *part = (double *)scratch("dots.part", sizeof(double) * (size_t)np * nb);
   hipLaunchKernelGGL(k_dot_part, dim3(np, nb), dim3(256), 0, s, n, x, ldx, y, ldy, part, np);
   hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part, np, scale, res);
@@ -1302,13 +1370,14 @@ void launch_pupdate_dots(long n, int nb, const double *x, const double *t, doubl
 }
 void launch_pair_update_dots(long n, int nb, const double *a, const double *c, double *x,
                              const double *y, double *u, const double *w, long ld, double scale,
-                             double *res1, double *res2, hipStream_t s) {
+                             double *res1, double *res2, hipStream_t s, const double *a2,
+                             const double *c2) {
   if (n * nb == 0) return;
   const int np = dot_nparts(n);
   Timed tt("band_update_dots", 8.0 * (double)n * nb * 6.0);
   double *part = (double *)scratch("pair.part", sizeof(double) * (size_t)2 * np * nb);
   hipLaunchKernelGGL(k_pair_update_dots, dim3(np, nb), dim3(256), 0, s, n, a, c, x, y, u, w, ld,
-                     part, part + (long)np * nb, np);
+                     part, part + (long)np * nb, np, a2, c2);
   hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part, np, scale, res1);
   hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part + (long)np * nb, np, scale,
                      res2);

@@ -154,6 +154,12 @@ struct StencilCall {
   // residual-preconditioner form (batched plans only): in = sign (in - eps_b in2)
   const double *in2 = nullptr, *eps = nullptr;
   double sign = 1.0;
+  // combination form (tiled path only): the input band is ca_b in + cb_b in2, never stored;
+  // dot_partial2 (same layout as dot_partial) receives the partials of <in2|out>
+  const double *ca = nullptr, *cb = nullptr;
+  double *dot_partial2 = nullptr;
+  // timing tag of the stencil kernel (tiled path); work = input bands in + out, plus V_eff
+  const char *tag = nullptr;
 };
 
 // Several systems' stencils as one persistent launch (device tables built once, so the
@@ -174,7 +180,7 @@ void launch_stencil(const StencilCall &c, hipStream_t s);
 // true when launch_stencil takes the tiled path, whose projector pass reads band-fastest coef
 bool stencil_band_fastest_coef(const Grid &g, bool halo_in);
 // partial-sum layout of launch_stencil's dot_partial: nb * nparts
-int stencil_nparts(const Grid &g, int nf, int nb);
+int stencil_nparts(const Grid &g, int nf, int nb, bool comb = false);  // comb: combination form
 // fused r = sign*(h - eps s), out = K r, optional per-band partials of <r|K r>
 void launch_precond_resid_dot(const Grid &g, const double *h, const double *sp, long ld,
                               const double *eps, double sign, double *out, int nb,
@@ -206,7 +212,11 @@ void launch_expand(const NonlocalPlan &pl, long n, const double *coef, long ld_c
 int expand_nparts(const NonlocalPlan &pl);
 void launch_expand_rows(const NonlocalPlan &pl, const double *coef, int nb, double *out,
                         long ld_out, const double *psi, long ld_psi, double *part, long part_ld,
-                        int part_off, hipStream_t s);  // coef band-fastest [slot][j][b]
+                        int part_off, hipStream_t s,
+                        const double *in2 = nullptr, const double *ca = nullptr,
+                        const double *cb = nullptr, double *part2 = nullptr);
+// (coef band-fastest [slot][j][b]; with in2: the partials use ca_b psi + cb_b in2 as the band
+// and part2 receives <in2|added>)
 
 // batched reductions: res[b] = scale * sum_i x_b[i] * y_b[i] (deterministic two-stage)
 void launch_dots(long n, const double *x, long ldx, const double *y, long ldy, int nb,
@@ -224,10 +234,12 @@ void launch_scale(long n, int nb, const double *f, double *x, long ldx, hipStrea
 void launch_pupdate_dots(long n, int nb, const double *x, const double *t, double *P, long ld,
                          const double *sv, double scale, double *res1, double *res2,
                          hipStream_t s);
-// x = a_b x + c_b y, u = a_b u + c_b w; res1[b] = scale <x_b|x_b>, res2[b] = scale <x_b|u_b>
+// x = a_b x + c_b y, u = a2_b u + c2_b w (a2, c2 null: a, c); res1[b] = scale <x_b|x_b>,
+// res2[b] = scale <x_b|u_b>
 void launch_pair_update_dots(long n, int nb, const double *a, const double *c, double *x,
                              const double *y, double *u, const double *w, long ld, double scale,
-                             double *res1, double *res2, hipStream_t s);
+                             double *res1, double *res2, hipStream_t s,
+                             const double *a2 = nullptr, const double *c2 = nullptr);
 void launch_copy2d(long n, int nb, const double *x, long ldx, double *y, long ldy, hipStream_t s);
 void launch_fill(long n, double v, double *x, hipStream_t s);
 void launch_fill_normal(double *x, long n, unsigned long long seed, hipStream_t s);

@@ -99,6 +99,15 @@ __global__ void k_pnorm(int nb, const double *d, const double *pp, const double 
   mdfac[b] = -d[b] * f;
 }
 
+// psi <- al psi + be p with p = fac P + mdf psi, as psi <- (al + be mdf) psi + (be fac) P
+__global__ void k_comb_coef(int nb, const double *al, const double *be, const double *fac,
+                            const double *mdf, double *a2, double *c2) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  a2[b] = al[b] + be[b] * mdf[b];
+  c2[b] = be[b] * fac[b];
+}
+
 // ksdiis8f.F90:285-600 per band, nitermax = 2 (one trial step), expressed on the basis
 // (phi0, Kr): returns coefficients (ca, cb) of the new band and its residual measure.
 // Inputs per band (already multiplied by dV): e = eps, hkk=<Kr|HKr>, h0k=<phi0|HKr>,
@@ -522,13 +531,28 @@ void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, 
     c.dot_ld = np_total;
     np = np_total;
   }
-  {
-    // algorithmic bytes: psi in + H psi out per band, plus veff and the projector tables once
-    const double nsum = (double)pl.nent;
-    Timed t("hpsi_stencil", 8.0 * ((double)nb * (2.0 * g.n() + nsum) + g.n() + pl.nprjmx * nsum));
-    launch_stencil(c, s);
-  }
+  c.tag = "hpsi_stencil";  // the stencil kernel; the projector pass has its own tag
+  launch_stencil(c, s);
   if (eps_out) launch_reduce_partials(part, np, nb, g.dv(), eps_out, s);
+}
+
+void Ctx::hpsi_comb(int ns, const double *veffs, const double *x, const double *y, long ld, int nb,
+                    const double *ca, const double *cb, const double *rp, long ld_rp, double *out,
+                    double *pap, double *sap) {
+  const long n1 = (long)pl.nprjmx * pl.num_ppcell;
+  double *coef = buf("coef", (size_t)std::max<long>(1, n1) * nb);
+  launch_coef(pl, rp, ld_rp, nb, dij, false, ns, 1.0 / g.dv(), coef, n1, s, true);
+  const int np = stencil_nparts(g, g.nf, nb, true) + expand_nparts(pl);
+  double *part = buf("hpsi.part", (size_t)2 * np * nb);
+  launch_fill((long)2 * np * nb, 0.0, part, s);
+  StencilCall c;
+  c.g = &g; c.in = x; c.in2 = y; c.ca = ca; c.cb = cb; c.ld_in = ld; c.out = out; c.ld_out = ld;
+  c.veff = veffs; c.nb = nb; c.pl = &pl; c.coef = coef; c.ld_coef = n1; c.coef_band_fastest = true;
+  c.dot_partial = part; c.dot_partial2 = part + (long)np * nb; c.dot_ld = np;
+  c.tag = "hp_comb_stencil";
+  launch_stencil(c, s);
+  launch_reduce_partials(part, np, nb, g.dv(), pap, s);
+  launch_reduce_partials(part + (long)np * nb, np, nb, g.dv(), sap, s);
 }
 
 void Ctx::spsi(const double *psi, long ld, int nb, const double *rp, long ld_rp, double *out,
@@ -909,6 +933,8 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
   // S = 1: fuse the direction update with its dot products and the band update with the next
   // iteration's <psi|psi> and <psi|H psi> (same arithmetic, fewer passes over HBM)
   const bool fused = c.q_zero;
+  // the combination stencil (p formed on the fly) needs the tiled stencil path
+  const bool comb = fused && stencil_band_fastest_coef(c.g, false) && param("bcg_comb", 1.0) >= 1.0;
   double *eh_next = c.buf("bcg.eh", (size_t)M);
   int *active = (int *)scratch("bcg.active", sizeof(int) * M);
   std::vector<int> act_h(M, 1);
@@ -978,12 +1004,26 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
       c.dots(a.psi, n, a.spsi, n, M, gg);
     }
     hipLaunchKernelGGL(k_pnorm, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, dd, pp, gg, fac, mdf, c.flag(1));
-    launch_axpby(n, M, fac, 1.0, PA, n, mdf, 1.0, a.psi, n, KR, n, c.s);  // KR <- p
-    c.proj(KR, n, M, rpp, n1);
-    c.hpsi(a.ns, a.veffs, KR, n, M, rpp, n1, HP, n, false, pap);
-    c.dots(a.psi, n, HP, n, M, sap);
+    if (comb) {
+      // p = fac P + mdf psi is never stored: <proj|p> = fac <proj|P> + mdf <proj|psi> (rsp),
+      // and one stencil pass forms p on the fly and gives H p, dV <p|H p>, dV <psi|H p>
+      c.proj(PA, n, M, rpp, n1);
+      if (n1 > 0) launch_axpby(n1, M, fac, 1.0, rpp, n1, mdf, 1.0, a.rsp, n1, rpp, n1, c.s);
+      c.hpsi_comb(a.ns, a.veffs, PA, a.psi, n, M, fac, mdf, rpp, n1, HP, pap, sap);
+    } else {
+      launch_axpby(n, M, fac, 1.0, PA, n, mdf, 1.0, a.psi, n, KR, n, c.s);  // KR <- p
+      c.proj(KR, n, M, rpp, n1);
+      c.hpsi(a.ns, a.veffs, KR, n, M, rpp, n1, HP, n, false, pap);
+      c.dots(a.psi, n, HP, n, M, sap);
+    }
     hipLaunchKernelGGL(k_kscg05, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, a.sval, pap, sap, active, al, be);
-    if (fused) {
+    if (comb) {
+      // psi <- al psi + be p = (al + be mdf) psi + (be fac) P; H psi <- al H psi + be H p;
+      // with dV <psi|psi> and dV <psi|H psi> of the new bands
+      double *a2 = c.buf("bcg.a2", (size_t)2 * M), *c2 = a2 + M;
+      hipLaunchKernelGGL(k_comb_coef, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, al, be, fac, mdf, a2, c2);
+      launch_pair_update_dots(n, M, a2, c2, a.psi, PA, a.hpsi, HP, n, dv, gg, eh_next, c.s, al, be);
+    } else if (fused) {
       // psi, H psi updated together, with dV <psi|psi> and dV <psi|H psi> of the new bands
       launch_pair_update_dots(n, M, al, be, a.psi, KR, a.hpsi, HP, n, dv, gg, eh_next, c.s);
     } else {

@@ -21,6 +21,11 @@ struct Ctx {
   void proj(const double *psi, long ld, int nb, double *rp, long ld_rp);
   void hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, const double *rp,
             long ld_rp, double *out, long ld_out, bool accumulate, double *eps_out);
+  // H p for p = ca_b x + cb_b y formed on the fly (tiled path; rp = <proj|p>): out = H p,
+  // pap = dV <p|H p>, sap = dV <y|H p>
+  void hpsi_comb(int ns, const double *veffs, const double *x, const double *y, long ld, int nb,
+                 const double *ca, const double *cb, const double *rp, long ld_rp, double *out,
+                 double *pap, double *sap);
   void spsi(const double *psi, long ld, int nb, const double *rp, long ld_rp, double *out,
             long ld_out);
   void dots(const double *x, long ldx, const double *y, long ldy, int nb, double *res);

@@ -112,6 +112,15 @@ def run_mode(s, dev, st, kk, synth, precision, steps, it0):
     return dt, tim, hart, st["sval"].numpy()
 
 
+def forces(s, dev, st, kk, synth):
+    """force_eig_r (FP64) of the current bands: (natom, 3) Ha/bohr."""
+    kw = synth.abi_args(s, **dev)
+    fx, fy, fz = np.zeros(s.natom), np.zeros(s.natom), np.zeros(s.natom)
+    kk.call("force_eig_r", **kw, svecre=st["svecre"], sval=st["sval"], fnele=st["fnele"], fatx=fx,
+            faty=fy, fatz=fz)
+    return np.stack([fx, fy, fz], axis=1)
+
+
 # timing tag -> kernel whose calibrated HBM bytes per launch the committed counter profile holds
 TRAFFIC_KERNEL = {"gemm_mixed_nn": "k_gemm_nn<3>", "gemm_mixed_tn": "k_gemm_tnf<3>",
                   "hpsi_stencil": "k_stencil_lds<4, 2, 0>", "hp_comb_stencil": "k_stencil_lds<4, 1, 2>",
@@ -224,8 +233,15 @@ def main():
     kk.set_param("kscg_mode", 1)
     kk.set_param("precision", 0)
     log(f"[bench] warmup: {args.warmup} FP64 steps")
+    cold = None
     for it in range(1, args.warmup + 1):
-        scf_step(s, dev, st, kk, synth, it)
+        if it == 1:  # the first Hartree solve starts from V_H = 0
+            kk.timing_reset()
+            kk.set_param("timing", 1)
+        h, _ = scf_step(s, dev, st, kk, synth, it)
+        if it == 1:
+            kk.set_param("timing", 0)
+            cold = {"iterations": h, "ms": round(kk.timing().get("phase_hartree", (0.0,))[0], 2)}
     kk.synchronize()
     snap = snapshot(s, st, kk)  # (also the known-traffic copy kernel counter runs calibrate on)
     res = {}
@@ -234,19 +250,26 @@ def main():
             restore(snap, st, kk)
         log(f"[bench] {mode}: timed {args.steps} steps")
         dt, tim, hart, sval = run_mode(s, dev, st, kk, synth, mode, args.steps, args.warmup)
-        res[mode] = dict(dt=dt, tim=tim, hart=hart, sval=sval)
+        res[mode] = dict(dt=dt, tim=tim, hart=hart, sval=sval,
+                         force=forces(s, dev, st, kk, synth) if args.precision == "both" else None)
         log(f"[bench] {mode}: {dt:.4f} s/step, hartree its {hart}")
         for k, v in sorted(tim.items(), key=lambda kv: -kv[1][0]):
-            log(f"    {k:16s} {v[0] / args.steps:9.2f} ms/step  {v[1] // args.steps:6d} launches/step")
+            if not k.startswith("phase_"):
+                log(f"    {k:16s} {v[0] / args.steps:9.2f} ms/step  {v[1] // args.steps:6d} launches/step")
+        log("    phases: " + ", ".join(f"{k[6:]} {v[0] / args.steps:.1f} ms" for k, v in tim.items()
+                                       if k.startswith("phase_")))
     if snap:
         for d in snap.values():
             d.free()
     head = "mixed" if "mixed" in res else "fp64"
     tim = res[head]["tim"]
-    eig_err = None
+    eig_err = force_err = None
     if "mixed" in res and "fp64" in res:
         eig_err = float(np.abs(res["mixed"]["sval"] - res["fp64"]["sval"]).max())
-    dominant = max(tim.items(), key=lambda kv: kv[1][0])[0] if tim else None
+        force_err = float(np.abs(res["mixed"]["force"] - res["fp64"]["force"]).max())
+    phases = {k[6:]: round(v[0] / args.steps, 2) for k, v in tim.items() if k.startswith("phase_")}
+    kern = {k: v for k, v in tim.items() if not k.startswith("phase_")}
+    dominant = max(kern.items(), key=lambda kv: kv[1][0])[0] if kern else None
     bound = {"gemm_f64": "mfma", "gemm_mixed_tn": "mfma", "gemm_mixed_nn": "mfma", "trsm_f64": "mfma", "syevd": "mfma",
              "potrf": "mfma"}.get(dominant, "hbm")
     roof = roofline_entry(tim, dominant, bound) if dominant else None
@@ -287,6 +310,9 @@ def main():
         "fp64_s_per_step": round(res["fp64"]["dt"], 4) if "fp64" in res else None,
         "mixed_s_per_step": round(res["mixed"]["dt"], 4) if "mixed" in res else None,
         "eig_err_mixed_vs_fp64_Ha": eig_err,
+        "force_err_mixed_vs_fp64_Ha_per_bohr": force_err,
+        "phases_ms_per_step": phases,
+        "hartree_cold_first_step": cold,
         "roofline": roof,
         "kernels": extra,
         "cpu_baseline": cpu,

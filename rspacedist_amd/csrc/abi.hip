@@ -623,13 +623,16 @@ int kukan_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, in
       const bool qz = cc.q_zero;
       if (l_rrb) {
         cc.q_zero = qz;
+        Timed ph("phase_rr", 0.0);
+        // block CG recomputes H psi from the rotated bands: hsvre is scratch here
         rayleigh_ritz(cc, neigmx, ns, vs, true, Ps.d() + off, Sp.d() + off, Rs.d() + off1, Hp.d(),
-                      Sv.d() + offs, false);
+                      Sv.d() + offs, false, block);
       }
       if (iscf <= ncgscf || iscf % nretcg == 0) {
         KscgArgs a{neigmx, ns, northo, nkscg, nsdmax, nprecon_cg, key_ortho_cmpt_innerproduct,
                    key_ortho_nocmpt_innerproduct, epssd, vs, Ps.d() + off, Sp.d() + off, Hp.d(),
                    Rs.d() + off1, Sv.d() + offs, Re.d() + offs, ksconv, ksitmax};
+        Timed ph("phase_cg", 0.0);
         if (block) kscg_block(cc, a, !l_rrb);
         else kscg_sequential(cc, a);
       } else {
@@ -645,6 +648,7 @@ int kukan_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, in
     }
     if (l_reo)
       for (int ns = 1; ns <= nums; ns++) {
+        Timed ph("phase_reorder", 0.0);
         const size_t off = (size_t)n * neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
         const size_t off1 = (size_t)n1 * neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
         const size_t offs = (size_t)neigmx * ((ns - 1) + (size_t)nums * (nk - 1));

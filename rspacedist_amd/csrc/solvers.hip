@@ -740,7 +740,8 @@ void ortho_r_01(Ctx &c, int M, int l_inp, int nflag, int key_cmpt, double *psi, 
 // ============================================================================= Rayleigh-Ritz
 // rayleigh-ritz8f.F90:22-125: A = dV Psi^T (H Psi), eigh (lower), rotate Psi, S Psi, <p|psi>.
 void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, double *psi,
-                   double *spsi, double *rsp, double *hpsi, double *sval, bool keep_h) {
+                   double *spsi, double *rsp, double *hpsi, double *sval, bool keep_h,
+                   bool h_scratch) {
   const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
   rocblas_handle h = lib().blas;
   if (l_rrb) c.hpsi(ns, veffs, psi, n, M, rsp, n1, hpsi, n, false, nullptr);
@@ -760,6 +761,21 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   sync();
   if (hinfo != 0) throw Error(3, "rayleighritz_diag_r: error in matrix diagonalization");
   launch_canon_sign(M, A, c.s);
+  if (h_scratch && !keep_h && mixed_for(n) && !(c.q_zero == false && spsi != psi)) {
+    // split products, H psi not needed afterwards: Psi rotated in place (the product reads
+    // it only through the pre-split copy), no band-sized temporary or copy
+    sub_nn(c, n, M, M, one, psi, n, A, M, zero, psi, n, s_psi);
+    if (spsi != psi)
+      HIPCK(hipMemcpyAsync(spsi, psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+    if (n1 > 0) {
+      double *R = c.buf("rr.R", (size_t)n1 * M);
+      dgemm(rocblas_operation_none, rocblas_operation_none, (int)n1, M, M, one, rsp, (int)n1, A, M,
+            zero, R, (int)n1);
+      HIPCK(hipMemcpyAsync(rsp, R, sizeof(double) * n1 * M, hipMemcpyDeviceToDevice, c.s));
+    }
+    renormalize(c, M, psi, spsi, rsp, nullptr);
+    return;
+  }
   if (keep_h && mixed_for(n)) {
     // block path, split products: Psi, H Psi (and S Psi) rotated in place (each product reads
     // its X only through the pre-split copy), <p|psi> through a small buffer
@@ -1053,6 +1069,7 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
   }
   *a.ksitmax = std::max(*a.ksitmax, iiter);
   if (c.read_flag(1)) throw Error(2, "kscg_r: pvec has negative norm");
+  Timed ph("phase_gs", 0.0);
   for (int loop = 0; loop < a.northo; loop++) cholqr(c, M, a.psi, a.spsi, a.rsp, a.hpsi);
 }
 
@@ -1180,6 +1197,7 @@ void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *sp
 // scf_hartree8f.F90:11-812: b = 2 pi rho (+ Dirichlet planes), periodic mean removed, CG on
 // -1/2 lap_nfh with the reference's avhs recurrence and stopping rule.
 int hartree(const HartreeArgs &a) {
+  Timed ph("phase_hartree", 0.0);
   hipStream_t s = lib().stream;
   Grid g{a.nx, a.ny, a.nz, a.nfh, a.nperi, 2.0 * a.xmax / a.nx, 2.0 * a.ymax / a.ny,
          2.0 * a.zmax / a.nz};

@@ -68,8 +68,11 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi)
 void gs_one(Ctx &c, int l, double *psi, double *spsi, double *rsp);
 void ortho_r_01(Ctx &c, int M, int l_inp, int nflag, int key_cmpt, double *psi, double *spsi,
                 double *rsp);
+// h_scratch: hpsi's content after the call is not used by the caller (the block CG
+// recomputes H psi), so the split-precision path rotates psi in place without filling it
 void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, double *psi,
-                   double *spsi, double *rsp, double *hpsi, double *sval, bool keep_h);
+                   double *spsi, double *rsp, double *hpsi, double *sval, bool keep_h,
+                   bool h_scratch = false);
 void kscg_sequential(Ctx &c, const KscgArgs &a);
 void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first);
 void ksdiis(Ctx &c, const DiisArgs &a);

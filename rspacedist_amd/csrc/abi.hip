@@ -633,7 +633,9 @@ int kukan_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, in
                    key_ortho_nocmpt_innerproduct, epssd, vs, Ps.d() + off, Sp.d() + off, Hp.d(),
                    Rs.d() + off1, Sv.d() + offs, Re.d() + offs, ksconv, ksitmax};
         Timed ph("phase_cg", 0.0);
-        if (block) kscg_block(cc, a, !l_rrb);
+        // H psi is needed afterwards only by a Rayleigh-Ritz without its own H psi (l_rra); the
+        // next scf_diag recomputes it (RR with l_rrb, or the block CG's first pass)
+        if (block) kscg_block(cc, a, !l_rrb, l_rra);
         else kscg_sequential(cc, a);
       } else {
         DiisArgs a{neigmx, ns, ndiismax, nprecon_diis, northo, key_ortho_cmpt_innerproduct,
@@ -653,7 +655,7 @@ int kukan_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, in
         const size_t off1 = (size_t)n1 * neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
         const size_t offs = (size_t)neigmx * ((ns - 1) + (size_t)nums * (nk - 1));
         reorder(c, neigmx, Sv.d() + offs, Re.d() + offs, Ps.d() + off, Sp.d() + off, Rs.d() + off1,
-                Hp.d(), block);
+                Hp.d(), false, block);  // (l_reo excludes l_rra: nothing reads hsvre after this)
       }
   }
   Rs.finish(); Ps.finish(); Sp.finish(); Hp.finish(); Sv.finish(); Re.finish();

@@ -929,7 +929,7 @@ void kscg_sequential(Ctx &c, const KscgArgs &a) {
 // Block CG: every band advances at once; the reference's per-band projections against
 // psi_1..l become one projection against the whole block (two GEMMs), the 2x2 Rayleigh-
 // Ritz step of kscg_r_05 runs per band, and Cholesky-QR restores orthonormality at the end.
-void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
+void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
   const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
   const int M = a.neigmx;
   rocblas_handle h = lib().blas;
@@ -1070,7 +1070,7 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first) {
   *a.ksitmax = std::max(*a.ksitmax, iiter);
   if (c.read_flag(1)) throw Error(2, "kscg_r: pvec has negative norm");
   Timed ph("phase_gs", 0.0);
-  for (int loop = 0; loop < a.northo; loop++) cholqr(c, M, a.psi, a.spsi, a.rsp, a.hpsi);
+  for (int loop = 0; loop < a.northo; loop++) cholqr(c, M, a.psi, a.spsi, a.rsp, keep_h ? a.hpsi : nullptr);
 }
 
 // ============================================================================= ksdiis_r
@@ -1127,7 +1127,7 @@ void ksdiis(Ctx &c, const DiisArgs &a) {
 
 // ============================================================================= reorderstates
 void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *spsi, double *rsp,
-             double *hpsi, bool move_h) {
+             double *hpsi, bool move_h, bool h_scratch) {
   const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
   std::vector<double> sv(M), rs(M);
   HIPCK(hipMemcpyAsync(sv.data(), sval, sizeof(double) * M, hipMemcpyDeviceToHost, c.s));
@@ -1158,7 +1158,7 @@ void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *sp
     launch_permute_cols(n1, M, pd, rsp, n1, R, n1, c.s);
     HIPCK(hipMemcpyAsync(rsp, R, sizeof(double) * n1 * M, hipMemcpyDeviceToDevice, c.s));
   }
-  if (!move_h) {  // hsvre is the work array, as in the reference: it ends as the permuted S psi
+  if (!move_h && !h_scratch) {  // hsvre is the work array (reference): ends as the permuted S psi
     launch_permute_cols(n, M, pd, psi, n, hpsi, n, c.s);
     HIPCK(hipMemcpyAsync(psi, hpsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
     if (spsi != psi) {  // S psi may alias psi (S = 1); then hpsi already holds it permuted
@@ -1168,7 +1168,7 @@ void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *sp
     sync();
     return;
   }
-  // block path: H psi is kept consistent with psi, and only the bands that change place move
+  // block path: only the bands that change place move (H psi with them when it is kept)
   std::vector<int> mv;
   for (int j = 0; j < M; j++)
     if (perm[j] != j) mv.push_back(j);
@@ -1188,7 +1188,7 @@ void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *sp
     };
     move(psi);
     if (spsi != psi) move(spsi);  // S psi may alias psi (S = 1); move it only once
-    move(hpsi);
+    if (move_h) move(hpsi);
   }
   sync();
 }

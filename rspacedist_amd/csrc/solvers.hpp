@@ -74,13 +74,16 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
                    double *spsi, double *rsp, double *hpsi, double *sval, bool keep_h,
                    bool h_scratch = false);
 void kscg_sequential(Ctx &c, const KscgArgs &a);
-void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first);
+// keep_h: leave H psi of the returned bands in a.hpsi (kscg_r's hsvre); false when the caller
+// recomputes it (scf_diag's block path), which saves rotating it in the closing Cholesky-QR
+void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h = true);
 void ksdiis(Ctx &c, const DiisArgs &a);
-// reorderstates8f.F90:8-124.  move_h = false: hpsi is the work array and ends as the permuted
-// S psi (the reference's hsvre); true (block path): hpsi holds H psi and is permuted with psi,
-// and only displaced bands are moved.
+// reorderstates8f.F90:8-124.  Default: hpsi is the work array and ends as the permuted S psi
+// (the reference's hsvre).  move_h (block path): hpsi holds H psi and is permuted with psi;
+// h_scratch (block path of scf_diag): hpsi's content is not used afterwards.  In both block
+// forms only displaced bands move.
 void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *spsi, double *rsp,
-             double *hpsi, bool move_h = false);
+             double *hpsi, bool move_h = false, bool h_scratch = false);
 int hartree(const HartreeArgs &a);
 void correct_potential(long n, double dvol_over_omega, double *vh);
 void charge(long n, int M, const double *psi, const double *f, double *rho);

@@ -578,6 +578,74 @@ __global__ __launch_bounds__(256) void k_split_tn(const double *A, long lda, lon
   }
 }
 
+// X diag(s) (R x M, column-major) split once into both fragment layouts: the TN layout
+// (fragment (i / 32, k / 16) of X as the first operand of X^T B, KC = Rp / 16 per band block)
+// and the NN layout (fragment (row / 32, band / 16) of X as the left factor of X C, KCn per
+// row block).  Work-group: 64 rows x 64 bands through LDS; each thread writes two 16-byte lane
+// slots of each layout.  Bands at or beyond 16 KCn have no NN fragments.
+__global__ __launch_bounds__(256) void k_split_both(const double *X, long ldx, long R, int M,
+                                                    const double *sc, long KC, int KCn,
+                                                    _Float16 *th, _Float16 *tl, _Float16 *nh,
+                                                    _Float16 *nl) {
+  __shared__ float tile[64][65];  // [band][row]
+  const long r0 = (long)blockIdx.x * 64;
+  const int b0 = blockIdx.y * 64;
+  const int t = threadIdx.x;
+  // loads: a wave reads one band's 64-row run (512 contiguous bytes) per instruction
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int e = q * 256 + t, bl = e >> 6, rl = e & 63;
+    const int b = b0 + bl;
+    const long r = r0 + rl;
+    tile[bl][rl] = (b < M && r < R) ? (float)(X[(long)b * ldx + r] * sc[b]) : 0.f;
+  }
+  __syncthreads();
+  auto split8 = [&](const float *v, half8 &h, half8 &l) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      float f = v[j];
+      asm volatile("" : "+v"(f));
+      const _Float16 hh = (_Float16)f;
+      h[j] = hh;
+      l[j] = (_Float16)(f - (float)hh);
+    }
+  };
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const int slot = t + 256 * u;  // 8 fragments x 64 lanes
+    const int lane = slot & 63, fr = slot >> 6;
+    // TN: band block fr >> 2 (of 2), row chunk fr & 3 (of 4)
+    {
+      const int bb = fr >> 2, kc = fr & 3;
+      const int bl = 32 * bb + (lane & 31), rl = 16 * kc + 8 * (lane >> 5);
+      float v[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) v[j] = tile[bl][rl + j];
+      half8 h, l;
+      split8(v, h, l);
+      const long off = ((((long)b0 / 32 + bb) * KC + r0 / 16 + kc) * 64 + lane) * 8;
+      *(half8 *)&th[off] = h;
+      *(half8 *)&tl[off] = l;
+    }
+    // NN: row block fr >> 2 (of 2), band chunk fr & 3 (of 4)
+    {
+      const int rb = fr >> 2, bc = fr & 3;
+      const int rl = 32 * rb + (lane & 31), bl = 16 * bc + 8 * (lane >> 5);
+      const int bcg = b0 / 16 + bc;
+      if (bcg < KCn) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = tile[bl + j][rl];
+        half8 h, l;
+        split8(v, h, l);
+        const long off = (((r0 / 32 + rb) * KCn + bcg) * 64 + lane) * 8;
+        *(half8 *)&nh[off] = h;
+        *(half8 *)&nl[off] = l;
+      }
+    }
+  }
+}
+
 struct TNFArgs {
   int M1, M2;
   const _Float16 *ah, *al, *bh, *bl;  // fragments, (M1p / 32) x KC and (M2p / 32) x KC
@@ -638,17 +706,21 @@ void colscale(const double *A, long lda, long rows, int cols, double *scale, hip
 // B is A with the same scales), then k_gemm_tnf and k_tn_reduce.
 static void gemm_mixed_tn_presplit(int M1, int M2, long K, double alpha, const double *A, long lda,
                                    const double *sa, const double *B, long ldb, const double *sb,
-                                   double beta, double *C, long ldc, int nterms, hipStream_t s) {
-  const long Kp = (K + SPT_K - 1) / SPT_K * SPT_K, KC = Kp / 16;  // whole split tiles
+                                   double beta, double *C, long ldc, int nterms, hipStream_t s,
+                                   const SplitBands *pa) {
+  const long Kp = (K + 255) / 256 * 256, KC = Kp / 16;  // whole split tiles (as split_bands)
   const int M1p = (M1 + 255) / 256 * 256, M2p = (M2 + 255) / 256 * 256;
   const bool same = A == B && sa == sb && lda == ldb && M1 == M2;
-  _Float16 *ah = (_Float16 *)scratch("gemm.tnah", sizeof(_Float16) * (size_t)M1p * Kp);
-  _Float16 *al = (_Float16 *)scratch("gemm.tnal", sizeof(_Float16) * (size_t)M1p * Kp);
+  if (pa && !(pa->X == A && pa->ldx == lda && pa->R == K && pa->M == M1 && pa->s == sa))
+    throw Error(1, "gemm_mixed_tn: pre-split operand does not match A");
+  _Float16 *ah = pa ? pa->th : (_Float16 *)scratch("gemm.tnah", sizeof(_Float16) * (size_t)M1p * Kp);
+  _Float16 *al = pa ? pa->tl : (_Float16 *)scratch("gemm.tnal", sizeof(_Float16) * (size_t)M1p * Kp);
   _Float16 *bh = ah, *bl = al;
   {
-    Timed t("gemm_split", (8.0 * K + 4.0 * Kp) * (same ? M1 : M1 + M2));
-    hipLaunchKernelGGL(k_split_tn, dim3((unsigned)(Kp / SPT_K), (unsigned)(M1p / 32)), dim3(256), 0, s,
-                       A, lda, K, M1, sa, KC, ah, al);
+    Timed t("gemm_split", (8.0 * K + 4.0 * Kp) * ((pa ? 0 : M1) + (same ? 0 : M2)));
+    if (!pa)
+      hipLaunchKernelGGL(k_split_tn, dim3((unsigned)(Kp / SPT_K), (unsigned)(M1p / 32)), dim3(256), 0,
+                         s, A, lda, K, M1, sa, KC, ah, al);
     if (!same) {
       bh = (_Float16 *)scratch("gemm.tnbh", sizeof(_Float16) * (size_t)M2p * Kp);
       bl = (_Float16 *)scratch("gemm.tnbl", sizeof(_Float16) * (size_t)M2p * Kp);
@@ -678,6 +750,23 @@ static void gemm_mixed_tn_presplit(int M1, int M2, long K, double alpha, const d
   HIPCK(hipGetLastError());
 }
 
+void split_bands(const double *X, long ldx, long R, int M, const double *s, SplitBands &out,
+                 hipStream_t st) {
+  if (R <= 0 || M <= 0) throw Error(1, "split_bands: empty operand");
+  const long Rp = (R + 255) / 256 * 256, KC = Rp / 16;
+  const int Mp = (M + 255) / 256 * 256, Kpn = (M + 63) / 64 * 64;
+  // the same buffers the products would split into (TN's first operand, NN's left factor)
+  out.X = X; out.ldx = ldx; out.R = R; out.M = M; out.s = s;
+  out.th = (_Float16 *)scratch("gemm.tnah", sizeof(_Float16) * (size_t)Mp * Rp);
+  out.tl = (_Float16 *)scratch("gemm.tnal", sizeof(_Float16) * (size_t)Mp * Rp);
+  out.nh = (_Float16 *)scratch("gemm.nnxh", sizeof(_Float16) * (size_t)Kpn * Rp);
+  out.nl = (_Float16 *)scratch("gemm.nnxl", sizeof(_Float16) * (size_t)Kpn * Rp);
+  Timed t("gemm_split", 8.0 * (double)R * M + 8.0 * (double)Rp * Mp);
+  hipLaunchKernelGGL(k_split_both, dim3((unsigned)(Rp / 64), (unsigned)(Mp / 64)), dim3(256), 0, st, X,
+                     ldx, R, M, s, KC, Kpn / 16, out.th, out.tl, out.nh, out.nl);
+  HIPCK(hipGetLastError());
+}
+
 void maxscale(const unsigned long long *cmax, int cols, double *scale, hipStream_t s) {
   if (cols <= 0) return;
   hipLaunchKernelGGL(k_maxscale, dim3(nblk(cols, 256)), dim3(256), 0, s, cols, cmax, scale);
@@ -692,10 +781,10 @@ void normscale(const double *g, int cols, double dv, double *scale, hipStream_t 
 
 void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                    const double *sa, const double *B, long ldb, const double *sb, double beta,
-                   double *C, long ldc, int nterms, hipStream_t s) {
+                   double *C, long ldc, int nterms, hipStream_t s, const SplitBands *pa) {
   if (M1 <= 0 || M2 <= 0 || K <= 0) throw Error(1, "gemm_mixed_tn: empty product");
-  if (param("tn_presplit", 1.0) >= 1.0) {
-    gemm_mixed_tn_presplit(M1, M2, K, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, nterms, s);
+  if (pa || param("tn_presplit", 1.0) >= 1.0) {
+    gemm_mixed_tn_presplit(M1, M2, K, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, nterms, s, pa);
     return;
   }
   TNArgs p{};
@@ -725,22 +814,24 @@ void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long l
 
 void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ldx,
                    const double *sx, const double *C, long ldc, double beta, double *Y, long ldy,
-                   int nterms, hipStream_t s) {
+                   int nterms, hipStream_t s, const SplitBands *px) {
   if (R <= 0 || M2 <= 0 || K <= 0) throw Error(1, "gemm_mixed_nn: empty product");
   const int Kp = (K + 63) / 64 * 64;  // whole 64-k split tiles (and 16-k stages)
   const int M2p = (M2 + NN_BN - 1) / NN_BN * NN_BN;
   const long Rp = (R + NN_BM - 1) / NN_BM * NN_BM;
   _Float16 *ch = (_Float16 *)scratch("gemm.nnch", sizeof(_Float16) * (size_t)Kp * M2p);
   _Float16 *cl = (_Float16 *)scratch("gemm.nncl", sizeof(_Float16) * (size_t)Kp * M2p);
-  _Float16 *xh = (_Float16 *)scratch("gemm.nnxh", sizeof(_Float16) * (size_t)Kp * Rp);
-  _Float16 *xl = (_Float16 *)scratch("gemm.nnxl", sizeof(_Float16) * (size_t)Kp * Rp);
+  if (px && !(px->X == X && px->ldx == ldx && px->R == R && px->M == K && px->s == sx))
+    throw Error(1, "gemm_mixed_nn: pre-split operand does not match X");
+  _Float16 *xh = px ? px->nh : (_Float16 *)scratch("gemm.nnxh", sizeof(_Float16) * (size_t)Kp * Rp);
+  _Float16 *xl = px ? px->nl : (_Float16 *)scratch("gemm.nnxl", sizeof(_Float16) * (size_t)Kp * Rp);
   double *tc = (double *)scratch("gemm.nntc", sizeof(double) * M2p);
   double *ys = (double *)scratch("gemm.nnys", sizeof(double) * M2p);
   hipLaunchKernelGGL(k_nn_cscale, dim3(M2), dim3(256), 0, s, C, ldc, K, M2, sx, alpha, tc, ys);
   hipLaunchKernelGGL(k_nn_prep, dim3(nblk((long)Kp * M2p, 256)), dim3(256), 0, s, C, ldc, K, M2, sx,
                      tc, Kp, M2p, ch, cl);
   HIPCK(hipGetLastError());
-  {
+  if (!px) {
     Timed t("gemm_split", 8.0 * (double)R * K + 4.0 * (double)Rp * Kp);
     hipLaunchKernelGGL(k_split_t, dim3((unsigned)(Rp / 64), (unsigned)(Kp / 64)), dim3(256), 0, s, X,
                        ldx, R, K, sx, Kp, xh, xl);

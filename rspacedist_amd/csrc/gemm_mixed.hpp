@@ -3,6 +3,19 @@
 #include <hip/hip_runtime.h>
 
 namespace kukan {
+// A band block X (R x M, column-major) scaled by s and split once into the fragment layouts
+// of both products: as the first operand of X^T B (TN) and as the left factor of X C (NN).
+// The buffers are the ones the products split into themselves, so the split is valid until
+// the next product that splits its own operand.
+struct SplitBands {
+  const double *X = nullptr;
+  long ldx = 0, R = 0;
+  int M = 0;
+  const double *s = nullptr;
+  _Float16 *th = nullptr, *tl = nullptr, *nh = nullptr, *nl = nullptr;
+};
+void split_bands(const double *X, long ldx, long R, int M, const double *s, SplitBands &out,
+                 hipStream_t st);
 // scale[j] = 2^(14 - e_j) with max_i |A(i,j)| < 2^e_j (1 for a zero column)
 void colscale(const double *A, long lda, long rows, int cols, double *scale, hipStream_t s);
 // the same scales from column maxima given as FP64 bit patterns
@@ -13,10 +26,10 @@ void normscale(const double *g, int cols, double dv, double *scale, hipStream_t 
 // C(M1 x M2) = alpha A^T B + beta C with column scales sa (A) and sb (B)
 void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                    const double *sa, const double *B, long ldb, const double *sb, double beta,
-                   double *C, long ldc, int nterms, hipStream_t s);
+                   double *C, long ldc, int nterms, hipStream_t s, const SplitBands *pa = nullptr);
 // Y(R x M2) = alpha X C + beta Y with column scales sx of X (length K).  X is read only by
 // the pre-split pass that precedes the product, so Y may be X itself (same ld, beta = 0).
 void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ldx,
                    const double *sx, const double *C, long ldc, double beta, double *Y, long ldy,
-                   int nterms, hipStream_t s);
+                   int nterms, hipStream_t s, const SplitBands *px = nullptr);
 }  // namespace kukan

@@ -595,12 +595,13 @@ const double *mixed_scale(const char *tag, const double *X, long ldx, long rows,
 
 void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                  const double *B, long ldb, double beta, double *C, long ldc, const double *sa,
-                 const double *sb) {
+                 const double *sb, const SplitBands *pa) {
   hipStream_t s = lib().stream;
   if (mixed_for(K)) {
+    if (pa) sa = pa->s;
     if (!sa) sa = mixed_scale("tn.a", A, lda, K, M1);
     if (!sb) sb = (B == A && ldb == lda && M2 == M1) ? sa : mixed_scale("tn.b", B, ldb, K, M2);
-    gemm_mixed_tn(M1, M2, K, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, mixed_terms(), s);
+    gemm_mixed_tn(M1, M2, K, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, mixed_terms(), s, pa);
     return;
   }
   dgemm(rocblas_operation_transpose, rocblas_operation_none, M1, M2, (int)K, alpha, A, (int)lda, B,
@@ -608,11 +609,13 @@ void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda
 }
 
 void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx, const double *C,
-                 long ldc, double beta, double *Y, long ldy, const double *sx) {
+                 long ldc, double beta, double *Y, long ldy, const double *sx,
+                 const SplitBands *px) {
   hipStream_t s = lib().stream;
   if (mixed_for(R)) {
+    if (px) sx = px->s;
     if (!sx) sx = mixed_scale("nn.x", X, ldx, R, K);
-    gemm_mixed_nn(R, K, M2, alpha, X, ldx, sx, C, ldc, beta, Y, ldy, mixed_terms(), s);
+    gemm_mixed_nn(R, K, M2, alpha, X, ldx, sx, C, ldc, beta, Y, ldy, mixed_terms(), s, px);
     return;
   }
   dgemm(rocblas_operation_none, rocblas_operation_none, (int)R, M2, K, alpha, X, (int)ldx, C,
@@ -621,13 +624,33 @@ void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx,
 
 static void sub_tn(Ctx &, int M1, int M2, long K, double alpha, const double *A, long lda,
                    const double *B, long ldb, double beta, double *C, long ldc,
-                   const double *sa = nullptr, const double *sb = nullptr) {
-  subspace_tn(M1, M2, K, alpha, A, lda, B, ldb, beta, C, ldc, sa, sb);
+                   const double *sa = nullptr, const double *sb = nullptr,
+                   const SplitBands *pa = nullptr) {
+  subspace_tn(M1, M2, K, alpha, A, lda, B, ldb, beta, C, ldc, sa, sb, pa);
 }
 static void sub_nn(Ctx &, long R, int K, int M2, double alpha, const double *X, long ldx,
                    const double *C, long ldc, double beta, double *Y, long ldy,
-                   const double *sx = nullptr) {
-  subspace_nn(R, K, M2, alpha, X, ldx, C, ldc, beta, Y, ldy, sx);
+                   const double *sx = nullptr, const SplitBands *px = nullptr) {
+  subspace_nn(R, K, M2, alpha, X, ldx, C, ldc, beta, Y, ldy, sx, px);
+}
+
+// Psi split once for the TN and the NN product that follow (split-precision runs only); a
+// split scale from the band norms when they are known (|psi| <= sqrt(g / dV)), else from
+// the column maxima.
+static const SplitBands *split_psi(Ctx &c, const char *tag, const double *psi, int M,
+                                   const double *norms, SplitBands &sp) {
+  const long n = c.g.n();
+  if (!mixed_for(n)) return nullptr;
+  const double *sc;
+  if (norms) {
+    double *b = c.buf((std::string("split.") + tag).c_str(), M);
+    normscale(norms, M, c.g.dv(), b, c.s);
+    sc = b;
+  } else {
+    sc = mixed_scale(tag, psi, n, n, M);
+  }
+  split_bands(psi, n, n, M, sc, sp, c.s);
+  return &sp;
 }
 
 // After a split-precision rotation the band norms carry ~1e-7 relative errors, which would
@@ -650,13 +673,17 @@ static void renormalize(Ctx &c, int M, double *psi, double *spsi, double *rsp, d
 // One pass of S-orthonormalisation of bands [0, M): G = dV Psi^T S Psi = R^T R,
 // Psi, S Psi, <p|psi> (and H Psi if given) <- X R^-1.  Same Q as Gram-Schmidt in exact
 // arithmetic (orthogonalization8f.F90:93-376 with l_inp = 0).
-void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi) {
+void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi,
+            const double *norms) {
   const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
   rocblas_handle h = lib().blas;
   double *G = c.buf("cholqr.G", (size_t)M * M);
   const double dv = c.g.dv(), one = 1.0;
-  const double *s_psi = mixed_scale("cholqr.psi", psi, n, n, M);
-  sub_tn(c, M, M, n, dv, psi, n, spsi, n, 0.0, G, M, s_psi, nullptr);  // spsi == psi reuses s_psi
+  // psi split once for G's TN and the first rotation (psi's own, next)
+  SplitBands spl;
+  const SplitBands *pre = split_psi(c, "cholqr.psi", psi, M, norms, spl);
+  const double *s_psi = pre ? pre->s : nullptr;
+  sub_tn(c, M, M, n, dv, psi, n, spsi, n, 0.0, G, M, s_psi, nullptr, pre);  // spsi == psi: one split
   int *info = (int *)scratch("cholqr.info", sizeof(int));
   {
     Timed t("potrf", (double)M * M * M / 3.0);
@@ -682,10 +709,12 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi)
     }
     hipLaunchKernelGGL(k_zero_lower, dim3(nblk((long)M * M, 256)), dim3(256), 0, c.s, M, Ri);
     // in place: the split product reads X only through its pre-split copy
-    auto apply = [&](double *X, const double *sx) { sub_nn(c, n, M, M, 1.0, X, n, Ri, M, 0.0, X, n, sx); };
-    apply(psi, s_psi);
-    if (spsi != psi) apply(spsi, nullptr);
-    if (hpsi) apply(hpsi, nullptr);
+    auto apply = [&](double *X, const double *sx, const SplitBands *px) {
+      sub_nn(c, n, M, M, 1.0, X, n, Ri, M, 0.0, X, n, sx, px);
+    };
+    apply(psi, s_psi, pre);
+    if (spsi != psi) apply(spsi, nullptr, nullptr);
+    if (hpsi) apply(hpsi, nullptr, nullptr);
     if (rsp && n1 > 0) tr(rsp, n1, n1);
     renormalize(c, M, psi, spsi, rsp, hpsi);
     return;
@@ -748,9 +777,12 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   double *A = c.buf("rr.A", (size_t)M * M);
   double *E = c.buf("rr.E", (size_t)M);
   const double dv = c.g.dv(), zero = 0.0, one = 1.0;
-  const double *s_psi = mixed_scale("rr.psi", psi, n, n, M);
+  // psi split once for this TN and the rotation's NN (no other product runs in between)
+  SplitBands spl;
+  const SplitBands *pre = split_psi(c, "rr.psi", psi, M, nullptr, spl);
+  const double *s_psi = pre ? pre->s : nullptr;
   const double *s_h = mixed_scale("rr.h", hpsi, n, n, M);
-  sub_tn(c, M, M, n, dv, psi, n, hpsi, n, zero, A, M, s_psi, s_h);
+  sub_tn(c, M, M, n, dv, psi, n, hpsi, n, zero, A, M, s_psi, s_h, pre);
   int *info = (int *)scratch("rr.info", sizeof(int));
   {
     Timed t("syevd", 4.0 * (double)M * M * M);
@@ -764,7 +796,7 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   if (h_scratch && !keep_h && mixed_for(n) && !(c.q_zero == false && spsi != psi)) {
     // split products, H psi not needed afterwards: Psi rotated in place (the product reads
     // it only through the pre-split copy), no band-sized temporary or copy
-    sub_nn(c, n, M, M, one, psi, n, A, M, zero, psi, n, s_psi);
+    sub_nn(c, n, M, M, one, psi, n, A, M, zero, psi, n, s_psi, pre);
     if (spsi != psi)
       HIPCK(hipMemcpyAsync(spsi, psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
     if (n1 > 0) {
@@ -778,9 +810,10 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   }
   if (keep_h && mixed_for(n)) {
     // block path, split products: Psi, H Psi (and S Psi) rotated in place (each product reads
-    // its X only through the pre-split copy), <p|psi> through a small buffer
+    // its X only through the pre-split copy), <p|psi> through a small buffer; psi first, while
+    // its shared split is still in place
+    sub_nn(c, n, M, M, one, psi, n, A, M, zero, psi, n, s_psi, pre);
     sub_nn(c, n, M, M, one, hpsi, n, A, M, zero, hpsi, n, s_h);
-    sub_nn(c, n, M, M, one, psi, n, A, M, zero, psi, n, s_psi);
     if (c.q_zero && spsi != psi)
       HIPCK(hipMemcpyAsync(spsi, psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
     else if (spsi != psi)
@@ -801,7 +834,7 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
     Wh = c.buf("tmp.nM", (size_t)n * M);  // shared with Cholesky-QR (never live together)
     sub_nn(c, n, M, M, one, hpsi, n, A, M, zero, Wh, n, s_h);
   }
-  sub_nn(c, n, M, M, one, psi, n, A, M, zero, W, n, s_psi);
+  sub_nn(c, n, M, M, one, psi, n, A, M, zero, W, n, s_psi, keep_h ? nullptr : pre);
   HIPCK(hipMemcpyAsync(psi, W, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
   if (c.q_zero && spsi != psi) {
     HIPCK(hipMemcpyAsync(spsi, psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
@@ -985,21 +1018,24 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
     // Kr orthogonal to the block in the S metric: C = dV (S Psi)^T Kr ; Kr -= Psi C.  Split
     // scales without passes over the bands: Psi's from dV <psi|psi> (S = 1), K r's from the
     // fused maxima.
-    const double *s_psi = nullptr, *s_kr = nullptr;
+    const double *s_kr = nullptr;
+    SplitBands spl;
+    const SplitBands *pre = nullptr;
     if (mixed) {
-      if (fused) {
-        double *sp = c.buf("bcg.spsi", M);
-        normscale(gg, M, dv, sp, c.s);
-        s_psi = sp;
-      } else {
-        s_psi = mixed_scale("bcg.psi", a.psi, n, n, M);
-      }
       double *sk = c.buf("bcg.skr", M);
       maxscale(krmax, M, sk, c.s);
       s_kr = sk;
+      // S = 1: S psi equals psi, so one split of psi serves both products
+      if (c.q_zero) pre = split_psi(c, "bcg.psi", a.psi, M, fused ? gg : nullptr, spl);
     }
-    sub_tn(c, M, M, n, dv, a.spsi, n, KR, n, zero, C, M, a.spsi == a.psi ? s_psi : nullptr, s_kr);
-    sub_nn(c, n, M, M, mone, a.psi, n, C, M, one, KR, n, s_psi);
+    if (pre) {
+      sub_tn(c, M, M, n, dv, a.psi, n, KR, n, zero, C, M, nullptr, s_kr, pre);
+      sub_nn(c, n, M, M, mone, a.psi, n, C, M, one, KR, n, nullptr, pre);
+    } else {
+      const double *s_psi = mixed_scale("bcg.psi", a.psi, n, n, M);
+      sub_tn(c, M, M, n, dv, a.spsi, n, KR, n, zero, C, M, a.spsi == a.psi ? s_psi : nullptr, s_kr);
+      sub_nn(c, n, M, M, mone, a.psi, n, C, M, one, KR, n, s_psi);
+    }
     // P = Kr + beta P
     hipLaunchKernelGGL(k_ratio, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, rnew, rold, doreset ? 1 : 0, tb);
     if (fused) {
@@ -1070,7 +1106,9 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
   *a.ksitmax = std::max(*a.ksitmax, iiter);
   if (c.read_flag(1)) throw Error(2, "kscg_r: pvec has negative norm");
   Timed ph("phase_gs", 0.0);
-  for (int loop = 0; loop < a.northo; loop++) cholqr(c, M, a.psi, a.spsi, a.rsp, keep_h ? a.hpsi : nullptr);
+  for (int loop = 0; loop < a.northo; loop++)  // gg: dV <psi|psi> from the last band update
+    cholqr(c, M, a.psi, a.spsi, a.rsp, keep_h ? a.hpsi : nullptr,
+           fused && loop == 0 && iiter > 0 ? gg : nullptr);
 }
 
 // ============================================================================= ksdiis_r

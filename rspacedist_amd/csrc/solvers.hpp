@@ -1,5 +1,6 @@
 // solvers.hpp — device-pointer implementations of the coarse routines.
 #pragma once
+#include "gemm_mixed.hpp"
 #include "kukan_internal.hpp"
 
 namespace kukan {
@@ -59,12 +60,15 @@ struct HartreeArgs {
 // null means "compute here".  mixed_scale returns null in FP64 mode.
 void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                  const double *B, long ldb, double beta, double *C, long ldc,
-                 const double *sa = nullptr, const double *sb = nullptr);
+                 const double *sa = nullptr, const double *sb = nullptr,
+                 const SplitBands *pa = nullptr);
 void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx, const double *C,
-                 long ldc, double beta, double *Y, long ldy, const double *sx = nullptr);
+                 long ldc, double beta, double *Y, long ldy, const double *sx = nullptr,
+                 const SplitBands *px = nullptr);
 const double *mixed_scale(const char *tag, const double *X, long ldx, long rows, int cols);
 
-void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi);
+void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi,
+            const double *norms = nullptr);
 void gs_one(Ctx &c, int l, double *psi, double *spsi, double *rsp);
 void ortho_r_01(Ctx &c, int M, int l_inp, int nflag, int key_cmpt, double *psi, double *spsi,
                 double *rsp);

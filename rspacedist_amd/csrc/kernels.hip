@@ -651,6 +651,29 @@ __global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz
   }
 }
 
+// sum over the active (slot, j) of coef(slot, j, b) rp(slot, j, b), per band: 64 bands per
+// work-group, the atoms split over its 4 waves and the 4 sums added in a fixed order.
+__global__ __launch_bounds__(256) void k_proj_dots(const int *act, int nact, int nprjmx,
+                                                   const double *coef, int nb, const double *rp,
+                                                   long ld_rp, double *part, long part_ld,
+                                                   int part_off) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int b = blockIdx.x * 64 + lane;
+  double acc = 0.0;
+  if (b < nb) {
+    for (int a = w; a < nact; a += 4) {
+      const int slot = act[5 * a], iprj = act[5 * a + 2];
+      const long j0 = (long)nprjmx * slot;
+      for (int j = 0; j < iprj; j++) acc += coef[(j0 + j) * nb + b] * rp[(long)b * ld_rp + j0 + j];
+    }
+  }
+  red[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && b < nb)
+    part[(long)b * part_ld + part_off] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
 __global__ void k_flag_rows(const int *row_ptr, long n, int *flag) {
   const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (r < n) flag[r] = row_ptr[r + 1] > row_ptr[r] ? 1 : 0;
@@ -1067,9 +1090,15 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
     if (c.pl && c.coef && c.pl->nent > 0) {  // nonlocal part: separate owner-computes pass
       const bool comb = c.ca != nullptr;
       const long pld = c.dot_ld ? c.dot_ld : stencil_parts_lds(g, nf, c.nb, comb);
-      launch_expand_rows(*c.pl, c.coef, c.nb, c.out, c.ld_out, c.in, c.ld_in, c.dot_partial, pld,
-                         stencil_parts_lds(g, nf, c.nb, comb), s, comb ? c.in2 : nullptr, c.ca, c.cb,
-                         c.dot_partial2);
+      const int off = stencil_parts_lds(g, nf, c.nb, comb);
+      if (c.rp && c.dot_partial && (!comb || c.rp2)) {
+        launch_expand_rows(*c.pl, c.coef, c.nb, c.out, c.ld_out, nullptr, 0, nullptr, 0, 0, s);
+        launch_proj_dots(*c.pl, c.coef, c.nb, c.rp, c.ld_rp, c.dot_partial, pld, off, s);
+        if (comb) launch_proj_dots(*c.pl, c.coef, c.nb, c.rp2, c.ld_rp, c.dot_partial2, pld, off, s);
+      } else {
+        launch_expand_rows(*c.pl, c.coef, c.nb, c.out, c.ld_out, c.in, c.ld_in, c.dot_partial, pld,
+                           off, s, comb ? c.in2 : nullptr, c.ca, c.cb, c.dot_partial2);
+      }
     }
     return;
   }
@@ -1289,6 +1318,14 @@ void launch_expand(const NonlocalPlan &pl, long n, const double *coef, long ld_c
 }
 
 int expand_nparts(const NonlocalPlan &pl) { return (int)std::max<long>(1, (pl.nnz + 255) / 256); }
+
+void launch_proj_dots(const NonlocalPlan &pl, const double *coef, int nb, const double *rp,
+                      long ld_rp, double *part, long part_ld, int part_off, hipStream_t s) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_proj_dots, dim3(nblk(nb, 64)), dim3(256), 0, s, pl.act, pl.nact,
+                     pl.nprjmx, coef, nb, rp, ld_rp, part, part_ld, part_off);
+  HIPCK(hipGetLastError());
+}
 
 void launch_expand_rows(const NonlocalPlan &pl, const double *coef, int nb, double *out,
                         long ld_out, const double *psi, long ld_psi, double *part, long part_ld,

@@ -160,6 +160,12 @@ struct StencilCall {
   double *dot_partial2 = nullptr;
   // timing tag of the stencil kernel (tiled path); work = input bands in + out, plus V_eff
   const char *tag = nullptr;
+  // optional projections <p|in> (ld_rp per band) the coefficients were built from; with
+  // them the projector part's partials come from projector space (one column at the
+  // expansion's partial offset) instead of re-reading the bands.  rp2 = <p|in2> for
+  // dot_partial2 of the combination form.
+  const double *rp = nullptr, *rp2 = nullptr;
+  long ld_rp = 0;
 };
 
 // Several systems' stencils as one persistent launch (device tables built once, so the
@@ -207,9 +213,14 @@ void launch_coef(const NonlocalPlan &pl, const double *rp, long ld_rp, int nb, c
 // out_b(r) += sum_{entries of r} sum_j coef_b(j, slot) vnl(k, j, slot)
 void launch_expand(const NonlocalPlan &pl, long n, const double *coef, long ld_coef, int nb,
                    double *out, long ld_out, hipStream_t s);
-// Same over the compacted rows, 8 bands per thread; optional per-band partials of
+// Same over the compacted rows, 16 bands per thread; optional per-band partials of
 // <psi_b|added part> written at part[b*part_ld + part_off + block].
 int expand_nparts(const NonlocalPlan &pl);
+// part[b*part_ld + part_off] = sum over active (slot, j) of coef(slot, j, b) * rp(slot, j, b):
+// sum_r in_b(r) added_b(r) by linearity (rp = <p|in>, added = sum coef vnl), no grid pass.
+// coef band-fastest [slot][j][b]; rp [b][slot][j] with ld_rp per band.
+void launch_proj_dots(const NonlocalPlan &pl, const double *coef, int nb, const double *rp,
+                      long ld_rp, double *part, long part_ld, int part_off, hipStream_t s);
 void launch_expand_rows(const NonlocalPlan &pl, const double *coef, int nb, double *out,
                         long ld_out, const double *psi, long ld_psi, double *part, long part_ld,
                         int part_off, hipStream_t s,

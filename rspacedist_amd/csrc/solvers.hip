@@ -513,7 +513,8 @@ void Ctx::proj(const double *psi, long ld, int nb, double *rp, long ld_rp) {
 }
 
 void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, const double *rp,
-               long ld_rp, double *out, long ld_out, bool accumulate, double *eps_out) {
+               long ld_rp, double *out, long ld_out, bool accumulate, double *eps_out,
+               bool rp_exact) {
   const long n1 = (long)pl.nprjmx * pl.num_ppcell;
   double *coef = buf("coef", (size_t)std::max<long>(1, n1) * nb);
   const bool bf = stencil_band_fastest_coef(g, false);
@@ -523,13 +524,16 @@ void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, 
   c.accumulate = accumulate; c.pl = &pl; c.coef = coef; c.ld_coef = n1; c.coef_band_fastest = bf;
   double *part = nullptr;
   int np = stencil_nparts(g, g.nf, nb);
+  const bool pdots = rp_exact && !accumulate && param("nl_proj_dots", 1.0) >= 1.0;
   if (eps_out) {
-    const int np_total = np + expand_nparts(pl);  // tiled path appends expansion partials
+    // the tiled path appends the expansion's partials (one column from projector space)
+    const int np_total = np + (pdots ? 1 : expand_nparts(pl));
     part = buf("hpsi.part", (size_t)np_total * nb);
     launch_fill((long)np_total * nb, 0.0, part, s);
     c.dot_partial = part;
     c.dot_ld = np_total;
     np = np_total;
+    if (pdots) { c.rp = rp; c.ld_rp = ld_rp; }
   }
   c.tag = "hpsi_stencil";  // the stencil kernel; the projector pass has its own tag
   launch_stencil(c, s);
@@ -538,17 +542,19 @@ void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, 
 
 void Ctx::hpsi_comb(int ns, const double *veffs, const double *x, const double *y, long ld, int nb,
                     const double *ca, const double *cb, const double *rp, long ld_rp, double *out,
-                    double *pap, double *sap) {
+                    double *pap, double *sap, const double *rpy) {
   const long n1 = (long)pl.nprjmx * pl.num_ppcell;
   double *coef = buf("coef", (size_t)std::max<long>(1, n1) * nb);
   launch_coef(pl, rp, ld_rp, nb, dij, false, ns, 1.0 / g.dv(), coef, n1, s, true);
-  const int np = stencil_nparts(g, g.nf, nb, true) + expand_nparts(pl);
+  const bool pdots = rpy && param("nl_proj_dots", 1.0) >= 1.0;
+  const int np = stencil_nparts(g, g.nf, nb, true) + (pdots ? 1 : expand_nparts(pl));
   double *part = buf("hpsi.part", (size_t)2 * np * nb);
   launch_fill((long)2 * np * nb, 0.0, part, s);
   StencilCall c;
   c.g = &g; c.in = x; c.in2 = y; c.ca = ca; c.cb = cb; c.ld_in = ld; c.out = out; c.ld_out = ld;
   c.veff = veffs; c.nb = nb; c.pl = &pl; c.coef = coef; c.ld_coef = n1; c.coef_band_fastest = true;
   c.dot_partial = part; c.dot_partial2 = part + (long)np * nb; c.dot_ld = np;
+  if (pdots) { c.rp = rp; c.rp2 = rpy; c.ld_rp = ld_rp; }
   c.tag = "hp_comb_stencil";
   launch_stencil(c, s);
   launch_reduce_partials(part, np, nb, g.dv(), pap, s);
@@ -1061,11 +1067,11 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
       // and one stencil pass forms p on the fly and gives H p, dV <p|H p>, dV <psi|H p>
       c.proj(PA, n, M, rpp, n1);
       if (n1 > 0) launch_axpby(n1, M, fac, 1.0, rpp, n1, mdf, 1.0, a.rsp, n1, rpp, n1, c.s);
-      c.hpsi_comb(a.ns, a.veffs, PA, a.psi, n, M, fac, mdf, rpp, n1, HP, pap, sap);
+      c.hpsi_comb(a.ns, a.veffs, PA, a.psi, n, M, fac, mdf, rpp, n1, HP, pap, sap, a.rsp);
     } else {
       launch_axpby(n, M, fac, 1.0, PA, n, mdf, 1.0, a.psi, n, KR, n, c.s);  // KR <- p
       c.proj(KR, n, M, rpp, n1);
-      c.hpsi(a.ns, a.veffs, KR, n, M, rpp, n1, HP, n, false, pap);
+      c.hpsi(a.ns, a.veffs, KR, n, M, rpp, n1, HP, n, false, pap, true);
       c.dots(a.psi, n, HP, n, M, sap);
     }
     hipLaunchKernelGGL(k_kscg05, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, a.sval, pap, sap, active, al, be);
@@ -1419,7 +1425,7 @@ void band_energy(Ctx &c, int M, const double *veff, const double *psi, double *e
   double *rp = c.buf("adj.rp", (size_t)std::max<long>(1, n1) * M);
   double *hp = c.buf("tmp.nM", (size_t)n * M);
   c.proj(psi, n, M, rp, n1);
-  c.hpsi(1, veff, psi, n, M, rp, n1, hp, n, false, eband);
+  c.hpsi(1, veff, psi, n, M, rp, n1, hp, n, false, eband, true);
 }
 
 void energy_adjoint(Ctx &c, int M, const double *psi, const double *f, const double *dvnl,

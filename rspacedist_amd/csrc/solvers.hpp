@@ -20,13 +20,17 @@ struct Ctx {
   void clear_flag(int i = 0);
   int read_flag(int i = 0);
   void proj(const double *psi, long ld, int nb, double *rp, long ld_rp);
+  // rp_exact: rp was just computed from psi, so eps_out's projector part may be taken from
+  // projector space (sum coef rp) instead of a second pass over the bands
   void hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, const double *rp,
-            long ld_rp, double *out, long ld_out, bool accumulate, double *eps_out);
+            long ld_rp, double *out, long ld_out, bool accumulate, double *eps_out,
+            bool rp_exact = false);
   // H p for p = ca_b x + cb_b y formed on the fly (tiled path; rp = <proj|p>): out = H p,
-  // pap = dV <p|H p>, sap = dV <y|H p>
+  // pap = dV <p|H p>, sap = dV <y|H p>.  With rpy = <proj|y> the projector parts of pap and
+  // sap come from projector space.
   void hpsi_comb(int ns, const double *veffs, const double *x, const double *y, long ld, int nb,
                  const double *ca, const double *cb, const double *rp, long ld_rp, double *out,
-                 double *pap, double *sap);
+                 double *pap, double *sap, const double *rpy = nullptr);
   void spsi(const double *psi, long ld, int nb, const double *rp, long ld_rp, double *out,
             long ld_out);
   void dots(const double *x, long ldx, const double *y, long ldy, int nb, double *res);

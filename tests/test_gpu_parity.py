@@ -269,6 +269,34 @@ def test_block_cg_combination_stencil_matches_materialised_direction(kk, orc):
     np.testing.assert_allclose(a["svecre"], b["svecre"], rtol=0, atol=1e-9 * np.abs(b["svecre"]).max())
 
 
+@pytest.mark.parametrize("comb", [1, 0])
+def test_projector_space_partials_match_grid_partials(kk, orc, comb):
+    """<p|H p> and <psi|H p> in the block CG take their projector part from projector space
+    (sum coef <proj|.>, no second pass over the bands); the grid-pass partials give the same
+    steps to rounding."""
+    s = synth.make_system(16, 0.4, 3, 12, seed=83)
+    base = _state(s, orc)
+    kw = synth.abi_args(s, nsdmax=4, epssd=1e-12)
+    out = []
+    kk.set_param("kscg_mode", 1)
+    kk.set_param("bcg_comb", comb)
+    try:
+        for pd in (1, 0):
+            kk.set_param("nl_proj_dots", pd)
+            st = {k: (v.copy(order="F") if isinstance(v, np.ndarray) else v) for k, v in base.items()}
+            for it in range(1, 4):
+                kw["iscf"] = it
+                kk.call("scf_diag", **kw, **st)
+            out.append(st)
+    finally:
+        kk.set_param("kscg_mode", 0)
+        kk.set_param("bcg_comb", 1)
+        kk.set_param("nl_proj_dots", 1)
+    a, b = out
+    np.testing.assert_allclose(a["sval"], b["sval"], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(a["svecre"], b["svecre"], rtol=0, atol=1e-9 * np.abs(b["svecre"]).max())
+
+
 def test_charge_force_energy(kk, orc):
     s, rng = _nl_system(sym=True)
     s.sss[:] = 0.02

@@ -156,8 +156,9 @@ __global__ __launch_bounds__(256) void k_stencil(SP p) {
 //     (x wraps inside the row), giving the x and y neighbours;
 //   * blocks are mapped XCD-aware: all tiles and z-ranges of a band run on one XCD
 //     (block id mod 8), so the halo rows and veff/projector reads hit that XCD's L2;
-//   * MODE 1 reads sign*(h - eps_b s) instead of a band (fused residual for K r); MODE 2 reads
-//     ca_b x + cb_b y (a combination of two bands formed on the fly) and also sums <y|out>.
+//   * MODE 1 reads sign*(h - eps_b s) instead of a band (fused residual for K r); MODE 2 gives
+//     H (ca_b x + cb_b y) by linearity, as ca_b H x + cb_b (H y) with H y read (hin), so the
+//     combination is never stored and the stencil loads only x; it also sums <y|out>.
 struct SL {
   int nx, ny, nz, px, py, pz, TY, ntile, nzch, zc, nbands;
   const double *in, *in2;
@@ -182,6 +183,7 @@ struct SL {
   unsigned long long *omax;  // optional: per band max |out| as FP64 bits (atomic max)
   const double *ca, *cb;     // MODE 2: the input is ca_b in + cb_b in2
   double *dotp2;             // MODE 2: partials of <in2|out> (in2 at the centre), as dotp
+  const double *hin;         // MODE 2: H in2 (same band stride as in)
 };
 
 // Block = TY full x-rows of one band, marching over a z-range.  The 2 NF + 1 values of each
@@ -202,6 +204,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
   const int y0 = tile * TY, z0 = zch * p.zc, z1 = min(nz, z0 + p.zc);
   const double *in = p.in + (long)b * p.ld_in;
   const double *in2 = (RES || COMB) ? p.in2 + (long)b * p.ld_in : nullptr;
+  const double *hin = COMB ? p.hin + (long)b * p.ld_in : nullptr;
   const double e = RES ? p.eps[b] : 0.0;
   const double fa = COMB ? p.ca[b] : 0.0, fb = COMB ? p.cb[b] : 0.0;
   double *out = p.out + (long)b * p.ld_out;
@@ -223,7 +226,6 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
   };
   auto ld = [&](long off) -> double {
     if (RES) return p.sign * (in[off] - e * in2[off]);
-    if (COMB) return fa * in[off] + fb * in2[off];
     return in[off];
   };
   // tile points
@@ -277,10 +279,9 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
 #pragma unroll
     for (int q = 0; q < P; q++) col[q][m + 1] = (lv[q] && zo >= 0) ? ld(zo + oxy[q]) : 0.0;
   }
-  // MODE 2: the second band's own value at the centre (for <in2|out>), prefetched like V_eff;
-  // it was read NF planes earlier for the combination, so it usually comes from L2
+  // MODE 2: the second band and its H at the centre, prefetched like V_eff
   constexpr int YP = COMB ? P : 1;
-  double yf[YP];
+  double yf[YP], hf[YP];
   double nxt[P], hvv[H], vf[P];
   auto prefetch = [&](int z) {
     const long zn = zoff(z + NF), zh = (long)z * plane;
@@ -288,6 +289,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
     for (int q = 0; q < P; q++) {
       nxt[q] = (lv[q] && zn >= 0) ? ld(zn + oxy[q]) : 0.0;
       if (COMB) yf[q] = sv[q] ? in2[zh + oxy[q]] : 0.0;
+      if (COMB) hf[q] = sv[q] ? hin[zh + oxy[q]] : 0.0;
       vf[q] = (p.veff && sv[q]) ? p.veff[zh + oxy[q]] : 0.0;
     }
 #pragma unroll
@@ -296,13 +298,14 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
   prefetch(z0);
   double dsum = 0.0, dsum2 = 0.0, amax = 0.0;
   for (int z = z0; z < z1; z++) {
-    double vcur[P], ycur[YP];
+    double vcur[P], ycur[YP], hcur[YP];
 #pragma unroll
     for (int q = 0; q < P; q++) {
 #pragma unroll
       for (int m = 0; m < 2 * NF; m++) col[q][m] = col[q][m + 1];
       col[q][2 * NF] = nxt[q];
       if (COMB) ycur[q] = yf[q];
+      if (COMB) hcur[q] = hf[q];
       vcur[q] = vf[q];
       if (t + 512 * q < nx * TY) put(lcol[q], xq[q], col[q][NF]);
     }
@@ -324,7 +327,8 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
       for (int m = 1; m <= NF; m++) s += p.cy[m] * (lr[-m * W] + lr[m * W]);
 #pragma unroll
       for (int m = 1; m <= NF; m++) s += p.cz[m] * (col[q][NF - m] + col[q][NF + m]);
-      if (p.dotp) dsum += c * s;
+      if (COMB) s = fa * s + fb * hcur[q];                 // H (fa x + fb y)
+      if (p.dotp) dsum += (COMB ? fa * c + fb * ycur[q] : c) * s;
       if (COMB) dsum2 += ycur[q] * s;
       if (p.omax) amax = fmax(amax, fabs(s));
       const long r = zh + oxy[q];
@@ -346,8 +350,17 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
   }
 }
 
+// waves per SIMD the register budget is sized for: 4 where the z-columns fit in 128 VGPRs
+// (a 512-thread block is 2 waves per SIMD, so only even counts matter)
 template <int NF, int P, int MODE>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NF <= 4 && (MODE == 2 ? P == 1 : P <= 2) ? 4 : 2))) void k_stencil_lds(SL p) {
+constexpr int stencil_waves() {
+  if (NF > 4) return 2;
+  if (MODE == 2) return P == 1 ? 4 : 2;
+  return P <= 2 ? 4 : 2;
+}
+
+template <int NF, int P, int MODE>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(stencil_waves<NF, P, MODE>()))) void k_stencil_lds(SL p) {
   extern __shared__ double lds[];
   __shared__ double red[8];
   if (p.skip && *p.skip) return;  // uniform: a device-side loop has finished
@@ -976,7 +989,8 @@ static LdsPlan lds_plan(const Grid &g, int NF, int nb, int force_p = 0) {
 }
 static bool lds_ok(const Grid &g) { return g.nx <= 256 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8; }
 
-// points per thread of the combination-form stencil (param comb_p, default 1)
+// points per thread of the combination-form stencil (param comb_p, default 1: measured faster
+// than 2 points at two waves per SIMD)
 static int comb_points() { return (int)param("comb_p", 1.0); }
 
 int stencil_parts_lds(const Grid &g, int NF, int nb, bool comb = false) {
@@ -1077,12 +1091,13 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
     q.dotp = c.dot_partial;
     q.dot_ld = c.dot_ld;
     if (c.ca) {
-      if (!c.dot_partial || !c.dot_partial2 || c.accumulate)
-        throw Error(1, "stencil: the combination form needs both partial buffers and no accumulate");
-      q.in2 = c.in2; q.ca = c.ca; q.cb = c.cb; q.dotp2 = c.dot_partial2;
+      if (!c.dot_partial || !c.dot_partial2 || !c.hin || c.accumulate)
+        throw Error(1, "stencil: the combination form needs H in2, both partial buffers and no accumulate");
+      q.in2 = c.in2; q.ca = c.ca; q.cb = c.cb; q.dotp2 = c.dot_partial2; q.hin = c.hin;
     }
-    if (c.tag) {
-      Timed t(c.tag, 8.0 * ((double)c.nb * (c.ca ? 3.0 : 2.0) * g.n() + (c.veff ? g.n() : 0)));
+    if (c.tag) {  // band arrays read and written: in (+ in2 + H in2) and out
+      const double arrays = c.ca ? 4.0 : 2.0;
+      Timed t(c.tag, 8.0 * ((double)c.nb * arrays * g.n() + (c.veff ? g.n() : 0)));
       launch_lds(q, g, nf, c.nb, false, s);
     } else {
       launch_lds(q, g, nf, c.nb, false, s);

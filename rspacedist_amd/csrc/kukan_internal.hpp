@@ -155,9 +155,13 @@ struct StencilCall {
   const double *in2 = nullptr, *eps = nullptr;
   double sign = 1.0;
   // combination form (tiled path only): the input band is ca_b in + cb_b in2, never stored;
-  // dot_partial2 (same layout as dot_partial) receives the partials of <in2|out>
+  // out = ca_b (H in) + cb_b hin with hin = H in2 (by linearity), so the stencil loads only
+  // in, and coef holds the projector coefficients of ca_b in alone; dot_partial (with the
+  // combination at the centre) and dot_partial2 (same layout, in2 at the centre) receive the
+  // partials of <ca in + cb in2|out> and <in2|out>
   const double *ca = nullptr, *cb = nullptr;
   double *dot_partial2 = nullptr;
+  const double *hin = nullptr;
   // timing tag of the stencil kernel (tiled path); work = input bands in + out, plus V_eff
   const char *tag = nullptr;
   // optional projections <p|in> (ld_rp per band) the coefficients were built from; with

@@ -540,12 +540,14 @@ void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, 
   if (eps_out) launch_reduce_partials(part, np, nb, g.dv(), eps_out, s);
 }
 
-void Ctx::hpsi_comb(int ns, const double *veffs, const double *x, const double *y, long ld, int nb,
-                    const double *ca, const double *cb, const double *rp, long ld_rp, double *out,
-                    double *pap, double *sap, const double *rpy) {
+void Ctx::hpsi_comb(int ns, const double *veffs, const double *x, const double *y,
+                    const double *hy, long ld, int nb, const double *ca, const double *cb,
+                    const double *rpx, const double *rp, long ld_rp, double *out, double *pap,
+                    double *sap, const double *rpy) {
   const long n1 = (long)pl.nprjmx * pl.num_ppcell;
   double *coef = buf("coef", (size_t)std::max<long>(1, n1) * nb);
-  launch_coef(pl, rp, ld_rp, nb, dij, false, ns, 1.0 / g.dv(), coef, n1, s, true);
+  // the projector term of ca x alone: cb H y already holds cb's
+  launch_coef(pl, rpx, ld_rp, nb, dij, false, ns, 1.0 / g.dv(), coef, n1, s, true);
   const bool pdots = rpy && param("nl_proj_dots", 1.0) >= 1.0;
   const int np = stencil_nparts(g, g.nf, nb, true) + (pdots ? 1 : expand_nparts(pl));
   double *part = buf("hpsi.part", (size_t)2 * np * nb);
@@ -555,6 +557,7 @@ void Ctx::hpsi_comb(int ns, const double *veffs, const double *x, const double *
   c.veff = veffs; c.nb = nb; c.pl = &pl; c.coef = coef; c.ld_coef = n1; c.coef_band_fastest = true;
   c.dot_partial = part; c.dot_partial2 = part + (long)np * nb; c.dot_ld = np;
   if (pdots) { c.rp = rp; c.rp2 = rpy; c.ld_rp = ld_rp; }
+  c.hin = hy;
   c.tag = "hp_comb_stencil";
   launch_stencil(c, s);
   launch_reduce_partials(part, np, nb, g.dv(), pap, s);
@@ -980,6 +983,7 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
   double *HP = c.buf("bcg.HP", (size_t)n * M);     // H p
   double *SK = c.q_zero ? nullptr : c.buf("bcg.SK", (size_t)n * M);
   double *rpp = c.buf("bcg.rpp", (size_t)std::max<long>(1, n1) * M);
+  double *rpx = c.buf("bcg.rpx", (size_t)std::max<long>(1, n1) * M);
   double *C = c.buf("bcg.C", (size_t)M * M);
   double *sc = c.buf("bcg.sc", (size_t)M * 12);
   double *rnew = sc, *rold = sc + M, *tb = sc + 2 * M, *dd = sc + 3 * M, *pp = sc + 4 * M;
@@ -1064,10 +1068,15 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
     hipLaunchKernelGGL(k_pnorm, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, dd, pp, gg, fac, mdf, c.flag(1));
     if (comb) {
       // p = fac P + mdf psi is never stored: <proj|p> = fac <proj|P> + mdf <proj|psi> (rsp),
-      // and one stencil pass forms p on the fly and gives H p, dV <p|H p>, dV <psi|H p>
+      // and one stencil pass over P gives H p = fac H P + mdf H psi (H psi is maintained),
+      // dV <p|H p> and dV <psi|H p>; the projector coefficients are those of fac P (rpx)
       c.proj(PA, n, M, rpp, n1);
-      if (n1 > 0) launch_axpby(n1, M, fac, 1.0, rpp, n1, mdf, 1.0, a.rsp, n1, rpp, n1, c.s);
-      c.hpsi_comb(a.ns, a.veffs, PA, a.psi, n, M, fac, mdf, rpp, n1, HP, pap, sap, a.rsp);
+      if (n1 > 0) {
+        launch_axpby(n1, M, fac, 1.0, rpp, n1, nullptr, 0.0, rpp, n1, rpx, n1, c.s);
+        launch_axpby(n1, M, nullptr, 1.0, rpx, n1, mdf, 1.0, a.rsp, n1, rpp, n1, c.s);
+      }
+      c.hpsi_comb(a.ns, a.veffs, PA, a.psi, a.hpsi, n, M, fac, mdf, rpx, rpp, n1, HP, pap, sap,
+                  a.rsp);
     } else {
       launch_axpby(n, M, fac, 1.0, PA, n, mdf, 1.0, a.psi, n, KR, n, c.s);  // KR <- p
       c.proj(KR, n, M, rpp, n1);

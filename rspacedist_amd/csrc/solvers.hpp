@@ -25,12 +25,14 @@ struct Ctx {
   void hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, const double *rp,
             long ld_rp, double *out, long ld_out, bool accumulate, double *eps_out,
             bool rp_exact = false);
-  // H p for p = ca_b x + cb_b y formed on the fly (tiled path; rp = <proj|p>): out = H p,
-  // pap = dV <p|H p>, sap = dV <y|H p>.  With rpy = <proj|y> the projector parts of pap and
+  // H p for p = ca_b x + cb_b y, p never stored (tiled path): out = H p = ca_b H x + hy with
+  // hy = H y, pap = dV <p|H p>, sap = dV <y|H p>.  rpx = ca_b <proj|x> (the projector
+  // coefficients of ca_b x), rp = <proj|p>; with rpy = <proj|y> the projector parts of pap and
   // sap come from projector space.
-  void hpsi_comb(int ns, const double *veffs, const double *x, const double *y, long ld, int nb,
-                 const double *ca, const double *cb, const double *rp, long ld_rp, double *out,
-                 double *pap, double *sap, const double *rpy = nullptr);
+  void hpsi_comb(int ns, const double *veffs, const double *x, const double *y, const double *hy,
+                 long ld, int nb, const double *ca, const double *cb, const double *rpx,
+                 const double *rp, long ld_rp, double *out, double *pap, double *sap,
+                 const double *rpy = nullptr);
   void spsi(const double *psi, long ld, int nb, const double *rp, long ld_rp, double *out,
             long ld_out);
   void dots(const double *x, long ldx, const double *y, long ldy, int nb, double *res);

@@ -390,31 +390,45 @@ typedef _Float16 FragBuf[NN_NS][32][512];
 // columns 128 (w >> 2) of a 256 x 256 tile) += the products of nst 16-deep stages.  The tile's
 // A fragments of stage st are at a[h|l] + (idx KC + st) 512 (idx = row block 0..7), the B
 // fragments likewise; the caller offsets a* / b* to its first row / column block and first
-// stage.  Each wave issues 4 LDS-DMA loads per stage, NN_NS - 1 stages ahead of the one it
-// computes; at the top of an iteration it waits for its own loads of that stage (vmcnt counts
-// them in issue order) and a barrier publishes everyone's.  The buffer refilled after the
-// barrier is the one every wave finished reading in the previous iteration (its reads were
-// consumed by MFMAs, and waited for, before that barrier).  The plain s_barrier (no fence)
-// keeps the later stages in flight; empty asm statements keep the compiler from moving LDS
-// accesses across it.  On return all loads have landed and the block is synchronised.
+// stage.  Each wave issues 4 LDS-DMA loads per stage (buffer_load ... lds through one buffer
+// descriptor per fragment row: the MUBUF form, unlike the global one, leaves the compiler's
+// LDS-read waits graded), NN_NS - 1 stages ahead of the one it computes.  Stage st + 1's
+// fragments are read from LDS while stage st multiplies: the next A fragments after the first
+// MFMA group, each B fragment as soon as its last MFMA of stage st is issued (at most 12 LDS
+// reads outstanding, within the 4-bit lgkm counter).  Before reading stage st + 1 a wave waits
+// for its own loads of that stage (vmcnt counts them in issue order) and a barrier publishes
+// everyone's; the buffer refilled after that barrier held stage st - 1, which every wave had
+// read into registers and consumed before arriving.  The plain s_barrier (no fence) keeps the
+// later stages in flight; empty asm statements keep the compiler from moving LDS accesses
+// across it.  The accumulation order per output (hi hi, hi lo, [lo lo,] lo hi) is the same in
+// every stage.  On return all loads have landed and the block is synchronised.
+// Fragment rows are addressed through 32-bit buffer offsets: KC 16-deep stages of 1 KB each.
+static void frag_rows_ok(long KC, const char *who) {
+  if (KC * 1024 > 0x7fffffffL) throw Error(1, std::string(who) + ": operand too long for the split-precision product");
+}
+
 template <int NT>
 __device__ inline void frag_gemm(const _Float16 *ah_g, const _Float16 *al_g, const _Float16 *bh_g,
                                  const _Float16 *bl_g, long KC, int nst, FragBuf &buf,
                                  f16v (&acc)[2][4]) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
-  // this wave's 4 fragments per stage: f = 4 w + i; f / 8 = A hi, A lo, B hi, B lo
-  const _Float16 *src[4];
+  // this wave's 4 fragments per stage: f = 4 w + i; f / 8 = A hi, A lo, B hi, B lo.  One
+  // buffer descriptor per fragment row (uniform per wave); a row spans KC * 1 KB, which the
+  // host keeps below 2^31 bytes (frag_rows_ok)
+  const int ws = __builtin_amdgcn_readfirstlane(w);
+  __amdgpu_buffer_rsrc_t rs[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    const int f = 4 * w + i, op = f >> 3, idx = f & 7;
+    const int f = 4 * ws + i, op = f >> 3, idx = f & 7;
     const _Float16 *base = op == 0 ? ah_g : op == 1 ? al_g : op == 2 ? bh_g : bl_g;
-    src[i] = base + (long)idx * KC * 512 + 8 * lane;
+    rs[i] = __builtin_amdgcn_make_buffer_rsrc((void *)(base + (long)idx * KC * 512), (short)0,
+                                              (int)(KC * 1024), 0x00020000);
   }
   auto dma = [&](int st) {
     _Float16(*d)[512] = buf[st % NN_NS];
 #pragma unroll
     for (int i = 0; i < 4; i++)
-      __builtin_amdgcn_global_load_lds(src[i] + (long)st * 512, NN_LDS(&d[4 * w + i][0]), 16, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs[i], NN_LDS(&d[4 * ws + i][0]), 16, st * 1024 + 16 * lane, 0, 0, 0);
   };
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -425,19 +439,20 @@ __device__ inline void frag_gemm(const _Float16 *ah_g, const _Float16 *al_g, con
 #pragma unroll
   for (int i = 0; i < NN_NS - 1; i++)
     if (i < nst) dma(i);
-  for (int st = 0; st < nst; st++) {
-    // own loads of stage st done: later stages still in flight = min(NN_NS - 2, nst - 1 - st)
-    const int later = min(NN_NS - 2, nst - 1 - st);
+  {
+    const int later = min(NN_NS - 2, nst - 1);
     if (later >= 3) wait_vm<12>();
     else if (later == 2) wait_vm<8>();
     else if (later == 1) wait_vm<4>();
     else wait_vm<0>();
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-    if (st + NN_NS - 1 < nst) dma(st + NN_NS - 1);
-    const _Float16(*C)[512] = buf[st % NN_NS];
-    half8 ah[2], al[2], bh[4], bl[4];
+  }
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  half8 ah[2], al[2], bh[4], bl[4];
+  {
+    const _Float16(*C)[512] = buf[0];
 #pragma unroll
     for (int a = 0; a < 2; a++) {
       ah[a] = *(const half8 *)&C[2 * wi + a][8 * lane];
@@ -448,27 +463,57 @@ __device__ inline void frag_gemm(const _Float16 *ah_g, const _Float16 *al_g, con
       bh[b] = *(const half8 *)&C[16 + 4 * wj + b][8 * lane];
       bl[b] = *(const half8 *)&C[24 + 4 * wj + b][8 * lane];
     }
+  }
+  auto stage = [&](auto more_t, int st) {
+    constexpr bool more = decltype(more_t)::value;
+    half8 ahn[2], aln[2];
+    const _Float16(*C1)[512] = buf[(st + 1) % NN_NS];
+    if (more) {
+      // stage st + 1 landed (own loads) and published; stages after it stay in flight
+      const int later = min(NN_NS - 3, nst - 2 - st);
+      if (later >= 2) wait_vm<8>();
+      else if (later == 1) wait_vm<4>();
+      else wait_vm<0>();
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+      // the buffer refilled held stage st - 1, whose fragments every wave has consumed
+      if (st + NN_NS - 1 < nst) dma(st + NN_NS - 1);
+    }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int b = 0; b < 4; b++)
+    for (int b = 0; b < 4; b++) {
 #pragma unroll
       for (int a = 0; a < 2; a++) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
 #pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
       for (int a = 0; a < 2; a++) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-#pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
-      for (int a = 0; a < 2; a++) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
-    if (NT == 4) {
-#pragma unroll
-      for (int b = 0; b < 4; b++)
+      if (NT == 4) {
 #pragma unroll
         for (int a = 0; a < 2; a++) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bl[b], acc[a][b], 0, 0, 0);
+      }
+      if (more && b == 0) {
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          ahn[a] = *(const half8 *)&C1[2 * wi + a][8 * lane];
+          aln[a] = *(const half8 *)&C1[8 + 2 * wi + a][8 * lane];
+        }
+      }
+      if (more) bl[b] = *(const half8 *)&C1[24 + 4 * wj + b][8 * lane];
+#pragma unroll
+      for (int a = 0; a < 2; a++) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
+      if (more) bh[b] = *(const half8 *)&C1[16 + 4 * wj + b][8 * lane];
+      __builtin_amdgcn_sched_barrier(0);
     }
-    // this stage's LDS reads are complete before anyone passes the next barrier
-    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-  }
+    if (more) {
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        ah[a] = ahn[a];
+        al[a] = aln[a];
+      }
+    }
+  };
+  for (int st = 0; st + 1 < nst; st++) stage(std::true_type{}, st);
+  if (nst > 0) stage(std::false_type{}, nst - 1);
   wait_vm<0>();
   __syncthreads();
 }
@@ -709,6 +754,7 @@ static void gemm_mixed_tn_presplit(int M1, int M2, long K, double alpha, const d
                                    double beta, double *C, long ldc, int nterms, hipStream_t s,
                                    const SplitBands *pa) {
   const long Kp = (K + 255) / 256 * 256, KC = Kp / 16;  // whole split tiles (as split_bands)
+  frag_rows_ok(KC, "gemm_mixed_tn");
   const int M1p = (M1 + 255) / 256 * 256, M2p = (M2 + 255) / 256 * 256;
   const bool same = A == B && sa == sb && lda == ldb && M1 == M2;
   if (pa && !(pa->X == A && pa->ldx == lda && pa->R == K && pa->M == M1 && pa->s == sa))
@@ -819,6 +865,7 @@ void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ld
   const int Kp = (K + 63) / 64 * 64;  // whole 64-k split tiles (and 16-k stages)
   const int M2p = (M2 + NN_BN - 1) / NN_BN * NN_BN;
   const long Rp = (R + NN_BM - 1) / NN_BM * NN_BM;
+  frag_rows_ok(Kp / 16, "gemm_mixed_nn");
   _Float16 *ch = (_Float16 *)scratch("gemm.nnch", sizeof(_Float16) * (size_t)Kp * M2p);
   _Float16 *cl = (_Float16 *)scratch("gemm.nncl", sizeof(_Float16) * (size_t)Kp * M2p);
   if (px && !(px->X == X && px->ldx == ldx && px->R == R && px->M == K && px->s == sx))

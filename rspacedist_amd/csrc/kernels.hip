@@ -1013,8 +1013,8 @@ static void launch_lds_nf(const SL &p, const LdsPlan &L, int nb, hipStream_t s) 
 
 // Generic LDS launch: coefficients in p; in2/eps for the residual mode.
 static void launch_lds(SL &p, const Grid &g, int NF, int nb, bool res, hipStream_t s) {
-  // the combination form holds two loads per prefetched value: one point per thread keeps it
-  // within the register budget of four waves per SIMD
+  // the combination form prefetches two more values per point (the second band and its H):
+  // one point per thread keeps it within the register budget of four waves per SIMD
   LdsPlan L = lds_plan(g, NF, nb, p.ca ? comb_points() : 0);
   if (L.P > 4) throw Error(1, "stencil tile does not fit the thread block");
   p.TY = L.TY; p.ntile = L.ntile; p.nzch = L.nzch; p.zc = L.zc; p.nbands = nb;

@@ -992,7 +992,7 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
   // S = 1: fuse the direction update with its dot products and the band update with the next
   // iteration's <psi|psi> and <psi|H psi> (same arithmetic, fewer passes over HBM)
   const bool fused = c.q_zero;
-  // the combination stencil (p formed on the fly) needs the tiled stencil path
+  // the combination stencil (H p = fac H P + mdf H psi, p never stored) needs the tiled path
   const bool comb = fused && stencil_band_fastest_coef(c.g, false) && param("bcg_comb", 1.0) >= 1.0;
   double *eh_next = c.buf("bcg.eh", (size_t)M);
   int *active = (int *)scratch("bcg.active", sizeof(int) * M);

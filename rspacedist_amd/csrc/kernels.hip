@@ -184,6 +184,7 @@ struct SL {
   const double *ca, *cb;     // MODE 2: the input is ca_b in + cb_b in2
   double *dotp2;             // MODE 2: partials of <in2|out> (in2 at the centre), as dotp
   const double *hin;         // MODE 2: H in2 (same band stride as in)
+  unsigned long long *cmax;  // optional: per band max |c| (centre input value) as FP64 bits
 };
 
 // Block = TY full x-rows of one band, marching over a z-range.  The 2 NF + 1 values of each
@@ -296,7 +297,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
     for (int h = 0; h < H; h++) hvv[h] = hv_ok[h] ? ld(zh + hxy[h]) : 0.0;
   };
   prefetch(z0);
-  double dsum = 0.0, dsum2 = 0.0, amax = 0.0;
+  double dsum = 0.0, dsum2 = 0.0, amax = 0.0, cmx = 0.0;
   for (int z = z0; z < z1; z++) {
     double vcur[P], ycur[YP], hcur[YP];
 #pragma unroll
@@ -330,6 +331,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
       if (COMB) s = fa * s + fb * hcur[q];                 // H (fa x + fb y)
       if (p.dotp) dsum += (COMB ? fa * c + fb * ycur[q] : c) * s;
       if (COMB) dsum2 += ycur[q] * s;
+      if (p.cmax) cmx = fmax(cmx, fabs(c));
       if (p.omax) amax = fmax(amax, fabs(s));
       const long r = zh + oxy[q];
       out[r] = p.accumulate ? out[r] + s : s;
@@ -343,6 +345,10 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
   if (COMB) {
     const double tot = block_sum<512>(dsum2, red);
     if (t == 0) p.dotp2[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
+  }
+  if (p.cmax) {
+    const double m = block_max<512>(cmx, red);
+    if (t == 0) atomicMax(p.cmax + b, (unsigned long long)__double_as_longlong(m));
   }
   if (p.omax) {  // non-negative doubles order like their bit patterns
     const double m = block_max<512>(amax, red);
@@ -594,10 +600,12 @@ __global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz
                                                      long ld_out, const double *psi, long ld_psi,
                                                      double *part, long part_ld, int part_off,
                                                      const double *in2, const double *ca,
-                                                     const double *cb, double *part2) {
+                                                     const double *cb, double *part2,
+                                                     double *maxpart) {
   // coef is [slot][j][b] (bands fastest): the EB coefficients of one (slot, j) are contiguous
   // and shared by all rows of the same atom, so they come from L1.
   __shared__ double red[EB][4];
+  __shared__ double redm[EB][4];
   const long i = blockIdx.x * 256L + threadIdx.x;
   const int b0 = blockIdx.y * EB;
   const bool full = b0 + EB <= nb;
@@ -629,6 +637,21 @@ __global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz
 #pragma unroll
     for (int q = 0; q < EB; q++)
       if (b0 + q < nb) out[(long)(b0 + q) * ld_out + r] += acc[q];
+  }
+  if (maxpart) {  // per band: the work-group's largest |added value|
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+      double m = (r >= 0 && b0 + q < nb) ? fabs(acc[q]) : 0.0;
+      for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+      if (lane == 0) redm[q][wv] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x < EB && b0 + (int)threadIdx.x < nb) {
+      const int q = threadIdx.x;
+      maxpart[(long)(b0 + q) * gridDim.x + blockIdx.x] =
+          fmax(fmax(redm[q][0], redm[q][1]), fmax(redm[q][2], redm[q][3]));
+    }
   }
   if (part) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -685,6 +708,28 @@ __global__ __launch_bounds__(256) void k_proj_dots(const int *act, int nact, int
   __syncthreads();
   if (w == 0 && b < nb)
     part[(long)b * part_ld + part_off] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
+// scale[b] = 2^(14 - e) with m_b < 2^e, m_b = smax_b (FP64 bits) + max over the expansion's
+// partial maxima: a bound on max |stencil part + projector part| without a pass over the bands
+__global__ __launch_bounds__(256) void k_bound_scale(const unsigned long long *smax,
+                                                     const double *maxpart, int np, double *scale) {
+  __shared__ double red[4];
+  const int b = blockIdx.x;
+  double m = 0.0;
+  if (maxpart)
+    for (int i = threadIdx.x; i < np; i += 256) m = fmax(m, maxpart[(long)b * np + i]);
+  m = block_max<256>(m, red);
+  if (threadIdx.x == 0) {
+    const double bnd = __longlong_as_double((long long)smax[b]) + m;
+    double sc = 1.0;
+    if (bnd > 0.0) {
+      int e;
+      frexp(bnd, &e);
+      sc = ldexp(1.0, 14 - e);
+    }
+    scale[b] = sc;
+  }
 }
 
 __global__ void k_flag_rows(const int *row_ptr, long n, int *flag) {
@@ -1090,6 +1135,8 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
     q.num_list = p.num_list; q.nprjmx = p.nprjmx; q.iprj_slot = p.iprj_slot;
     q.dotp = c.dot_partial;
     q.dot_ld = c.dot_ld;
+    q.cmax = c.in_max;
+    q.omax = c.out_max;
     if (c.ca) {
       if (!c.dot_partial || !c.dot_partial2 || !c.hin || c.accumulate)
         throw Error(1, "stencil: the combination form needs H in2, both partial buffers and no accumulate");
@@ -1107,17 +1154,20 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
       const long pld = c.dot_ld ? c.dot_ld : stencil_parts_lds(g, nf, c.nb, comb);
       const int off = stencil_parts_lds(g, nf, c.nb, comb);
       if (c.rp && c.dot_partial && (!comb || c.rp2)) {
-        launch_expand_rows(*c.pl, c.coef, c.nb, c.out, c.ld_out, nullptr, 0, nullptr, 0, 0, s);
+        launch_expand_rows(*c.pl, c.coef, c.nb, c.out, c.ld_out, nullptr, 0, nullptr, 0, 0, s,
+                           nullptr, nullptr, nullptr, nullptr, c.expand_max);
         launch_proj_dots(*c.pl, c.coef, c.nb, c.rp, c.ld_rp, c.dot_partial, pld, off, s);
         if (comb) launch_proj_dots(*c.pl, c.coef, c.nb, c.rp2, c.ld_rp, c.dot_partial2, pld, off, s);
       } else {
         launch_expand_rows(*c.pl, c.coef, c.nb, c.out, c.ld_out, c.in, c.ld_in, c.dot_partial, pld,
-                           off, s, comb ? c.in2 : nullptr, c.ca, c.cb, c.dot_partial2);
+                           off, s, comb ? c.in2 : nullptr, c.ca, c.cb, c.dot_partial2, c.expand_max);
       }
     }
     return;
   }
   if (c.ca) throw Error(1, "stencil: the combination form runs on the tiled path only");
+  if (c.in_max || c.out_max || c.expand_max)
+    throw Error(1, "stencil: norm and max outputs run on the tiled path only");
   p.zc = 8;
   p.nzch = (int)nblk(g.nz, p.zc);
   p.dotp = c.dot_partial;
@@ -1345,10 +1395,11 @@ void launch_proj_dots(const NonlocalPlan &pl, const double *coef, int nb, const 
 void launch_expand_rows(const NonlocalPlan &pl, const double *coef, int nb, double *out,
                         long ld_out, const double *psi, long ld_psi, double *part, long part_ld,
                         int part_off, hipStream_t s, const double *in2, const double *ca,
-                        const double *cb, double *part2) {
+                        const double *cb, double *part2, double *maxpart) {
   if (nb == 0) return;
   const int np = expand_nparts(pl);
   if (pl.nnz == 0) {
+    if (maxpart) launch_fill((long)nb * np, 0.0, maxpart, s);
     if (part)
       for (int b = 0; b < nb; b++) launch_fill(1, 0.0, part + (long)b * part_ld + part_off, s);
     if (part2)
@@ -1359,7 +1410,14 @@ void launch_expand_rows(const NonlocalPlan &pl, const double *coef, int nb, doub
   constexpr int EB = 16;
   hipLaunchKernelGGL(k_expand_rows<EB>, dim3(np, nblk(nb, EB)), dim3(256), 0, s, pl.nzrows, pl.nnz,
                      pl.row_ptr, pl.ent, pl.iprj_slot, pl.vnl, pl.num_list, pl.nprjmx, coef, nb,
-                     out, ld_out, psi, ld_psi, part, part_ld, part_off, in2, ca, cb, part2);
+                     out, ld_out, psi, ld_psi, part, part_ld, part_off, in2, ca, cb, part2, maxpart);
+  HIPCK(hipGetLastError());
+}
+
+void launch_bound_scale(int nb, const unsigned long long *smax, const double *maxpart, int np,
+                        double *scale, hipStream_t s) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_bound_scale, dim3(nb), dim3(256), 0, s, smax, maxpart, np, scale);
   HIPCK(hipGetLastError());
 }
 

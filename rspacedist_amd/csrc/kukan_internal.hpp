@@ -170,6 +170,12 @@ struct StencilCall {
   // dot_partial2 of the combination form.
   const double *rp = nullptr, *rp2 = nullptr;
   long ld_rp = 0;
+  // optional (tiled path): per-band max |in| and max |stencil part| as FP64 bits (atomic max;
+  // zero them first), and per-band, per-work-group maxima of |projector part|
+  // ([band][expand_nparts])
+  unsigned long long *in_max = nullptr;
+  unsigned long long *out_max = nullptr;
+  double *expand_max = nullptr;
 };
 
 // Several systems' stencils as one persistent launch (device tables built once, so the
@@ -229,7 +235,11 @@ void launch_expand_rows(const NonlocalPlan &pl, const double *coef, int nb, doub
                         long ld_out, const double *psi, long ld_psi, double *part, long part_ld,
                         int part_off, hipStream_t s,
                         const double *in2 = nullptr, const double *ca = nullptr,
-                        const double *cb = nullptr, double *part2 = nullptr);
+                        const double *cb = nullptr, double *part2 = nullptr,
+                        double *maxpart = nullptr);
+// scale[b] = 2^(14 - e) with smax_b (FP64 bits) + max_i maxpart[b][i] < 2^e (maxpart may be null)
+void launch_bound_scale(int nb, const unsigned long long *smax, const double *maxpart, int np,
+                        double *scale, hipStream_t s);
 // (coef band-fastest [slot][j][b]; with in2: the partials use ca_b psi + cb_b in2 as the band
 // and part2 receives <in2|added>)
 

@@ -514,7 +514,7 @@ void Ctx::proj(const double *psi, long ld, int nb, double *rp, long ld_rp) {
 
 void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, const double *rp,
                long ld_rp, double *out, long ld_out, bool accumulate, double *eps_out,
-               bool rp_exact) {
+               bool rp_exact, double *pscale_out, double *hscale_out) {
   const long n1 = (long)pl.nprjmx * pl.num_ppcell;
   double *coef = buf("coef", (size_t)std::max<long>(1, n1) * nb);
   const bool bf = stencil_band_fastest_coef(g, false);
@@ -535,9 +535,36 @@ void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, 
     np = np_total;
     if (pdots) { c.rp = rp; c.ld_rp = ld_rp; }
   }
+  // psi's column maxima and a bound on out's from the same pass (tiled path)
+  const bool tiled = bf && !accumulate;
+  const int nx = expand_nparts(pl);
+  double *xmax = nullptr;
+  unsigned long long *smax = nullptr, *pmax = nullptr;
+  if (tiled && pscale_out) {
+    pmax = (unsigned long long *)buf("hpsi.pmax", (size_t)nb);
+    launch_fill(nb, 0.0, (double *)pmax, s);  // +0.0 has all bits zero
+    c.in_max = pmax;
+  }
+  if (tiled && hscale_out) {
+    smax = (unsigned long long *)buf("hpsi.smax", (size_t)nb);
+    launch_fill(nb, 0.0, (double *)smax, s);  // +0.0 has all bits zero
+    c.out_max = smax;
+    if (pl.nent > 0) {
+      xmax = buf("hpsi.xmax", (size_t)nx * nb);
+      c.expand_max = xmax;
+    }
+  }
   c.tag = "hpsi_stencil";  // the stencil kernel; the projector pass has its own tag
   launch_stencil(c, s);
   if (eps_out) launch_reduce_partials(part, np, nb, g.dv(), eps_out, s);
+  if (pscale_out) {
+    if (pmax) maxscale(pmax, nb, pscale_out, s);
+    else colscale(psi, ld, g.n(), nb, pscale_out, s);
+  }
+  if (hscale_out) {
+    if (smax) launch_bound_scale(nb, smax, xmax, nx, hscale_out, s);
+    else colscale(out, ld_out, g.n(), nb, hscale_out, s);
+  }
 }
 
 void Ctx::hpsi_comb(int ns, const double *veffs, const double *x, const double *y,
@@ -662,6 +689,15 @@ static const SplitBands *split_psi(Ctx &c, const char *tag, const double *psi, i
   return &sp;
 }
 
+// Psi split with given column scales (split-precision runs only)
+static const SplitBands *split_psi_scaled(Ctx &c, const double *psi, int M, const double *sc,
+                                          SplitBands &sp) {
+  const long n = c.g.n();
+  if (!mixed_for(n)) return nullptr;
+  split_bands(psi, n, n, M, sc, sp, c.s);
+  return &sp;
+}
+
 // After a split-precision rotation the band norms carry ~1e-7 relative errors, which would
 // enter the (unnormalised) Rayleigh quotients dV <psi|H|psi> at first order.  Restore
 // dV <psi|S|psi> = 1 per band in FP64, scaling S psi, <p|psi> and H psi alike.
@@ -782,15 +818,23 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
                    bool h_scratch) {
   const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
   rocblas_handle h = lib().blas;
-  if (l_rrb) c.hpsi(ns, veffs, psi, n, M, rsp, n1, hpsi, n, false, nullptr);
+  // split-precision: the H psi pass also gives psi's column maxima and a bound on those of
+  // H psi (their split scales), so neither needs a pass of its own
+  double *psc = nullptr, *hsc = nullptr;
+  if (l_rrb && mixed_for(n) && param("rr_fused_scales", 1.0) >= 1.0) {
+    psc = c.buf("rr.pscale", M);
+    hsc = c.buf("rr.hscale", M);
+  }
+  if (l_rrb) c.hpsi(ns, veffs, psi, n, M, rsp, n1, hpsi, n, false, nullptr, false, psc, hsc);
   double *A = c.buf("rr.A", (size_t)M * M);
   double *E = c.buf("rr.E", (size_t)M);
   const double dv = c.g.dv(), zero = 0.0, one = 1.0;
   // psi split once for this TN and the rotation's NN (no other product runs in between)
   SplitBands spl;
-  const SplitBands *pre = split_psi(c, "rr.psi", psi, M, nullptr, spl);
+  const SplitBands *pre = psc ? split_psi_scaled(c, psi, M, psc, spl)
+                              : split_psi(c, "rr.psi", psi, M, nullptr, spl);
   const double *s_psi = pre ? pre->s : nullptr;
-  const double *s_h = mixed_scale("rr.h", hpsi, n, n, M);
+  const double *s_h = hsc ? hsc : mixed_scale("rr.h", hpsi, n, n, M);
   sub_tn(c, M, M, n, dv, psi, n, hpsi, n, zero, A, M, s_psi, s_h, pre);
   int *info = (int *)scratch("rr.info", sizeof(int));
   {

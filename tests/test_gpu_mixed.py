@@ -168,3 +168,30 @@ def test_mixed_scf_diag_converges_to_dense(kk, orc):
     st = _run_scf(kk, s, H.init_state(s, orc, smooth=True), 40, 1)
     w = scipy.linalg.eigh(H.dense_h(s), eigvals_only=True)[:8]
     np.testing.assert_allclose(st["sval"], w, atol=1e-6)
+
+
+@pytest.mark.parametrize("n,m", [(16, 24), (15, 20)])
+def test_rayleigh_ritz_scales_from_h_psi_pass(kk, orc, n, m):
+    """The split-precision Rayleigh-Ritz takes psi's split scale from its column maxima and
+    H psi's from a bound on its column maxima (stencil part + projector part), both produced
+    by the H psi pass, instead of two column-max passes.  psi's scales are the same; H psi's
+    differ by a power of two at most, so the two runs are two split-precision runs of the
+    same steps: eigenvalues agree to the split-precision bar (1e-6 Ha) and both keep
+    orthonormal bands."""
+    s = synth.make_system(n, 0.4, 3, m, seed=53 + n)
+    st0 = H.init_state(s, orc, smooth=True)
+    copy = lambda st: {k: v.copy(order="F") for k, v in st.items()}
+    out = []
+    try:
+        for fused in (1, 0):
+            kk.set_param("rr_fused_scales", fused)
+            out.append(_run_scf(kk, s, copy(st0), 3, 1))
+    finally:
+        kk.set_param("rr_fused_scales", 1)
+    a, b = out
+    print(f"max |d eig| {np.abs(a['sval'] - b['sval']).max():.2e} Ha")
+    np.testing.assert_allclose(a["sval"], b["sval"], rtol=0, atol=1e-6)
+    for st in (a, b):
+        g = st["svecre"].T @ st["ssvre"] * s.dv
+        np.testing.assert_allclose(np.diag(g), 1.0, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(g, np.eye(m), rtol=0, atol=1e-5)

@@ -1307,7 +1307,10 @@ int hartree(const HartreeArgs &a) {
   double *avhs = (double *)scratch("hart.avhs", sizeof(double) * n);
   double *sc = (double *)scratch("hart.sc", sizeof(double) * 16);
   const int np = stencil_nparts(g, a.nfh, 1);
-  const int npu = (int)std::min<long>(1024, std::max<long>(1, (n + 8191) / 8192));
+  // update work-groups: 4096 points each (512 at 128^3; 8192 points each, one work-group per
+  // CU, left the update latency-bound: a 128^3 solve took 33.9 ms against 29.7)
+  const long chunk = (long)param("hart_upd_chunk", 4096.0);
+  const int npu = (int)std::min<long>(1024, std::max<long>(1, (n + chunk - 1) / chunk));
   double *part = (double *)scratch("hart.part", sizeof(double) * std::max(np, npu));
   hipLaunchKernelGGL(k_hartree_b, dim3(nblk(n, 256)), dim3(256), 0, s, n, a.rho, b);
   double w[9];

@@ -273,30 +273,19 @@ __global__ void k_tn_reduce(const float *part, int nsplit, int M1, int M2, doubl
   *c = beta == 0.0 ? v : v + beta * *c;
 }
 
-// ----------------------------------------------------------------------------- fragment order
-// Fragment (blk, st) = 32 rows (or columns) blk x 16-deep stage st, 512 halves.  Fragments are
-// grouped in tiles of 8 consecutive blocks (256 rows); inside a tile they are stage-major, so
-// the 8 fragments a work-group needs per stage are one 8 KB run and a tile's K loop reads one
-// contiguous stream.  KC = stages per block.
-__host__ __device__ inline long frag_index(long blk, long st, long KC) {
-  return ((blk >> 3) * KC + st) * 8 + (blk & 7);
-}
-
 // ----------------------------------------------------------------------------- NN
 // C'' = C / sx * tc split into hi/lo halves in fragment layout: fragment (j / 32, k / 16) of
-// KC = Kp / 16 stages per column block, lane (j % 32) + 32 ((k % 16) / 8), half k % 8.
-// Columns are padded to M2p and k to Kp with zeros.
+// KC = Kp / 16 per column block, lane (j % 32) + 32 ((k % 16) / 8), half k % 8.  Columns
+// are padded to M2p and k to Kp with zeros.
 __global__ void k_nn_prep(const double *C, long ldc, int K, int M2, const double *sx,
                           const double *tc, int Kp, int M2p, _Float16 *ch, _Float16 *cl) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;  // = fragment element index
   if (t >= (long)Kp * M2p) return;
   const int hlf = (int)(t & 7), lane = (int)((t >> 3) & 63);
-  const long frag = t >> 9;  // = frag_index(blk, st, KC)
+  const long frag = t >> 9;
   const int KC = Kp / 16;
-  const long grp = frag >> 3;
-  const int blk = (int)((grp / KC) * 8 + (frag & 7)), st = (int)(grp % KC);
-  const int j = blk * 32 + (lane & 31);
-  const int k = st * 16 + 8 * (lane >> 5) + hlf;
+  const int j = (int)(frag / KC) * 32 + (lane & 31);
+  const int k = (int)(frag % KC) * 16 + 8 * (lane >> 5) + hlf;
   double x = 0.0;
   if (k < K && j < M2) x = C[k + ldc * (long)j] / sx[k] * tc[j];
   const _Float16 h = to_half(x);
@@ -367,7 +356,7 @@ __global__ __launch_bounds__(256) void k_split_t(const double *X, long ldx, long
       h[j] = hh;
       l[j] = (_Float16)(f - (float)hh);
     }
-    const long off = frag_index(r0 / 32 + rb, k0 / 16 + kc, KC) * 512 + 8 * lane;
+    const long off = (((r0 / 32 + rb) * KC) + k0 / 16 + kc) * 512 + 8 * lane;
     *(half8 *)&xh[off] = h;
     *(half8 *)&xl[off] = l;
   }
@@ -399,11 +388,11 @@ typedef _Float16 FragBuf[NN_NS][32][512];
 
 // The fragment-GEMM main loop shared by NN and pre-split TN: acc (wave w: rows 64 (w & 3),
 // columns 128 (w >> 2) of a 256 x 256 tile) += the products of nst 16-deep stages.  The tile's
-// A fragments of stage st are at a[h|l] + (8 st + idx) 512 (idx = row block 0..7, the fragment
-// order of frag_index), the B fragments likewise; the caller offsets a* / b* to its tile and
-// first stage.  Each wave issues 4 LDS-DMA loads per stage (buffer_load ... lds through one
-// buffer descriptor per fragment slot: the MUBUF form, unlike the global one, leaves the
-// compiler's LDS-read waits graded), NN_NS - 1 stages ahead of the one it computes.  Stage st + 1's
+// A fragments of stage st are at a[h|l] + (idx KC + st) 512 (idx = row block 0..7), the B
+// fragments likewise; the caller offsets a* / b* to its first row / column block and first
+// stage.  Each wave issues 4 LDS-DMA loads per stage (buffer_load ... lds through one buffer
+// descriptor per fragment row: the MUBUF form, unlike the global one, leaves the compiler's
+// LDS-read waits graded), NN_NS - 1 stages ahead of the one it computes.  Stage st + 1's
 // fragments are read from LDS while stage st multiplies: the next A fragments after the first
 // MFMA group, each B fragment as soon as its last MFMA of stage st is issued (at most 12 LDS
 // reads outstanding, within the 4-bit lgkm counter).  Before reading stage st + 1 a wave waits
@@ -413,32 +402,33 @@ typedef _Float16 FragBuf[NN_NS][32][512];
 // later stages in flight; empty asm statements keep the compiler from moving LDS accesses
 // across it.  The accumulation order per output (hi hi, hi lo, [lo lo,] lo hi) is the same in
 // every stage.  On return all loads have landed and the block is synchronised.
-// A work-group's stages are addressed through 32-bit buffer offsets, 8 KB per stage.
-static void frag_rows_ok(long nst, const char *who) {
-  if (nst * 8192 > 0x7fffffffL) throw Error(1, std::string(who) + ": operand too long for the split-precision product");
+// Fragment rows are addressed through 32-bit buffer offsets: KC 16-deep stages of 1 KB each.
+static void frag_rows_ok(long KC, const char *who) {
+  if (KC * 1024 > 0x7fffffffL) throw Error(1, std::string(who) + ": operand too long for the split-precision product");
 }
 
 template <int NT>
 __device__ inline void frag_gemm(const _Float16 *ah_g, const _Float16 *al_g, const _Float16 *bh_g,
-                                 const _Float16 *bl_g, int nst, FragBuf &buf, f16v (&acc)[2][4]) {
+                                 const _Float16 *bl_g, long KC, int nst, FragBuf &buf,
+                                 f16v (&acc)[2][4]) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
   // this wave's 4 fragments per stage: f = 4 w + i; f / 8 = A hi, A lo, B hi, B lo.  One
-  // buffer descriptor per fragment slot (uniform per wave), spanning the work-group's nst
-  // stages of 8 KB (below 2^31 bytes: frag_rows_ok)
+  // buffer descriptor per fragment row (uniform per wave); a row spans KC * 1 KB, which the
+  // host keeps below 2^31 bytes (frag_rows_ok)
   const int ws = __builtin_amdgcn_readfirstlane(w);
   __amdgpu_buffer_rsrc_t rs[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int f = 4 * ws + i, op = f >> 3, idx = f & 7;
     const _Float16 *base = op == 0 ? ah_g : op == 1 ? al_g : op == 2 ? bh_g : bl_g;
-    rs[i] = __builtin_amdgcn_make_buffer_rsrc((void *)(base + (long)idx * 512), (short)0,
-                                              nst * 8192, 0x00020000);
+    rs[i] = __builtin_amdgcn_make_buffer_rsrc((void *)(base + (long)idx * KC * 512), (short)0,
+                                              (int)(KC * 1024), 0x00020000);
   }
   auto dma = [&](int st) {
     _Float16(*d)[512] = buf[st % NN_NS];
 #pragma unroll
     for (int i = 0; i < 4; i++)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs[i], NN_LDS(&d[4 * ws + i][0]), 16, st * 8192 + 16 * lane, 0, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs[i], NN_LDS(&d[4 * ws + i][0]), 16, st * 1024 + 16 * lane, 0, 0, 0);
   };
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -543,9 +533,9 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
   const long KC = p.Kp / 16;
   f16v acc[2][4];
-  // this row block's and column tile's fragment tiles (frag_index order)
-  frag_gemm<NT>(p.xh + rb * KC * 4096, p.xl + rb * KC * 4096, p.ch + (long)ct * KC * 4096,
-                p.cl + (long)ct * KC * 4096, (int)KC, buf, acc);
+  frag_gemm<NT>(p.xh + (r0 / 32) * KC * 512, p.xl + (r0 / 32) * KC * 512,
+                p.ch + (long)(c0 / 32) * KC * 512, p.cl + (long)(c0 / 32) * KC * 512, KC, (int)KC,
+                buf, acc);
   // epilogue: per 32-column block b, each wave parks its 64 x 32 piece in LDS as
   // [column][row] (pitch 65 floats: conflict-free) and writes it back column by column.
   // For beta != 0 the block's 32 Y values per lane are loaded before the parking, as
@@ -627,7 +617,7 @@ __global__ __launch_bounds__(256) void k_split_tn(const double *A, long lda, lon
       h[j] = hh;
       l[j] = (_Float16)(v - (float)hh);
     }
-    const long off = frag_index(ib, kc0 + f, KC) * 512 + 8 * lane;
+    const long off = (((long)ib * KC + kc0 + f) * 64 + lane) * 8;
     *(half8 *)&xh[off] = h;
     *(half8 *)&xl[off] = l;
   }
@@ -678,7 +668,7 @@ __global__ __launch_bounds__(256) void k_split_both(const double *X, long ldx, l
       for (int j = 0; j < 8; j++) v[j] = tile[bl][rl + j];
       half8 h, l;
       split8(v, h, l);
-      const long off = frag_index((long)b0 / 32 + bb, r0 / 16 + kc, KC) * 512 + 8 * lane;
+      const long off = ((((long)b0 / 32 + bb) * KC + r0 / 16 + kc) * 64 + lane) * 8;
       *(half8 *)&th[off] = h;
       *(half8 *)&tl[off] = l;
     }
@@ -693,7 +683,7 @@ __global__ __launch_bounds__(256) void k_split_both(const double *X, long ldx, l
         for (int j = 0; j < 8; j++) v[j] = tile[bl + j][rl];
         half8 h, l;
         split8(v, h, l);
-        const long off = frag_index(r0 / 32 + rb, bcg, KCn) * 512 + 8 * lane;
+        const long off = (((r0 / 32 + rb) * KCn + bcg) * 64 + lane) * 8;
         *(half8 *)&nh[off] = h;
         *(half8 *)&nl[off] = l;
       }
@@ -727,8 +717,8 @@ __global__ __launch_bounds__(512) void k_gemm_tnf(TNFArgs p) {
   const int nst = (int)min((long)p.kper_st, p.KC - st0);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
   f16v acc[2][4];
-  const long ao = ((long)(i0 / 256) * p.KC + st0) * 4096, bo = ((long)(j0 / 256) * p.KC + st0) * 4096;
-  frag_gemm<NT>(p.ah + ao, p.al + ao, p.bh + bo, p.bl + bo, nst, buf, acc);
+  const long ao = ((long)(i0 / 32) * p.KC + st0) * 512, bo = ((long)(j0 / 32) * p.KC + st0) * 512;
+  frag_gemm<NT>(p.ah + ao, p.al + ao, p.bh + bo, p.bl + bo, p.KC, nst, buf, acc);
   float *out = p.part + (long)split * p.M1 * p.M2;
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -764,6 +754,7 @@ static void gemm_mixed_tn_presplit(int M1, int M2, long K, double alpha, const d
                                    double beta, double *C, long ldc, int nterms, hipStream_t s,
                                    const SplitBands *pa) {
   const long Kp = (K + 255) / 256 * 256, KC = Kp / 16;  // whole split tiles (as split_bands)
+  frag_rows_ok(KC, "gemm_mixed_tn");
   const int M1p = (M1 + 255) / 256 * 256, M2p = (M2 + 255) / 256 * 256;
   const bool same = A == B && sa == sb && lda == ldb && M1 == M2;
   if (pa && !(pa->X == A && pa->ldx == lda && pa->R == K && pa->M == M1 && pa->s == sa))
@@ -793,7 +784,6 @@ static void gemm_mixed_tn_presplit(int M1, int M2, long K, double alpha, const d
   const long max_split = std::max<long>(1, (1L << 29) / ((long)M1 * M2));
   p.kper_st = (int)std::max<long>(256, (KC + max_split - 1) / max_split);
   p.nsplit = (int)((KC + p.kper_st - 1) / p.kper_st);
-  frag_rows_ok(p.kper_st, "gemm_mixed_tn");
   p.part = (float *)scratch("gemm.tnpart", sizeof(float) * (size_t)p.nsplit * M1 * M2);
   const int tiles = p.tiles_i * p.tiles_j;
   const unsigned grid = (unsigned)((long)(p.nsplit + 7) / 8 * 8 * tiles);

@@ -185,6 +185,7 @@ struct SL {
   double *dotp2;             // MODE 2: partials of <in2|out> (in2 at the centre), as dotp
   const double *hin;         // MODE 2: H in2 (same band stride as in)
   unsigned long long *cmax;  // optional: per band max |c| (centre input value) as FP64 bits
+  double *dotp3;             // optional: partials of <c|c>, as dotp
 };
 
 // Block = TY full x-rows of one band, marching over a z-range.  The 2 NF + 1 values of each
@@ -297,7 +298,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
     for (int h = 0; h < H; h++) hvv[h] = hv_ok[h] ? ld(zh + hxy[h]) : 0.0;
   };
   prefetch(z0);
-  double dsum = 0.0, dsum2 = 0.0, amax = 0.0, cmx = 0.0;
+  double dsum = 0.0, dsum2 = 0.0, dsum3 = 0.0, amax = 0.0, cmx = 0.0;
   for (int z = z0; z < z1; z++) {
     double vcur[P], ycur[YP], hcur[YP];
 #pragma unroll
@@ -332,6 +333,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
       if (p.dotp) dsum += (COMB ? fa * c + fb * ycur[q] : c) * s;
       if (COMB) dsum2 += ycur[q] * s;
       if (p.cmax) cmx = fmax(cmx, fabs(c));
+      if (p.dotp3) dsum3 += c * c;
       if (p.omax) amax = fmax(amax, fabs(s));
       const long r = zh + oxy[q];
       out[r] = p.accumulate ? out[r] + s : s;
@@ -345,6 +347,10 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
   if (COMB) {
     const double tot = block_sum<512>(dsum2, red);
     if (t == 0) p.dotp2[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
+  }
+  if (p.dotp3) {
+    const double tot = block_sum<512>(dsum3, red);
+    if (t == 0) p.dotp3[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
   }
   if (p.cmax) {
     const double m = block_max<512>(cmx, red);
@@ -1136,6 +1142,7 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
     q.dotp = c.dot_partial;
     q.dot_ld = c.dot_ld;
     q.cmax = c.in_max;
+    q.dotp3 = c.norm_partial;
     q.omax = c.out_max;
     if (c.ca) {
       if (!c.dot_partial || !c.dot_partial2 || !c.hin || c.accumulate)
@@ -1166,7 +1173,7 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
     return;
   }
   if (c.ca) throw Error(1, "stencil: the combination form runs on the tiled path only");
-  if (c.in_max || c.out_max || c.expand_max)
+  if (c.in_max || c.out_max || c.expand_max || c.norm_partial)
     throw Error(1, "stencil: norm and max outputs run on the tiled path only");
   p.zc = 8;
   p.nzch = (int)nblk(g.nz, p.zc);

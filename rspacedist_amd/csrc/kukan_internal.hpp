@@ -174,6 +174,7 @@ struct StencilCall {
   // zero them first), and per-band, per-work-group maxima of |projector part|
   // ([band][expand_nparts])
   unsigned long long *in_max = nullptr;
+  double *norm_partial = nullptr;  // optional (tiled path): partials of <in|in>, as dot_partial
   unsigned long long *out_max = nullptr;
   double *expand_max = nullptr;
 };

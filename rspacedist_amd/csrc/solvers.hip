@@ -514,7 +514,7 @@ void Ctx::proj(const double *psi, long ld, int nb, double *rp, long ld_rp) {
 
 void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, const double *rp,
                long ld_rp, double *out, long ld_out, bool accumulate, double *eps_out,
-               bool rp_exact, double *pscale_out, double *hscale_out) {
+               bool rp_exact, double *pscale_out, double *hscale_out, double *norm_out) {
   const long n1 = (long)pl.nprjmx * pl.num_ppcell;
   double *coef = buf("coef", (size_t)std::max<long>(1, n1) * nb);
   const bool bf = stencil_band_fastest_coef(g, false);
@@ -540,6 +540,13 @@ void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, 
   const int nx = expand_nparts(pl);
   double *xmax = nullptr;
   unsigned long long *smax = nullptr, *pmax = nullptr;
+  const int nq = c.dot_ld ? (int)c.dot_ld : stencil_nparts(g, g.nf, nb);
+  double *npart = nullptr;
+  if (tiled && norm_out) {  // same layout as the <psi|out> partials
+    npart = buf("hpsi.npart", (size_t)nq * nb);
+    launch_fill((long)nq * nb, 0.0, npart, s);
+    c.norm_partial = npart;
+  }
   if (tiled && pscale_out) {
     pmax = (unsigned long long *)buf("hpsi.pmax", (size_t)nb);
     launch_fill(nb, 0.0, (double *)pmax, s);  // +0.0 has all bits zero
@@ -557,6 +564,10 @@ void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, 
   c.tag = "hpsi_stencil";  // the stencil kernel; the projector pass has its own tag
   launch_stencil(c, s);
   if (eps_out) launch_reduce_partials(part, np, nb, g.dv(), eps_out, s);
+  if (norm_out) {
+    if (npart) launch_reduce_partials(npart, nq, nb, g.dv(), norm_out, s);
+    else dots(psi, ld, psi, ld, nb, norm_out);
+  }
   if (pscale_out) {
     if (pmax) maxscale(pmax, nb, pscale_out, s);
     else colscale(psi, ld, g.n(), nb, pscale_out, s);
@@ -1045,20 +1056,30 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
   std::vector<double> res_h(M);
   c.clear_flag(1);
   // H Psi for all bands with their <p|psi>
-  c.hpsi(a.ns, a.veffs, a.psi, n, M, a.rsp, n1, a.hpsi, n, false, nullptr);
+  // H Psi; with S = 1 the same pass gives the first iteration's dV <psi|H psi> and
+  // dV <psi|psi> (no passes of their own)
+  double *eh0 = nullptr, *gg0 = nullptr;
+  if (fused) {
+    eh0 = c.buf("bcg.eh0", (size_t)M);
+    gg0 = c.buf("bcg.gg0", (size_t)M);
+  }
+  c.hpsi(a.ns, a.veffs, a.psi, n, M, a.rsp, n1, a.hpsi, n, false, eh0, false, nullptr, nullptr, gg0);
   int iiter = 0;
   bool all_conv = false;
   while (iiter < a.nsdmax && !all_conv) {
     iiter++;
     const bool doreset = (iiter == 1) || (iiter == 10) || ((iiter - 1) % 200 == 0);
     HIPCK(hipMemcpyAsync(rold, rnew, sizeof(double) * M, hipMemcpyDeviceToDevice, c.s));
-    // dV <psi|H psi>: from the previous iteration's fused band update when S = 1
-    if (fused && iiter > 1) {
-      HIPCK(hipMemcpyAsync(a.sval, eh_next, sizeof(double) * M, hipMemcpyDeviceToDevice, c.s));
+    // dV <psi|H psi> when S = 1: from the previous iteration's fused band update (the first
+    // iteration's from the H psi pass)
+    if (fused) {
+      HIPCK(hipMemcpyAsync(a.sval, iiter > 1 ? eh_next : eh0, sizeof(double) * M,
+                           hipMemcpyDeviceToDevice, c.s));
     } else {
       c.dots(a.psi, n, a.hpsi, n, M, a.sval);
     }
-    if (fused && iiter == 1) c.dots(a.psi, n, a.spsi, n, M, gg);  // later: from the band update
+    if (fused && iiter == 1)  // later: from the band update
+      HIPCK(hipMemcpyAsync(gg, gg0, sizeof(double) * M, hipMemcpyDeviceToDevice, c.s));
     // K r with r = -(H psi - eps S psi), and <r|Kr> (unprojected): residual measure and CG ratio;
     // for split-precision products also the per-band max |K r|
     const bool mixed = mixed_for(n);

@@ -25,9 +25,11 @@ struct Ctx {
   // scales 2^(14 - e) of psi from its column maxima; hscale_out: the same for out from a bound
   // on its column maxima (both from the same pass on the tiled path, else from passes of
   // their own)
+  // norm_out: dV <psi|psi> per band (from the same pass on the tiled path)
   void hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, const double *rp,
             long ld_rp, double *out, long ld_out, bool accumulate, double *eps_out,
-            bool rp_exact = false, double *pscale_out = nullptr, double *hscale_out = nullptr);
+            bool rp_exact = false, double *pscale_out = nullptr, double *hscale_out = nullptr,
+            double *norm_out = nullptr);
   // H p for p = ca_b x + cb_b y, p never stored (tiled path): out = H p = ca_b H x + hy with
   // hy = H y, pap = dV <p|H p>, sap = dV <y|H p>.  rpx = ca_b <proj|x> (the projector
   // coefficients of ca_b x), rp = <proj|p>; with rpy = <proj|y> the projector parts of pap and

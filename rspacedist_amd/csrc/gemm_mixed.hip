@@ -372,6 +372,7 @@ struct NNArgs {
   double *Y;
   long ldy;
   int row_blocks, col_tiles;
+  double *colsq;  // optional (beta = 0): sums of Y^2 per column and row block, [col][row block]
 };
 
 // s_waitcnt with only the vector-memory count constrained (expcnt, lgkmcnt at their maxima)
@@ -542,6 +543,8 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
   // independent loads whose latency overlaps the LDS round trip.
   constexpr int EP = 65;
   float *W = reinterpret_cast<float *>(&buf[0][0][0]) + w * 32 * EP;
+  // column sums of squares of the parked values, [row group wi][tile column], after the W areas
+  double *cs = reinterpret_cast<double *>(reinterpret_cast<float *>(&buf[0][0][0]) + 8 * 32 * EP);
   const long m = r0 + 64 * wi + lane;
   const bool mrow = m < p.R;
 #pragma unroll
@@ -561,6 +564,17 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
         W[(lane & 31) * EP + ml] = acc[a][b][r];
       }
     __syncthreads();
+    if (p.colsq) {  // lane: column lane % 32, rows 32 (lane / 32) .. +31 of the wave's 64
+      const int q = lane & 31, h = lane >> 5;
+      double s2 = 0.0;
+#pragma unroll 8
+      for (int r = 0; r < 32; r++) {
+        const double x = (double)W[q * EP + 32 * h + r];
+        s2 += x * x;
+      }
+      s2 += __shfl_down(s2, 32, 64);
+      if (lane < 32) cs[wi * 256 + 128 * wj + 32 * b + q] = s2;
+    }
 #pragma unroll
     for (int q = 0; q < 32; q++) {
       const int n = nb0 + q;
@@ -570,6 +584,11 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
       }
     }
     __syncthreads();
+  }
+  if (p.colsq && t < 256 && c0 + t < p.M2) {  // the values written are ys W exactly
+    const double y2 = p.ys[c0 + t] * p.ys[c0 + t];
+    p.colsq[(long)(c0 + t) * p.row_blocks + rb] =
+        y2 * (((cs[t] + cs[256 + t]) + cs[512 + t]) + cs[768 + t]);
   }
 }
 
@@ -860,7 +879,8 @@ void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long l
 
 void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ldx,
                    const double *sx, const double *C, long ldc, double beta, double *Y, long ldy,
-                   int nterms, hipStream_t s, const SplitBands *px) {
+                   int nterms, hipStream_t s, const SplitBands *px, double *colsq, double dv) {
+  if (colsq && beta != 0.0) throw Error(1, "gemm_mixed_nn: column norms need beta = 0");
   if (R <= 0 || M2 <= 0 || K <= 0) throw Error(1, "gemm_mixed_nn: empty product");
   const int Kp = (K + 63) / 64 * 64;  // whole 64-k split tiles (and 16-k stages)
   const int M2p = (M2 + NN_BN - 1) / NN_BN * NN_BN;
@@ -891,10 +911,18 @@ void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ld
   p.col_tiles = M2p / NN_BN;
   const long groups = (p.row_blocks + 7) / 8;
   const unsigned grid = (unsigned)(groups * 8 * p.col_tiles);
-  Timed t("gemm_mixed_nn", 2.0 * (nterms >= 4 ? 4 : 3) * (double)R * M2 * K);  // f16 MFMA flops
-  if (nterms >= 4) hipLaunchKernelGGL(k_gemm_nn<4>, dim3(grid), dim3(512), 0, s, p);
-  else hipLaunchKernelGGL(k_gemm_nn<3>, dim3(grid), dim3(512), 0, s, p);
-  HIPCK(hipGetLastError());
+  double *part = nullptr;
+  if (colsq) {
+    part = (double *)scratch("gemm.nn.colsq", sizeof(double) * (size_t)p.row_blocks * M2);
+    p.colsq = part;
+  }
+  {
+    Timed t("gemm_mixed_nn", 2.0 * (nterms >= 4 ? 4 : 3) * (double)R * M2 * K);  // f16 MFMA flops
+    if (nterms >= 4) hipLaunchKernelGGL(k_gemm_nn<4>, dim3(grid), dim3(512), 0, s, p);
+    else hipLaunchKernelGGL(k_gemm_nn<3>, dim3(grid), dim3(512), 0, s, p);
+    HIPCK(hipGetLastError());
+  }
+  if (colsq) launch_reduce_partials(part, p.row_blocks, M2, dv, colsq, s);
 }
 
 }  // namespace kukan

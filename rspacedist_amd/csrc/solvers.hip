@@ -657,14 +657,15 @@ void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda
 
 void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx, const double *C,
                  long ldc, double beta, double *Y, long ldy, const double *sx,
-                 const SplitBands *px) {
+                 const SplitBands *px, double *colsq, double dv) {
   hipStream_t s = lib().stream;
   if (mixed_for(R)) {
     if (px) sx = px->s;
     if (!sx) sx = mixed_scale("nn.x", X, ldx, R, K);
-    gemm_mixed_nn(R, K, M2, alpha, X, ldx, sx, C, ldc, beta, Y, ldy, mixed_terms(), s, px);
+    gemm_mixed_nn(R, K, M2, alpha, X, ldx, sx, C, ldc, beta, Y, ldy, mixed_terms(), s, px, colsq, dv);
     return;
   }
+  if (colsq) throw Error(1, "subspace_nn: column norms come from the split-precision product only");
   dgemm(rocblas_operation_none, rocblas_operation_none, (int)R, M2, K, alpha, X, (int)ldx, C,
         (int)ldc, beta, Y, (int)ldy);
 }
@@ -675,10 +676,12 @@ static void sub_tn(Ctx &, int M1, int M2, long K, double alpha, const double *A,
                    const SplitBands *pa = nullptr) {
   subspace_tn(M1, M2, K, alpha, A, lda, B, ldb, beta, C, ldc, sa, sb, pa);
 }
-static void sub_nn(Ctx &, long R, int K, int M2, double alpha, const double *X, long ldx,
+// colsq: dV sum Y^2 per column of the result, from the product's epilogue (split precision)
+static void sub_nn(Ctx &c, long R, int K, int M2, double alpha, const double *X, long ldx,
                    const double *C, long ldc, double beta, double *Y, long ldy,
-                   const double *sx = nullptr, const SplitBands *px = nullptr) {
-  subspace_nn(R, K, M2, alpha, X, ldx, C, ldc, beta, Y, ldy, sx, px);
+                   const double *sx = nullptr, const SplitBands *px = nullptr,
+                   double *colsq = nullptr) {
+  subspace_nn(R, K, M2, alpha, X, ldx, C, ldc, beta, Y, ldy, sx, px, colsq, c.g.dv());
 }
 
 // Psi split once for the TN and the NN product that follow (split-precision runs only); a
@@ -712,11 +715,14 @@ static const SplitBands *split_psi_scaled(Ctx &c, const double *psi, int M, cons
 // After a split-precision rotation the band norms carry ~1e-7 relative errors, which would
 // enter the (unnormalised) Rayleigh quotients dV <psi|H|psi> at first order.  Restore
 // dV <psi|S|psi> = 1 per band in FP64, scaling S psi, <p|psi> and H psi alike.
-static void renormalize(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi) {
+// norms: dV <psi|S psi> when already known (from the product that wrote psi).
+static void renormalize(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi,
+                        const double *norms = nullptr) {
   const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
   double *f = c.buf("renorm.f", M);
   c.clear_flag(2);
-  c.dots(psi, n, spsi, n, M, f);
+  if (norms) HIPCK(hipMemcpyAsync(f, norms, sizeof(double) * M, hipMemcpyDeviceToDevice, c.s));
+  else c.dots(psi, n, spsi, n, M, f);
   hipLaunchKernelGGL(k_rsqrt, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, f, c.flag(2));
   if (c.read_flag(2)) throw Error(2, "orthogonalization_r_01 (1): wf has negative norm");
   launch_scale(n, M, f, psi, n, c.s);
@@ -768,11 +774,13 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi,
     auto apply = [&](double *X, const double *sx, const SplitBands *px) {
       sub_nn(c, n, M, M, 1.0, X, n, Ri, M, 0.0, X, n, sx, px);
     };
-    apply(psi, s_psi, pre);
+    // with S psi = psi the product that writes psi also gives its norms
+    double *nrm = spsi == psi ? c.buf("cholqr.norms", M) : nullptr;
+    sub_nn(c, n, M, M, 1.0, psi, n, Ri, M, 0.0, psi, n, s_psi, pre, nrm);
     if (spsi != psi) apply(spsi, nullptr, nullptr);
     if (hpsi) apply(hpsi, nullptr, nullptr);
     if (rsp && n1 > 0) tr(rsp, n1, n1);
-    renormalize(c, M, psi, spsi, rsp, hpsi);
+    renormalize(c, M, psi, spsi, rsp, hpsi, nrm);
     return;
   }
   tr(psi, n, n);
@@ -860,7 +868,9 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
   if (h_scratch && !keep_h && mixed_for(n) && !(c.q_zero == false && spsi != psi)) {
     // split products, H psi not needed afterwards: Psi rotated in place (the product reads
     // it only through the pre-split copy), no band-sized temporary or copy
-    sub_nn(c, n, M, M, one, psi, n, A, M, zero, psi, n, s_psi, pre);
+    // (S psi = psi here: the product that writes psi also gives its norms)
+    double *nrm = c.buf("rr.norms", M);
+    sub_nn(c, n, M, M, one, psi, n, A, M, zero, psi, n, s_psi, pre, nrm);
     if (spsi != psi)
       HIPCK(hipMemcpyAsync(spsi, psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
     if (n1 > 0) {
@@ -869,7 +879,7 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
             zero, R, (int)n1);
       HIPCK(hipMemcpyAsync(rsp, R, sizeof(double) * n1 * M, hipMemcpyDeviceToDevice, c.s));
     }
-    renormalize(c, M, psi, spsi, rsp, nullptr);
+    renormalize(c, M, psi, spsi, rsp, nullptr, nrm);
     return;
   }
   if (keep_h && mixed_for(n)) {

@@ -123,7 +123,7 @@ def forces(s, dev, st, kk, synth):
 
 # timing tag -> kernel whose calibrated HBM bytes per launch the committed counter profile holds
 TRAFFIC_KERNEL = {"gemm_mixed_nn": "k_gemm_nn<3>", "gemm_mixed_tn": "k_gemm_tnf<3>",
-                  "hpsi_stencil": "k_stencil_lds<4, 2, 0>", "hp_comb_stencil": "k_stencil_lds<4, 1, 2>",
+                  "hpsi_stencil": "k_stencil_lds<4, 2, 3>", "hp_comb_stencil": "k_stencil_lds<4, 1, 2>",
                   "precond_resid": "k_stencil_lds<1, 2, 1>",
                   "band_update_dots": "k_pair_update_dots", "p_update_dots": "k_pupdate_dots",
                   "dots": "k_dot_part", "gemm_scale": "k_colmax", "nl_expand": "k_expand_rows<16>",

@@ -199,6 +199,9 @@ template <int NF, int P, int MODE>
 __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, double *lds,
                                      double *red) {
   constexpr bool RES = MODE == 1, COMB = MODE == 2;
+  // MODE 3: plain, also the partials of <c|c> (no max); MODE 4: plain, also the maxima of |c|
+  // and |out| (no partials) - each keeps the plain kernel's accumulator count
+  constexpr bool NORMS = MODE == 3, MAXES = MODE == 4;
   constexpr int H = NF;  // halo values per thread: 2 NF rows x nx <= 512 NF for nx <= 256
   const int nx = p.nx, ny = p.ny, nz = p.nz, TY = p.TY, W = nx + 2 * NF;
   const long plane = (long)nx * ny;
@@ -330,17 +333,17 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
 #pragma unroll
       for (int m = 1; m <= NF; m++) s += p.cz[m] * (col[q][NF - m] + col[q][NF + m]);
       if (COMB) s = fa * s + fb * hcur[q];                 // H (fa x + fb y)
-      if (p.dotp) dsum += (COMB ? fa * c + fb * ycur[q] : c) * s;
+      if (!MAXES && p.dotp) dsum += (COMB ? fa * c + fb * ycur[q] : c) * s;
       if (COMB) dsum2 += ycur[q] * s;
-      if (p.cmax) cmx = fmax(cmx, fabs(c));
-      if (p.dotp3) dsum3 += c * c;
-      if (p.omax) amax = fmax(amax, fabs(s));
+      if (MAXES) cmx = fmax(cmx, fabs(c));
+      if (NORMS) dsum3 += c * c;
+      if (!NORMS && p.omax) amax = fmax(amax, fabs(s));
       const long r = zh + oxy[q];
       out[r] = p.accumulate ? out[r] + s : s;
     }
     __syncthreads();
   }
-  if (p.dotp) {
+  if (!MAXES && p.dotp) {
     const double tot = block_sum<512>(dsum, red);
     if (t == 0) p.dotp[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
   }
@@ -348,15 +351,15 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
     const double tot = block_sum<512>(dsum2, red);
     if (t == 0) p.dotp2[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
   }
-  if (p.dotp3) {
+  if (NORMS) {
     const double tot = block_sum<512>(dsum3, red);
     if (t == 0) p.dotp3[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
   }
-  if (p.cmax) {
+  if (MAXES) {
     const double m = block_max<512>(cmx, red);
     if (t == 0) atomicMax(p.cmax + b, (unsigned long long)__double_as_longlong(m));
   }
-  if (p.omax) {  // non-negative doubles order like their bit patterns
+  if (!NORMS && p.omax) {  // non-negative doubles order like their bit patterns
     const double m = block_max<512>(amax, red);
     if (t == 0) atomicMax(p.omax + b, (unsigned long long)__double_as_longlong(m));
   }
@@ -1075,6 +1078,22 @@ static void launch_lds(SL &p, const Grid &g, int NF, int nb, bool res, hipStream
   } else if (p.ca) {
     switch (NF) {
 #define CASE(N) case N: launch_lds_nf<N, 2>(p, L, nb, s); break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+      default: throw Error(1, "error in kslaplacian_r! This subroutine is for nf in 1-8.");
+    }
+  } else if (p.dotp3) {  // plain with norm partials
+    if (p.cmax || p.omax) throw Error(1, "stencil: norms and maxima in one launch");
+    switch (NF) {
+#define CASE(N) case N: launch_lds_nf<N, 3>(p, L, nb, s); break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+      default: throw Error(1, "error in kslaplacian_r! This subroutine is for nf in 1-8.");
+    }
+  } else if (p.cmax) {  // plain with column maxima
+    if (p.dotp) throw Error(1, "stencil: maxima and partials in one launch");
+    switch (NF) {
+#define CASE(N) case N: launch_lds_nf<N, 4>(p, L, nb, s); break;
       CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
       default: throw Error(1, "error in kslaplacian_r! This subroutine is for nf in 1-8.");

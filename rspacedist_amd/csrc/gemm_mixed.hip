@@ -261,11 +261,15 @@ __global__ __launch_bounds__(512) void k_gemm_tn(TNArgs p) {
       }
 }
 
+// tri = 1 / 2: only the 256 x 256 tiles on and below / above the diagonal were computed;
+// the others are left as they were
 __global__ void k_tn_reduce(const float *part, int nsplit, int M1, int M2, double alpha,
-                            const double *sa, const double *sb, double beta, double *C, long ldc) {
+                            const double *sa, const double *sb, double beta, double *C, long ldc,
+                            int tri) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= (long)M1 * M2) return;
   const int i = (int)(t / M2), j = (int)(t % M2);
+  if ((tri == 1 && (i >> 8) < (j >> 8)) || (tri == 2 && (i >> 8) > (j >> 8))) return;
   double s = 0.0;
   for (int q = 0; q < nsplit; q++) s += (double)part[(long)q * M1 * M2 + t];
   const double v = alpha * s / (sa[i] * sb[j]);
@@ -717,6 +721,7 @@ struct TNFArgs {
   float *part;  // [split][M1][M2], M2 fastest
   int kper_st;  // stages (16 k) per split
   int nsplit, tiles_i, tiles_j;
+  int tri;  // 0: all tiles; 1 / 2: the tiles on and below / above the diagonal (M1 = M2)
 };
 
 // Split-K TN from fragments: work-group = one 256 x 256 output tile over one K-range of
@@ -727,11 +732,22 @@ template <int NT>
 __global__ __launch_bounds__(512) void k_gemm_tnf(TNFArgs p) {
   __shared__ __attribute__((aligned(16))) FragBuf buf;
   const int L = blockIdx.x, xcd = L & 7, jj = L >> 3;
-  const int tiles = p.tiles_i * p.tiles_j;
+  const int tiles = p.tri ? p.tiles_i * (p.tiles_i + 1) / 2 : p.tiles_i * p.tiles_j;
   const int split = (jj / tiles) * 8 + xcd;
   if (split >= p.nsplit) return;
   const int tile = jj % tiles;
-  const int i0 = (tile % p.tiles_i) * 256, j0 = (tile / p.tiles_i) * 256;
+  int ti = tile % p.tiles_i, tj = tile / p.tiles_i;
+  if (p.tri) {  // tile = ti (ti + 1) / 2 + tj with tj <= ti (lower); transposed for upper
+    ti = 0;
+    while ((ti + 1) * (ti + 2) / 2 <= tile) ti++;
+    tj = tile - ti * (ti + 1) / 2;
+    if (p.tri == 2) {
+      const int x = ti;
+      ti = tj;
+      tj = x;
+    }
+  }
+  const int i0 = ti * 256, j0 = tj * 256;
   const long st0 = (long)split * p.kper_st;
   const int nst = (int)min((long)p.kper_st, p.KC - st0);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
@@ -771,7 +787,8 @@ void colscale(const double *A, long lda, long rows, int cols, double *scale, hip
 static void gemm_mixed_tn_presplit(int M1, int M2, long K, double alpha, const double *A, long lda,
                                    const double *sa, const double *B, long ldb, const double *sb,
                                    double beta, double *C, long ldc, int nterms, hipStream_t s,
-                                   const SplitBands *pa) {
+                                   const SplitBands *pa, int tri) {
+  if (tri && M1 != M2) throw Error(1, "gemm_mixed_tn: a triangle needs a square product");
   const long Kp = (K + 255) / 256 * 256, KC = Kp / 16;  // whole split tiles (as split_bands)
   frag_rows_ok(KC, "gemm_mixed_tn");
   const int M1p = (M1 + 255) / 256 * 256, M2p = (M2 + 255) / 256 * 256;
@@ -798,20 +815,23 @@ static void gemm_mixed_tn_presplit(int M1, int M2, long K, double alpha, const d
   p.M1 = M1; p.M2 = M2; p.ah = ah; p.al = al; p.bh = bh; p.bl = bl; p.KC = KC;
   p.tiles_i = M1p / 256;
   p.tiles_j = M2p / 256;
+  p.tri = tri;
   // stages per K-range: 256 (4096 k, FP32 accumulation depth), more if the FP32 partials
   // would exceed 2^29 values (2 GiB)
   const long max_split = std::max<long>(1, (1L << 29) / ((long)M1 * M2));
   p.kper_st = (int)std::max<long>(256, (KC + max_split - 1) / max_split);
   p.nsplit = (int)((KC + p.kper_st - 1) / p.kper_st);
   p.part = (float *)scratch("gemm.tnpart", sizeof(float) * (size_t)p.nsplit * M1 * M2);
-  const int tiles = p.tiles_i * p.tiles_j;
+  const int tiles = tri ? p.tiles_i * (p.tiles_i + 1) / 2 : p.tiles_i * p.tiles_j;
   const unsigned grid = (unsigned)((long)(p.nsplit + 7) / 8 * 8 * tiles);
-  Timed t("gemm_mixed_tn", 2.0 * (nterms >= 4 ? 4 : 3) * (double)M1 * M2 * K);  // f16 MFMA flops
+  // f16 MFMA flops of the tiles computed
+  const double frac = tri ? (double)tiles / ((double)p.tiles_i * p.tiles_j) : 1.0;
+  Timed t("gemm_mixed_tn", frac * 2.0 * (nterms >= 4 ? 4 : 3) * (double)M1 * M2 * K);
   if (nterms >= 4) hipLaunchKernelGGL(k_gemm_tnf<4>, dim3(grid), dim3(512), 0, s, p);
   else hipLaunchKernelGGL(k_gemm_tnf<3>, dim3(grid), dim3(512), 0, s, p);
   HIPCK(hipGetLastError());
   hipLaunchKernelGGL(k_tn_reduce, dim3(nblk((long)M1 * M2, 256)), dim3(256), 0, s, p.part, p.nsplit,
-                     M1, M2, alpha, sa, sb, beta, C, ldc);
+                     M1, M2, alpha, sa, sb, beta, C, ldc, tri);
   HIPCK(hipGetLastError());
 }
 
@@ -846,10 +866,11 @@ void normscale(const double *g, int cols, double dv, double *scale, hipStream_t 
 
 void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                    const double *sa, const double *B, long ldb, const double *sb, double beta,
-                   double *C, long ldc, int nterms, hipStream_t s, const SplitBands *pa) {
+                   double *C, long ldc, int nterms, hipStream_t s, const SplitBands *pa, int tri) {
   if (M1 <= 0 || M2 <= 0 || K <= 0) throw Error(1, "gemm_mixed_tn: empty product");
   if (pa || param("tn_presplit", 1.0) >= 1.0) {
-    gemm_mixed_tn_presplit(M1, M2, K, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, nterms, s, pa);
+    gemm_mixed_tn_presplit(M1, M2, K, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, nterms, s, pa,
+                           tri);
     return;
   }
   TNArgs p{};
@@ -873,7 +894,7 @@ void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long l
   else hipLaunchKernelGGL(k_gemm_tn<3>, dim3(grid), dim3(512), 0, s, p);
   HIPCK(hipGetLastError());
   hipLaunchKernelGGL(k_tn_reduce, dim3(nblk((long)M1 * M2, 256)), dim3(256), 0, s, p.part, nsplit,
-                     M1, M2, alpha, sa, sb, beta, C, ldc);
+                     M1, M2, alpha, sa, sb, beta, C, ldc, 0);
   HIPCK(hipGetLastError());
 }
 

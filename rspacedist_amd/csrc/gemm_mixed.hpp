@@ -23,10 +23,13 @@ void maxscale(const unsigned long long *cmax, int cols, double *scale, hipStream
 // scale[j] = 2^(13 - e_j) with sqrt(g_j / dv) < 2^e_j: a bound for columns with
 // dv sum_i A(i,j)^2 = g_j, no pass over A
 void normscale(const double *g, int cols, double dv, double *scale, hipStream_t s);
-// C(M1 x M2) = alpha A^T B + beta C with column scales sa (A) and sb (B)
+// C(M1 x M2) = alpha A^T B + beta C with column scales sa (A) and sb (B).  tri = 1 / 2 (M1 = M2):
+// only the 256 x 256 tiles on and below / above the diagonal are computed and written (for a
+// consumer that reads the lower / upper triangle alone)
 void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                    const double *sa, const double *B, long ldb, const double *sb, double beta,
-                   double *C, long ldc, int nterms, hipStream_t s, const SplitBands *pa = nullptr);
+                   double *C, long ldc, int nterms, hipStream_t s, const SplitBands *pa = nullptr,
+                   int tri = 0);
 // Y(R x M2) = alpha X C + beta Y with column scales sx of X (length K).  X is read only by
 // the pre-split pass that precedes the product, so Y may be X itself (same ld, beta = 0).
 // colsq (beta = 0 only): dv sum_i Y(i,j)^2 of the written Y per column, from the epilogue.

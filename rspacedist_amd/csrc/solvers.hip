@@ -642,13 +642,14 @@ const double *mixed_scale(const char *tag, const double *X, long ldx, long rows,
 
 void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                  const double *B, long ldb, double beta, double *C, long ldc, const double *sa,
-                 const double *sb, const SplitBands *pa) {
+                 const double *sb, const SplitBands *pa, int tri) {
   hipStream_t s = lib().stream;
   if (mixed_for(K)) {
     if (pa) sa = pa->s;
     if (!sa) sa = mixed_scale("tn.a", A, lda, K, M1);
     if (!sb) sb = (B == A && ldb == lda && M2 == M1) ? sa : mixed_scale("tn.b", B, ldb, K, M2);
-    gemm_mixed_tn(M1, M2, K, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, mixed_terms(), s, pa);
+    gemm_mixed_tn(M1, M2, K, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, mixed_terms(), s, pa,
+                  param("tn_triangle", 1.0) >= 1.0 ? tri : 0);
     return;
   }
   dgemm(rocblas_operation_transpose, rocblas_operation_none, M1, M2, (int)K, alpha, A, (int)lda, B,
@@ -673,8 +674,8 @@ void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx,
 static void sub_tn(Ctx &, int M1, int M2, long K, double alpha, const double *A, long lda,
                    const double *B, long ldb, double beta, double *C, long ldc,
                    const double *sa = nullptr, const double *sb = nullptr,
-                   const SplitBands *pa = nullptr) {
-  subspace_tn(M1, M2, K, alpha, A, lda, B, ldb, beta, C, ldc, sa, sb, pa);
+                   const SplitBands *pa = nullptr, int tri = 0) {
+  subspace_tn(M1, M2, K, alpha, A, lda, B, ldb, beta, C, ldc, sa, sb, pa, tri);
 }
 // colsq: dV sum Y^2 per column of the result, from the product's epilogue (split precision)
 static void sub_nn(Ctx &c, long R, int K, int M2, double alpha, const double *X, long ldx,
@@ -745,7 +746,8 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi,
   SplitBands spl;
   const SplitBands *pre = split_psi(c, "cholqr.psi", psi, M, norms, spl);
   const double *s_psi = pre ? pre->s : nullptr;
-  sub_tn(c, M, M, n, dv, psi, n, spsi, n, 0.0, G, M, s_psi, nullptr, pre);  // spsi == psi: one split
+  // dpotrf reads the upper triangle only
+  sub_tn(c, M, M, n, dv, psi, n, spsi, n, 0.0, G, M, s_psi, nullptr, pre, 2);  // spsi == psi: one split
   int *info = (int *)scratch("cholqr.info", sizeof(int));
   {
     Timed t("potrf", (double)M * M * M / 3.0);
@@ -854,7 +856,7 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
                               : split_psi(c, "rr.psi", psi, M, nullptr, spl);
   const double *s_psi = pre ? pre->s : nullptr;
   const double *s_h = hsc ? hsc : mixed_scale("rr.h", hpsi, n, n, M);
-  sub_tn(c, M, M, n, dv, psi, n, hpsi, n, zero, A, M, s_psi, s_h, pre);
+  sub_tn(c, M, M, n, dv, psi, n, hpsi, n, zero, A, M, s_psi, s_h, pre, 1);  // dsyevd reads the lower triangle
   int *info = (int *)scratch("rr.info", sizeof(int));
   {
     Timed t("syevd", 4.0 * (double)M * M * M);

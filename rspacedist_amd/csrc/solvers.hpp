@@ -69,10 +69,11 @@ struct HartreeArgs {
 // per params "precision" / "mixed_min_rows" / "mixed_terms".
 // Column scales sa/sb/sx may be passed in (from mixed_scale) when an operand is reused;
 // null means "compute here".  mixed_scale returns null in FP64 mode.
+// tri (split precision): 1 / 2 computes only the tiles on and below / above the diagonal
 void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda,
                  const double *B, long ldb, double beta, double *C, long ldc,
                  const double *sa = nullptr, const double *sb = nullptr,
-                 const SplitBands *pa = nullptr);
+                 const SplitBands *pa = nullptr, int tri = 0);
 // colsq (split precision, beta = 0): dv sum_i Y(i,j)^2 per column of the result
 void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx, const double *C,
                  long ldc, double beta, double *Y, long ldy, const double *sx = nullptr,

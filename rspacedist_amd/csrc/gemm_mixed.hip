@@ -377,6 +377,7 @@ struct NNArgs {
   long ldy;
   int row_blocks, col_tiles;
   double *colsq;  // optional (beta = 0): sums of Y^2 per column and row block, [col][row block]
+  int cupper;     // C upper triangular: column tile ct needs only k < 256 (ct + 1)
 };
 
 // s_waitcnt with only the vector-memory count constrained (expcnt, lgkmcnt at their maxima)
@@ -537,9 +538,12 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
   const int c0 = ct * NN_BN;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
   const long KC = p.Kp / 16;
+  // with C upper triangular, rows k >= c0 + 256 of C are zero for these columns: their
+  // products would add exact zeros, so the stages are skipped
+  const int nst = p.cupper ? (int)min(KC, (long)(c0 + NN_BN) / 16) : (int)KC;
   f16v acc[2][4];
   frag_gemm<NT>(p.xh + (r0 / 32) * KC * 512, p.xl + (r0 / 32) * KC * 512,
-                p.ch + (long)(c0 / 32) * KC * 512, p.cl + (long)(c0 / 32) * KC * 512, KC, (int)KC,
+                p.ch + (long)(c0 / 32) * KC * 512, p.cl + (long)(c0 / 32) * KC * 512, KC, nst,
                 buf, acc);
   // epilogue: per 32-column block b, each wave parks its 64 x 32 piece in LDS as
   // [column][row] (pitch 65 floats: conflict-free) and writes it back column by column.
@@ -900,7 +904,8 @@ void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long l
 
 void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ldx,
                    const double *sx, const double *C, long ldc, double beta, double *Y, long ldy,
-                   int nterms, hipStream_t s, const SplitBands *px, double *colsq, double dv) {
+                   int nterms, hipStream_t s, const SplitBands *px, double *colsq, double dv,
+                   bool c_upper) {
   if (colsq && beta != 0.0) throw Error(1, "gemm_mixed_nn: column norms need beta = 0");
   if (R <= 0 || M2 <= 0 || K <= 0) throw Error(1, "gemm_mixed_nn: empty product");
   const int Kp = (K + 63) / 64 * 64;  // whole 64-k split tiles (and 16-k stages)
@@ -928,6 +933,7 @@ void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ld
   NNArgs p{};
   p.R = R; p.M2 = M2; p.Kp = Kp; p.xh = xh; p.xl = xl; p.ch = ch; p.cl = cl;
   p.ys = ys; p.beta = beta; p.Y = Y; p.ldy = ldy;
+  p.cupper = c_upper ? 1 : 0;
   p.row_blocks = (int)(Rp / NN_BM);
   p.col_tiles = M2p / NN_BN;
   const long groups = (p.row_blocks + 7) / 8;
@@ -937,8 +943,15 @@ void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ld
     part = (double *)scratch("gemm.nn.colsq", sizeof(double) * (size_t)p.row_blocks * M2);
     p.colsq = part;
   }
+  double frac = 1.0;  // share of the K stages computed
+  if (c_upper) {
+    const long KC = Kp / 16;
+    double done = 0.0;
+    for (int ct = 0; ct < p.col_tiles; ct++) done += (double)std::min(KC, (long)(ct + 1) * NN_BN / 16);
+    frac = done / ((double)KC * p.col_tiles);
+  }
   {
-    Timed t("gemm_mixed_nn", 2.0 * (nterms >= 4 ? 4 : 3) * (double)R * M2 * K);  // f16 MFMA flops
+    Timed t("gemm_mixed_nn", frac * 2.0 * (nterms >= 4 ? 4 : 3) * (double)R * M2 * K);  // f16 flops
     if (nterms >= 4) hipLaunchKernelGGL(k_gemm_nn<4>, dim3(grid), dim3(512), 0, s, p);
     else hipLaunchKernelGGL(k_gemm_nn<3>, dim3(grid), dim3(512), 0, s, p);
     HIPCK(hipGetLastError());

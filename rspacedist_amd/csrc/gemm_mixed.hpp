@@ -33,8 +33,9 @@ void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long l
 // Y(R x M2) = alpha X C + beta Y with column scales sx of X (length K).  X is read only by
 // the pre-split pass that precedes the product, so Y may be X itself (same ld, beta = 0).
 // colsq (beta = 0 only): dv sum_i Y(i,j)^2 of the written Y per column, from the epilogue.
+// c_upper: C is upper triangular (zero below the diagonal), so column j needs only k <= j.
 void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ldx,
                    const double *sx, const double *C, long ldc, double beta, double *Y, long ldy,
                    int nterms, hipStream_t s, const SplitBands *px = nullptr,
-                   double *colsq = nullptr, double dv = 1.0);
+                   double *colsq = nullptr, double dv = 1.0, bool c_upper = false);
 }  // namespace kukan

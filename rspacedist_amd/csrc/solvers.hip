@@ -658,12 +658,13 @@ void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda
 
 void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx, const double *C,
                  long ldc, double beta, double *Y, long ldy, const double *sx,
-                 const SplitBands *px, double *colsq, double dv) {
+                 const SplitBands *px, double *colsq, double dv, bool c_upper) {
   hipStream_t s = lib().stream;
   if (mixed_for(R)) {
     if (px) sx = px->s;
     if (!sx) sx = mixed_scale("nn.x", X, ldx, R, K);
-    gemm_mixed_nn(R, K, M2, alpha, X, ldx, sx, C, ldc, beta, Y, ldy, mixed_terms(), s, px, colsq, dv);
+    gemm_mixed_nn(R, K, M2, alpha, X, ldx, sx, C, ldc, beta, Y, ldy, mixed_terms(), s, px, colsq, dv,
+                  c_upper && param("nn_triangle", 1.0) >= 1.0);
     return;
   }
   if (colsq) throw Error(1, "subspace_nn: column norms come from the split-precision product only");
@@ -681,8 +682,8 @@ static void sub_tn(Ctx &, int M1, int M2, long K, double alpha, const double *A,
 static void sub_nn(Ctx &c, long R, int K, int M2, double alpha, const double *X, long ldx,
                    const double *C, long ldc, double beta, double *Y, long ldy,
                    const double *sx = nullptr, const SplitBands *px = nullptr,
-                   double *colsq = nullptr) {
-  subspace_nn(R, K, M2, alpha, X, ldx, C, ldc, beta, Y, ldy, sx, px, colsq, c.g.dv());
+                   double *colsq = nullptr, bool c_upper = false) {
+  subspace_nn(R, K, M2, alpha, X, ldx, C, ldc, beta, Y, ldy, sx, px, colsq, c.g.dv(), c_upper);
 }
 
 // Psi split once for the TN and the NN product that follow (split-precision runs only); a
@@ -774,11 +775,11 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi,
     hipLaunchKernelGGL(k_zero_lower, dim3(nblk((long)M * M, 256)), dim3(256), 0, c.s, M, Ri);
     // in place: the split product reads X only through its pre-split copy
     auto apply = [&](double *X, const double *sx, const SplitBands *px) {
-      sub_nn(c, n, M, M, 1.0, X, n, Ri, M, 0.0, X, n, sx, px);
+      sub_nn(c, n, M, M, 1.0, X, n, Ri, M, 0.0, X, n, sx, px, nullptr, true);
     };
     // with S psi = psi the product that writes psi also gives its norms
     double *nrm = spsi == psi ? c.buf("cholqr.norms", M) : nullptr;
-    sub_nn(c, n, M, M, 1.0, psi, n, Ri, M, 0.0, psi, n, s_psi, pre, nrm);
+    sub_nn(c, n, M, M, 1.0, psi, n, Ri, M, 0.0, psi, n, s_psi, pre, nrm, true);  // R^-1 upper
     if (spsi != psi) apply(spsi, nullptr, nullptr);
     if (hpsi) apply(hpsi, nullptr, nullptr);
     if (rsp && n1 > 0) tr(rsp, n1, n1);

@@ -77,7 +77,8 @@ void subspace_tn(int M1, int M2, long K, double alpha, const double *A, long lda
 // colsq (split precision, beta = 0): dv sum_i Y(i,j)^2 per column of the result
 void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx, const double *C,
                  long ldc, double beta, double *Y, long ldy, const double *sx = nullptr,
-                 const SplitBands *px = nullptr, double *colsq = nullptr, double dv = 1.0);
+                 const SplitBands *px = nullptr, double *colsq = nullptr, double dv = 1.0,
+                 bool c_upper = false);
 const double *mixed_scale(const char *tag, const double *X, long ldx, long rows, int cols);
 
 void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi,

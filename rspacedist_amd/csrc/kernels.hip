@@ -159,6 +159,8 @@ __global__ __launch_bounds__(256) void k_stencil(SP p) {
 //   * MODE 1 reads sign*(h - eps_b s) instead of a band (fused residual for K r); MODE 2 gives
 //     H (ca_b x + cb_b y) by linearity, as ca_b H x + cb_b (H y) with H y read (hin), so the
 //     combination is never stored and the stencil loads only x; it also sums <y|out>.
+//     MODE 5 is MODE 2 without y: for a symmetric H, <y|H p> = ca_b <H y|x> + cb_b <y|H y>,
+//     so the block sums <ca_b x|out> and <ca_b x|H y> from x and H y alone and y is not read.
 struct SL {
   int nx, ny, nz, px, py, pz, TY, ntile, nzch, zc, nbands;
   const double *in, *in2;
@@ -198,7 +200,7 @@ struct SL {
 template <int NF, int P, int MODE>
 __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, double *lds,
                                      double *red) {
-  constexpr bool RES = MODE == 1, COMB = MODE == 2;
+  constexpr bool RES = MODE == 1, COMB = MODE == 2 || MODE == 5, SYM = MODE == 5;
   // MODE 3: plain, also the partials of <c|c> (no max); MODE 4: plain, also the maxima of |c|
   // and |out| (no partials) - each keeps the plain kernel's accumulator count
   constexpr bool NORMS = MODE == 3, MAXES = MODE == 4;
@@ -208,7 +210,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
   const int per = p.ntile * p.nzch;
   const int y0 = tile * TY, z0 = zch * p.zc, z1 = min(nz, z0 + p.zc);
   const double *in = p.in + (long)b * p.ld_in;
-  const double *in2 = (RES || COMB) ? p.in2 + (long)b * p.ld_in : nullptr;
+  const double *in2 = (RES || (COMB && !SYM)) ? p.in2 + (long)b * p.ld_in : nullptr;
   const double *hin = COMB ? p.hin + (long)b * p.ld_in : nullptr;
   const double e = RES ? p.eps[b] : 0.0;
   const double fa = COMB ? p.ca[b] : 0.0, fb = COMB ? p.cb[b] : 0.0;
@@ -293,7 +295,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
 #pragma unroll
     for (int q = 0; q < P; q++) {
       nxt[q] = (lv[q] && zn >= 0) ? ld(zn + oxy[q]) : 0.0;
-      if (COMB) yf[q] = sv[q] ? in2[zh + oxy[q]] : 0.0;
+      if (COMB && !SYM) yf[q] = sv[q] ? in2[zh + oxy[q]] : 0.0;
       if (COMB) hf[q] = sv[q] ? hin[zh + oxy[q]] : 0.0;
       vf[q] = (p.veff && sv[q]) ? p.veff[zh + oxy[q]] : 0.0;
     }
@@ -309,7 +311,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
 #pragma unroll
       for (int m = 0; m < 2 * NF; m++) col[q][m] = col[q][m + 1];
       col[q][2 * NF] = nxt[q];
-      if (COMB) ycur[q] = yf[q];
+      if (COMB && !SYM) ycur[q] = yf[q];
       if (COMB) hcur[q] = hf[q];
       vcur[q] = vf[q];
       if (t + 512 * q < nx * TY) put(lcol[q], xq[q], col[q][NF]);
@@ -333,8 +335,8 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
 #pragma unroll
       for (int m = 1; m <= NF; m++) s += p.cz[m] * (col[q][NF - m] + col[q][NF + m]);
       if (COMB) s = fa * s + fb * hcur[q];                 // H (fa x + fb y)
-      if (!MAXES && p.dotp) dsum += (COMB ? fa * c + fb * ycur[q] : c) * s;
-      if (COMB) dsum2 += ycur[q] * s;
+      if (!MAXES && p.dotp) dsum += (SYM ? fa * c : COMB ? fa * c + fb * ycur[q] : c) * s;
+      if (COMB) dsum2 += SYM ? fa * c * hcur[q] : ycur[q] * s;
       if (MAXES) cmx = fmax(cmx, fabs(c));
       if (NORMS) dsum3 += c * c;
       if (!NORMS && p.omax) amax = fmax(amax, fabs(s));
@@ -370,7 +372,7 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
 template <int NF, int P, int MODE>
 constexpr int stencil_waves() {
   if (NF > 4) return 2;
-  if (MODE == 2) return P == 1 ? 4 : 2;
+  if (MODE == 2 || MODE == 5) return P == 1 ? 4 : 2;
   return P <= 2 ? 4 : 2;
 }
 
@@ -1075,9 +1077,16 @@ static void launch_lds(SL &p, const Grid &g, int NF, int nb, bool res, hipStream
   if (res) {
     if (NF != 1) throw Error(1, "fused residual stencil supports radius 1 only");
     launch_lds_nf<1, 1>(p, L, nb, s);
-  } else if (p.ca) {
+  } else if (p.ca && p.in2) {
     switch (NF) {
 #define CASE(N) case N: launch_lds_nf<N, 2>(p, L, nb, s); break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+      default: throw Error(1, "error in kslaplacian_r! This subroutine is for nf in 1-8.");
+    }
+  } else if (p.ca) {  // symmetric form: y is not read
+    switch (NF) {
+#define CASE(N) case N: launch_lds_nf<N, 5>(p, L, nb, s); break;
       CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
       default: throw Error(1, "error in kslaplacian_r! This subroutine is for nf in 1-8.");
@@ -1169,22 +1178,24 @@ void launch_stencil(const StencilCall &c, hipStream_t s) {
       q.in2 = c.in2; q.ca = c.ca; q.cb = c.cb; q.dotp2 = c.dot_partial2; q.hin = c.hin;
     }
     if (c.tag) {  // band arrays read and written: in (+ in2 + H in2) and out
-      const double arrays = c.ca ? 4.0 : 2.0;
+      const double arrays = c.ca ? (c.in2 ? 4.0 : 3.0) : 2.0;
       Timed t(c.tag, 8.0 * ((double)c.nb * arrays * g.n() + (c.veff ? g.n() : 0)));
       launch_lds(q, g, nf, c.nb, false, s);
     } else {
       launch_lds(q, g, nf, c.nb, false, s);
     }
     if (c.pl && c.coef && c.pl->nent > 0) {  // nonlocal part: separate owner-computes pass
-      const bool comb = c.ca != nullptr;
+      const bool comb = c.ca != nullptr, sym = comb && !c.in2;
       const long pld = c.dot_ld ? c.dot_ld : stencil_parts_lds(g, nf, c.nb, comb);
       const int off = stencil_parts_lds(g, nf, c.nb, comb);
-      if (c.rp && c.dot_partial && (!comb || c.rp2)) {
+      if (c.rp && c.dot_partial && (!comb || sym || c.rp2)) {
         launch_expand_rows(*c.pl, c.coef, c.nb, c.out, c.ld_out, nullptr, 0, nullptr, 0, 0, s,
                            nullptr, nullptr, nullptr, nullptr, c.expand_max);
         launch_proj_dots(*c.pl, c.coef, c.nb, c.rp, c.ld_rp, c.dot_partial, pld, off, s);
-        if (comb) launch_proj_dots(*c.pl, c.coef, c.nb, c.rp2, c.ld_rp, c.dot_partial2, pld, off, s);
+        // symmetric form: <ca x|H y> already holds the projector part (H y is complete)
+        if (comb && !sym) launch_proj_dots(*c.pl, c.coef, c.nb, c.rp2, c.ld_rp, c.dot_partial2, pld, off, s);
       } else {
+        if (sym) throw Error(1, "stencil: the symmetric combination form needs projector-space dots");
         launch_expand_rows(*c.pl, c.coef, c.nb, c.out, c.ld_out, c.in, c.ld_in, c.dot_partial, pld,
                            off, s, comb ? c.in2 : nullptr, c.ca, c.cb, c.dot_partial2, c.expand_max);
       }

@@ -158,7 +158,9 @@ struct StencilCall {
   // out = ca_b (H in) + cb_b hin with hin = H in2 (by linearity), so the stencil loads only
   // in, and coef holds the projector coefficients of ca_b in alone; dot_partial (with the
   // combination at the centre) and dot_partial2 (same layout, in2 at the centre) receive the
-  // partials of <ca in + cb in2|out> and <in2|out>
+  // partials of <ca in + cb in2|out> and <in2|out>.  With in2 = nullptr (symmetric form, for a
+  // symmetric H; needs rp): dot_partial receives <ca in|out> (rp = <p|ca in>) and
+  // dot_partial2 <ca in|hin>, and in2 is never read
   const double *ca = nullptr, *cb = nullptr;
   double *dot_partial2 = nullptr;
   const double *hin = nullptr;

@@ -108,6 +108,26 @@ __global__ void k_comb_coef(int nb, const double *al, const double *be, const do
   c2[b] = be[b] * fac[b];
 }
 
+// flag = 1 if some dij(:, :, s, a) is not exactly symmetric (count = nprjmx^2 nums natom)
+__global__ void k_dij_asym(const double *D, int np, long count, int *flag) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const long m = i / ((long)np * np);
+  const int r = (int)(i % ((long)np * np)), a = r % np, b = r / np;
+  const double *Dm = D + m * np * np;
+  if (Dm[a + (long)np * b] != Dm[b + (long)np * a]) *flag = 1;
+}
+
+// symmetric form of the combination dots (p = fac x + mdf y, H symmetric, all times dV):
+// sap = <fac x|H y> + mdf <y|H y>, then pap = <fac x|H p> + mdf sap
+__global__ void k_comb_sym(int nb, const double *mdf, const double *eh, double *pap, double *sap) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const double sy = sap[b] + mdf[b] * eh[b];
+  sap[b] = sy;
+  pap[b] += mdf[b] * sy;
+}
+
 // ksdiis8f.F90:285-600 per band, nitermax = 2 (one trial step), expressed on the basis
 // (phi0, Kr): returns coefficients (ca, cb) of the new band and its residual measure.
 // Inputs per band (already multiplied by dV): e = eps, hkk=<Kr|HKr>, h0k=<phi0|HKr>,
@@ -578,28 +598,45 @@ void Ctx::hpsi(int ns, const double *veffs, const double *psi, long ld, int nb, 
   }
 }
 
+bool Ctx::dij_symmetric() {
+  if (dij_sym < 0) {
+    const long count = (long)pl.nprjmx * pl.nprjmx * pl.nums * pl.natom;
+    dij_sym = 1;
+    if (dij && count > 0) {
+      clear_flag(3);
+      hipLaunchKernelGGL(k_dij_asym, dim3(nblk(count, 256)), dim3(256), 0, s, dij, pl.nprjmx,
+                         count, flag(3));
+      dij_sym = read_flag(3) ? 0 : 1;
+    }
+  }
+  return dij_sym == 1;
+}
+
 void Ctx::hpsi_comb(int ns, const double *veffs, const double *x, const double *y,
                     const double *hy, long ld, int nb, const double *ca, const double *cb,
                     const double *rpx, const double *rp, long ld_rp, double *out, double *pap,
-                    double *sap, const double *rpy) {
+                    double *sap, const double *rpy, const double *eh) {
   const long n1 = (long)pl.nprjmx * pl.num_ppcell;
   double *coef = buf("coef", (size_t)std::max<long>(1, n1) * nb);
   // the projector term of ca x alone: cb H y already holds cb's
   launch_coef(pl, rpx, ld_rp, nb, dij, false, ns, 1.0 / g.dv(), coef, n1, s, true);
   const bool pdots = rpy && param("nl_proj_dots", 1.0) >= 1.0;
+  // H symmetric (dij symmetric): <y|H p> = <H y|ca x> + cb <y|H y>, so y is not read
+  const bool sym = pdots && eh && param("comb_sym", 1.0) >= 1.0 && dij_symmetric();
   const int np = stencil_nparts(g, g.nf, nb, true) + (pdots ? 1 : expand_nparts(pl));
   double *part = buf("hpsi.part", (size_t)2 * np * nb);
   launch_fill((long)2 * np * nb, 0.0, part, s);
   StencilCall c;
-  c.g = &g; c.in = x; c.in2 = y; c.ca = ca; c.cb = cb; c.ld_in = ld; c.out = out; c.ld_out = ld;
+  c.g = &g; c.in = x; c.in2 = sym ? nullptr : y; c.ca = ca; c.cb = cb; c.ld_in = ld; c.out = out; c.ld_out = ld;
   c.veff = veffs; c.nb = nb; c.pl = &pl; c.coef = coef; c.ld_coef = n1; c.coef_band_fastest = true;
   c.dot_partial = part; c.dot_partial2 = part + (long)np * nb; c.dot_ld = np;
-  if (pdots) { c.rp = rp; c.rp2 = rpy; c.ld_rp = ld_rp; }
+  if (pdots) { c.rp = sym ? rpx : rp; c.rp2 = sym ? nullptr : rpy; c.ld_rp = ld_rp; }
   c.hin = hy;
   c.tag = "hp_comb_stencil";
   launch_stencil(c, s);
   launch_reduce_partials(part, np, nb, g.dv(), pap, s);
   launch_reduce_partials(part + (long)np * nb, np, nb, g.dv(), sap, s);
+  if (sym) hipLaunchKernelGGL(k_comb_sym, dim3(nblk(nb, 64)), dim3(64), 0, s, nb, cb, eh, pap, sap);
 }
 
 void Ctx::spsi(const double *psi, long ld, int nb, const double *rp, long ld_rp, double *out,
@@ -1154,7 +1191,7 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
         launch_axpby(n1, M, nullptr, 1.0, rpx, n1, mdf, 1.0, a.rsp, n1, rpp, n1, c.s);
       }
       c.hpsi_comb(a.ns, a.veffs, PA, a.psi, a.hpsi, n, M, fac, mdf, rpx, rpp, n1, HP, pap, sap,
-                  a.rsp);
+                  a.rsp, a.sval);
     } else {
       launch_axpby(n, M, fac, 1.0, PA, n, mdf, 1.0, a.psi, n, KR, n, c.s);  // KR <- p
       c.proj(KR, n, M, rpp, n1);

@@ -15,6 +15,8 @@ struct Ctx {
   hipStream_t s;
   const double *dij = nullptr, *sss = nullptr;
   bool q_zero = true;  // S = 1 (all sss entries zero): S psi == psi
+  int dij_sym = -1;    // dij exactly symmetric: 1 / 0, -1 = not yet checked
+  bool dij_symmetric();
   double *buf(const char *tag, size_t count);
   int *flag(int i = 0);
   void clear_flag(int i = 0);
@@ -33,11 +35,12 @@ struct Ctx {
   // H p for p = ca_b x + cb_b y, p never stored (tiled path): out = H p = ca_b H x + hy with
   // hy = H y, pap = dV <p|H p>, sap = dV <y|H p>.  rpx = ca_b <proj|x> (the projector
   // coefficients of ca_b x), rp = <proj|p>; with rpy = <proj|y> the projector parts of pap and
-  // sap come from projector space.
+  // sap come from projector space.  With eh = dV <y|H y> as well and dij symmetric, sap is
+  // formed as dV <H y|ca x> + cb eh (H symmetric) and y is not read.
   void hpsi_comb(int ns, const double *veffs, const double *x, const double *y, const double *hy,
                  long ld, int nb, const double *ca, const double *cb, const double *rpx,
                  const double *rp, long ld_rp, double *out, double *pap, double *sap,
-                 const double *rpy = nullptr);
+                 const double *rpy = nullptr, const double *eh = nullptr);
   void spsi(const double *psi, long ld, int nb, const double *rp, long ld_rp, double *out,
             long ld_out);
   void dots(const double *x, long ldx, const double *y, long ldy, int nb, double *res);

@@ -297,6 +297,39 @@ def test_projector_space_partials_match_grid_partials(kk, orc, comb):
     np.testing.assert_allclose(a["svecre"], b["svecre"], rtol=0, atol=1e-9 * np.abs(b["svecre"]).max())
 
 
+@pytest.mark.parametrize("asym", [0, 1])
+def test_symmetric_combination_dots(kk, orc, asym):
+    """With dij symmetric, H is symmetric and the block CG forms dV <psi|H p> as
+    dV <H psi|fac P> + mdf dV <psi|H psi>, so the H p stencil does not read psi; the steps agree
+    with the direct sums to rounding.  An asymmetric dij keeps the direct sums (bitwise equal)."""
+    s = synth.make_system(16, 0.4, 3, 12, seed=85)
+    if asym:
+        s.dij = np.asfortranarray(s.dij.copy())
+        s.dij[0, 1] += 0.05
+    base = _state(s, orc)
+    kw = synth.abi_args(s, nsdmax=4, epssd=1e-12)
+    out = []
+    kk.set_param("kscg_mode", 1)
+    try:
+        for sym in (1, 0):
+            kk.set_param("comb_sym", sym)
+            st = {k: (v.copy(order="F") if isinstance(v, np.ndarray) else v) for k, v in base.items()}
+            for it in range(1, 4):
+                kw["iscf"] = it
+                kk.call("scf_diag", **kw, **st)
+            out.append(st)
+    finally:
+        kk.set_param("kscg_mode", 0)
+        kk.set_param("comb_sym", 1)
+    a, b = out
+    if asym:
+        np.testing.assert_array_equal(a["sval"], b["sval"])
+        np.testing.assert_array_equal(a["svecre"], b["svecre"])
+    else:
+        np.testing.assert_allclose(a["sval"], b["sval"], rtol=0, atol=1e-11)
+        np.testing.assert_allclose(a["svecre"], b["svecre"], rtol=0, atol=1e-9 * np.abs(b["svecre"]).max())
+
+
 def test_charge_force_energy(kk, orc):
     s, rng = _nl_system(sym=True)
     s.sss[:] = 0.02

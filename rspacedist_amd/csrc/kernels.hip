@@ -161,6 +161,11 @@ __global__ __launch_bounds__(256) void k_stencil(SP p) {
 //     combination is never stored and the stencil loads only x; it also sums <y|out>.
 //     MODE 5 is MODE 2 without y: for a symmetric H, <y|H p> = ca_b <H y|x> + cb_b <y|H y>,
 //     so the block sums <ca_b x|out> and <ca_b x|H y> from x and H y alone and y is not read.
+//   * MODE 6 (radius 1) is the block CG's band update fused with the next residual
+//     preconditioner: every value it loads is formed as xn = ca_b x + cb_b y (the new band),
+//     un = ha_b h + hb_b w (its H) and r = sign (un - eps_b xn); out = K r, and the centre's
+//     xn, un go to out_x, out_h (other buffers than x, h: neighbouring blocks still read those),
+//     with partials <r|K r> (dotp), <xn|xn> (dotp2), <xn|un> (dotp3) and max |K r| (omax).
 struct SL {
   int nx, ny, nz, px, py, pz, TY, ntile, nzch, zc, nbands;
   const double *in, *in2;
@@ -188,6 +193,8 @@ struct SL {
   const double *hin;         // MODE 2: H in2 (same band stride as in)
   unsigned long long *cmax;  // optional: per band max |c| (centre input value) as FP64 bits
   double *dotp3;             // optional: partials of <c|c>, as dotp
+  const double *hin2, *ha, *hb;  // MODE 6: w, and the coefficients of un
+  double *out_x, *out_h;         // MODE 6: xn and un at the centre
 };
 
 // Block = TY full x-rows of one band, marching over a z-range.  The 2 NF + 1 values of each
@@ -200,7 +207,8 @@ struct SL {
 template <int NF, int P, int MODE>
 __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, double *lds,
                                      double *red) {
-  constexpr bool RES = MODE == 1, COMB = MODE == 2 || MODE == 5, SYM = MODE == 5;
+  constexpr bool RES = MODE == 1, COMB = MODE == 2 || MODE == 5, SYM = MODE == 5, UPD = MODE == 6;
+  static_assert(!UPD || NF == 1, "the fused update is radius 1");
   // MODE 3: plain, also the partials of <c|c> (no max); MODE 4: plain, also the maxima of |c|
   // and |out| (no partials) - each keeps the plain kernel's accumulator count
   constexpr bool NORMS = MODE == 3, MAXES = MODE == 4;
@@ -210,10 +218,12 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
   const int per = p.ntile * p.nzch;
   const int y0 = tile * TY, z0 = zch * p.zc, z1 = min(nz, z0 + p.zc);
   const double *in = p.in + (long)b * p.ld_in;
-  const double *in2 = (RES || (COMB && !SYM)) ? p.in2 + (long)b * p.ld_in : nullptr;
-  const double *hin = COMB ? p.hin + (long)b * p.ld_in : nullptr;
-  const double e = RES ? p.eps[b] : 0.0;
-  const double fa = COMB ? p.ca[b] : 0.0, fb = COMB ? p.cb[b] : 0.0;
+  const double *in2 = (RES || UPD || (COMB && !SYM)) ? p.in2 + (long)b * p.ld_in : nullptr;
+  const double *hin = (COMB || UPD) ? p.hin + (long)b * p.ld_in : nullptr;
+  const double *hin2 = UPD ? p.hin2 + (long)b * p.ld_in : nullptr;
+  const double e = (RES || UPD) ? p.eps[b] : 0.0;
+  const double fa = (COMB || UPD) ? p.ca[b] : 0.0, fb = (COMB || UPD) ? p.cb[b] : 0.0;
+  const double ga = UPD ? p.ha[b] : 0.0, gb = UPD ? p.hb[b] : 0.0;
   double *out = p.out + (long)b * p.ld_out;
   const int t = threadIdx.x;
   // y of a row index relative to the grid, wrapped once when periodic; -1 = zero row
@@ -233,7 +243,17 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
   };
   auto ld = [&](long off) -> double {
     if (RES) return p.sign * (in[off] - e * in2[off]);
+    if (UPD) {
+      const double xn = fa * in[off] + fb * in2[off], un = ga * hin[off] + gb * hin2[off];
+      return p.sign * (un - e * xn);
+    }
     return in[off];
+  };
+  // MODE 6: the same, also returning xn and un
+  auto ld3 = [&](long off, double &xn, double &un) -> double {
+    xn = fa * in[off] + fb * in2[off];
+    un = ga * hin[off] + gb * hin2[off];
+    return p.sign * (un - e * xn);
   };
   // tile points
   int oxy[P], lcol[P];
@@ -280,11 +300,22 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
     }
   };
   double col[P][2 * NF + 1];
+  // MODE 6: xn, un of the next centre plane (a delay line one plane deep) and of the plane
+  // being prefetched
+  constexpr int UP = UPD ? P : 1;
+  double xd[UP], ud[UP], xnx[UP], unx[UP];
 #pragma unroll
   for (int m = 0; m < 2 * NF; m++) {
     const long zo = zoff(z0 - NF + m);
 #pragma unroll
-    for (int q = 0; q < P; q++) col[q][m + 1] = (lv[q] && zo >= 0) ? ld(zo + oxy[q]) : 0.0;
+    for (int q = 0; q < P; q++) {
+      if (UPD && m == NF) {  // the first centre plane z0 (in range: zo >= 0)
+        col[q][m + 1] = lv[q] ? ld3(zo + oxy[q], xd[q], ud[q]) : 0.0;
+        if (!lv[q]) xd[q] = ud[q] = 0.0;
+      } else {
+        col[q][m + 1] = (lv[q] && zo >= 0) ? ld(zo + oxy[q]) : 0.0;
+      }
+    }
   }
   // MODE 2: the second band and its H at the centre, prefetched like V_eff
   constexpr int YP = COMB ? P : 1;
@@ -294,7 +325,12 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
     const long zn = zoff(z + NF), zh = (long)z * plane;
 #pragma unroll
     for (int q = 0; q < P; q++) {
-      nxt[q] = (lv[q] && zn >= 0) ? ld(zn + oxy[q]) : 0.0;
+      if (UPD) {
+        nxt[q] = (lv[q] && zn >= 0) ? ld3(zn + oxy[q], xnx[q], unx[q]) : 0.0;
+        if (!(lv[q] && zn >= 0)) xnx[q] = unx[q] = 0.0;
+      } else {
+        nxt[q] = (lv[q] && zn >= 0) ? ld(zn + oxy[q]) : 0.0;
+      }
       if (COMB && !SYM) yf[q] = sv[q] ? in2[zh + oxy[q]] : 0.0;
       if (COMB) hf[q] = sv[q] ? hin[zh + oxy[q]] : 0.0;
       vf[q] = (p.veff && sv[q]) ? p.veff[zh + oxy[q]] : 0.0;
@@ -305,12 +341,18 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
   prefetch(z0);
   double dsum = 0.0, dsum2 = 0.0, dsum3 = 0.0, amax = 0.0, cmx = 0.0;
   for (int z = z0; z < z1; z++) {
-    double vcur[P], ycur[YP], hcur[YP];
+    double vcur[P], ycur[YP], hcur[YP], xcen[UP], ucen[UP];
 #pragma unroll
     for (int q = 0; q < P; q++) {
 #pragma unroll
       for (int m = 0; m < 2 * NF; m++) col[q][m] = col[q][m + 1];
       col[q][2 * NF] = nxt[q];
+      if (UPD) {  // radius 1: the plane just prefetched is the next centre
+        xcen[q] = xd[q];
+        ucen[q] = ud[q];
+        xd[q] = xnx[q];
+        ud[q] = unx[q];
+      }
       if (COMB && !SYM) ycur[q] = yf[q];
       if (COMB) hcur[q] = hf[q];
       vcur[q] = vf[q];
@@ -339,9 +381,17 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
       if (COMB) dsum2 += SYM ? fa * c * hcur[q] : ycur[q] * s;
       if (MAXES) cmx = fmax(cmx, fabs(c));
       if (NORMS) dsum3 += c * c;
+      if (UPD) {
+        dsum2 += xcen[q] * xcen[q];
+        dsum3 += xcen[q] * ucen[q];
+      }
       if (!NORMS && p.omax) amax = fmax(amax, fabs(s));
       const long r = zh + oxy[q];
       out[r] = p.accumulate ? out[r] + s : s;
+      if (UPD) {
+        p.out_x[(long)b * p.ld_out + r] = xcen[q];
+        p.out_h[(long)b * p.ld_out + r] = ucen[q];
+      }
     }
     __syncthreads();
   }
@@ -349,11 +399,11 @@ __device__ inline void stencil_block(const SL &p, int b, int tile, int zch, doub
     const double tot = block_sum<512>(dsum, red);
     if (t == 0) p.dotp[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
   }
-  if (COMB) {
+  if (COMB || UPD) {
     const double tot = block_sum<512>(dsum2, red);
     if (t == 0) p.dotp2[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
   }
-  if (NORMS) {
+  if (NORMS || UPD) {
     const double tot = block_sum<512>(dsum3, red);
     if (t == 0) p.dotp3[(long)b * (p.dot_ld ? p.dot_ld : per) + tile + (long)p.ntile * zch] = tot;
   }
@@ -892,21 +942,23 @@ __global__ __launch_bounds__(256) void k_pair_update_dots(long n, const double *
                                                           const double *w, long ld,
                                                           double *part1, double *part2,
                                                           int nparts, const double *a2,
-                                                          const double *c2) {
+                                                          const double *c2, double *xo,
+                                                          double *uo) {
   __shared__ double red[4];
   const int b = blockIdx.y;
   const long chunk = (n + nparts - 1) / nparts;
   const long i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
   const double ab = a[b], cb = c[b];
   const double ab2 = a2 ? a2[b] : ab, cb2 = c2 ? c2[b] : cb;
-  double *xb = x + (long)b * ld, *ub = u + (long)b * ld;
+  const double *xb = x + (long)b * ld, *ub = u + (long)b * ld;
   const double *yb = y + (long)b * ld, *wb = w + (long)b * ld;
+  double *xob = xo + (long)b * ld, *uob = uo + (long)b * ld;
   double s1 = 0.0, s2 = 0.0;
   for (long i = i0 + threadIdx.x; i < i1; i += 256) {
     const double xv = ab * xb[i] + cb * yb[i];
     const double uv = ab2 * ub[i] + cb2 * wb[i];
-    xb[i] = xv;
-    ub[i] = uv;
+    xob[i] = xv;
+    uob[i] = uv;
     s1 += xv * xv;
     s2 += xv * uv;
   }
@@ -1363,6 +1415,35 @@ void launch_precond_resid_dot(const Grid &g, const double *h, const double *sp, 
   launch_lds(q, g, 1, nb, true, s);
 }
 
+void launch_update_precond(const Grid &g, const double *x, const double *y, const double *h,
+                           const double *w, long ld, const double *ca, const double *cb,
+                           const double *ha, const double *hb, const double *eps, double sign,
+                           double *x_out, double *h_out, double *out, int nb, double *part_r,
+                           double *part_xx, double *part_xh, unsigned long long *out_max,
+                           hipStream_t s) {
+  if (!lds_ok(g) || param("stencil_lds", 1.0) < 1.0)
+    throw Error(1, "fused band update: tiled stencil path only");
+  if (x_out == x || h_out == h || out == x || out == h)
+    throw Error(1, "fused band update: outputs must not alias the bands it reads");
+  Timed t("update_precond", 8.0 * (double)g.n() * nb * 7.0);
+  SL q{};
+  q.nx = g.nx; q.ny = g.ny; q.nz = g.nz;
+  q.px = g.nperi >= 1; q.py = g.nperi >= 2; q.pz = g.nperi >= 3;
+  q.in = x; q.in2 = y; q.hin = h; q.hin2 = w; q.ld_in = ld;
+  q.ca = ca; q.cb = cb; q.ha = ha; q.hb = hb; q.eps = eps; q.sign = sign;
+  q.out = out; q.ld_out = ld; q.out_x = x_out; q.out_h = h_out;
+  q.c0 = 0.5;
+  q.cx[1] = q.cy[1] = q.cz[1] = 1.0 / 12.0;
+  q.dotp = part_r; q.dotp2 = part_xx; q.dotp3 = part_xh;
+  q.omax = out_max;
+  if (out_max) launch_fill(nb, 0.0, (double *)out_max, s);  // all-zero bits
+  // points per thread: param upd_p (1, 2 or 4; 0 = the stencils' default)
+  LdsPlan L = lds_plan(g, 1, nb, (int)param("upd_p", 0.0));
+  q.TY = L.TY; q.ntile = L.ntile; q.nzch = L.nzch; q.zc = L.zc; q.nbands = nb;
+  launch_lds_nf<1, 6>(q, L, nb, s);
+  HIPCK(hipGetLastError());
+}
+
 void launch_precond_resid(const Grid &g, const double *h, const double *sp, long ld,
                           const double *eps, double sign, double *r_out, double *out, int nb,
                           hipStream_t s) {
@@ -1518,13 +1599,15 @@ void launch_pupdate_dots(long n, int nb, const double *x, const double *t, doubl
 void launch_pair_update_dots(long n, int nb, const double *a, const double *c, double *x,
                              const double *y, double *u, const double *w, long ld, double scale,
                              double *res1, double *res2, hipStream_t s, const double *a2,
-                             const double *c2) {
+                             const double *c2, double *x_out, double *u_out) {
   if (n * nb == 0) return;
+  if (!x_out) x_out = x;
+  if (!u_out) u_out = u;
   const int np = dot_nparts(n);
   Timed tt("band_update_dots", 8.0 * (double)n * nb * 6.0);
   double *part = (double *)scratch("pair.part", sizeof(double) * (size_t)2 * np * nb);
   hipLaunchKernelGGL(k_pair_update_dots, dim3(np, nb), dim3(256), 0, s, n, a, c, x, y, u, w, ld,
-                     part, part + (long)np * nb, np, a2, c2);
+                     part, part + (long)np * nb, np, a2, c2, x_out, u_out);
   hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part, np, scale, res1);
   hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part + (long)np * nb, np, scale,
                      res2);

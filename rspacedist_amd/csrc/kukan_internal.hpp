@@ -205,6 +205,16 @@ void launch_precond_resid_dot(const Grid &g, const double *h, const double *sp, 
                               const double *eps, double sign, double *out, int nb,
                               double *dot_partial, hipStream_t s,
                               unsigned long long *out_max = nullptr);
+// band update fused with the next residual preconditioner (tiled path, S = 1): per band
+// xn = ca x + cb y, un = ha h + hb w into x_out, h_out (not x, h), out = K r with
+// r = sign (un - eps xn); partials (stencil_nparts(g, 1, nb) per band) of <r|K r>, <xn|xn>,
+// <xn|un>, and max |K r| as FP64 bits
+void launch_update_precond(const Grid &g, const double *x, const double *y, const double *h,
+                           const double *w, long ld, const double *ca, const double *cb,
+                           const double *ha, const double *hb, const double *eps, double sign,
+                           double *x_out, double *h_out, double *out, int nb, double *part_r,
+                           double *part_xx, double *part_xh, unsigned long long *out_max,
+                           hipStream_t s);
 
 void launch_halo_fill(double *v, int nperi, int na, int nb, int nc, int nf, int nf1, int nsnd,
                       hipStream_t s);
@@ -263,11 +273,12 @@ void launch_pupdate_dots(long n, int nb, const double *x, const double *t, doubl
                          const double *sv, double scale, double *res1, double *res2,
                          hipStream_t s);
 // x = a_b x + c_b y, u = a2_b u + c2_b w (a2, c2 null: a, c); res1[b] = scale <x_b|x_b>,
-// res2[b] = scale <x_b|u_b>
+// res2[b] = scale <x_b|u_b>.  x_out / u_out (null: x / u) receive the new x and u.
 void launch_pair_update_dots(long n, int nb, const double *a, const double *c, double *x,
                              const double *y, double *u, const double *w, long ld, double scale,
                              double *res1, double *res2, hipStream_t s,
-                             const double *a2 = nullptr, const double *c2 = nullptr);
+                             const double *a2 = nullptr, const double *c2 = nullptr,
+                             double *x_out = nullptr, double *u_out = nullptr);
 void launch_copy2d(long n, int nb, const double *x, long ldx, double *y, long ldy, hipStream_t s);
 void launch_fill(long n, double v, double *x, hipStream_t s);
 void launch_fill_normal(double *x, long n, unsigned long long seed, hipStream_t s);

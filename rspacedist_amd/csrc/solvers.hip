@@ -128,6 +128,16 @@ __global__ void k_comb_sym(int nb, const double *mdf, const double *eh, double *
   pap[b] += mdf[b] * sy;
 }
 
+// eps of psi_new = al psi + be p for a symmetric H (all times dV):
+// al^2 <psi|H psi> + 2 al be <psi|H p> + be^2 <p|H p>
+__global__ void k_eps_next(int nb, const double *al, const double *be, const double *eh,
+                           const double *sap, const double *pap, double *eps) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const double x = al[b], y = be[b];
+  eps[b] = x * x * eh[b] + 2.0 * x * y * sap[b] + y * y * pap[b];
+}
+
 // ksdiis8f.F90:285-600 per band, nitermax = 2 (one trial step), expressed on the basis
 // (phi0, Kr): returns coefficients (ca, cb) of the new band and its residual measure.
 // Inputs per band (already multiplied by dV): e = eps, hkk=<Kr|HKr>, h0k=<phi0|HKr>,
@@ -1114,6 +1124,18 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
     gg0 = c.buf("bcg.gg0", (size_t)M);
   }
   c.hpsi(a.ns, a.veffs, a.psi, n, M, a.rsp, n1, a.hpsi, n, false, eh0, false, nullptr, nullptr, gg0);
+  // With H symmetric the band update of one iteration and the residual preconditioner of the
+  // next run as one stencil pass (launch_update_precond), with the next eps formed as
+  // al^2 eps + 2 al be <psi|H p> + be^2 <p|H p>.  That pass writes the new bands to the other
+  // buffer of a pair, so psi and H psi alternate between the caller's arrays and bcg.psi2 /
+  // bcg.h2; the last band update writes back into the caller's.
+  const bool fuse_upd = comb && a.spsi == a.psi && a.nsdmax > 1 &&
+                        param("bcg_fuse_update", 1.0) >= 1.0 && c.dij_symmetric();
+  double *psi = a.psi, *hpsi = a.hpsi, *spsi = a.spsi;
+  double *psi2 = fuse_upd ? c.buf("bcg.psi2", (size_t)n * M) : nullptr;
+  double *h2 = fuse_upd ? c.buf("bcg.h2", (size_t)n * M) : nullptr;
+  double *rnext = c.buf("bcg.rnext", (size_t)M), *epsn = c.buf("bcg.epsn", (size_t)M);
+  bool kr_ready = false;  // K r of the current bands (and rnext, krmax) came with the update
   int iiter = 0;
   bool all_conv = false;
   while (iiter < a.nsdmax && !all_conv) {
@@ -1126,7 +1148,7 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
       HIPCK(hipMemcpyAsync(a.sval, iiter > 1 ? eh_next : eh0, sizeof(double) * M,
                            hipMemcpyDeviceToDevice, c.s));
     } else {
-      c.dots(a.psi, n, a.hpsi, n, M, a.sval);
+      c.dots(psi, n, hpsi, n, M, a.sval);
     }
     if (fused && iiter == 1)  // later: from the band update
       HIPCK(hipMemcpyAsync(gg, gg0, sizeof(double) * M, hipMemcpyDeviceToDevice, c.s));
@@ -1134,10 +1156,12 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
     // for split-precision products also the per-band max |K r|
     const bool mixed = mixed_for(n);
     unsigned long long *krmax = mixed ? (unsigned long long *)c.buf("bcg.krmax", M) : nullptr;
-    {
+    if (kr_ready) {
+      HIPCK(hipMemcpyAsync(rnew, rnext, sizeof(double) * M, hipMemcpyDeviceToDevice, c.s));
+    } else {
       const int np = stencil_nparts(c.g, 1, M);
       double *part = c.buf("bcg.kpart", (size_t)np * M);
-      launch_precond_resid_dot(c.g, a.hpsi, a.spsi, n, a.sval, -1.0, KR, M, part, c.s, krmax);
+      launch_precond_resid_dot(c.g, hpsi, spsi, n, a.sval, -1.0, KR, M, part, c.s, krmax);
       launch_reduce_partials(part, np, M, c.g.dv(), rnew, c.s);
     }
     // Kr orthogonal to the block in the S metric: C = dV (S Psi)^T Kr ; Kr -= Psi C.  Split
@@ -1151,21 +1175,21 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
       maxscale(krmax, M, sk, c.s);
       s_kr = sk;
       // S = 1: S psi equals psi, so one split of psi serves both products
-      if (c.q_zero) pre = split_psi(c, "bcg.psi", a.psi, M, fused ? gg : nullptr, spl);
+      if (c.q_zero) pre = split_psi(c, "bcg.psi", psi, M, fused ? gg : nullptr, spl);
     }
     if (pre) {
-      sub_tn(c, M, M, n, dv, a.psi, n, KR, n, zero, C, M, nullptr, s_kr, pre);
-      sub_nn(c, n, M, M, mone, a.psi, n, C, M, one, KR, n, nullptr, pre);
+      sub_tn(c, M, M, n, dv, psi, n, KR, n, zero, C, M, nullptr, s_kr, pre);
+      sub_nn(c, n, M, M, mone, psi, n, C, M, one, KR, n, nullptr, pre);
     } else {
-      const double *s_psi = mixed_scale("bcg.psi", a.psi, n, n, M);
-      sub_tn(c, M, M, n, dv, a.spsi, n, KR, n, zero, C, M, a.spsi == a.psi ? s_psi : nullptr, s_kr);
-      sub_nn(c, n, M, M, mone, a.psi, n, C, M, one, KR, n, s_psi);
+      const double *s_psi = mixed_scale("bcg.psi", psi, n, n, M);
+      sub_tn(c, M, M, n, dv, spsi, n, KR, n, zero, C, M, spsi == psi ? s_psi : nullptr, s_kr);
+      sub_nn(c, n, M, M, mone, psi, n, C, M, one, KR, n, s_psi);
     }
     // P = Kr + beta P
     hipLaunchKernelGGL(k_ratio, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, rnew, rold, doreset ? 1 : 0, tb);
     if (fused) {
       // P = Kr + beta P together with d = dV <S psi|P> and dV <P|P> (S = 1) in one pass
-      launch_pupdate_dots(n, M, KR, doreset ? nullptr : tb, PA, n, a.spsi, dv, dd, pp, c.s);
+      launch_pupdate_dots(n, M, KR, doreset ? nullptr : tb, PA, n, spsi, dv, dd, pp, c.s);
     } else {
       if (doreset) launch_copy2d(n, M, KR, n, PA, n, c.s);
       else launch_axpby(n, M, nullptr, 1.0, KR, n, tb, 1.0, PA, n, PA, n, c.s);
@@ -1176,9 +1200,9 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
         c.spsi(PA, n, M, rpp, n1, SK, n);
         SPA = SK;
       }
-      c.dots(a.spsi, n, PA, n, M, dd);
+      c.dots(spsi, n, PA, n, M, dd);
       c.dots(PA, n, SPA, n, M, pp);
-      c.dots(a.psi, n, a.spsi, n, M, gg);
+      c.dots(psi, n, spsi, n, M, gg);
     }
     hipLaunchKernelGGL(k_pnorm, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, dd, pp, gg, fac, mdf, c.flag(1));
     if (comb) {
@@ -1190,13 +1214,13 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
         launch_axpby(n1, M, fac, 1.0, rpp, n1, nullptr, 0.0, rpp, n1, rpx, n1, c.s);
         launch_axpby(n1, M, nullptr, 1.0, rpx, n1, mdf, 1.0, a.rsp, n1, rpp, n1, c.s);
       }
-      c.hpsi_comb(a.ns, a.veffs, PA, a.psi, a.hpsi, n, M, fac, mdf, rpx, rpp, n1, HP, pap, sap,
+      c.hpsi_comb(a.ns, a.veffs, PA, psi, hpsi, n, M, fac, mdf, rpx, rpp, n1, HP, pap, sap,
                   a.rsp, a.sval);
     } else {
-      launch_axpby(n, M, fac, 1.0, PA, n, mdf, 1.0, a.psi, n, KR, n, c.s);  // KR <- p
+      launch_axpby(n, M, fac, 1.0, PA, n, mdf, 1.0, psi, n, KR, n, c.s);  // KR <- p
       c.proj(KR, n, M, rpp, n1);
       c.hpsi(a.ns, a.veffs, KR, n, M, rpp, n1, HP, n, false, pap, true);
-      c.dots(a.psi, n, HP, n, M, sap);
+      c.dots(psi, n, HP, n, M, sap);
     }
     hipLaunchKernelGGL(k_kscg05, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, a.sval, pap, sap, active, al, be);
     if (comb) {
@@ -1204,13 +1228,37 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
       // with dV <psi|psi> and dV <psi|H psi> of the new bands
       double *a2 = c.buf("bcg.a2", (size_t)2 * M), *c2 = a2 + M;
       hipLaunchKernelGGL(k_comb_coef, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, al, be, fac, mdf, a2, c2);
-      launch_pair_update_dots(n, M, a2, c2, a.psi, PA, a.hpsi, HP, n, dv, gg, eh_next, c.s, al, be);
+      if (fuse_upd && iiter < a.nsdmax) {
+        // this update and the next iteration's K r in one pass, into the other buffers
+        hipLaunchKernelGGL(k_eps_next, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, al, be, a.sval, sap,
+                           pap, epsn);
+        double *po = psi == a.psi ? psi2 : a.psi, *ho = hpsi == a.hpsi ? h2 : a.hpsi;
+        const int np = stencil_nparts(c.g, 1, M);
+        double *part = c.buf("bcg.upart", (size_t)3 * np * M);
+        launch_update_precond(c.g, psi, PA, hpsi, HP, n, a2, c2, al, be, epsn, -1.0, po, ho, KR, M,
+                              part, part + (long)np * M, part + 2L * np * M, krmax, c.s);
+        launch_reduce_partials(part, np, M, dv, rnext, c.s);
+        launch_reduce_partials(part + (long)np * M, np, M, dv, gg, c.s);
+        launch_reduce_partials(part + 2L * np * M, np, M, dv, eh_next, c.s);
+        psi = po;
+        hpsi = ho;
+        spsi = psi;
+        kr_ready = true;
+      } else {
+        // in place, or back into the caller's arrays after fused updates
+        launch_pair_update_dots(n, M, a2, c2, psi, PA, hpsi, HP, n, dv, gg, eh_next, c.s, al, be,
+                                a.psi, a.hpsi);
+        psi = a.psi;
+        hpsi = a.hpsi;
+        spsi = a.spsi;
+        kr_ready = false;
+      }
     } else if (fused) {
       // psi, H psi updated together, with dV <psi|psi> and dV <psi|H psi> of the new bands
-      launch_pair_update_dots(n, M, al, be, a.psi, KR, a.hpsi, HP, n, dv, gg, eh_next, c.s);
+      launch_pair_update_dots(n, M, al, be, psi, KR, hpsi, HP, n, dv, gg, eh_next, c.s);
     } else {
-      launch_axpby(n, M, al, 1.0, a.psi, n, be, 1.0, KR, n, a.psi, n, c.s);
-      launch_axpby(n, M, al, 1.0, a.hpsi, n, be, 1.0, HP, n, a.hpsi, n, c.s);
+      launch_axpby(n, M, al, 1.0, psi, n, be, 1.0, KR, n, psi, n, c.s);
+      launch_axpby(n, M, al, 1.0, hpsi, n, be, 1.0, HP, n, hpsi, n, c.s);
     }
     if (a.spsi != a.psi) {
       if (c.q_zero) {
@@ -1232,6 +1280,10 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
       *a.ksconv = *a.ksconv && (cv || iiter < a.nsdmax);
     }
     HIPCK(hipMemcpyAsync(active, act_h.data(), sizeof(int) * M, hipMemcpyHostToDevice, c.s));
+  }
+  if (psi != a.psi) {  // ended after a fused update: the bands back into the caller's arrays
+    HIPCK(hipMemcpyAsync(a.psi, psi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
+    HIPCK(hipMemcpyAsync(a.hpsi, hpsi, sizeof(double) * n * M, hipMemcpyDeviceToDevice, c.s));
   }
   *a.ksitmax = std::max(*a.ksitmax, iiter);
   if (c.read_flag(1)) throw Error(2, "kscg_r: pvec has negative norm");

@@ -330,6 +330,35 @@ def test_symmetric_combination_dots(kk, orc, asym):
         np.testing.assert_allclose(a["svecre"], b["svecre"], rtol=0, atol=1e-9 * np.abs(b["svecre"]).max())
 
 
+@pytest.mark.parametrize("epssd", [1e-12, 3.0])
+def test_fused_band_update_and_preconditioner(kk, orc, epssd):
+    """With H symmetric the block CG's band update and the next residual preconditioner run as
+    one stencil pass, the new bands alternating between two buffers; the steps agree with the
+    separate passes to rounding.  epssd = 3 makes every band converge early, so the loop also
+    ends after a fused update (the bands are copied back into the caller's arrays)."""
+    s = synth.make_system(16, 0.4, 3, 12, seed=87)
+    base = _state(s, orc)
+    kw = synth.abi_args(s, nsdmax=4, epssd=epssd)
+    out = []
+    kk.set_param("kscg_mode", 1)
+    try:
+        for fu in (1, 0):
+            kk.set_param("bcg_fuse_update", fu)
+            st = {k: (v.copy(order="F") if isinstance(v, np.ndarray) else v) for k, v in base.items()}
+            for it in range(1, 4):
+                kw["iscf"] = it
+                kk.call("scf_diag", **kw, **st)
+            out.append(st)
+    finally:
+        kk.set_param("kscg_mode", 0)
+        kk.set_param("bcg_fuse_update", 1)
+    a, b = out
+    np.testing.assert_allclose(a["sval"], b["sval"], rtol=0, atol=1e-11)
+    for k, v in b.items():
+        if isinstance(v, np.ndarray) and v.dtype == np.float64 and v.size > 1:
+            np.testing.assert_allclose(a[k], v, rtol=0, atol=1e-9 * max(1.0, np.abs(v).max()), err_msg=k)
+
+
 def test_charge_force_energy(kk, orc):
     s, rng = _nl_system(sym=True)
     s.sss[:] = 0.02

@@ -123,8 +123,8 @@ def forces(s, dev, st, kk, synth):
 
 # timing tag -> kernel whose calibrated HBM bytes per launch the committed counter profile holds
 TRAFFIC_KERNEL = {"gemm_mixed_nn": "k_gemm_nn<3>", "gemm_mixed_tn": "k_gemm_tnf<3>",
-                  "hpsi_stencil": "k_stencil_lds<4, 2, 3>", "hp_comb_stencil": "k_stencil_lds<4, 1, 2>",
-                  "precond_resid": "k_stencil_lds<1, 2, 1>",
+                  "hpsi_stencil": "k_stencil_lds<4, 2, 3>", "hp_comb_stencil": "k_stencil_lds<4, 1, 5>",
+                  "precond_resid": "k_stencil_lds<1, 2, 1>", "update_precond": "k_stencil_lds<1, 1, 6>",
                   "band_update_dots": "k_pair_update_dots", "p_update_dots": "k_pupdate_dots",
                   "dots": "k_dot_part", "gemm_scale": "k_colmax", "nl_expand": "k_expand_rows<16>",
                   "nl_gather": "k_gather<9, 4>"}
@@ -274,7 +274,7 @@ def main():
              "potrf": "mfma"}.get(dominant, "hbm")
     roof = roofline_entry(tim, dominant, bound) if dominant else None
     extra = {t: roofline_entry(tim, t, "hbm") for t in ("band_update_dots", "p_update_dots", "hpsi_stencil",
-                                                        "hp_comb_stencil", "precond_resid", "dots", "axpby", "nl_gather", "nl_expand",
+                                                        "hp_comb_stencil", "precond_resid", "update_precond", "dots", "axpby", "nl_gather", "nl_expand",
                                                         "gemm_scale")
              if t in tim and t != dominant}
     for t in ("gemm_f64", "gemm_mixed_tn", "gemm_mixed_nn"):

@@ -207,14 +207,15 @@ void launch_precond_resid_dot(const Grid &g, const double *h, const double *sp, 
                               unsigned long long *out_max = nullptr);
 // band update fused with the next residual preconditioner (tiled path, S = 1): per band
 // xn = ca x + cb y, un = ha h + hb w into x_out, h_out (not x, h), out = K r with
-// r = sign (un - eps xn); partials (stencil_nparts(g, 1, nb) per band) of <r|K r>, <xn|xn>,
-// <xn|un>, and max |K r| as FP64 bits
+// r = sign (un - eps xn); partials of <r|K r>, <xn|xn>, <xn|un> (update_precond_nparts(g, nb)
+// per band at stride part_ld, which must be at least that), and max |K r| as FP64 bits
+int update_precond_nparts(const Grid &g, int nb);
 void launch_update_precond(const Grid &g, const double *x, const double *y, const double *h,
                            const double *w, long ld, const double *ca, const double *cb,
                            const double *ha, const double *hb, const double *eps, double sign,
                            double *x_out, double *h_out, double *out, int nb, double *part_r,
-                           double *part_xx, double *part_xh, unsigned long long *out_max,
-                           hipStream_t s);
+                           double *part_xx, double *part_xh, long part_ld,
+                           unsigned long long *out_max, hipStream_t s);
 
 void launch_halo_fill(double *v, int nperi, int na, int nb, int nc, int nf, int nf1, int nsnd,
                       hipStream_t s);

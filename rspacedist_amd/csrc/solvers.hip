@@ -1233,10 +1233,10 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
         hipLaunchKernelGGL(k_eps_next, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, al, be, a.sval, sap,
                            pap, epsn);
         double *po = psi == a.psi ? psi2 : a.psi, *ho = hpsi == a.hpsi ? h2 : a.hpsi;
-        const int np = stencil_nparts(c.g, 1, M);
+        const int np = update_precond_nparts(c.g, M);
         double *part = c.buf("bcg.upart", (size_t)3 * np * M);
         launch_update_precond(c.g, psi, PA, hpsi, HP, n, a2, c2, al, be, epsn, -1.0, po, ho, KR, M,
-                              part, part + (long)np * M, part + 2L * np * M, krmax, c.s);
+                              part, part + (long)np * M, part + 2L * np * M, np, krmax, c.s);
         launch_reduce_partials(part, np, M, dv, rnext, c.s);
         launch_reduce_partials(part + (long)np * M, np, M, dv, gg, c.s);
         launch_reduce_partials(part + 2L * np * M, np, M, dv, eh_next, c.s);

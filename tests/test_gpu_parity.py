@@ -330,17 +330,21 @@ def test_symmetric_combination_dots(kk, orc, asym):
         np.testing.assert_allclose(a["svecre"], b["svecre"], rtol=0, atol=1e-9 * np.abs(b["svecre"]).max())
 
 
-@pytest.mark.parametrize("epssd", [1e-12, 3.0])
-def test_fused_band_update_and_preconditioner(kk, orc, epssd):
+@pytest.mark.parametrize("n,epssd,upd_p", [(16, 1e-12, 1), (16, 3.0, 1), (32, 1e-12, 1), (32, 1e-12, 2)])
+def test_fused_band_update_and_preconditioner(kk, orc, n, epssd, upd_p):
     """With H symmetric the block CG's band update and the next residual preconditioner run as
     one stencil pass, the new bands alternating between two buffers; the steps agree with the
     separate passes to rounding.  epssd = 3 makes every band converge early, so the loop also
-    ends after a fused update (the bands are copied back into the caller's arrays)."""
-    s = synth.make_system(16, 0.4, 3, 12, seed=87)
+    ends after a fused update (the bands are copied back into the caller's arrays).  At 32^3
+    one and two points per thread give different tilings (two y-tiles or one, 16 or 8
+    work-groups per band) from the separate preconditioner's, so the partial sums' layout of
+    the fused pass is checked against its own tiling."""
+    s = synth.make_system(n, 0.4, 3, 12, seed=87)
     base = _state(s, orc)
     kw = synth.abi_args(s, nsdmax=4, epssd=epssd)
     out = []
     kk.set_param("kscg_mode", 1)
+    kk.set_param("upd_p", upd_p)
     try:
         for fu in (1, 0):
             kk.set_param("bcg_fuse_update", fu)
@@ -352,6 +356,7 @@ def test_fused_band_update_and_preconditioner(kk, orc, epssd):
     finally:
         kk.set_param("kscg_mode", 0)
         kk.set_param("bcg_fuse_update", 1)
+        kk.set_param("upd_p", 1)
     a, b = out
     np.testing.assert_allclose(a["sval"], b["sval"], rtol=0, atol=1e-11)
     for k, v in b.items():

@@ -929,7 +929,7 @@ void rayleigh_ritz(Ctx &c, int M, int ns, const double *veffs, bool l_rrb, doubl
             zero, R, (int)n1);
       HIPCK(hipMemcpyAsync(rsp, R, sizeof(double) * n1 * M, hipMemcpyDeviceToDevice, c.s));
     }
-    renormalize(c, M, psi, spsi, rsp, nullptr, nrm);
+    if (param("rr_renorm", 1.0) >= 1.0) renormalize(c, M, psi, spsi, rsp, nullptr, nrm);
     return;
   }
   if (keep_h && mixed_for(n)) {

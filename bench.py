@@ -160,9 +160,13 @@ def roofline_entry(tim, tag, bound):
 
 def cpu_baseline(cfg, budget_s=30.0):
     """The FP64 oracle (a C/OpenMP restatement of the reference's band-sequential path) on
-    this host: the same 128^3 grid and atoms with a small number of bands, scaled to the
-    full band count as a*M + b*M^2 from two sizes; the Hartree solve is timed for a fixed
-    number of CG iterations and scaled to the GPU's iteration count by the caller."""
+    this host.  At a fixed atom density the band solve costs a*M*N (stencil, projectors, dot
+    products) + b*M^2*N (Gram-Schmidt against lower bands, the RR products and rotation) + the
+    M^3 dense eigensolve, so it is measured on a grid of 1/8 the volume (same spacing, 1/8 of
+    the atoms) at 32, 64 and 128 bands, fitted as T = a M + b M^2, scaled by the volume ratio
+    and evaluated at the full band count; the oracle's eigensolver is timed at half the band
+    count and scaled by 8.  The Hartree solve is timed on the full grid for a fixed number of
+    CG iterations and scaled to the GPU's iteration count by the caller."""
     threads = min(16, os.cpu_count() or 1)
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     from oracle import oracle
@@ -170,10 +174,13 @@ def cpu_baseline(cfg, budget_s=30.0):
     lib = oracle.lib()
     threads = lib.orc_num_threads()
     c = synth.CONFIGS[cfg]
-    out = {}
+    M = c["nbands"]
+    n_s = max(8, c["n"] // 2)
+    vol = (c["n"] / n_s) ** 3
+    natom_s = max(1, int(round(c["natom"] / vol)))
     times = {}
-    for m in (4, 8, 16):
-        s = synth.make_system(c["n"], c["h"], c["natom"], m, seed=cfg)
+    for m in sorted({min(M, k) for k in (32, 64, 128)}):
+        s = synth.make_system(n_s, c["h"], natom_s, m, seed=cfg)
         psi = synth.random_bands(s, nbands=m)
         st = dict(svecre=psi, ssvre=np.zeros_like(psi, order="F"),
                   rspsep=np.zeros((9, s.natom, m), order="F"), hsvre=np.zeros_like(psi, order="F"),
@@ -184,14 +191,21 @@ def cpu_baseline(cfg, budget_s=30.0):
         t0 = time.perf_counter()
         oracle.call("scf_diag", **kw, **st)
         times[m] = time.perf_counter() - t0
-    # T(M)/M = a + b M by least squares over the samples
+    # T(M) = a M + b M^2 by least squares over the samples (no constant term)
     ms_ = np.array(sorted(times), dtype=float)
-    per = np.array([times[int(m)] / m for m in ms_])
-    b, a = np.polyfit(ms_, per, 1)
-    b = max(float(b), 0.0)
-    a = float(a)
-    M = c["nbands"]
-    diag = a * M + b * M * M
+    tt = np.array([times[int(m)] for m in ms_])
+    if len(ms_) >= 2:
+        (a, b), *_ = np.linalg.lstsq(np.stack([ms_, ms_ ** 2], axis=1), tt, rcond=None)
+    else:
+        a, b = tt[0] / ms_[0], 0.0
+    a, b = max(float(a), 0.0), max(float(b), 0.0)
+    # the RR eigensolve at the full band count: the oracle's tred2/tql2 at M/2, times 8
+    mh = max(2, M // 2)
+    A = np.random.default_rng(1).standard_normal((mh, mh))
+    t0 = time.perf_counter()
+    oracle.dsyev(A + A.T)
+    eig_s = 8.0 * (time.perf_counter() - t0)
+    diag = vol * (a * M + b * M * M) + eig_s
     # Hartree: fixed 10 CG iterations on the full grid
     n = c["n"]
     xmax = 0.5 * n * c["h"]
@@ -204,7 +218,8 @@ def cpu_baseline(cfg, budget_s=30.0):
                 rho_aug_dense=rho, boundx=z, boundy=z, boundz=z, vh_dense=vh)
     hart_per_it = (time.perf_counter() - t0) / 10
     out = dict(threads=threads, diag_s=diag, samples={str(k): round(v, 3) for k, v in times.items()},
-               hartree_per_iter_s=hart_per_it, a=a, b=b)
+               hartree_per_iter_s=hart_per_it, a=a, b=b, n_s=n_s, natom_s=natom_s, vol=vol,
+               eig_s=eig_s, mh=mh)
     return out
 
 
@@ -287,10 +302,13 @@ def main():
         hits = float(np.mean(res[head]["hart"])) if res[head]["hart"] else 1.0
         val = cb["diag_s"] + cb["hartree_per_iter_s"] * hits
         cpu = {"value": round(val, 2), "unit": "s/SCF-step", "cores": cb["threads"], "kind": "port",
-               "sample": (f"oracle scf_diag (band-sequential CG, FP64) on the full 128^3 grid with "
-                          f"4, 8 and 16 bands ({cb['samples']} s), fitted a*M+b*M^2 and evaluated at "
-                          f"M={s.nbands}; plus 10 oracle Hartree CG iterations "
-                          f"({cb['hartree_per_iter_s']:.3f} s/it) x {hits:.0f} GPU iterations")}
+               "sample": (f"oracle scf_diag (band-sequential CG, FP64) on a {cb['n_s']}^3 grid "
+                          f"({cb['natom_s']} atoms, 1/{cb['vol']:.0f} of the volume) with "
+                          f"{', '.join(cb['samples'])} bands ({cb['samples']} s), fitted a*M+b*M^2, "
+                          f"scaled by {cb['vol']:.0f} and evaluated at M={s.nbands}; plus the oracle "
+                          f"eigensolver at M={cb['mh']} x 8 ({cb['eig_s']:.2f} s); plus 10 oracle "
+                          f"Hartree CG iterations on the full grid ({cb['hartree_per_iter_s']:.3f} s/it) "
+                          f"x {hits:.0f} GPU iterations")}
     line = {
         "metric": "SCF-step s at 128^3x1024 bands: mixed vs FP64, eigval err <=1e-6 Ha",
         "value": round(res[head]["dt"], 4),

@@ -1177,13 +1177,25 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
       // S = 1: S psi equals psi, so one split of psi serves both products
       if (c.q_zero) pre = split_psi(c, "bcg.psi", psi, M, fused ? gg : nullptr, spl);
     }
+    // bcg_proj 1 (default): band b's K r against bands 1..b, the set kscg_r projects on
+    // (kscg8f.F90:246-261); C is then upper triangular, so the TN product computes only the
+    // tiles on and above the diagonal and the rotation skips the zero stages.  0: all bands.
+    const bool lower_only = param("bcg_proj", 1.0) >= 1.0;
+    const int tri = lower_only ? 2 : 0;
+    auto zero_lower = [&]() {
+      if (lower_only)
+        hipLaunchKernelGGL(k_zero_lower, dim3(nblk((long)M * M, 256)), dim3(256), 0, c.s, M, C);
+    };
     if (pre) {
-      sub_tn(c, M, M, n, dv, psi, n, KR, n, zero, C, M, nullptr, s_kr, pre);
-      sub_nn(c, n, M, M, mone, psi, n, C, M, one, KR, n, nullptr, pre);
+      sub_tn(c, M, M, n, dv, psi, n, KR, n, zero, C, M, nullptr, s_kr, pre, tri);
+      zero_lower();
+      sub_nn(c, n, M, M, mone, psi, n, C, M, one, KR, n, nullptr, pre, nullptr, lower_only);
     } else {
       const double *s_psi = mixed_scale("bcg.psi", psi, n, n, M);
-      sub_tn(c, M, M, n, dv, spsi, n, KR, n, zero, C, M, spsi == psi ? s_psi : nullptr, s_kr);
-      sub_nn(c, n, M, M, mone, psi, n, C, M, one, KR, n, s_psi);
+      sub_tn(c, M, M, n, dv, spsi, n, KR, n, zero, C, M, spsi == psi ? s_psi : nullptr, s_kr,
+             nullptr, tri);
+      zero_lower();
+      sub_nn(c, n, M, M, mone, psi, n, C, M, one, KR, n, s_psi, nullptr, nullptr, lower_only);
     }
     // P = Kr + beta P
     hipLaunchKernelGGL(k_ratio, dim3(nblk(M, 64)), dim3(64), 0, c.s, M, rnew, rold, doreset ? 1 : 0, tb);

@@ -228,8 +228,11 @@ def test_scf_diag_sequential_matches_oracle_over_steps(kk, orc):
     np.testing.assert_allclose(res["gpu"]["sval"], res["orc"]["sval"], rtol=1e-9, atol=1e-10)
 
 
-def test_scf_diag_block_converges_to_dense_eigenvalues(kk, orc):
+@pytest.mark.parametrize("proj", [1, 0])
+def test_scf_diag_block_converges_to_dense_eigenvalues(kk, orc, proj):
+    """Block CG with K r projected on bands 1..b (as kscg_r) or on all bands."""
     kk.set_param("kscg_mode", 1)
+    kk.set_param("bcg_proj", proj)
     try:
         s = synth.make_system(10, 0.5, 3, 8, seed=8)
         st = _state(s, orc)
@@ -242,6 +245,7 @@ def test_scf_diag_block_converges_to_dense_eigenvalues(kk, orc):
         q = st["svecre"]
         np.testing.assert_allclose(q.T @ q * s.dv, np.eye(8), atol=1e-12)
     finally:
+        kk.set_param("bcg_proj", 1)
         kk.set_param("kscg_mode", 0)
 
 

@@ -389,6 +389,14 @@ __device__ inline void wait_vm() {
 
 #define NN_LDS(x) ((__attribute__((address_space(3))) void *)(x))
 
+// LDS written by lanes of this wave, next read by other lanes of the same wave (no other wave
+// touches the area): a wavefront-scope fence pair around the wave barrier
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 constexpr int NN_NS = 5;
 typedef _Float16 FragBuf[NN_NS][32][512];
 
@@ -548,7 +556,9 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
   // epilogue: per 32-column block b, each wave parks its 64 x 32 piece in LDS as
   // [column][row] (pitch 65 floats: conflict-free) and writes it back column by column.
   // For beta != 0 the block's 32 Y values per lane are loaded before the parking, as
-  // independent loads whose latency overlaps the LDS round trip.
+  // independent loads whose latency overlaps the LDS round trip.  Each wave's area (and its
+  // column sums) is its own, so the waves synchronise only with themselves until the column
+  // sums are combined.
   constexpr int EP = 65;
   float *W = reinterpret_cast<float *>(&buf[0][0][0]) + w * 32 * EP;
   // column sums of squares of the parked values, [row group wi][tile column], after the W areas
@@ -571,7 +581,7 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
         const int ml = 32 * a + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         W[(lane & 31) * EP + ml] = acc[a][b][r];
       }
-    __syncthreads();
+    wave_lds_sync();
     if (p.colsq) {  // lane: column lane % 32, rows 32 (lane / 32) .. +31 of the wave's 64
       const int q = lane & 31, h = lane >> 5;
       double s2 = 0.0;
@@ -591,8 +601,9 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
         p.Y[m + p.ldy * (long)n] = p.beta == 0.0 ? v : v + p.beta * yo[q];
       }
     }
-    __syncthreads();
+    wave_lds_sync();  // the next block's parking overwrites this wave's area
   }
+  if (p.colsq) __syncthreads();  // column sums of all waves
   if (p.colsq && t < 256 && c0 + t < p.M2) {  // the values written are ys W exactly
     const double y2 = p.ys[c0 + t] * p.ys[c0 + t];
     p.colsq[(long)(c0 + t) * p.row_blocks + rb] =

@@ -270,7 +270,9 @@ __global__ void k_tn_reduce(const float *part, int nsplit, int M1, int M2, doubl
   if (t >= (long)M1 * M2) return;
   const int i = (int)(t / M2), j = (int)(t % M2);
   if ((tri == 1 && (i >> 8) < (j >> 8)) || (tri == 2 && (i >> 8) > (j >> 8))) return;
+  // in split order (unrolled so that several partial loads are in flight)
   double s = 0.0;
+#pragma unroll 16
   for (int q = 0; q < nsplit; q++) s += (double)part[(long)q * M1 * M2 + t];
   const double v = alpha * s / (sa[i] * sb[j]);
   double *c = C + i + ldc * (long)j;

@@ -42,7 +42,8 @@ SIGMA_P = 0.5
 R_CUT = 2.0
 NPRJ = 9
 
-# BASELINE configurations: (n, h, bands, atoms)
+# Benchmark workloads: 40^3 CPU-sized case, 96^3 x 512 bands, 128^3 x 1024 bands (the
+# split-precision step), 96^3 x 256 bands (forces and adjoints)
 CONFIGS = {
     0: dict(n=40, h=0.25, nbands=32, natom=8),
     1: dict(n=96, h=0.20, nbands=512, natom=64),

@@ -417,7 +417,9 @@ typedef _Float16 FragBuf[NN_NS][32][512];
 // read into registers and consumed before arriving.  The plain s_barrier (no fence) keeps the
 // later stages in flight; empty asm statements keep the compiler from moving LDS accesses
 // across it.  The accumulation order per output (hi hi, hi lo, [lo lo,] lo hi) is the same in
-// every stage.  On return all loads have landed and the block is synchronised.
+// every stage.  On return this wave's loads have landed; the block is not synchronised: the
+// last barrier was at the start of stage nst - 2, so slower waves may still read the ring slot
+// of stage nst - 1 from LDS (no load is in flight), and every other slot is free.
 // Fragment rows are addressed through 32-bit buffer offsets: KC 16-deep stages of 1 KB each.
 static void frag_rows_ok(long KC, const char *who) {
   if (KC * 1024 > 0x7fffffffL) throw Error(1, std::string(who) + ": operand too long for the split-precision product");
@@ -531,7 +533,6 @@ __device__ inline void frag_gemm(const _Float16 *ah_g, const _Float16 *al_g, con
   for (int st = 0; st + 1 < nst; st++) stage(std::true_type{}, st);
   if (nst > 0) stage(std::false_type{}, nst - 1);
   wait_vm<0>();
-  __syncthreads();
 }
 
 // Y tile 256 rows x 256 columns (frag_gemm over all of K).  The col_tiles of one row block
@@ -561,10 +562,17 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
   // independent loads whose latency overlaps the LDS round trip.  Each wave's area (and its
   // column sums) is its own, so the waves synchronise only with themselves until the column
   // sums are combined.
+  // The areas lie in the ring slots other than that of the last stage (which slower waves may
+  // still read): three 8320-byte wave areas per 32 KB slot in the three slots after it, the
+  // column sums in the fourth, so a wave starts its epilogue without waiting for the others.
   constexpr int EP = 65;
-  float *W = reinterpret_cast<float *>(&buf[0][0][0]) + w * 32 * EP;
-  // column sums of squares of the parked values, [row group wi][tile column], after the W areas
-  double *cs = reinterpret_cast<double *>(reinterpret_cast<float *>(&buf[0][0][0]) + 8 * 32 * EP);
+  static_assert(NN_NS == 5 && 3 * 32 * EP * sizeof(float) <= sizeof(buf[0]) &&
+                    1024 * sizeof(double) <= sizeof(buf[0]),
+                "epilogue areas: four free ring slots, three wave areas per slot");
+  const int last = (nst - 1) % NN_NS;  // nst >= 1
+  float *W = reinterpret_cast<float *>(&buf[(last + 1 + w / 3) % NN_NS][0][0]) + (w % 3) * 32 * EP;
+  // column sums of squares of the parked values, [row group wi][tile column]
+  double *cs = reinterpret_cast<double *>(&buf[(last + 4) % NN_NS][0][0]);
   const long m = r0 + 64 * wi + lane;
   const bool mrow = m < p.R;
 #pragma unroll

@@ -1,0 +1,196 @@
+"""Writes oracle/_ref/ref_bridge.F90: C-callable (bind(C)) wrappers around the reference's
+own Fortran routines, compiled from /root/reference by oracle/Makefile (target `ref`).
+
+TEST INFRASTRUCTURE ONLY.  The wrappers take exactly the C-ABI argument lists of
+include/kukan_hip.h (the tables in rspacedist_amd/abi.py: scalars by value, arrays as
+pointers, Fortran logicals as int) and call the reference routine of the same name, so a test
+can run the reference, the oracle restatement and the GPU library on the same arguments.
+Nothing here is a stand-in for a reference source: the reference files are compiled as they
+are and linked against the image's real MPICH and MKL (one rank, MPI_COMM_WORLD).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+from rspacedist_amd.abi import SIGNATURES  # noqa: E402
+
+# routine -> (reference module, statements before the call, statements after the call)
+ROUTINES = {
+    # one rank: the exchange posts its buffers in overlap_finitedifference_r and writes the
+    # halos in overlap_fdcheck_r, so the wrapper runs both, as every caller does
+    "overlap_finitedifference_r": ("mod_overlap_finitedifference",
+                                   ["call overlap_finitedifference_init(ncpa, ncpb, ncpc, nf, 1, 0)"],
+                                   ["call overlap_fdcheck_r(nperi, ncpa, ncpb, ncpc, nf, nf1, nsnd, v_cc)",
+                                    "call overlap_finitedifference_final()"]),
+    "kslaplacian_r": ("mod_kslaplacian", ["call kslaplacian_initialize(nf)"],
+                      ["call kslaplacian_finalize()"]),
+    "ksprecondition": ("mod_ksprecondition", [], []),
+    "nonlocaloperation_r_01": ("mod_nonlocaloperation", [], []),
+    "nonlocaloperation_r_02": ("mod_nonlocaloperation", [], []),
+    "kscg_r": ("mod_kscg", [], []),
+    "ksdiis_r": ("mod_ksdiis", [], []),
+    "orthogonalization_r": ("mod_orthogonalization", [], []),
+    "orthogonalization_r_01": ("mod_orthogonalization", [], []),
+    "rayleighritz_r": ("mod_rayleighritz", [], []),
+    "reorderstates": ("mod_reorderstates", [], []),
+    "scf_diag": ("mod_scf_diag", [], []),
+    "scf_hartree": ("mod_scf_hartree", [], []),
+    "tools_correctpotential": ("mod_tools", [], []),
+    "scf_charge_smt_r": ("mod_scf_charge", [], []),
+    "force_eig_r": ("mod_force", [], []),
+    "totalenergy_eig": ("mod_totalenergy", [], []),
+}
+# C int arguments that are Fortran logicals in the reference
+LOGICALS = {"ksconv", "l_hsv", "l_rrb"}
+# arrays the reference declares complex*16 (the real path passes dummies)
+COMPLEX = {"cspsep", "sveccm", "ssvcm", "hsvcm", "cspsepsoc", "svecsoc"}
+# atom-indexed reductions use mpicom_atom(natom); routines with natom get it sized first
+NATOM_ARG = "natom"
+
+
+def wrapper(name):
+    mod, pre, post = ROUTINES[name]
+    args = SIGNATURES[name]
+    names = [a for a, _ in args]
+    decl, conv_in, conv_out, actual = [], [], [], []
+    for a, t in args:
+        if t == "i":
+            if a in LOGICALS:
+                decl.append(f"integer(c_int), value :: {a}")
+                decl.append(f"logical :: l__{a}")
+                conv_in.append(f"l__{a} = {a} /= 0")
+                actual.append(f"l__{a}")
+            else:
+                decl.append(f"integer(c_int), value :: {a}")
+                actual.append(a)
+        elif t == "d":
+            decl.append(f"real(c_double), value :: {a}")
+            actual.append(a)
+        elif t == "I":
+            decl.append(f"integer(c_int) :: {a}(*)")
+            actual.append(a)
+        elif t == "D":
+            decl.append(f"real(c_double) :: {a}(*)")
+            actual.append(a)
+        elif t == "V":
+            kind = "complex(c_double_complex)" if a in COMPLEX else "real(c_double)"
+            decl.append(f"{kind} :: {a}(*)")
+            actual.append(a)
+        elif t == "oi":
+            decl.append(f"integer(c_int) :: {a}")
+            if a in LOGICALS:
+                decl.append(f"logical :: l__{a}")
+                conv_in.append(f"l__{a} = {a} /= 0")
+                conv_out.append(f"{a} = merge(1, 0, l__{a})")
+                actual.append(f"l__{a}")
+            else:
+                actual.append(a)
+        elif t == "od":
+            decl.append(f"real(c_double) :: {a}")
+            actual.append(a)
+        else:
+            raise ValueError(t)
+    head = f"integer(c_int) function ref_{name}({', '.join(names)}) bind(C, name='ref_{name}')"
+    lines = [_wrap(head), "  use iso_c_binding", f"  use {mod}, only: {name}"]
+    for p in pre + post:
+        callee = p.split("call ")[1].split("(")[0]
+        lines.append(f"  use {mod}, only: {callee}")
+    if NATOM_ARG in names:
+        lines.append("  use kukan_ref_setup, only: ref_atoms")
+    lines.append("  implicit none")
+    lines += ["  " + d for d in decl]
+    if NATOM_ARG in names:
+        lines.append(f"  call ref_atoms({NATOM_ARG})")
+    lines += ["  " + c for c in conv_in]
+    lines += ["  " + p for p in pre]
+    call = f"  call {name}({', '.join(actual)})"
+    lines.append(_wrap(call))
+    lines += ["  " + p for p in post]
+    lines += ["  " + c for c in conv_out]
+    lines.append(f"  ref_{name} = 0")
+    lines.append(f"end function ref_{name}")
+    return "\n".join(lines)
+
+
+def _wrap(line, width=100):
+    out, cur = [], ""
+    for tok in line.split(", "):
+        piece = tok if not cur else cur + ", " + tok
+        if len(piece) > width:
+            out.append(cur + ", &")
+            cur = "      " + tok
+        else:
+            cur = piece
+    out.append(cur)
+    return "\n".join(out)
+
+
+HEAD = """! Generated by oracle/gen_ref_bridge.py: bind(C) wrappers around the reference routines.
+module kukan_ref_setup
+  use iso_c_binding
+  implicit none
+contains
+  ! MPI on one rank and the single-rank communicator layout of mod_mpi
+  integer(c_int) function ref_init() bind(C, name='ref_init')
+    use mod_mpi
+    use mod_stopp, only: ndisp_stopp
+    implicit none
+    integer :: ierr
+    logical :: up
+    call mpi_initialized(up, ierr)
+    if (.not. up) call mpi_init(ierr)
+    myrank_glbl = 0; myr_space = 0; myrx = 0; myry = 0; myrz = 0
+    myr_kpt = 0; myr_rhpt = 0; myr_grn = 0; myr_ene = 0
+    nprocworld = 1; nprocw = 1; nprocs = 1; nprocx = 1; nprocy = 1; nprocz = 1
+    nprock = 1; nprocsk = 1; nprocg = 1; nprocrhpt = 1; nprocene = 1
+    nprocx_inv = 1; nprocy_inv = 1; nprocz_inv = 1; nprocg_inv = 1
+    mpicom_space = mpi_comm_world; mpicom_kpt = mpi_comm_world; mpicom_rhpt = mpi_comm_world
+    mpicom_grn = mpi_comm_world; mpicom_ene = mpi_comm_world
+    ndisp_stopp = 6
+    comcount = 0.0d0; comtime = 0.0d0
+    ! unit 11 is the main program's log file (scf_diag8f.F90:123 writes to it): discard it
+    open(unit=11, file='/dev/null', status='unknown')
+    ref_init = 0
+  end function ref_init
+
+  subroutine ref_atoms(natom)
+    use mod_mpi
+    implicit none
+    integer, intent(in) :: natom
+    if (allocated(mpicom_atom)) then
+      if (size(mpicom_atom) >= natom) return
+      deallocate(mpicom_atom)
+    end if
+    allocate(mpicom_atom(max(1, natom)))
+    mpicom_atom = mpi_comm_world
+  end subroutine ref_atoms
+
+  ! tools8f.F90:144 returns the weights as nine scalars; here as acfd(0:8)
+  integer(c_int) function ref_tools_definefdcoef(nf, acfd) bind(C, name='ref_tools_definefdcoef')
+    use mod_tools, only: tools_definefdcoef
+    implicit none
+    integer(c_int), value :: nf
+    real(c_double) :: acfd(0:8)
+    acfd = 0.0d0
+    call tools_definefdcoef(nf, acfd(0), acfd(1), acfd(2), acfd(3), acfd(4), acfd(5), acfd(6), &
+                            acfd(7), acfd(8))
+    ref_tools_definefdcoef = 0
+  end function ref_tools_definefdcoef
+end module kukan_ref_setup
+"""
+
+
+def main(out):
+    body = [HEAD] + [wrapper(name) for name in ROUTINES]
+    text = "\n\n".join(body) + "\n"
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    with open(out, "w") as fh:
+        fh.write(text)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(HERE, "_ref", "ref_bridge.F90"))

@@ -16,7 +16,8 @@ shapes the reference allocates (dim_global8f.F90:28-45, Fortran order):
 * index lists ``lstvec2(num_list,num_ppcell)`` (1-based, x fastest) and offsets
   ``lstx/lsty/lstz`` built in the order of pseudocalc8f.F90:954-1033 (z outer, x inner);
 * ``dij(nprjmx,nprjmx,nums,natom)``: one seeded symmetric 9x9 with entries U[-0.5,0.5]
-  replicated per atom; ``sss = 0`` (S = 1).
+  replicated per atom; ``sss = 0`` (S = 1) unless a positive semi-definite q is asked for
+  (``q_scale``, the S != 1 case of PAW data).
 
 The smooth cut-off makes the projectors continuous functions of the atom position, so
 energies can be differentiated by central differences.
@@ -299,8 +300,17 @@ def seeded_dmat(rng):
     return np.triu(a) + np.triu(a, 1).T
 
 
-def make_system(n, h, natom, nbands, seed, nf=4, atoms=None, dmat=None):
-    """Build a System; ``atoms``/``dmat`` override the seeded draws (used for finite differences)."""
+def seeded_qmat(seed, scale):
+    """A seeded symmetric positive semi-definite 9x9 augmentation matrix q (sss of one
+    species): scale * R R^T / 9 with R uniform in [-0.5, 0.5].  With q >= 0 the overlap
+    S = 1 + sum_a sum_ij |p_i> q_ij <p_j| is positive definite, as PAW overlaps are."""
+    r = np.random.default_rng(seed + 104729).uniform(-0.5, 0.5, size=(NPRJ, NPRJ))
+    return scale * (r @ r.T) / NPRJ
+
+
+def make_system(n, h, natom, nbands, seed, nf=4, atoms=None, dmat=None, q_scale=0.0):
+    """Build a System; ``atoms``/``dmat`` override the seeded draws (used for finite differences).
+    ``q_scale`` > 0 gives a nonzero overlap matrix sss (seeded_qmat), i.e. S != 1."""
     rng = np.random.default_rng(seed)
     L = n * h
     xmax = 0.5 * L
@@ -317,6 +327,8 @@ def make_system(n, h, natom, nbands, seed, nf=4, atoms=None, dmat=None):
         setattr(s, k, v)
     s.dij = np.asfortranarray(np.repeat(dmat[:, :, None, None], natom, axis=3).reshape(NPRJ, NPRJ, 1, natom))
     s.sss = np.zeros((NPRJ, NPRJ, 1), order="F")
+    if q_scale:
+        s.sss[:, :, 0] = seeded_qmat(seed, q_scale)
     return s
 
 

@@ -49,6 +49,13 @@ def dense_h(s):
     return H
 
 
+def dense_s(s):
+    """Dense S = 1 + P Q P^T / dV (Q = sss of the single species, replicated per atom)."""
+    P = dense_projectors(s)
+    Qbig = np.kron(np.eye(s.natom), s.sss[:, :, 0])
+    return np.eye(s.npts) + P @ Qbig @ P.T / s.dv
+
+
 def halo(v, nf, nsnd, nperi=3):
     """Halo'd copy of interior v (n,n,n) with periodic faces filled (np.pad wrap, faces only)."""
     n1, n2, n3 = v.shape

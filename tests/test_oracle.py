@@ -264,6 +264,58 @@ def test_scf_diag_diis_converges_to_dense_eigenvalues(orc):
     np.testing.assert_allclose(st["sval"], w, atol=1e-8)
 
 
+def test_s_metric_orthogonalization_is_gram_schmidt_in_s(orc):
+    """S != 1 (sss positive semi-definite): orthogonalization_r gives Q^T S Q = I, S psi and
+    <p|psi> consistent with the dense S and projectors, and the same Q as Gram-Schmidt in
+    the S inner product (the Cholesky factor of X^T S X, orthogonalization8f.F90:144-256)."""
+    s = synth.make_system(10, 0.5, 2, 40, seed=6, q_scale=2.0)
+    psi0 = synth.random_bands(s)
+    st = {"svecre": psi0.copy(order="F"), "ssvre": np.zeros_like(psi0, order="F"),
+          "rspsep": np.zeros((9, s.natom, 40), order="F")}
+    orc.call("orthogonalization_r", **synth.abi_args(s), nflag=1, **st)
+    Sd = H.dense_s(s)
+    q = st["svecre"]
+    np.testing.assert_allclose(q.T @ Sd @ q * s.dv, np.eye(40), atol=1e-12)
+    np.testing.assert_allclose(st["ssvre"], Sd @ q, atol=1e-12)
+    P = H.dense_projectors(s)
+    np.testing.assert_allclose(st["rspsep"].reshape(9 * s.natom, 40, order="F"), P.T @ q, atol=1e-11)
+    # Gram-Schmidt in the S metric: X = Q R with R = chol(dV X^T S X)^T upper triangular
+    R = np.linalg.cholesky(psi0.T @ Sd @ psi0 * s.dv).T
+    np.testing.assert_allclose(q, scipy.linalg.solve_triangular(R, psi0.T, trans="T").T, atol=1e-10)
+    # a dropped S term is visible: the plain Gram-Schmidt Q differs
+    assert np.abs(q.T @ q * s.dv - np.eye(40)).max() > 1e-3
+
+
+def test_s_metric_rayleigh_ritz_matches_dense_projection(orc):
+    s = synth.make_system(8, 0.55, 2, 6, seed=7, q_scale=2.0)
+    st = H.init_state(s, orc)
+    psi0 = st["svecre"].copy()
+    orc.call("rayleighritz_r", **synth.abi_args(s), l_rrb=1, **{k: st[k] for k in
+             ("rspsep", "svecre", "ssvre", "hsvre", "sval")})
+    Hd, Sd = H.dense_h(s), H.dense_s(s)
+    w, z = scipy.linalg.eigh(psi0.T @ Hd @ psi0 * s.dv, psi0.T @ Sd @ psi0 * s.dv)
+    np.testing.assert_allclose(st["sval"], w, rtol=1e-11, atol=1e-11)
+    q = st["svecre"]
+    np.testing.assert_allclose(q.T @ Sd @ q * s.dv, np.eye(6), atol=1e-11)
+    np.testing.assert_allclose(st["ssvre"], Sd @ q, atol=1e-11)
+
+
+@pytest.mark.parametrize("ncgscf", [200, 2])  # CG (kscg_r), then DIIS after two CG steps
+def test_s_metric_scf_diag_converges_to_generalized_eigenvalues(orc, ncgscf):
+    """kscg_r / ksdiis_r with S != 1 converge to the generalized eigenpairs H psi = eps S psi
+    of the dense H and S (scipy.linalg.eigh(H, S)); they differ from the S = 1 eigenvalues
+    by ~1e-2 Ha here, so a dropped or misplaced S term fails."""
+    s = synth.make_system(10, 0.5, 3, 8, seed=8, q_scale=2.0)
+    st = _converge(orc, s, 60 if ncgscf > 2 else 100, ncgscf=ncgscf, nsdmax=4, epssd=1e-12)
+    Hd, Sd = H.dense_h(s), H.dense_s(s)
+    w = scipy.linalg.eigh(Hd, Sd, eigvals_only=True)[:8]
+    w1 = scipy.linalg.eigh(Hd, eigvals_only=True)[:8]
+    assert np.abs(w - w1).max() > 1e-3
+    np.testing.assert_allclose(st["sval"], w, atol=1e-8)
+    q = st["svecre"]
+    np.testing.assert_allclose(q.T @ Sd @ q * s.dv, np.eye(8), atol=1e-12)
+
+
 def _dense_precond(n):
     I = np.eye(n)
     S = np.roll(I, 1, axis=0) + np.roll(I, -1, axis=0)

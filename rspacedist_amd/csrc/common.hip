@@ -114,6 +114,11 @@ Timed::~Timed() {
   }
 }
 
+void timing_add_work(const char *tag, double work) {
+  if (param("timing", 0.0) < 1.0 || lib().capturing) return;
+  timing_table()[tag].work += work;
+}
+
 void timing_collect() {
   auto &p = pending();
   if (p.empty()) return;

@@ -418,8 +418,11 @@ typedef _Float16 FragBuf[NN_NS][32][512];
 // later stages in flight; empty asm statements keep the compiler from moving LDS accesses
 // across it.  The accumulation order per output (hi hi, hi lo, [lo lo,] lo hi) is the same in
 // every stage.  On return this wave's loads have landed; the block is not synchronised: the
-// last barrier was at the start of stage nst - 2, so slower waves may still read the ring slot
-// of stage nst - 1 from LDS (no load is in flight), and every other slot is free.
+// last barrier was at the start of stage nst - 2.  Slower waves may still read the ring slot
+// of stage nst - 1 (its fragments are read during stage nst - 2), and their reads of stage
+// nst - 2's slot, issued during stage nst - 3, may still be outstanding when they pass that
+// barrier.  The slots of stages nst - 3 and earlier were read into registers and consumed by
+// MFMAs before every wave reached the barrier, so those three slots are free.
 // Fragment rows are addressed through 32-bit buffer offsets: KC 16-deep stages of 1 KB each.
 static void frag_rows_ok(long KC, const char *who) {
   if (KC * 1024 > 0x7fffffffL) throw Error(1, std::string(who) + ": operand too long for the split-precision product");
@@ -562,17 +565,21 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
   // independent loads whose latency overlaps the LDS round trip.  Each wave's area (and its
   // column sums) is its own, so the waves synchronise only with themselves until the column
   // sums are combined.
-  // The areas lie in the ring slots other than that of the last stage (which slower waves may
-  // still read): three 8320-byte wave areas per 32 KB slot in the three slots after it, the
-  // column sums in the fourth, so a wave starts its epilogue without waiting for the others.
+  // The areas lie in the three ring slots of stages nst - 3, nst - 4 and nst - 5 (slots last +
+  // 3, + 2, + 1), which no wave reads after the last barrier (stage nst - 1's slot may still
+  // be read, and stage nst - 2's may still have reads in flight): three 8320-byte wave areas
+  // per 32 KB slot, waves 6 and 7 with the column sums behind them in the third, so a wave
+  // starts its epilogue without waiting for the others.
   constexpr int EP = 65;
   static_assert(NN_NS == 5 && 3 * 32 * EP * sizeof(float) <= sizeof(buf[0]) &&
-                    1024 * sizeof(double) <= sizeof(buf[0]),
-                "epilogue areas: four free ring slots, three wave areas per slot");
+                    2 * 32 * EP * sizeof(float) + 1024 * sizeof(double) <= sizeof(buf[0]),
+                "epilogue areas: three free ring slots, three wave areas per slot");
   const int last = (nst - 1) % NN_NS;  // nst >= 1
   float *W = reinterpret_cast<float *>(&buf[(last + 1 + w / 3) % NN_NS][0][0]) + (w % 3) * 32 * EP;
-  // column sums of squares of the parked values, [row group wi][tile column]
-  double *cs = reinterpret_cast<double *>(&buf[(last + 4) % NN_NS][0][0]);
+  // column sums of squares of the parked values, [row group wi][tile column], after the two
+  // wave areas of slot last + 3 (2 * 32 * EP floats = 16640 bytes, 8-byte aligned)
+  double *cs = reinterpret_cast<double *>(reinterpret_cast<float *>(&buf[(last + 3) % NN_NS][0][0]) +
+                                          2 * 32 * EP);
   const long m = r0 + 64 * wi + lane;
   const bool mrow = m < p.R;
 #pragma unroll
@@ -849,9 +856,11 @@ static void gemm_mixed_tn_presplit(int M1, int M2, long K, double alpha, const d
   p.part = (float *)scratch("gemm.tnpart", sizeof(float) * (size_t)p.nsplit * M1 * M2);
   const int tiles = tri ? p.tiles_i * (p.tiles_i + 1) / 2 : p.tiles_i * p.tiles_j;
   const unsigned grid = (unsigned)((long)(p.nsplit + 7) / 8 * 8 * tiles);
-  // f16 MFMA flops of the tiles computed
+  // algorithmic flops 2 M1 M2 K (the full product, whatever the triangle or split); the
+  // f16 MFMA flops of the tiles computed go to "gemm_mixed_tn.mfma"
   const double frac = tri ? (double)tiles / ((double)p.tiles_i * p.tiles_j) : 1.0;
-  Timed t("gemm_mixed_tn", frac * 2.0 * (nterms >= 4 ? 4 : 3) * (double)M1 * M2 * K);
+  timing_add_work("gemm_mixed_tn.mfma", frac * 2.0 * (nterms >= 4 ? 4 : 3) * (double)M1 * M2 * K);
+  Timed t("gemm_mixed_tn", 2.0 * (double)M1 * M2 * K);
   if (nterms >= 4) hipLaunchKernelGGL(k_gemm_tnf<4>, dim3(grid), dim3(512), 0, s, p);
   else hipLaunchKernelGGL(k_gemm_tnf<3>, dim3(grid), dim3(512), 0, s, p);
   HIPCK(hipGetLastError());
@@ -913,7 +922,8 @@ void gemm_mixed_tn(int M1, int M2, long K, double alpha, const double *A, long l
   p.nsplit = nsplit;
   p.nsplit_pad = (nsplit + 7) / 8 * 8;
   p.part = (float *)scratch("gemm.tnpart", sizeof(float) * (size_t)nsplit * M1 * M2);
-  Timed t("gemm_mixed_tn", 2.0 * (nterms >= 4 ? 4 : 3) * (double)M1 * M2 * K);  // f16 MFMA flops
+  timing_add_work("gemm_mixed_tn.mfma", 2.0 * (nterms >= 4 ? 4 : 3) * (double)M1 * M2 * K);
+  Timed t("gemm_mixed_tn", 2.0 * (double)M1 * M2 * K);  // algorithmic flops
   const unsigned grid = (unsigned)((long)p.nsplit_pad * tiles);
   if (nterms >= 4) hipLaunchKernelGGL(k_gemm_tn<4>, dim3(grid), dim3(512), 0, s, p);
   else hipLaunchKernelGGL(k_gemm_tn<3>, dim3(grid), dim3(512), 0, s, p);
@@ -972,7 +982,9 @@ void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ld
     frac = done / ((double)KC * p.col_tiles);
   }
   {
-    Timed t("gemm_mixed_nn", frac * 2.0 * (nterms >= 4 ? 4 : 3) * (double)R * M2 * K);  // f16 flops
+    // algorithmic flops 2 R M2 K; the f16 MFMA flops of the stages computed beside them
+    timing_add_work("gemm_mixed_nn.mfma", frac * 2.0 * (nterms >= 4 ? 4 : 3) * (double)R * M2 * K);
+    Timed t("gemm_mixed_nn", 2.0 * (double)R * M2 * K);
     if (nterms >= 4) hipLaunchKernelGGL(k_gemm_nn<4>, dim3(grid), dim3(512), 0, s, p);
     else hipLaunchKernelGGL(k_gemm_nn<3>, dim3(grid), dim3(512), 0, s, p);
     HIPCK(hipGetLastError());

@@ -1551,7 +1551,7 @@ void launch_bound_scale(int nb, const unsigned long long *smax, const double *ma
   HIPCK(hipGetLastError());
 }
 
-static int dot_nparts(long n) {
+int dot_nparts(long n) {
   long p = (n + 8191) / 8192;
   if (p < 1) p = 1;
   if (p > 1024) p = 1024;
@@ -1559,11 +1559,11 @@ static int dot_nparts(long n) {
 }
 
 void launch_dots(long n, const double *x, long ldx, const double *y, long ldy, int nb,
-                 double scale, double *res, hipStream_t s) {
+                 double scale, double *res, hipStream_t s, double *part) {
   if (nb == 0) return;
   const int np = dot_nparts(n);
   Timed t("dots", (x == y && ldx == ldy ? 8.0 : 16.0) * (double)n * nb);
-  double *part = (double *)scratch("dots.part", sizeof(double) * (size_t)np * nb);
+  if (!part) part = (double *)scratch("dots.part", sizeof(double) * (size_t)np * nb);
   hipLaunchKernelGGL(k_dot_part, dim3(np, nb), dim3(256), 0, s, n, x, ldx, y, ldy, part, np);
   hipLaunchKernelGGL(k_reduce_parts, dim3(nb), dim3(256), 0, s, part, np, scale, res);
   HIPCK(hipGetLastError());

@@ -92,6 +92,10 @@ class Timed {
   hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 void timing_collect();
+// adds work (no time, no launch) to a tag's accumulator while timing is on: a second measure
+// of a timed launch, e.g. the MFMA flops a split-precision product issues beside its
+// algorithmic flops
+void timing_add_work(const char *tag, double work);
 struct TimingAcc {
   double ms = 0.0, work = 0.0;
   long count = 0;
@@ -257,9 +261,11 @@ void launch_bound_scale(int nb, const unsigned long long *smax, const double *ma
 // (coef band-fastest [slot][j][b]; with in2: the partials use ca_b psi + cb_b in2 as the band
 // and part2 receives <in2|added>)
 
-// batched reductions: res[b] = scale * sum_i x_b[i] * y_b[i] (deterministic two-stage)
+// batched reductions: res[b] = scale * sum_i x_b[i] * y_b[i] (deterministic two-stage);
+// part: room for dot_nparts(n) * nb partial sums, or null for the library's own buffer
 void launch_dots(long n, const double *x, long ldx, const double *y, long ldy, int nb,
-                 double scale, double *res, hipStream_t s);
+                 double scale, double *res, hipStream_t s, double *part = nullptr);
+int dot_nparts(long n);  // <= 1024
 // res[b] = scale * sum over partials (nb x nparts) in fixed order
 void launch_reduce_partials(const double *part, int nparts, int nb, double scale, double *res,
                             hipStream_t s);

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 
 #include "gemm_mixed.hpp"
 #include "kukan_internal.hpp"
@@ -444,6 +445,13 @@ __global__ void k_hartree_bound(int nx, int ny, int nz, int nfh, int axis, const
       b[ohi] -= w[mm] * bound[ihi];
     }
 }
+struct Nine {
+  double v[9];
+};
+__global__ void k_set9(Nine a, double *out) {
+  if (threadIdx.x < 9) out[threadIdx.x] = a.v[threadIdx.x];
+}
+
 __global__ void k_sub_scalar(long n, const double *s, double f, double *x) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i < n) x[i] = x[i] - s[0] * f;
@@ -784,18 +792,37 @@ static void renormalize(Ctx &c, int M, double *psi, double *spsi, double *rsp, d
 // One pass of S-orthonormalisation of bands [0, M): G = dV Psi^T S Psi = R^T R,
 // Psi, S Psi, <p|psi> (and H Psi if given) <- X R^-1.  Same Q as Gram-Schmidt in exact
 // arithmetic (orthogonalization8f.F90:93-376 with l_inp = 0).
+// fp64: the pass runs in FP64 also when the subspace products are split-precision (the
+// closing orthonormalisation of the band update: the bands it returns are orthonormal to
+// FP64 rounding, not to the split products' ~1e-7).  In FP64, G is one rocBLAS dgemm: at
+// 2,097,152 x 1024 dsyrk took 219 ms against about 62 ms for the full dgemm.
 void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi,
-            const double *norms) {
+            const double *norms, bool fp64) {
   const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
   rocblas_handle h = lib().blas;
   double *G = c.buf("cholqr.G", (size_t)M * M);
-  const double dv = c.g.dv(), one = 1.0;
-  // psi split once for G's TN and the first rotation (psi's own, next)
+  const double dv = c.g.dv(), one = 1.0, zero = 0.0;
+  const bool split = mixed_for(n) && !fp64;
+  if (fp64 && mixed_for(n)) {
+    // the split-precision rotations before this pass updated <p|psi> and S psi beside psi,
+    // each with its own ~1e-7 rounding: recompute both from psi in FP64 so that the pass
+    // S-orthonormalises psi itself (and <p|psi>, S psi match the returned bands)
+    if (rsp && n1 > 0) c.proj(psi, n, M, rsp, n1);
+    if (spsi != psi && rsp) c.spsi(psi, n, M, rsp, n1, spsi, n);
+  }
   SplitBands spl;
-  const SplitBands *pre = split_psi(c, "cholqr.psi", psi, M, norms, spl);
-  const double *s_psi = pre ? pre->s : nullptr;
-  // dpotrf reads the upper triangle only
-  sub_tn(c, M, M, n, dv, psi, n, spsi, n, 0.0, G, M, s_psi, nullptr, pre, 2);  // spsi == psi: one split
+  const SplitBands *pre = nullptr;
+  const double *s_psi = nullptr;
+  if (split) {
+    // psi split once for G's TN and the first rotation (psi's own, next)
+    pre = split_psi(c, "cholqr.psi", psi, M, norms, spl);
+    s_psi = pre ? pre->s : nullptr;
+    // dpotrf reads the upper triangle only
+    sub_tn(c, M, M, n, dv, psi, n, spsi, n, 0.0, G, M, s_psi, nullptr, pre, 2);  // spsi == psi: one split
+  } else {
+    dgemm(rocblas_operation_transpose, rocblas_operation_none, M, M, (int)n, dv, psi, (int)n, spsi,
+          (int)n, zero, G, M);
+  }
   int *info = (int *)scratch("cholqr.info", sizeof(int));
   {
     Timed t("potrf", (double)M * M * M / 3.0);
@@ -810,7 +837,7 @@ void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi,
     BLASCK(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                          rocblas_diagonal_non_unit, (int)rows, M, &one, G, M, X, (int)ldx));
   };
-  if (mixed_for(n)) {
+  if (split) {
     // R^-1 in FP64 (upper triangular), then X <- X R^-1 as a split-FP16 product
     double *Ri = c.buf("cholqr.Ri", (size_t)M * M);
     HIPCK(hipMemcpy2DAsync(Ri, sizeof(double) * M, G, sizeof(double) * M, sizeof(double) * M, M,
@@ -1089,7 +1116,6 @@ void kscg_sequential(Ctx &c, const KscgArgs &a) {
 void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
   const long n = c.g.n(), n1 = (long)c.pl.nprjmx * c.pl.num_ppcell;
   const int M = a.neigmx;
-  rocblas_handle h = lib().blas;
   const double dv = c.g.dv(), zero = 0.0, one = 1.0, mone = -1.0;
   if (ortho_first)
     for (int loop = 0; loop < a.northo; loop++) cholqr(c, M, a.psi, a.spsi, a.rsp, nullptr);
@@ -1300,9 +1326,12 @@ void kscg_block(Ctx &c, const KscgArgs &a, bool ortho_first, bool keep_h) {
   *a.ksitmax = std::max(*a.ksitmax, iiter);
   if (c.read_flag(1)) throw Error(2, "kscg_r: pvec has negative norm");
   Timed ph("phase_gs", 0.0);
+  // the last pass in FP64 (param final_ortho_fp64, default 1): the returned bands are
+  // orthonormal to FP64 rounding also when the subspace products are split-precision
+  const bool fp64_last = param("final_ortho_fp64", 1.0) >= 1.0;
   for (int loop = 0; loop < a.northo; loop++)  // gg: dV <psi|psi> from the last band update
     cholqr(c, M, a.psi, a.spsi, a.rsp, keep_h ? a.hpsi : nullptr,
-           fused && loop == 0 && iiter > 0 ? gg : nullptr);
+           fused && loop == 0 && iiter > 0 ? gg : nullptr, fp64_last && loop == a.northo - 1);
 }
 
 // ============================================================================= ksdiis_r
@@ -1428,25 +1457,80 @@ void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *sp
 // ============================================================================= Hartree
 // scf_hartree8f.F90:11-812: b = 2 pi rho (+ Dirichlet planes), periodic mean removed, CG on
 // -1/2 lap_nfh with the reference's avhs recurrence and stopping rule.
-int hartree(const HartreeArgs &a) {
-  Timed ph("phase_hartree", 0.0);
+// Layout of one solve's buffers: b, r, p, ap, avhs (n each), then sc (16), the stencil /
+// update partials (max(np, npu)), the update partials (npu), the dot partials
+// (dot_nparts(n)), the boundary weights (3 x 9) and the state (2 ints), each 256-byte aligned.
+namespace {
+struct HartreeBufs {
+  double *b, *r, *p, *ap, *avhs, *sc, *part, *upart, *dpart, *w;
+  int *state;
+  size_t bytes;
+};
+HartreeBufs hartree_layout(char *base, long n, int np, int npu) {
+  HartreeBufs o{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char *q = base ? base + off : nullptr;
+    off += (bytes + 255) / 256 * 256;
+    return q;
+  };
+  o.b = (double *)take(sizeof(double) * n);
+  o.r = (double *)take(sizeof(double) * n);
+  o.p = (double *)take(sizeof(double) * n);
+  o.ap = (double *)take(sizeof(double) * n);
+  o.avhs = (double *)take(sizeof(double) * n);
+  o.sc = (double *)take(sizeof(double) * 16);
+  o.part = (double *)take(sizeof(double) * std::max(np, npu));
+  o.upart = (double *)take(sizeof(double) * npu);
+  o.dpart = (double *)take(sizeof(double) * dot_nparts(n));
+  o.w = (double *)take(sizeof(double) * 27);
+  o.state = (int *)take(sizeof(int) * 2);
+  o.bytes = off;
+  return o;
+}
+int hartree_update_parts(long n) {
+  const long chunk = (long)param("hart_upd_chunk", 4096.0);
+  return (int)std::min<long>(1024, std::max<long>(1, (n + chunk - 1) / chunk));
+}
+}  // namespace
+
+size_t hartree_work_bytes(const Grid &g, int nfh) {
+  const long n = g.n();
+  return hartree_layout(nullptr, n, stencil_nparts(g, nfh, 1), hartree_update_parts(n)).bytes;
+}
+
+int hartree(const HartreeArgs &a, const HartreeWork *ws) {
+  // with a caller workspace nothing is timed (events are library resources) and nothing is
+  // allocated
+  std::unique_ptr<Timed> ph;
+  if (!ws) ph.reset(new Timed("phase_hartree", 0.0));
   hipStream_t s = lib().stream;
   Grid g{a.nx, a.ny, a.nz, a.nfh, a.nperi, 2.0 * a.xmax / a.nx, 2.0 * a.ymax / a.ny,
          2.0 * a.zmax / a.nz};
   const long n = g.n();
   const double omega = 8.0 * a.xmax * a.ymax * a.zmax;
-  double *b = (double *)scratch("hart.b", sizeof(double) * n);
-  double *r = (double *)scratch("hart.r", sizeof(double) * n);
-  double *p = (double *)scratch("hart.p", sizeof(double) * n);
-  double *ap = (double *)scratch("hart.ap", sizeof(double) * n);
-  double *avhs = (double *)scratch("hart.avhs", sizeof(double) * n);
-  double *sc = (double *)scratch("hart.sc", sizeof(double) * 16);
   const int np = stencil_nparts(g, a.nfh, 1);
   // update work-groups: 4096 points each (512 at 128^3; 8192 points each, one work-group per
   // CU, left the update latency-bound: a 128^3 solve took 33.9 ms against 29.7)
-  const long chunk = (long)param("hart_upd_chunk", 4096.0);
-  const int npu = (int)std::min<long>(1024, std::max<long>(1, (n + chunk - 1) / chunk));
-  double *part = (double *)scratch("hart.part", sizeof(double) * std::max(np, npu));
+  const int npu = hartree_update_parts(n);
+  HartreeBufs B;
+  if (ws) {
+    B = hartree_layout(ws->base, n, np, npu);
+    if (B.bytes > ws->bytes) throw Error(1, "scf_hartree: workspace too small for this grid");
+  } else {
+    B.b = (double *)scratch("hart.b", sizeof(double) * n);
+    B.r = (double *)scratch("hart.r", sizeof(double) * n);
+    B.p = (double *)scratch("hart.p", sizeof(double) * n);
+    B.ap = (double *)scratch("hart.ap", sizeof(double) * n);
+    B.avhs = (double *)scratch("hart.avhs", sizeof(double) * n);
+    B.sc = (double *)scratch("hart.sc", sizeof(double) * 16);
+    B.part = (double *)scratch("hart.part", sizeof(double) * std::max(np, npu));
+    B.upart = (double *)scratch("hart.upart", sizeof(double) * npu);
+    B.dpart = nullptr;  // launch_dots' own buffer
+    B.w = (double *)scratch("hart.w", sizeof(double) * 27);
+    B.state = (int *)scratch("hart.state", sizeof(int) * 2);
+  }
+  double *b = B.b, *r = B.r, *p = B.p, *ap = B.ap, *avhs = B.avhs, *sc = B.sc, *part = B.part;
   hipLaunchKernelGGL(k_hartree_b, dim3(nblk(n, 256)), dim3(256), 0, s, n, a.rho, b);
   double w[9];
   fd_coefficients(a.nfh, w);
@@ -1456,19 +1540,21 @@ int hartree(const HartreeArgs &a) {
     const int need[3] = {a.nperi == 0, a.nperi <= 1, a.nperi <= 2};
     for (int axis = 2; axis >= 0; axis--) {
       if (!need[axis]) continue;
+      // weights through a kernel argument copy: no host buffer has to outlive the launch
       double wc[9];
       for (int m = 0; m < 9; m++) wc[m] = w[m] / hs[axis] / hs[axis] * (-0.5);
-      double *wd = (double *)scratch("hart.w" + std::to_string(axis), sizeof(wc));
-      HIPCK(hipMemcpyAsync(wd, wc, sizeof(wc), hipMemcpyHostToDevice, s));
+      double *wd = B.w + 9 * axis;
+      hipLaunchKernelGGL(k_set9, dim3(1), dim3(9), 0, s, Nine{{wc[0], wc[1], wc[2], wc[3], wc[4],
+                         wc[5], wc[6], wc[7], wc[8]}}, wd);
       long faces = axis == 0 ? (long)a.ny * a.nz : axis == 1 ? (long)a.nx * a.nz : (long)a.nx * a.ny;
       hipLaunchKernelGGL(k_hartree_bound, dim3(nblk(faces, 256)), dim3(256), 0, s, a.nx, a.ny, a.nz,
                          a.nfh, axis, wd, bd[axis], b);
     }
   }
   if (a.nperi == 3) {
-    double *ones = (double *)scratch("hart.ones", sizeof(double) * n);
+    double *ones = ap;  // ap is free until the warm-start product below
     launch_fill(n, 1.0, ones, s);
-    launch_dots(n, b, n, ones, n, 1, g.dv() / omega, sc + 4, s);
+    launch_dots(n, b, n, ones, n, 1, g.dv() / omega, sc + 4, s, B.dpart);
     hipLaunchKernelGGL(k_sub_scalar, dim3(nblk(n, 256)), dim3(256), 0, s, n, sc + 4, 1.0, b);
   }
   // Av for the warm start
@@ -1484,10 +1570,10 @@ int hartree(const HartreeArgs &a) {
   // the block do nothing, so the result is that of the host-driven loop; the host reads the
   // state once per block.  With param "hartree_graph" (default 1) a block is captured once
   // as a HIP graph and replayed.
-  int *state = (int *)scratch("hart.state", sizeof(int) * 2);
-  double *upart = (double *)scratch("hart.upart", sizeof(double) * npu);
+  int *state = B.state;
+  double *upart = B.upart;
   hipLaunchKernelGGL(k_hcg_reset, dim3(1), dim3(1), 0, s, state);
-  launch_dots(n, r, n, r, n, 1, 1.0, sc + 0, s);  // r1r of iteration 1
+  launch_dots(n, r, n, r, n, 1, 1.0, sc + 0, s, B.dpart);  // r1r of iteration 1
   st.in = p; st.out = ap; st.dot_partial = part; st.skip = state;
   const double dvo = g.dv() / omega;
   const int blk = std::max(1, (int)param("hartree_block", 16.0));
@@ -1514,7 +1600,7 @@ int hartree(const HartreeArgs &a) {
     Key key;
   };
   static GraphCache gc;
-  const bool use_graph = param("hartree_graph", 1.0) >= 1.0;
+  const bool use_graph = !ws && param("hartree_graph", 1.0) >= 1.0;
   if (use_graph) {
     const Key key{{a.vh, b, r, p, ap, avhs, sc, part, upart, state},
                   {n, a.nx, a.ny, a.nz, a.nperi, a.nfh, blk, a.ncgres, a.ncgmin, a.ncgmax, np, npu},

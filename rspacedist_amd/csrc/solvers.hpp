@@ -84,8 +84,9 @@ void subspace_nn(long R, int K, int M2, double alpha, const double *X, long ldx,
                  bool c_upper = false);
 const double *mixed_scale(const char *tag, const double *X, long ldx, long rows, int cols);
 
+// fp64: FP64 pass also when the subspace products are split-precision
 void cholqr(Ctx &c, int M, double *psi, double *spsi, double *rsp, double *hpsi,
-            const double *norms = nullptr);
+            const double *norms = nullptr, bool fp64 = false);
 void gs_one(Ctx &c, int l, double *psi, double *spsi, double *rsp);
 void ortho_r_01(Ctx &c, int M, int l_inp, int nflag, int key_cmpt, double *psi, double *spsi,
                 double *rsp);
@@ -105,7 +106,17 @@ void ksdiis(Ctx &c, const DiisArgs &a);
 // forms only displaced bands move.
 void reorder(Ctx &c, int M, double *sval, double *resid, double *psi, double *spsi, double *rsp,
              double *hpsi, bool move_h = false, bool h_scratch = false);
-int hartree(const HartreeArgs &a);
+// Buffers of one Hartree solve carved from a caller's workspace instead of the library's
+// own allocations: 5 grid vectors and the small arrays (see hartree_work_bytes).
+struct HartreeWork {
+  char *base;
+  size_t bytes;
+};
+// bytes a Hartree solve on a grid of n points needs in a HartreeWork (0 if the grid's
+// partial-sum counts are unknown before set-up: never for the tiled stencil)
+size_t hartree_work_bytes(const Grid &g, int nfh);
+// ws: run on the caller's workspace with eager launches (no graph, no allocation)
+int hartree(const HartreeArgs &a, const HartreeWork *ws = nullptr);
 void correct_potential(long n, double dvol_over_omega, double *vh);
 void charge(long n, int M, const double *psi, const double *f, double *rho);
 // e_l = dV <psi_l|H|psi_l> (H = T + veff + nonlocal with dij)

@@ -238,6 +238,24 @@ int kukan_scf_hartree(int nd, int ncpx_d, int ncpy_d, int ncpz_d, int nfh, int n
                       double epsvh, const double *rho_aug_dense, const double *boundx,
                       const double *boundy, const double *boundz, double *vh_dense);
 
+/* No counterpart in the reference.  kukan_scf_hartree for callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same arguments, iterations and results as kukan_scf_hartree; it runs on a
+ * fixed workspace in device memory that the library holds per GPU for the life of the
+ * process (5 x 8 x KUKAN_HARTREE_NOALLOC_MAX_POINTS bytes + 1 MiB, about 41 MiB), calls no
+ * allocating routine and uses no HIP graph (eager launches on the library stream).
+ * Limits: at most KUKAN_HARTREE_NOALLOC_MAX_POINTS grid points (error 7 beyond, e.g. 96^3
+ * or 100^3 fit, 128^3 does not); rho_aug_dense, vh_dense and, when nd = 0 on a non-periodic
+ * cell, boundx/y/z must be device pointers (error 1 otherwise).  Every call on one GPU
+ * shares that GPU's workspace: calls that overlap on one GPU race.  Call kukan_init first
+ * (it sets the workspace up). */
+#define KUKAN_HARTREE_NOALLOC_MAX_POINTS 1048576
+int kukan_scf_hartree_noalloc(int nd, int ncpx_d, int ncpy_d, int ncpz_d, int nfh, int ndisp,
+                              int nperi, int ncgres, int ncgmin, int ncgmax, double xmax,
+                              double ymax, double zmax, double epsvh,
+                              const double *rho_aug_dense, const double *boundx,
+                              const double *boundy, const double *boundz, double *vh_dense);
+
 /* tools8f.F90:354-378 tools_correctpotential: vh_dense -= its cell average. */
 int kukan_tools_correctpotential(int ncpx_d, int ncpy_d, int ncpz_d, double xmax, double ymax,
                                  double zmax, double ddx, double ddy, double ddz,

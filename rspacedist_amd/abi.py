@@ -126,6 +126,12 @@ SIGNATURES = {
                     ("ncgmax", "i"), ("xmax", "d"), ("ymax", "d"), ("zmax", "d"), ("epsvh", "d"),
                     ("rho_aug_dense", "D"), ("boundx", "D"), ("boundy", "D"), ("boundz", "D"),
                     ("vh_dense", "D")],
+    # the allocation-free variant: same arguments, device pointers only
+    "scf_hartree_noalloc": [("nd", "i"), ("ncpx_d", "i"), ("ncpy_d", "i"), ("ncpz_d", "i"),
+                            ("nfh", "i"), ("ndisp", "i"), ("nperi", "i"), ("ncgres", "i"),
+                            ("ncgmin", "i"), ("ncgmax", "i"), ("xmax", "d"), ("ymax", "d"),
+                            ("zmax", "d"), ("epsvh", "d"), ("rho_aug_dense", "D"), ("boundx", "D"),
+                            ("boundy", "D"), ("boundz", "D"), ("vh_dense", "D")],
     "tools_correctpotential": [("ncpx_d", "i"), ("ncpy_d", "i"), ("ncpz_d", "i"), ("xmax", "d"),
                                ("ymax", "d"), ("zmax", "d"), ("ddx", "d"), ("ddy", "d"),
                                ("ddz", "d"), ("vh_dense", "D")],

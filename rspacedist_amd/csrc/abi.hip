@@ -63,6 +63,7 @@ int kukan_init(int device) {
     if (lib().stream && device != lib().device) throw Error(1, "kukan_init: already initialised on another device");
     lib().device = device;
     ensure_init();
+    noalloc_setup(lib().device);
     return 0;
   } catch (const std::exception &e) {
     lib().last_error = e.what();

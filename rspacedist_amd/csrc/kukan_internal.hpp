@@ -42,6 +42,9 @@ struct Lib {
 };
 Lib &lib();
 void ensure_init();
+// once per device, with that device current: the device address of the fixed workspace of
+// kukan_scf_hartree_noalloc (noalloc.hip)
+void noalloc_setup(int device);
 double param(const char *key, double dflt);
 void *scratch(const std::string &tag, size_t bytes);  // device memory, reused across calls
 void sync();

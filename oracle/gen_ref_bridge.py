@@ -16,7 +16,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 
-from rspacedist_amd.abi import SIGNATURES  # noqa: E402
+from rspacedist_amd.abi import BLOCH_SIGNATURES, SIGNATURES as _S  # noqa: E402
+
+SIGNATURES = dict(_S, **BLOCH_SIGNATURES)
 
 # routine -> (reference module, statements before the call, statements after the call)
 ROUTINES = {
@@ -43,11 +45,24 @@ ROUTINES = {
     "scf_charge_smt_r": ("mod_scf_charge", [], []),
     "force_eig_r": ("mod_force", [], []),
     "totalenergy_eig": ("mod_totalenergy", [], []),
+    # complex (Bloch) path
+    "overlap_finitedifference_c": ("mod_overlap_finitedifference",
+                                   ["call overlap_finitedifference_init(ncpa, ncpb, ncpc, nf, ncol, 1)"],
+                                   ["call overlap_fdcheck_c(nperi, ncpa, ncpb, ncpc, nf, nf1, nsnd, ncol, v_cc)",
+                                    "call overlap_finitedifference_final()"]),
+    "kslaplacian_c": ("mod_kslaplacian", ["call kslaplacian_initialize(nf)"],
+                      ["call kslaplacian_finalize()"]),
+    "ksprecondition_c": ("mod_ksprecondition", [], []),
+    "nonlocaloperation_c_01": ("mod_nonlocaloperation", [], []),
+    "nonlocaloperation_c_02": ("mod_nonlocaloperation", [], []),
+    "orthogonalization_c": ("mod_orthogonalization", [], []),
+    "orthogonalization_c_01": ("mod_orthogonalization", [], []),
+    "kscg_c": ("mod_kscg", [], []),
+    "rayleighritz_c": ("mod_rayleighritz", [], []),
+    "scf_charge_smt_c": ("mod_scf_charge", [], []),
 }
 # C int arguments that are Fortran logicals in the reference
 LOGICALS = {"ksconv", "l_hsv", "l_rrb"}
-# arrays the reference declares complex*16 (the real path passes dummies)
-COMPLEX = {"cspsep", "sveccm", "ssvcm", "hsvcm", "cspsepsoc", "svecsoc"}
 # atom-indexed reductions use mpicom_atom(natom); routines with natom get it sized first
 NATOM_ARG = "natom"
 
@@ -76,9 +91,8 @@ def wrapper(name):
         elif t == "D":
             decl.append(f"real(c_double) :: {a}(*)")
             actual.append(a)
-        elif t == "V":
-            kind = "complex(c_double_complex)" if a in COMPLEX else "real(c_double)"
-            decl.append(f"{kind} :: {a}(*)")
+        elif t == "Z":  # complex*16 in the reference
+            decl.append(f"complex(c_double_complex) :: {a}(*)")
             actual.append(a)
         elif t == "oi":
             decl.append(f"integer(c_int) :: {a}")

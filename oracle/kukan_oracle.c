@@ -31,6 +31,8 @@ static int fail(int code, const char *msg) {
   snprintf(g_err, sizeof g_err, "%s", msg);
   return code;
 }
+/* the same error slot for kukan_oracle_bloch.c */
+int orc_set_error(int code, const char *msg) { return fail(code, msg); }
 const char *orc_last_error(void) { return g_err; }
 int orc_num_threads(void) {
 #ifdef _OPENMP
@@ -1073,8 +1075,11 @@ int orc_reorderstates(int nrc, int num_ppcell, int nprjmx, int neigmx, int nums,
                       int numk, int ncpx, int ncpy, int ncpz, int ns1, int nk, double *sval,
                       double *residual_states, double *rspsep, void *cspsep, double *svecre,
                       void *sveccm, double *ssvre, void *ssvcm, double *hsvre, void *hsvcm) {
-  (void)numk; (void)cspsep; (void)sveccm; (void)ssvcm; (void)hsvcm;
-  if (nrc != 0) return fail(5, "reorderstates: complex path (nrc=1) is not part of this build");
+  (void)numk;
+  if (nrc != 0)
+    return orc_reorderstates_c(num_ppcell, nprjmx, neigmx, nums, ncol, ncpx, ncpy, ncpz, ns1, nk,
+                               sval, residual_states, (double *)cspsep, (double *)sveccm,
+                               (double *)ssvcm, (double *)hsvcm);
   const int M = neigmx, d2 = nums + 1 - ncol;
   const long n = (long)ncpx * ncpy * ncpz, n1 = (long)nprjmx * num_ppcell;
   double *sv = sval + (size_t)M * ((ns1 - 1) + (size_t)d2 * (nk - 1));
@@ -1601,11 +1606,9 @@ int orc_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, int 
                  double *svecre, void *sveccm, double *hsvre, void *hsvcm, double *ssvre,
                  void *ssvcm, double *sval, double *svalsoc, void *cspsepsoc, void *svecsoc,
                  double *residual_states, int *ksconv, int *ksitmax) {
-  (void)npxmax; (void)npymax; (void)npzmax; (void)key_soc_calc; (void)key_pp_paw;
-  (void)skpxx; (void)skpyy; (void)skpzz; (void)dijsoc; (void)natsoc; (void)lstx; (void)lsty;
-  (void)lstz; (void)natx; (void)naty; (void)natz; (void)svalsoc; (void)cspsepsoc; (void)svecsoc;
-  if (nrc != 0 || nso != 0 || ncol != 1)
-    return fail(5, "scf_diag: only the real path (nrc=0, nso=0, ncol=1) is part of this build");
+  (void)key_pp_paw; (void)svalsoc; (void)cspsepsoc; (void)svecsoc;
+  if (nso != 0 || ncol != 1)
+    return fail(5, "scf_diag: noncollinear spin and spin-orbit (nso=1 or ncol=2) are not part of this build");
   const int l_soc2nd = (nso == 1) && (ncol == 1);
   int l_rrb, l_rra, l_hsv, l_reo;
   if (nrrz != 0) {
@@ -1622,6 +1625,50 @@ int orc_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, int 
   *ksitmax = 0;
   const int d2 = nums + 1 - ncol;
   int rc = 0;
+  if (nrc != 0) { /* complex path (scf_diag8f.F90:213-312) */
+    if (!(iscf <= ncgscf || iscf % nretcg == 0))
+      return fail(5, "scf_diag: RMM-DIIS of the complex path (ksdiis_c) is not part of this build");
+    for (int nk = 1; nk <= numk && !rc; nk++) {
+      for (int ns = 1; ns <= nums + 1 - ncol && !rc; ns++) {
+        double *rs = residual_states + (size_t)neigmx * ((ns - 1) + (size_t)d2 * (nk - 1));
+        if (l_rrb)
+          rc = orc_rayleighritz_c(nso, natom, num_ppcell, num_spe, nprjmx, neigmx, numk, nums, ncol,
+                                  nspv, num_list, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, nperi, nk,
+                                  ns, nf, ndisp, key_natpri_in, key_natpri_inps, key_soc_calc, dx,
+                                  dy, dz, xmax, ymax, zmax, skpxx, skpyy, skpzz, nprj, natinf,
+                                  lstvec2, indspe, natpri, naps, natsoc, ntyppp, lstx, lsty, lstz,
+                                  natx, naty, natz, dij, dijsoc, veff, vnlocp, l_rrb,
+                                  (double *)cspsep, (double *)sveccm, (double *)ssvcm,
+                                  (double *)hsvcm, sval);
+        if (rc) break;
+        rc = orc_kscg_c(nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, northo, ns, nk, nf,
+                        nkscg, nsdmax, nprecon_cg, nprjmx, num_list, num_ppcell, ndisp, ncpx, ncpy,
+                        ncpz, npxmax, npymax, npzmax, key_ortho_nocmpt_innerproduct,
+                        key_ortho_cmpt_innerproduct, key_natpri_in, key_natpri_inps,
+                        key_natpri_out, key_soc_calc, dx, dy, dz, skpxx[nk - 1], skpyy[nk - 1],
+                        skpzz[nk - 1], epssd, indspe, natpri, naps, nprj, natinf, lstvec2, latom,
+                        natsoc, ntyppp, lstx, lsty, lstz, natx, naty, natz, dij, dijsoc, veff,
+                        vnlocp, sss, ksconv, ksitmax, sval, (double *)cspsep, (double *)sveccm,
+                        (double *)ssvcm, rs, (double *)hsvcm);
+        if (rc) break;
+        if (l_rra)
+          rc = orc_rayleighritz_c(nso, natom, num_ppcell, num_spe, nprjmx, neigmx, numk, nums, ncol,
+                                  nspv, num_list, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, nperi, nk,
+                                  ns, nf, ndisp, key_natpri_in, key_natpri_inps, key_soc_calc, dx,
+                                  dy, dz, xmax, ymax, zmax, skpxx, skpyy, skpzz, nprj, natinf,
+                                  lstvec2, indspe, natpri, naps, natsoc, ntyppp, lstx, lsty, lstz,
+                                  natx, naty, natz, dij, dijsoc, veff, vnlocp, l_rrb,
+                                  (double *)cspsep, (double *)sveccm, (double *)ssvcm,
+                                  (double *)hsvcm, sval);
+      }
+      if (!rc && l_reo)
+        for (int ns = 1; ns <= nums - ncol + 1 && !rc; ns++)
+          rc = orc_reorderstates(nrc, num_ppcell, nprjmx, neigmx, nums, ncol, numk, ncpx, ncpy, ncpz,
+                                 ns, nk, sval, residual_states, rspsep, cspsep, svecre, sveccm,
+                                 ssvre, ssvcm, hsvre, hsvcm);
+    }
+    return rc;
+  }
   for (int nk = 1; nk <= numk && !rc; nk++) {
     for (int ns = 1; ns <= nums && !rc; ns++) {
       double *rs = residual_states + (size_t)neigmx * ((ns - 1) + (size_t)d2 * (nk - 1));

@@ -155,6 +155,94 @@ int orc_totalenergy_eig(int nums, int ncol, int numk, int neigmx, const int *nws
  * largest magnitude (lowest index on ties) is positive. */
 int orc_dsyev(int n, double *a, double *w);
 
+/* complex (Bloch) path: kukan_oracle_bloch.c.  Complex arrays are complex*16, interleaved
+ * (re, im) doubles. */
+int orc_overlap_finitedifference_c(int nperi, int ncpa, int ncpb, int ncpc, int nf, int nf1,
+                                   int nsnd, int ncol, double *v_cc);
+int orc_kslaplacian_c(int ncpx, int ncpy, int ncpz, int neigmx, int ncol, int nvef, int nf,
+                      double dx, double dy, double dz, double xmax, double ymax, double zmax,
+                      double skpx, double skpy, double skpz, double *vcm, const double *veff,
+                      int l, double *avcm, double *workc);
+int orc_ksprecondition_c(int ncpx, int ncpy, int ncpz, int nf, int ncol, const double *vcm,
+                         double *avcm);
+int orc_nonlocaloperation_c_01(int natom, int nprjmx, int num_spe, int num_ppcell, int num_list,
+                               int neigmx, int ncol, int l, int ncpx, int ncpy, int ncpz,
+                               int key_natpri_in, int key_natpri_inps, double dx, double dy,
+                               double dz, double skpx, double skpy, double skpz, const int *nprj,
+                               const int *indspe, const int *natinf, const int *natpri,
+                               const int *naps, const int *lstvec2, const int *lstx,
+                               const int *lsty, const int *lstz, const int *natx, const int *naty,
+                               const int *natz, const double *vnlocp, const double *vcm2,
+                               double *cpsep, double *vcccm);
+int orc_nonlocaloperation_c_02(int nhs, int nso, int natom, int num_spe, int neigmx, int nums,
+                               int nprjmx, int ncol, int ns1, int l, int num_list, int num_ppcell,
+                               int ncpx, int ncpy, int ncpz, int npxmax, int npymax, int npzmax,
+                               int key_natpri_in, int key_natpri_inps, int key_soc_calc,
+                               double dx, double dy, double dz, double skpx, double skpy,
+                               double skpz, const int *nprj, const int *indspe, const int *natinf,
+                               const int *lstvec2, const int *natpri, const int *naps,
+                               const int *natsoc, const int *lstx, const int *lsty,
+                               const int *lstz, const int *natx, const int *naty, const int *natz,
+                               const double *vnlocp, const double *cpsep, const double *dijsss,
+                               const double *dijsoc, double *avcm, double *vcccm, double *avc);
+int orc_orthogonalization_c_01(int natom, int neigmx, int nprjmx, int nums, int ncol, int numk,
+                               int nk, int ns1, int l_inp, int num_list, int num_ppcell,
+                               int num_spe, int nflag, int ndisp, int ncpx, int ncpy, int ncpz,
+                               int npxmax, int npymax, int npzmax, int key_ortho_cmpt_innerproduct,
+                               int key_natpri_in, int key_natpri_inps, double dx, double dy,
+                               double dz, double skpx, double skpy, double skpz, const int *nprj,
+                               const int *indspe, const int *natpri, const int *naps,
+                               const int *natinf, const int *lstvec2, const int *latom,
+                               const int *lstx, const int *lsty, const int *lstz, const int *natx,
+                               const int *naty, const int *natz, const double *vnlocp,
+                               const double *sss, double *sveccm, double *ssvcm, double *cspsep);
+int orc_orthogonalization_c(int natom, int neigmx, int nprjmx, int nums, int ncol, int numk,
+                            int num_ppcell, int num_spe, int northo, int num_list, int nflag,
+                            int ndisp, int ncpx, int ncpy, int ncpz, int npxmax, int npymax,
+                            int npzmax, int key_ortho_cmpt_innerproduct, int key_natpri_in,
+                            int key_natpri_inps, double dx, double dy, double dz, const int *nprj,
+                            const int *indspe, const int *natpri, const int *naps,
+                            const int *natinf, const int *lstvec2, const int *latom,
+                            const int *ntyppp, const int *lstx, const int *lsty, const int *lstz,
+                            const int *natx, const int *naty, const int *natz,
+                            const double *skpxx, const double *skpyy, const double *skpzz,
+                            const double *vnlocp, const double *sss, double *sveccm,
+                            double *ssvcm, double *cspsep);
+int orc_kscg_c(int nso, int natom, int num_spe, int neigmx, int nums, int ncol, int nspv, int numk,
+               int nperi, int northo, int ns1, int nk, int nf, int nkscg, int nsdmax,
+               int nprecon_cg, int nprjmx, int num_list, int num_ppcell, int ndisp, int ncpx,
+               int ncpy, int ncpz, int npxmax, int npymax, int npzmax,
+               int key_ortho_nocmpt_innerproduct, int key_ortho_cmpt_innerproduct,
+               int key_natpri_in, int key_natpri_inps, int key_natpri_out, int key_soc_calc,
+               double dx, double dy, double dz, double skpx, double skpy, double skpz,
+               double epssd, const int *indspe, const int *natpri, const int *naps,
+               const int *nprj, const int *natinf, const int *lstvec2, const int *latom,
+               const int *natsoc, const int *ntyppp, const int *lstx, const int *lsty,
+               const int *lstz, const int *natx, const int *naty, const int *natz,
+               const double *dij, const double *dijsoc, const double *veff, const double *vnlocp,
+               const double *sss, int *ksconv, int *ksitmax, double *sval, double *cspsep,
+               double *sveccm, double *ssvcm, double *residual_states, double *hsvcm);
+int orc_rayleighritz_c(int nso, int natom, int num_ppcell, int num_spe, int nprjmx, int neigmx,
+                       int numk, int nums, int ncol, int nspv, int num_list, int ncpx, int ncpy,
+                       int ncpz, int npxmax, int npymax, int npzmax, int nperi, int nk, int ns1,
+                       int nf, int ndisp, int key_natpri_in, int key_natpri_inps,
+                       int key_soc_calc, double dx, double dy, double dz, double xmax,
+                       double ymax, double zmax, const double *skpxx, const double *skpyy,
+                       const double *skpzz, const int *nprj, const int *natinf,
+                       const int *lstvec2, const int *indspe, const int *natpri, const int *naps,
+                       const int *natsoc, const int *ntyppp, const int *lstx, const int *lsty,
+                       const int *lstz, const int *natx, const int *naty, const int *natz,
+                       const double *dij, const double *dijsoc, const double *veff,
+                       const double *vnlocp, int l_rrb, double *cspsep, double *sveccm,
+                       double *ssvcm, double *hsvcm, double *sval);
+int orc_reorderstates_c(int num_ppcell, int nprjmx, int neigmx, int nums, int ncol, int ncpx,
+                        int ncpy, int ncpz, int ns1, int nk, double *sval, double *residual_states,
+                        double *cspsep, double *sveccm, double *ssvcm, double *hsvcm);
+int orc_scf_charge_smt_c(int neigmx, int nums, int ncol, int nspv, int numk, int ncpx, int ncpy,
+                         int ncpz, const double *fnele, const double *sveccm, double *rhosmt);
+/* Hermitian eigensolver (cyclic Jacobi; upper triangle read), test helper */
+int orc_zheev(int n, double *a, double *w);
+
 #ifdef __cplusplus
 }
 #endif

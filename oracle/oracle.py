@@ -14,6 +14,7 @@ from rspacedist_amd import abi
 HERE = os.path.dirname(os.path.abspath(__file__))
 # routines only the oracle has: the one-system restatement of the batched band step
 SIGNATURES = dict(abi.SIGNATURES)
+SIGNATURES.update(abi.BLOCH_SIGNATURES)
 SIGNATURES["band_sd_step"] = [("n", "i"), ("h", "d"), ("nbands", "i"), ("natom", "i"),
                               ("num_list", "i"), ("lstvec2", "I"), ("natinf", "I"),
                               ("vnlocp", "D"), ("dij", "D"), ("veff", "D"), ("svecre", "D"),
@@ -34,6 +35,8 @@ def lib():
         _lib = ctypes.CDLL(LIB)
         _lib.orc_last_error.restype = ctypes.c_char_p
         _lib.orc_dsyev.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        if hasattr(_lib, "orc_zheev"):
+            _lib.orc_zheev.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         abi.bind(_lib, "orc_", SIGNATURES)
     return _lib
 
@@ -53,6 +56,18 @@ def dsyev(a):
     z = np.asfortranarray(a, dtype=np.float64).copy(order="F")
     w = np.zeros(z.shape[0])
     rc = lib().orc_dsyev(z.shape[0], z.ctypes.data, w.ctypes.data)
+    if rc:
+        raise RuntimeError(last_error())
+    return w, z
+
+
+def zheev(a):
+    """Hermitian eigen-decomposition with the oracle's cyclic Jacobi (upper triangle read;
+    ascending; each eigenvector's largest component real and positive)."""
+    import numpy as np
+    z = np.asfortranarray(a, dtype=np.complex128).copy(order="F")
+    w = np.zeros(z.shape[0])
+    rc = lib().orc_zheev(z.shape[0], z.ctypes.data, w.ctypes.data)
     if rc:
         raise RuntimeError(last_error())
     return w, z

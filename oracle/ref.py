@@ -13,6 +13,7 @@ import os
 from rspacedist_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = dict(abi.SIGNATURES, **abi.BLOCH_SIGNATURES)
 LIB = os.path.join(HERE, "_ref", "libkukan_ref.so")
 _lib = None
 
@@ -30,7 +31,7 @@ def lib():
         # and they resolve symbols of the MKL threading layer this library links
         L = ctypes.CDLL(LIB, mode=ctypes.RTLD_GLOBAL)
         L.ref_tools_definefdcoef.argtypes = [ctypes.c_int, ctypes.c_void_p]
-        abi.bind(L, "ref_")
+        abi.bind(L, "ref_", TABLE)
         if L.ref_init() != 0:
             raise RuntimeError("ref_init failed")
         _lib = L
@@ -39,7 +40,7 @@ def lib():
 
 def call(name, **kw):
     """Call the reference routine `name` with reference-named keyword arguments."""
-    return abi.call(lib(), "ref_", name, **kw)
+    return abi.call(lib(), "ref_", name, _table=TABLE, **kw)
 
 
 def definefdcoef(nf):

@@ -2,8 +2,8 @@
 
 Each entry is the reference Fortran routine's argument list in order, with a type code:
 ``i`` int32 scalar, ``d`` double scalar, ``I`` int32 array, ``D`` double array,
-``V`` opaque array (complex-path dummy), ``oi``/``od`` int/double output scalar
-(passed by pointer).  ``bind(lib, prefix)`` attaches ctypes prototypes for every routine
+``Z`` complex*16 array (complex128, optional where the real path passes a dummy),
+``oi``/``od`` int/double output scalar (passed by pointer).  ``bind(lib, prefix)`` attaches ctypes prototypes for every routine
 present in ``lib`` under ``prefix + name``; ``call`` marshals keyword arguments:
 numpy arrays (host memory, Fortran order) and torch tensors (``data_ptr()``, host or
 device) are passed as pointers, scalars by value.
@@ -96,9 +96,9 @@ SIGNATURES = {
     "reorderstates": [("nrc", "i"), ("num_ppcell", "i"), ("nprjmx", "i"), ("neigmx", "i"),
                       ("nums", "i"), ("ncol", "i"), ("numk", "i"), ("ncpx", "i"), ("ncpy", "i"),
                       ("ncpz", "i"), ("ns1", "i"), ("nk", "i"), ("sval", "D"),
-                      ("residual_states", "D"), ("rspsep", "D"), ("cspsep", "V"),
-                      ("svecre", "D"), ("sveccm", "V"), ("ssvre", "D"), ("ssvcm", "V"),
-                      ("hsvre", "D"), ("hsvcm", "V")],
+                      ("residual_states", "D"), ("rspsep", "D"), ("cspsep", "Z"),
+                      ("svecre", "D"), ("sveccm", "Z"), ("ssvre", "D"), ("ssvcm", "Z"),
+                      ("hsvre", "D"), ("hsvcm", "Z")],
     "scf_diag": [("nrc", "i"), ("nso", "i"), ("nprjmx", "i"), ("num_ppcell", "i"),
                  ("num_spe", "i"), ("iscf", "i"), ("ncgscf", "i"), ("nretcg", "i"),
                  ("natom", "i"), ("neigmx", "i"), ("nums", "i"), ("ncol", "i"), ("nspv", "i"),
@@ -116,10 +116,10 @@ SIGNATURES = {
                  ("alambda_min", "d"), ("alambda_max", "d"), ("indspe", "I"), ("natpri", "I"),
                  ("naps", "I"), ("nprj", "I"), ("natinf", "I"), ("lstvec2", "I"), ("latom", "I"),
                  ("natsoc", "I"), ("ntyppp", "I"), ("lstx", "I"), ("lsty", "I"), ("lstz", "I"),
-                 ("natx", "I"), ("naty", "I"), ("natz", "I"), ("rspsep", "D"), ("cspsep", "V"),
-                 ("svecre", "D"), ("sveccm", "V"), ("hsvre", "D"), ("hsvcm", "V"),
-                 ("ssvre", "D"), ("ssvcm", "V"), ("sval", "D"), ("svalsoc", "D"),
-                 ("cspsepsoc", "V"), ("svecsoc", "V"), ("residual_states", "D"),
+                 ("natx", "I"), ("naty", "I"), ("natz", "I"), ("rspsep", "D"), ("cspsep", "Z"),
+                 ("svecre", "D"), ("sveccm", "Z"), ("hsvre", "D"), ("hsvcm", "Z"),
+                 ("ssvre", "D"), ("ssvcm", "Z"), ("sval", "D"), ("svalsoc", "D"),
+                 ("cspsepsoc", "Z"), ("svecsoc", "Z"), ("residual_states", "D"),
                  ("ksconv", "oi"), ("ksitmax", "oi")],
     "scf_hartree": [("nd", "i"), ("ncpx_d", "i"), ("ncpy_d", "i"), ("ncpz_d", "i"), ("nfh", "i"),
                     ("ndisp", "i"), ("nperi", "i"), ("ncgres", "i"), ("ncgmin", "i"),
@@ -167,8 +167,103 @@ SIGNATURES = {
                         ("fnele", "D"), ("eneel", "od"), ("eneth", "od")],
 }
 
+# complex (Bloch) path: nrc = 1, collinear spin, no spin-orbit (the reference argument
+# lists; Z = complex*16)
+BLOCH_SIGNATURES = {
+    "overlap_finitedifference_c": [("nperi", "i"), ("ncpa", "i"), ("ncpb", "i"), ("ncpc", "i"),
+                                   ("nf", "i"), ("nf1", "i"), ("nsnd", "i"), ("ncol", "i"),
+                                   ("v_cc", "Z")],
+    "kslaplacian_c": [("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("neigmx", "i"), ("ncol", "i"),
+                      ("nvef", "i"), ("nf", "i"), ("dx", "d"), ("dy", "d"), ("dz", "d"),
+                      ("xmax", "d"), ("ymax", "d"), ("zmax", "d"), ("skpx", "d"), ("skpy", "d"),
+                      ("skpz", "d"), ("vcm", "Z"), ("veff", "D"), ("l", "i"), ("avcm", "Z"),
+                      ("workc", "Z")],
+    "ksprecondition_c": [("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("nf", "i"), ("ncol", "i"),
+                         ("vcm", "Z"), ("avcm", "Z")],
+    "nonlocaloperation_c_01": [("natom", "i"), ("nprjmx", "i"), ("num_spe", "i"),
+                               ("num_ppcell", "i"), ("num_list", "i"), ("neigmx", "i"),
+                               ("ncol", "i"), ("l", "i"), ("ncpx", "i"), ("ncpy", "i"),
+                               ("ncpz", "i"), ("key_natpri_in", "i"), ("key_natpri_inps", "i"),
+                               ("dx", "d"), ("dy", "d"), ("dz", "d"), ("skpx", "d"), ("skpy", "d"),
+                               ("skpz", "d"), ("nprj", "I"), ("indspe", "I"), ("natinf", "I"),
+                               ("natpri", "I"), ("naps", "I"), ("lstvec2", "I"), ("lstx", "I"),
+                               ("lsty", "I"), ("lstz", "I"), ("natx", "I"), ("naty", "I"),
+                               ("natz", "I"), ("vnlocp", "D"), ("vcm2", "Z"), ("cpsep", "Z"),
+                               ("vcccm", "Z")],
+    "nonlocaloperation_c_02": [("nhs", "i"), ("nso", "i"), ("natom", "i"), ("num_spe", "i"),
+                               ("neigmx", "i"), ("nums", "i"), ("nprjmx", "i"), ("ncol", "i"),
+                               ("ns1", "i"), ("l", "i"), ("num_list", "i"), ("num_ppcell", "i"),
+                               ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("npxmax", "i"),
+                               ("npymax", "i"), ("npzmax", "i"), ("key_natpri_in", "i"),
+                               ("key_natpri_inps", "i"), ("key_soc_calc", "i"), ("dx", "d"),
+                               ("dy", "d"), ("dz", "d"), ("skpx", "d"), ("skpy", "d"),
+                               ("skpz", "d"), ("nprj", "I"), ("indspe", "I"), ("natinf", "I"),
+                               ("lstvec2", "I"), ("natpri", "I"), ("naps", "I"), ("natsoc", "I"),
+                               ("lstx", "I"), ("lsty", "I"), ("lstz", "I"), ("natx", "I"),
+                               ("naty", "I"), ("natz", "I"), ("vnlocp", "D"), ("cpsep", "Z"),
+                               ("dijsss", "D"), ("dijsoc", "D"), ("avcm", "Z"), ("vcccm", "Z"),
+                               ("avc", "Z")],
+    "orthogonalization_c_01": [("natom", "i"), ("neigmx", "i"), ("nprjmx", "i"), ("nums", "i"),
+                               ("ncol", "i"), ("numk", "i"), ("nk", "i"), ("ns1", "i"),
+                               ("l_inp", "i"), ("num_list", "i"), ("num_ppcell", "i"),
+                               ("num_spe", "i"), ("nflag", "i"), ("ndisp", "i"), ("ncpx", "i"),
+                               ("ncpy", "i"), ("ncpz", "i"), ("npxmax", "i"), ("npymax", "i"),
+                               ("npzmax", "i"), ("key_ortho_cmpt_innerproduct", "i"),
+                               ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("dx", "d"),
+                               ("dy", "d"), ("dz", "d"), ("skpx", "d"), ("skpy", "d"),
+                               ("skpz", "d"), ("nprj", "I"), ("indspe", "I"), ("natpri", "I"),
+                               ("naps", "I"), ("natinf", "I"), ("lstvec2", "I"), ("latom", "I"),
+                               ("lstx", "I"), ("lsty", "I"), ("lstz", "I"), ("natx", "I"),
+                               ("naty", "I"), ("natz", "I"), ("vnlocp", "D"), ("sss", "D"),
+                               ("sveccm", "Z"), ("ssvcm", "Z"), ("cspsep", "Z")],
+    "orthogonalization_c": [("natom", "i"), ("neigmx", "i"), ("nprjmx", "i"), ("nums", "i"),
+                            ("ncol", "i"), ("numk", "i"), ("num_ppcell", "i"), ("num_spe", "i"),
+                            ("northo", "i"), ("num_list", "i"), ("nflag", "i"), ("ndisp", "i"),
+                            ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("npxmax", "i"),
+                            ("npymax", "i"), ("npzmax", "i"), ("key_ortho_cmpt_innerproduct", "i"),
+                            ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("dx", "d"),
+                            ("dy", "d"), ("dz", "d"), ("nprj", "I"), ("indspe", "I"),
+                            ("natpri", "I"), ("naps", "I"), ("natinf", "I"), ("lstvec2", "I"),
+                            ("latom", "I"), ("ntyppp", "I"), ("lstx", "I"), ("lsty", "I"),
+                            ("lstz", "I"), ("natx", "I"), ("naty", "I"), ("natz", "I"),
+                            ("skpxx", "D"), ("skpyy", "D"), ("skpzz", "D"), ("vnlocp", "D"),
+                            ("sss", "D"), ("sveccm", "Z"), ("ssvcm", "Z"), ("cspsep", "Z")],
+    "kscg_c": [("nso", "i"), ("natom", "i"), ("num_spe", "i"), ("neigmx", "i"), ("nums", "i"),
+               ("ncol", "i"), ("nspv", "i"), ("numk", "i"), ("nperi", "i"), ("northo", "i"),
+               ("ns1", "i"), ("nk", "i"), ("nf", "i"), ("nkscg", "i"), ("nsdmax", "i"),
+               ("nprecon_cg", "i"), ("nprjmx", "i"), ("num_list", "i"), ("num_ppcell", "i"),
+               ("ndisp", "i"), ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("npxmax", "i"),
+               ("npymax", "i"), ("npzmax", "i"), ("key_ortho_nocmpt_innerproduct", "i"),
+               ("key_ortho_cmpt_innerproduct", "i"), ("key_natpri_in", "i"),
+               ("key_natpri_inps", "i"), ("key_natpri_out", "i"), ("key_soc_calc", "i"),
+               ("dx", "d"), ("dy", "d"), ("dz", "d"), ("skpx", "d"), ("skpy", "d"), ("skpz", "d"),
+               ("epssd", "d"), ("indspe", "I"), ("natpri", "I"), ("naps", "I"), ("nprj", "I"),
+               ("natinf", "I"), ("lstvec2", "I"), ("latom", "I"), ("natsoc", "I"), ("ntyppp", "I"),
+               ("lstx", "I"), ("lsty", "I"), ("lstz", "I"), ("natx", "I"), ("naty", "I"),
+               ("natz", "I"), ("dij", "D"), ("dijsoc", "D"), ("veff", "D"), ("vnlocp", "D"),
+               ("sss", "D"), ("ksconv", "oi"), ("ksitmax", "oi"), ("sval", "D"), ("cspsep", "Z"),
+               ("sveccm", "Z"), ("ssvcm", "Z"), ("residual_states", "D"), ("hsvcm", "Z")],
+    "rayleighritz_c": [("nso", "i"), ("natom", "i"), ("num_ppcell", "i"), ("num_spe", "i"),
+                       ("nprjmx", "i"), ("neigmx", "i"), ("numk", "i"), ("nums", "i"),
+                       ("ncol", "i"), ("nspv", "i"), ("num_list", "i"), ("ncpx", "i"),
+                       ("ncpy", "i"), ("ncpz", "i"), ("npxmax", "i"), ("npymax", "i"),
+                       ("npzmax", "i"), ("nperi", "i"), ("nk", "i"), ("ns1", "i"), ("nf", "i"),
+                       ("ndisp", "i"), ("key_natpri_in", "i"), ("key_natpri_inps", "i"),
+                       ("key_soc_calc", "i"), ("dx", "d"), ("dy", "d"), ("dz", "d"), ("xmax", "d"),
+                       ("ymax", "d"), ("zmax", "d"), ("skpxx", "D"), ("skpyy", "D"),
+                       ("skpzz", "D"), ("nprj", "I"), ("natinf", "I"), ("lstvec2", "I"),
+                       ("indspe", "I"), ("natpri", "I"), ("naps", "I"), ("natsoc", "I"),
+                       ("ntyppp", "I"), ("lstx", "I"), ("lsty", "I"), ("lstz", "I"), ("natx", "I"),
+                       ("naty", "I"), ("natz", "I"), ("dij", "D"), ("dijsoc", "D"), ("veff", "D"),
+                       ("vnlocp", "D"), ("l_rrb", "i"), ("cspsep", "Z"), ("sveccm", "Z"),
+                       ("ssvcm", "Z"), ("hsvcm", "Z"), ("sval", "D")],
+    "scf_charge_smt_c": [("neigmx", "i"), ("nums", "i"), ("ncol", "i"), ("nspv", "i"),
+                         ("numk", "i"), ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"),
+                         ("fnele", "D"), ("sveccm", "Z"), ("rhosmt", "D")],
+}
+
 _CT = {"i": ctypes.c_int, "d": ctypes.c_double, "I": ctypes.c_void_p, "D": ctypes.c_void_p,
-       "V": ctypes.c_void_p, "oi": ctypes.POINTER(ctypes.c_int),
+       "Z": ctypes.c_void_p, "oi": ctypes.POINTER(ctypes.c_int),
        "od": ctypes.POINTER(ctypes.c_double)}
 
 
@@ -187,8 +282,8 @@ def _ptr(x, code, name):
     if x is None:
         return None
     if isinstance(x, np.ndarray):
-        want = np.int32 if code == "I" else np.float64
-        if code in ("I", "D") and x.dtype != want:
+        want = {"I": np.int32, "D": np.float64, "Z": np.complex128}.get(code)
+        if want is not None and x.dtype != want:
             raise TypeError(f"argument {name}: dtype {x.dtype}, expected {want}")
         if not (x.flags.f_contiguous or x.flags.c_contiguous):
             raise TypeError(f"argument {name}: array must be contiguous")
@@ -225,7 +320,7 @@ def call(lib, prefix, name, err=None, _table=None, **kw):
             outs[an] = box
             cargs.append(ctypes.byref(box))
         else:
-            if an not in kw and t != "V":
+            if an not in kw and t != "Z":
                 raise TypeError(f"{name}: missing array {an}")
             cargs.append(_ptr(kw.get(an), t, an))
     rc = fn(*cargs)

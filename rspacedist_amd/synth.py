@@ -384,3 +384,25 @@ def smooth_bands(psi, n, h, width=None):
         f = np.fft.fftn(psi[:, j].reshape((n, n, n), order="F"))
         out[:, j] = np.real(np.fft.ifftn(f * g)).ravel(order="F")
     return out
+
+
+def random_bands_c(sys: System, seed=None, nbands=None, smooth=True):
+    """Seeded complex N(0,1) bands (re and im independent), shape (n^3, M) Fortran order,
+    low-pass filtered like smooth_bands unless smooth is False (not yet orthonormal)."""
+    rng = np.random.default_rng(sys.seed + 7927 if seed is None else seed)
+    m = sys.nbands if nbands is None else nbands
+    re = rng.standard_normal((sys.npts, m))
+    im = rng.standard_normal((sys.npts, m))
+    if smooth:
+        re = smooth_bands(np.asfortranarray(re), sys.n, sys.h)
+        im = smooth_bands(np.asfortranarray(im), sys.n, sys.h)
+    return np.asfortranarray(re + 1j * im)
+
+
+def bloch_args(s: System, k, **extra):
+    """abi_args for the complex (Bloch) path at one k-point k = (kx, ky, kz) (1/bohr): the
+    scalar skpx/y/z of the band routines and the per-k-point arrays skpxx/yy/zz (numk = 1)."""
+    kx, ky, kz = (float(v) for v in k)
+    return abi_args(s, nrc=1, skpx=kx, skpy=ky, skpz=kz, skpxx=np.array([kx]),
+                    skpyy=np.array([ky]), skpzz=np.array([kz]), ns1=1, nso=0, ncol=1,
+                    natsoc=np.zeros(s.natom, dtype=np.int32), dijsoc=np.zeros(1), **extra)

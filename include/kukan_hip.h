@@ -200,14 +200,18 @@ int kukan_rayleighritz_r(int natom, int num_ppcell, int num_spe, int nprjmx, int
 int kukan_rayleighritz_blacs_ini(int neigmx, int nrc);
 int kukan_rayleighritz_blacs_fin(void);
 
-/* reorderstates8f.F90:8-124 reorderstates (real path nrc = 0): stable rank sort by sval. */
+/* reorderstates8f.F90:8-124 reorderstates: stable rank sort by sval of the real (nrc = 0:
+ * rspsep, svecre, ssvre, hsvre) or complex (nrc = 1: cspsep, sveccm, ssvcm, hsvcm) arrays;
+ * the other set is not read and may be null. */
 int kukan_reorderstates(int nrc, int num_ppcell, int nprjmx, int neigmx, int nums, int ncol,
                         int numk, int ncpx, int ncpy, int ncpz, int ns1, int nk, double *sval,
                         double *residual_states, double *rspsep, void *cspsep, double *svecre,
                         void *sveccm, double *ssvre, void *ssvcm, double *hsvre, void *hsvcm);
 
-/* scf_diag8f.F90:6-316 scf_diag, real path (nrc = 0, nso = 0, ncol = 1): optional RR
- * before (nrrz > 0), CG or DIIS, optional RR after (nrrz < 0), optional reorder. */
+/* scf_diag8f.F90:6-316 scf_diag (nso = 0, ncol = 1): optional RR before (nrrz > 0), CG or
+ * DIIS, optional RR after (nrrz < 0), optional reorder.  nrc = 0 works on the real arrays;
+ * nrc = 1 (scf_diag8f.F90:213-312) on the complex ones at the k-points skpxx/yy/zz(numk),
+ * with the complex CG only (the DIIS step of that branch returns error 5). */
 int kukan_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, int iscf,
                    int ncgscf, int nretcg, int natom, int neigmx, int nums, int ncol, int nspv,
                    int numk, int nperi, int northo, int nf, int ndisp, int num_list, int ncpx,
@@ -328,6 +332,122 @@ int kukan_force_eig_r(int nperi, int numk, int nums, int neigmx, int nf, int ncp
 int kukan_totalenergy_eig(int nums, int ncol, int numk, int neigmx, const int *nwskk,
                           int nwskptot, double tf, const double *sval, const double *fnele,
                           double *eneel, double *eneth);
+
+/* ===================================================================== complex (Bloch) path
+ * nrc = 1, collinear spin (ncol = 1), no spin-orbit (nso = 0; other values return error 5).
+ * Complex arrays are complex*16 (re, im doubles) in the reference's layouts, passed as void *. */
+
+/* overlap_finitedifference8f.F90:327-645 overlap_finitedifference_c: periodic halo fill of
+ * ncol complex components on one rank. */
+int kukan_overlap_finitedifference_c(int nperi, int ncpa, int ncpb, int ncpc, int nf, int nf1,
+                                     int nsnd, int ncol, void *v_cc);
+
+/* kslaplacian8f.F90:560-657 kslaplacian_c: vcm (halo'd) *= e^{ik.r} in place, workc =
+ * (-1/2 lap_nf + veff) vcm, avcm(:,l) += e^{-ik.r} workc. */
+int kukan_kslaplacian_c(int ncpx, int ncpy, int ncpz, int neigmx, int ncol, int nvef, int nf,
+                        double dx, double dy, double dz, double xmax, double ymax, double zmax,
+                        double skpx, double skpy, double skpz, void *vcm, const double *veff,
+                        int l, void *avcm, void *workc);
+
+/* ksprecondition8f.F90:42-73 ksprecondition_c: avcm = 1/2 vcm + 1/12 sum of 6 neighbours. */
+int kukan_ksprecondition_c(int ncpx, int ncpy, int ncpz, int nf, int ncol, const void *vcm,
+                           void *avcm);
+
+/* nonlocaloperation8f.F90:56-146 nonlocaloperation_c_01: cpsep(j, slot) +=
+ * sum_i vnlocp(i,j,slot) e^{ik.x_i} vcm2(lstvec2(i,slot), l).  vcccm is not written. */
+int kukan_nonlocaloperation_c_01(int natom, int nprjmx, int num_spe, int num_ppcell, int num_list,
+                                 int neigmx, int ncol, int l, int ncpx, int ncpy, int ncpz,
+                                 int key_natpri_in, int key_natpri_inps, double dx, double dy,
+                                 double dz, double skpx, double skpy, double skpz, const int *nprj,
+                                 const int *indspe, const int *natinf, const int *natpri,
+                                 const int *naps, const int *lstvec2, const int *lstx,
+                                 const int *lsty, const int *lstz, const int *natx,
+                                 const int *naty, const int *natz, const double *vnlocp,
+                                 const void *vcm2, void *cpsep, void *vcccm);
+
+/* nonlocaloperation8f.F90:238-393 nonlocaloperation_c_02: avcm(:,l) += conj-phased expansion
+ * of (dijsss cpsep / dV) (nhs = 1: dij of spin ns1; nhs = 0: sss).  vcccm, avc not written. */
+int kukan_nonlocaloperation_c_02(int nhs, int nso, int natom, int num_spe, int neigmx, int nums,
+                                 int nprjmx, int ncol, int ns1, int l, int num_list,
+                                 int num_ppcell, int ncpx, int ncpy, int ncpz, int npxmax,
+                                 int npymax, int npzmax, int key_natpri_in, int key_natpri_inps,
+                                 int key_soc_calc, double dx, double dy, double dz, double skpx,
+                                 double skpy, double skpz, const int *nprj, const int *indspe,
+                                 const int *natinf, const int *lstvec2, const int *natpri,
+                                 const int *naps, const int *natsoc, const int *lstx,
+                                 const int *lsty, const int *lstz, const int *natx,
+                                 const int *naty, const int *natz, const double *vnlocp,
+                                 const void *cpsep, const double *dijsss, const double *dijsoc,
+                                 void *avcm, void *vcccm, void *avc);
+
+/* orthogonalization8f.F90:379-724 orthogonalization_c_01: S-metric Gram-Schmidt of bands
+ * (all: l_inp = 0, as a Cholesky QR; or band l_inp against 1..l_inp-1). */
+int kukan_orthogonalization_c_01(int natom, int neigmx, int nprjmx, int nums, int ncol, int numk,
+                                 int nk, int ns1, int l_inp, int num_list, int num_ppcell,
+                                 int num_spe, int nflag, int ndisp, int ncpx, int ncpy, int ncpz,
+                                 int npxmax, int npymax, int npzmax,
+                                 int key_ortho_cmpt_innerproduct, int key_natpri_in,
+                                 int key_natpri_inps, double dx, double dy, double dz,
+                                 double skpx, double skpy, double skpz, const int *nprj,
+                                 const int *indspe, const int *natpri, const int *naps,
+                                 const int *natinf, const int *lstvec2, const int *latom,
+                                 const int *lstx, const int *lsty, const int *lstz,
+                                 const int *natx, const int *naty, const int *natz,
+                                 const double *vnlocp, const double *sss, void *sveccm,
+                                 void *ssvcm, void *cspsep);
+
+/* orthogonalization8f.F90:47-90 orthogonalization_c: northo passes of _c_01 over every
+ * k-point and spin. */
+int kukan_orthogonalization_c(int natom, int neigmx, int nprjmx, int nums, int ncol, int numk,
+                              int num_ppcell, int num_spe, int northo, int num_list, int nflag,
+                              int ndisp, int ncpx, int ncpy, int ncpz, int npxmax, int npymax,
+                              int npzmax, int key_ortho_cmpt_innerproduct, int key_natpri_in,
+                              int key_natpri_inps, double dx, double dy, double dz,
+                              const int *nprj, const int *indspe, const int *natpri,
+                              const int *naps, const int *natinf, const int *lstvec2,
+                              const int *latom, const int *ntyppp, const int *lstx,
+                              const int *lsty, const int *lstz, const int *natx, const int *naty,
+                              const int *natz, const double *skpxx, const double *skpyy,
+                              const double *skpzz, const double *vnlocp, const double *sss,
+                              void *sveccm, void *ssvcm, void *cspsep);
+
+/* kscg8f.F90:492-1032 kscg_c: band-sequential preconditioned CG of H_k at k = (skpx, skpy,
+ * skpz) for spin ns1, k-point nk. */
+int kukan_kscg_c(int nso, int natom, int num_spe, int neigmx, int nums, int ncol, int nspv,
+                 int numk, int nperi, int northo, int ns1, int nk, int nf, int nkscg, int nsdmax,
+                 int nprecon_cg, int nprjmx, int num_list, int num_ppcell, int ndisp, int ncpx,
+                 int ncpy, int ncpz, int npxmax, int npymax, int npzmax,
+                 int key_ortho_nocmpt_innerproduct, int key_ortho_cmpt_innerproduct,
+                 int key_natpri_in, int key_natpri_inps, int key_natpri_out, int key_soc_calc,
+                 double dx, double dy, double dz, double skpx, double skpy, double skpz,
+                 double epssd, const int *indspe, const int *natpri, const int *naps,
+                 const int *nprj, const int *natinf, const int *lstvec2, const int *latom,
+                 const int *natsoc, const int *ntyppp, const int *lstx, const int *lsty,
+                 const int *lstz, const int *natx, const int *naty, const int *natz,
+                 const double *dij, const double *dijsoc, const double *veff,
+                 const double *vnlocp, const double *sss, int *ksconv, int *ksitmax,
+                 double *sval, void *cspsep, void *sveccm, void *ssvcm,
+                 double *residual_states, void *hsvcm);
+
+/* rayleigh-ritz8f.F90:128-242 rayleighritz_c: subspace diagonalisation of H_k (k-point nk of
+ * skpxx/yy/zz), rotating sveccm, ssvcm and cspsep (hsvcm ends as the rotated S Psi). */
+int kukan_rayleighritz_c(int nso, int natom, int num_ppcell, int num_spe, int nprjmx, int neigmx,
+                         int numk, int nums, int ncol, int nspv, int num_list, int ncpx, int ncpy,
+                         int ncpz, int npxmax, int npymax, int npzmax, int nperi, int nk, int ns1,
+                         int nf, int ndisp, int key_natpri_in, int key_natpri_inps,
+                         int key_soc_calc, double dx, double dy, double dz, double xmax,
+                         double ymax, double zmax, const double *skpxx, const double *skpyy,
+                         const double *skpzz, const int *nprj, const int *natinf,
+                         const int *lstvec2, const int *indspe, const int *natpri,
+                         const int *naps, const int *natsoc, const int *ntyppp, const int *lstx,
+                         const int *lsty, const int *lstz, const int *natx, const int *naty,
+                         const int *natz, const double *dij, const double *dijsoc,
+                         const double *veff, const double *vnlocp, int l_rrb, void *cspsep,
+                         void *sveccm, void *ssvcm, void *hsvcm, double *sval);
+
+/* scf_charge8f.F90:324-381 scf_charge_smt_c: rhosmt(:,ns) = sum_{k,l} fnele |psi|^2. */
+int kukan_scf_charge_smt_c(int neigmx, int nums, int ncol, int nspv, int numk, int ncpx, int ncpy,
+                           int ncpz, const double *fnele, const void *sveccm, double *rhosmt);
 
 #ifdef __cplusplus
 }

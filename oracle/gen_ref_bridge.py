@@ -16,9 +16,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 
-from rspacedist_amd.abi import BLOCH_SIGNATURES, SIGNATURES as _S  # noqa: E402
+from rspacedist_amd.abi import SIGNATURES  # noqa: E402
 
-SIGNATURES = dict(_S, **BLOCH_SIGNATURES)
 
 # routine -> (reference module, statements before the call, statements after the call)
 ROUTINES = {

@@ -921,18 +921,17 @@ int orc_reorderstates_c(int num_ppcell, int nprjmx, int neigmx, int nums, int nc
  * rhosmt(:,ns) = sum_{k,l} fnele(l,ns,k) |psi|^2 */
 int orc_scf_charge_smt_c(int neigmx, int nums, int ncol, int nspv, int numk, int ncpx, int ncpy,
                          int ncpz, const double *fnele, const double *sveccm, double *rhosmt) {
-  (void)nspv;
   if (ncol != 1) return fail(5, "scf_charge_smt_c: noncollinear spin (ncol = 2) is not part of this build");
   const long n = (long)ncpx * ncpy * ncpz;
-  for (int ns = 1; ns <= nums; ns++) {
-    double *rho = rhosmt + (size_t)n * (ns - 1);
-    for (long i = 0; i < n; i++) rho[i] = 0.0;
-    for (int nk = 1; nk <= numk; nk++)
+  for (long i = 0; i < n * nspv; i++) rhosmt[i] = 0.0;
+  for (int nk = 1; nk <= numk; nk++)
+    for (int ns = 1; ns <= nums; ns++) {
+      double *rho = rhosmt + (size_t)n * (nspv > 1 ? ns - 1 : 0);
       for (int l = 1; l <= neigmx; l++) {
         const double f = fnele[(l - 1) + (size_t)neigmx * ((ns - 1) + (size_t)nums * (nk - 1))];
         const zd *p = (const zd *)sveccm + (size_t)n * ((l - 1) + (size_t)neigmx * ((ns - 1) + (size_t)nums * (nk - 1)));
         for (long i = 0; i < n; i++) rho[i] = rho[i] + f * (p[i].r * p[i].r + p[i].i * p[i].i);
       }
-  }
+    }
   return 0;
 }

@@ -14,7 +14,6 @@ from rspacedist_amd import abi
 HERE = os.path.dirname(os.path.abspath(__file__))
 # routines only the oracle has: the one-system restatement of the batched band step
 SIGNATURES = dict(abi.SIGNATURES)
-SIGNATURES.update(abi.BLOCH_SIGNATURES)
 SIGNATURES["band_sd_step"] = [("n", "i"), ("h", "d"), ("nbands", "i"), ("natom", "i"),
                               ("num_list", "i"), ("lstvec2", "I"), ("natinf", "I"),
                               ("vnlocp", "D"), ("dij", "D"), ("veff", "D"), ("svecre", "D"),

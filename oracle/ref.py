@@ -13,7 +13,7 @@ import os
 from rspacedist_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-TABLE = dict(abi.SIGNATURES, **abi.BLOCH_SIGNATURES)
+TABLE = abi.SIGNATURES
 LIB = os.path.join(HERE, "_ref", "libkukan_ref.so")
 _lib = None
 

@@ -261,6 +261,7 @@ BLOCH_SIGNATURES = {
                          ("numk", "i"), ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"),
                          ("fnele", "D"), ("sveccm", "Z"), ("rhosmt", "D")],
 }
+SIGNATURES.update(BLOCH_SIGNATURES)
 
 _CT = {"i": ctypes.c_int, "d": ctypes.c_double, "I": ctypes.c_void_p, "D": ctypes.c_void_p,
        "Z": ctypes.c_void_p, "oi": ctypes.POINTER(ctypes.c_int),

@@ -9,52 +9,14 @@
 #include <cstring>
 
 #include "kukan_hip.h"
+#include "abi_common.hpp"
 #include "kukan_internal.hpp"
 #include "solvers.hpp"
+#include "bloch.hpp"
 
 using namespace kukan;
 
-#define API_BEGIN \
-  try {           \
-    ensure_init();
-#define API_END                                  \
-  sync();                                        \
-  return 0;                                      \
-  }                                              \
-  catch (const Error &e) {                       \
-    lib().last_error = e.what();                 \
-    (void)hipStreamSynchronize(lib().stream);    \
-    return e.code;                               \
-  }                                              \
-  catch (const std::exception &e) {              \
-    lib().last_error = e.what();                 \
-    return 99;                                   \
-  }
-
-namespace {
-
-constexpr size_t D = sizeof(double), I = sizeof(int);
-
-bool sss_is_zero(const double *sss, size_t count) {
-  std::vector<double> h = host_doubles(sss, count);
-  for (double v : h)
-    if (v != 0.0) return false;
-  return true;
-}
-
-// Grid + nonlocal plan + tables for one call.
-void setup(Ctx &c, int natom, int nprjmx, int num_spe, int num_ppcell, int num_list, int nums,
-           int key_in, int key_inps, const int *nprj, const int *indspe, const int *natinf,
-           const int *natpri, const int *naps, const Arr &vnl, const Arr &lst, const double *dij,
-           const double *sss_dev, const double *sss_host) {
-  build_plan(c.pl, c.g, natom, nprjmx, num_spe, num_ppcell, num_list, nums, key_in, key_inps, nprj,
-             indspe, natinf, natpri, naps, vnl.d(), lst.i(), "plan");
-  c.dij = dij;
-  c.sss = sss_dev;
-  c.q_zero = sss_host ? sss_is_zero(sss_host, (size_t)nprjmx * nprjmx * num_spe) : true;
-}
-
-}  // namespace
+using namespace kukan::abi;
 
 extern "C" {
 
@@ -530,9 +492,13 @@ int kukan_reorderstates(int nrc, int num_ppcell, int nprjmx, int neigmx, int num
                         int numk, int ncpx, int ncpy, int ncpz, int ns1, int nk, double *sval,
                         double *residual_states, double *rspsep, void *cspsep, double *svecre,
                         void *sveccm, double *ssvre, void *ssvcm, double *hsvre, void *hsvcm) {
-  (void)cspsep; (void)sveccm; (void)ssvcm; (void)hsvcm;
   API_BEGIN
-  if (nrc != 0) throw Error(5, "reorderstates: complex path (nrc=1) is not part of this build");
+  if (nrc != 0) {
+    reorder_bloch(num_ppcell, nprjmx, neigmx, nums, ncol, numk, ncpx, ncpy, ncpz, ns1, nk, sval,
+                  residual_states, cspsep, sveccm, ssvcm, hsvcm);
+    sync();
+    return 0;
+  }
   Grid g{ncpx, ncpy, ncpz, 1, 3, 1.0, 1.0, 1.0};
   Ctx c(g);
   c.pl.nprjmx = nprjmx;
@@ -574,13 +540,45 @@ int kukan_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, in
                    void *cspsepsoc, void *svecsoc, double *residual_states, int *ksconv,
                    int *ksitmax) {
   (void)npxmax; (void)npymax; (void)npzmax; (void)key_natpri_out; (void)key_soc_calc;
-  (void)key_pp_paw; (void)skpxx; (void)skpyy; (void)skpzz; (void)dijsoc; (void)ndisp;
-  (void)ratio_diis; (void)latom; (void)natsoc; (void)ntyppp; (void)lstx; (void)lsty; (void)lstz;
-  (void)natx; (void)naty; (void)natz; (void)cspsep; (void)sveccm; (void)hsvcm; (void)ssvcm;
+  (void)key_pp_paw; (void)dijsoc; (void)ndisp; (void)ratio_diis; (void)latom; (void)natsoc;
+  (void)ntyppp;
   (void)svalsoc; (void)cspsepsoc; (void)svecsoc; (void)xmax; (void)ymax; (void)zmax;
   API_BEGIN
-  if (nrc != 0 || nso != 0 || ncol != 1)
-    throw Error(5, "scf_diag: only the real path (nrc=0, nso=0, ncol=1) is part of this build");
+  if (nso != 0 || ncol != 1)
+    throw Error(5, "scf_diag: noncollinear spin and spin-orbit (nso=1 or ncol=2) are not part of this library");
+  // scf_diag8f.F90:72-93
+  bool l_rrb, l_rra, l_hsv, l_reo;
+  const bool l_soc2nd = false;
+  if (nrrz != 0) {
+    const int az = std::abs(nrrz);
+    l_rrb = (iscf % az == 0) && nrrz > 0;
+    l_rra = (iscf % az == 0) && nrrz < 0;
+    l_hsv = l_rra || l_soc2nd || (nrrz > 0 && (iscf + 1) % nrrz == 0);
+  } else {
+    l_rrb = l_rra = false;
+    l_hsv = l_soc2nd;
+  }
+  l_reo = nchange > 0 ? ((iscf % nchange == 0) && !l_rra && !l_soc2nd) : false;
+  if (nrc != 0) {
+    BlochDiagArgs b{};
+    b.nso = nso; b.ncol = ncol; b.iscf = iscf; b.ncgscf = ncgscf; b.nretcg = nretcg;
+    b.natom = natom; b.num_spe = num_spe; b.neigmx = neigmx; b.nums = nums; b.nspv = nspv;
+    b.numk = numk; b.nperi = nperi; b.northo = northo; b.nf = nf; b.num_list = num_list;
+    b.num_ppcell = num_ppcell; b.nprjmx = nprjmx; b.ncpx = ncpx; b.ncpy = ncpy; b.ncpz = ncpz;
+    b.key_cmpt = key_ortho_cmpt_innerproduct; b.key_nocmpt = key_ortho_nocmpt_innerproduct;
+    b.key_in = key_natpri_in; b.key_inps = key_natpri_inps; b.nkscg = nkscg; b.nsdmax = nsdmax;
+    b.nprecon_cg = nprecon_cg; b.l_rrb = l_rrb; b.l_rra = l_rra; b.l_reo = l_reo;
+    b.dx = dx; b.dy = dy; b.dz = dz; b.epssd = epssd; b.skpxx = skpxx; b.skpyy = skpyy;
+    b.skpzz = skpzz; b.dij = dij; b.veff = veff; b.vnlocp = vnlocp; b.sss = sss;
+    b.indspe = indspe; b.natpri = natpri; b.naps = naps; b.nprj = nprj; b.natinf = natinf;
+    b.lstvec2 = lstvec2; b.lstx = lstx; b.lsty = lsty; b.lstz = lstz; b.natx = natx;
+    b.naty = naty; b.natz = natz; b.cspsep = cspsep; b.sveccm = sveccm; b.hsvcm = hsvcm;
+    b.ssvcm = ssvcm; b.sval = sval; b.residual_states = residual_states; b.ksconv = ksconv;
+    b.ksitmax = ksitmax;
+    scf_diag_bloch(b);
+    sync();
+    return 0;
+  }
   Grid g{ncpx, ncpy, ncpz, nf, nperi, dx, dy, dz};
   Ctx c(g);
   const long n = g.n(), n1 = (long)nprjmx * num_ppcell, nall = (long)neigmx * nums * numk;
@@ -597,19 +595,6 @@ int kukan_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, in
   Arr Re(residual_states, (size_t)nall * D, Arr::InOut);
   setup(c, natom, nprjmx, num_spe, num_ppcell, num_list, nums, key_natpri_in, key_natpri_inps,
         nprj, indspe, natinf, natpri, naps, vnl, lst, Dij.d(), Sss.d(), sss);
-  // scf_diag8f.F90:72-93
-  bool l_rrb, l_rra, l_hsv, l_reo;
-  const bool l_soc2nd = false;
-  if (nrrz != 0) {
-    const int az = std::abs(nrrz);
-    l_rrb = (iscf % az == 0) && nrrz > 0;
-    l_rra = (iscf % az == 0) && nrrz < 0;
-    l_hsv = l_rra || l_soc2nd || (nrrz > 0 && (iscf + 1) % nrrz == 0);
-  } else {
-    l_rrb = l_rra = false;
-    l_hsv = l_soc2nd;
-  }
-  l_reo = nchange > 0 ? ((iscf % nchange == 0) && !l_rra && !l_soc2nd) : false;
   *ksconv = 1;
   *ksitmax = 0;
   const bool block = param("kscg_mode", 0.0) >= 1.0;

@@ -184,3 +184,15 @@ def test_reorderstates_c_equals_reference(orc):
     o, r = both(orc, "reorderstates", arr, **synth.abi_args(s, neigmx=7, nrc=1), ns1=1)
     for k in ("sval", "residual_states", "cspsep", "sveccm", "ssvcm"):
         np.testing.assert_array_equal(o[k], r[k])
+
+
+@pytest.mark.parametrize("nspv", [1, 2])
+def test_scf_charge_smt_c_spin_mapping_equals_reference(orc, nspv):
+    """Two spins and two k-points: with nspv = 1 both spins add into one density."""
+    s = synth.make_system(6, 0.5, 1, 3, seed=3)
+    rng = np.random.default_rng(5)
+    psi = np.asfortranarray(rng.standard_normal((s.npts, 3, 2, 2)) + 1j * rng.standard_normal((s.npts, 3, 2, 2)))
+    f = np.asfortranarray(rng.random((3, 2, 2)))
+    o, r = both(orc, "scf_charge_smt_c", {"rhosmt": np.full((s.npts, nspv), 7.0, order="F")}, neigmx=3,
+                nums=2, ncol=1, nspv=nspv, numk=2, ncpx=s.n, ncpy=s.n, ncpz=s.n, fnele=f, sveccm=psi)
+    np.testing.assert_allclose(o["rhosmt"], r["rhosmt"], rtol=1e-14, atol=0)

@@ -210,8 +210,8 @@ int kukan_reorderstates(int nrc, int num_ppcell, int nprjmx, int neigmx, int num
 
 /* scf_diag8f.F90:6-316 scf_diag (nso = 0, ncol = 1): optional RR before (nrrz > 0), CG or
  * DIIS, optional RR after (nrrz < 0), optional reorder.  nrc = 0 works on the real arrays;
- * nrc = 1 (scf_diag8f.F90:213-312) on the complex ones at the k-points skpxx/yy/zz(numk),
- * with the complex CG only (the DIIS step of that branch returns error 5). */
+ * nrc = 1 (scf_diag8f.F90:213-312) on the complex ones at the k-points skpxx/yy/zz(numk)
+ * (kscg_c or ksdiis_c). */
 int kukan_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, int iscf,
                    int ncgscf, int nretcg, int natom, int neigmx, int nums, int ncol, int nspv,
                    int numk, int nperi, int northo, int nf, int ndisp, int num_list, int ncpx,
@@ -428,6 +428,26 @@ int kukan_kscg_c(int nso, int natom, int num_spe, int neigmx, int nums, int ncol
                  const double *vnlocp, const double *sss, int *ksconv, int *ksitmax,
                  double *sval, void *cspsep, void *sveccm, void *ssvcm,
                  double *residual_states, void *hsvcm);
+
+/* ksdiis8f.F90:715-1509 ksdiis_c: RMM-DIIS of H_k (two vectors per band, ndiismax - 1
+ * sweeps; with l_hsv the bands are orthonormalised and hsvcm = H psi after each sweep).  The
+ * grid spacing is 2 xmax / ncpx as in the reference.  Each new band's overall phase is chosen
+ * so that the incoming band's coefficient is real and non-negative. */
+int kukan_ksdiis_c(int l_hsv, int nso, int natom, int num_spe, int neigmx, int nums, int ncol,
+                   int nspv, int numk, int nperi, int ndiismax, int nprecon_diis, int nk, int ns1,
+                   int nf, int nprjmx, int num_list, int num_ppcell, int ndisp, int northo, int ncpx,
+                   int ncpy, int ncpz, int npxmax, int npymax, int npzmax,
+                   int key_ortho_cmpt_innerproduct, int key_natpri_in, int key_natpri_inps,
+                   int key_soc_calc, double eps_eig_diis, double alambda_diis, double ratio_diis,
+                   double alambda_min, double alambda_max, double epssd, double xmax, double ymax,
+                   double zmax, double skpx, double skpy, double skpz, const int *indspe,
+                   const int *natpri, const int *naps, const int *nprj, const int *natinf,
+                   const int *lstvec2, const int *latom, const int *natsoc, const int *ntyppp,
+                   const int *lstx, const int *lsty, const int *lstz, const int *natx,
+                   const int *naty, const int *natz, const double *dij, const double *dijsoc,
+                   const double *veff, const double *vnlocp, const double *sss, int *ksconv,
+                   int *ksitmax, void *cspsep, double *sval, void *sveccm, void *ssvcm,
+                   double *residual_states, void *hsvcm);
 
 /* rayleigh-ritz8f.F90:128-242 rayleighritz_c: subspace diagonalisation of H_k (k-point nk of
  * skpxx/yy/zz), rotating sveccm, ssvcm and cspsep (hsvcm ends as the rotated S Psi). */

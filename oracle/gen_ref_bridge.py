@@ -57,6 +57,7 @@ ROUTINES = {
     "orthogonalization_c": ("mod_orthogonalization", [], []),
     "orthogonalization_c_01": ("mod_orthogonalization", [], []),
     "kscg_c": ("mod_kscg", [], []),
+    "ksdiis_c": ("mod_ksdiis", [], []),
     "rayleighritz_c": ("mod_rayleighritz", [], []),
     "scf_charge_smt_c": ("mod_scf_charge", [], []),
 }

@@ -1626,8 +1626,7 @@ int orc_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, int 
   const int d2 = nums + 1 - ncol;
   int rc = 0;
   if (nrc != 0) { /* complex path (scf_diag8f.F90:213-312) */
-    if (!(iscf <= ncgscf || iscf % nretcg == 0))
-      return fail(5, "scf_diag: RMM-DIIS of the complex path (ksdiis_c) is not part of this build");
+    const int use_cg = iscf <= ncgscf || iscf % nretcg == 0;
     for (int nk = 1; nk <= numk && !rc; nk++) {
       for (int ns = 1; ns <= nums + 1 - ncol && !rc; ns++) {
         double *rs = residual_states + (size_t)neigmx * ((ns - 1) + (size_t)d2 * (nk - 1));
@@ -1641,15 +1640,26 @@ int orc_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, int 
                                   (double *)cspsep, (double *)sveccm, (double *)ssvcm,
                                   (double *)hsvcm, sval);
         if (rc) break;
-        rc = orc_kscg_c(nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, northo, ns, nk, nf,
-                        nkscg, nsdmax, nprecon_cg, nprjmx, num_list, num_ppcell, ndisp, ncpx, ncpy,
-                        ncpz, npxmax, npymax, npzmax, key_ortho_nocmpt_innerproduct,
-                        key_ortho_cmpt_innerproduct, key_natpri_in, key_natpri_inps,
-                        key_natpri_out, key_soc_calc, dx, dy, dz, skpxx[nk - 1], skpyy[nk - 1],
-                        skpzz[nk - 1], epssd, indspe, natpri, naps, nprj, natinf, lstvec2, latom,
-                        natsoc, ntyppp, lstx, lsty, lstz, natx, naty, natz, dij, dijsoc, veff,
-                        vnlocp, sss, ksconv, ksitmax, sval, (double *)cspsep, (double *)sveccm,
-                        (double *)ssvcm, rs, (double *)hsvcm);
+        if (use_cg)
+          rc = orc_kscg_c(nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, northo, ns, nk,
+                          nf, nkscg, nsdmax, nprecon_cg, nprjmx, num_list, num_ppcell, ndisp, ncpx,
+                          ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_nocmpt_innerproduct,
+                          key_ortho_cmpt_innerproduct, key_natpri_in, key_natpri_inps,
+                          key_natpri_out, key_soc_calc, dx, dy, dz, skpxx[nk - 1], skpyy[nk - 1],
+                          skpzz[nk - 1], epssd, indspe, natpri, naps, nprj, natinf, lstvec2, latom,
+                          natsoc, ntyppp, lstx, lsty, lstz, natx, naty, natz, dij, dijsoc, veff,
+                          vnlocp, sss, ksconv, ksitmax, sval, (double *)cspsep, (double *)sveccm,
+                          (double *)ssvcm, rs, (double *)hsvcm);
+        else /* scf_diag8f.F90:241-259 */
+          rc = orc_ksdiis_c(l_hsv, nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, ndiismax,
+                            nprecon_diis, nk, ns, nf, nprjmx, num_list, num_ppcell, ndisp, northo, ncpx,
+                            ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct,
+                            key_natpri_in, key_natpri_inps, key_soc_calc, eps_eig_diis, alambda_diis,
+                            ratio_diis, alambda_min, alambda_max, epssd, xmax, ymax, zmax,
+                            skpxx[nk - 1], skpyy[nk - 1], skpzz[nk - 1], indspe, natpri, naps, nprj,
+                            natinf, lstvec2, latom, natsoc, ntyppp, lstx, lsty, lstz, natx, naty, natz,
+                            dij, dijsoc, veff, vnlocp, sss, ksconv, ksitmax, (double *)cspsep, sval,
+                            (double *)sveccm, (double *)ssvcm, rs, (double *)hsvcm);
         if (rc) break;
         if (l_rra)
           rc = orc_rayleighritz_c(nso, natom, num_ppcell, num_spe, nprjmx, neigmx, numk, nums, ncol,

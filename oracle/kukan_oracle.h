@@ -242,6 +242,21 @@ int orc_scf_charge_smt_c(int neigmx, int nums, int ncol, int nspv, int numk, int
                          int ncpz, const double *fnele, const double *sveccm, double *rhosmt);
 /* Hermitian eigensolver (cyclic Jacobi; upper triangle read), test helper */
 int orc_zheev(int n, double *a, double *w);
+int orc_ksdiis_c(int l_hsv, int nso, int natom, int num_spe, int neigmx, int nums, int ncol,
+                 int nspv, int numk, int nperi, int ndiismax, int nprecon_diis, int nk, int ns1,
+                 int nf, int nprjmx, int num_list, int num_ppcell, int ndisp, int northo, int ncpx,
+                 int ncpy, int ncpz, int npxmax, int npymax, int npzmax,
+                 int key_ortho_cmpt_innerproduct, int key_natpri_in, int key_natpri_inps,
+                 int key_soc_calc, double eps_eig_diis, double alambda_diis, double ratio_diis,
+                 double alambda_min, double alambda_max, double epssd, double xmax, double ymax,
+                 double zmax, double skpx, double skpy, double skpz, const int *indspe,
+                 const int *natpri, const int *naps, const int *nprj, const int *natinf,
+                 const int *lstvec2, const int *latom, const int *natsoc, const int *ntyppp,
+                 const int *lstx, const int *lsty, const int *lstz, const int *natx, const int *naty,
+                 const int *natz, const double *dij, const double *dijsoc, const double *veff,
+                 const double *vnlocp, const double *sss, int *ksconv, int *ksitmax, double *cspsep,
+                 double *sval, double *sveccm, double *ssvcm, double *residual_states,
+                 double *hsvcm);
 
 #ifdef __cplusplus
 }

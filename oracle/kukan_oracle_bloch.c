@@ -935,3 +935,223 @@ int orc_scf_charge_smt_c(int neigmx, int nums, int ncol, int nspv, int numk, int
     }
   return 0;
 }
+
+/* ------------------------------------------------------------------ ksdiis8f.F90:715-1509
+ * ksdiis_c: RMM-DIIS of H_k for spin ns1, k-point nk (ncol = 1, nso = 0).  The reference
+ * works through the bands in blocks of nb0 = 64 for its MPI reductions; every operation inside
+ * a block acts on each band on its own, so the bands are taken one at a time here.  With
+ * nitermax = 2 (:738) the inner loop runs once (iiter = 1).  The DIIS combination's overall
+ * phase is the Hermitian eigensolver's free choice in the reference (zheev, :1216, :1262);
+ * here the coefficient of the incoming band is made real and non-negative. */
+int orc_ksdiis_c(int l_hsv, int nso, int natom, int num_spe, int neigmx, int nums, int ncol,
+                 int nspv, int numk, int nperi, int ndiismax, int nprecon_diis, int nk, int ns1,
+                 int nf, int nprjmx, int num_list, int num_ppcell, int ndisp, int northo, int ncpx,
+                 int ncpy, int ncpz, int npxmax, int npymax, int npzmax,
+                 int key_ortho_cmpt_innerproduct, int key_natpri_in, int key_natpri_inps,
+                 int key_soc_calc, double eps_eig_diis, double alambda_diis, double ratio_diis,
+                 double alambda_min, double alambda_max, double epssd, double xmax, double ymax,
+                 double zmax, double skpx, double skpy, double skpz, const int *indspe,
+                 const int *natpri, const int *naps, const int *nprj, const int *natinf,
+                 const int *lstvec2, const int *latom, const int *natsoc, const int *ntyppp,
+                 const int *lstx, const int *lsty, const int *lstz, const int *natx, const int *naty,
+                 const int *natz, const double *dij, const double *dijsoc, const double *veff,
+                 const double *vnlocp, const double *sss, int *ksconv, int *ksitmax, double *cspsep,
+                 double *sval, double *sveccm, double *ssvcm, double *residual_states,
+                 double *hsvcm) {
+  (void)numk; (void)ndisp; (void)npxmax; (void)npymax; (void)npzmax; (void)key_soc_calc;
+  (void)ratio_diis; (void)latom; (void)natsoc; (void)ntyppp; (void)dijsoc;
+  if (ncol != 1 || nso != 0)
+    return fail(5, "ksdiis_c: noncollinear spin and spin-orbit are not part of this build");
+  const double dx = 2.0 * xmax / ncpx, dy = 2.0 * ymax / ncpy, dz = 2.0 * zmax / ncpz;
+  zctx_t c;
+  zctx_set(&c, ncpx, ncpy, ncpz, nf, nperi, dx, dy, dz, natom, num_spe, nums, nprjmx, num_list,
+           num_ppcell, key_natpri_in, key_natpri_inps, nprj, indspe, natinf, lstvec2, natpri, naps,
+           lstx, lsty, lstz, natx, naty, natz, vnlocp, sss, skpx, skpy, skpz);
+  c.xmax = xmax; c.ymax = ymax; c.zmax = zmax;
+  c.dij = dij;
+  zctx_alloc(&c);
+  const int nsv = ns1 < nspv ? ns1 : nspv;
+  c.veff = veff + (size_t)(nsv - 1) * c.n;
+  const long n = c.n, n1 = c.n1;
+  const size_t off = (size_t)n * neigmx * ((ns1 - 1) + (size_t)nums * (nk - 1));
+  const size_t off1 = (size_t)n1 * neigmx * ((ns1 - 1) + (size_t)nums * (nk - 1));
+  zd *Psi = (zd *)sveccm + off, *SPsi = (zd *)ssvcm + off, *Cps = (zd *)cspsep + off1;
+  zd *Hps = (zd *)hsvcm;
+  double *sv = sval + (size_t)neigmx * ((ns1 - 1) + (size_t)nums * (nk - 1));
+  enum { NIT = 2 };
+  zd *phi = (zd *)zalloc(n * (NIT + 1)), *sphi = (zd *)zalloc(n * (NIT + 1));
+  zd *hphi = (zd *)zalloc(n * (NIT + 1)), *rvec = (zd *)zalloc(n * (NIT + 1));
+  zd *cphi = (zd *)zalloc(n1 * (NIT + 1));
+  zd *rv = (zd *)zalloc(n), *av = (zd *)zalloc(n), *asv = (zd *)zalloc(n), *cp = (zd *)zalloc(n1);
+#define PH(j) (phi + (size_t)n * (j))
+#define SPH(j) (sphi + (size_t)n * (j))
+#define HPH(j) (hphi + (size_t)n * (j))
+#define RV(j) (rvec + (size_t)n * (j))
+#define CPH(j) (cphi + (size_t)n1 * (j))
+  int rc = 0, jiter = 0;
+  while (jiter + 1 < ndiismax && !rc) {
+    jiter++;
+    for (int l = 1; l <= neigmx && !rc; l++) {
+      memcpy(PH(0), Psi + (size_t)n * (l - 1), sizeof(zd) * n);
+      memcpy(SPH(0), SPsi + (size_t)n * (l - 1), sizeof(zd) * n);
+      memcpy(CPH(0), Cps + (size_t)n1 * (l - 1), sizeof(zd) * n1);
+      const int iiter = 1;
+      /* eigenvalue and residual (:868-917): H phi_0 with the stored <p|phi_0> */
+      memcpy(cp, CPH(0), sizeof(zd) * n1);
+      if ((rc = zapply_h(&c, PH(0), cp, HPH(0)))) break;
+      const double valre = zdotr(n, PH(0), HPH(0)) * c.dv;
+      sv[l - 1] = valre;
+      for (long i = 0; i < n; i++) RV(0)[i] = zadd(HPH(0)[i], zscal(-valre, SPH(0)[i]));
+      /* precondition (:920-954), <p|K r>, H K r, S K r (:956-1063) */
+      if (nprecon_diis == 1) zprecond(&c, RV(0), rv);
+      else memcpy(rv, RV(0), sizeof(zd) * n);
+      zproj(&c, rv, cp);
+      if ((rc = zapply_h(&c, rv, cp, av))) break;
+      zapply_s(&c, rv, cp, asv);
+      /* step length from a trial step alambda_diis (:1065-1096) */
+      const double a1 = alambda_diis;
+      for (long i = 0; i < n1; i++) CPH(1)[i] = zadd(CPH(0)[i], zscal(a1, cp[i]));
+      for (long i = 0; i < n; i++) PH(1)[i] = zadd(PH(0)[i], zscal(a1, rv[i]));
+      for (long i = 0; i < n; i++) HPH(1)[i] = zadd(HPH(0)[i], zscal(a1, av[i]));
+      for (long i = 0; i < n; i++) SPH(1)[i] = zadd(SPH(0)[i], zscal(a1, asv[i]));
+      const double s1 = zdotr(n, rv, HPH(0)), s2 = zdotr(n, rv, SPH(0));
+      const double s3 = zdotr(n, PH(1), HPH(1)), s4 = zdotr(n, PH(1), SPH(1));
+      const double aaa = 2.0 * (s1 - sv[l - 1] * s2) * c.dv;
+      const double bbb = (s3 / s4 - sv[l - 1] - aaa * a1) / (a1 * a1);
+      double alambda = 0.5 * aaa / bbb;
+      if (alambda > alambda_max) alambda = alambda_max;
+      if (alambda < alambda_min) alambda = alambda_min;
+      /* trial phi (:1099-1116) and residuals (:1140-1149) */
+      for (long i = 0; i < n; i++) PH(iiter)[i] = zadd(PH(0)[i], zscal(alambda, rv[i]));
+      for (long i = 0; i < n; i++) HPH(iiter)[i] = zadd(HPH(0)[i], zscal(alambda, av[i]));
+      for (long i = 0; i < n; i++) SPH(iiter)[i] = zadd(SPH(0)[i], zscal(alambda, asv[i]));
+      for (long i = 0; i < n1; i++) CPH(iiter)[i] = zadd(CPH(0)[i], zscal(alambda, cp[i]));
+      for (int j = 0; j <= iiter; j++)
+        for (long i = 0; i < n; i++) RV(j)[i] = zadd(HPH(j)[i], zscal(-sv[l - 1], SPH(j)[i]));
+      /* DIIS matrices (:1154-1199): csmat(i,j) = dV <phi_i|S phi_j>, crmat(i,j) = dV <r_i|r_j> */
+      const int m = iiter + 1;
+      zd csm[(NIT + 1) * (NIT + 1)], crm[(NIT + 1) * (NIT + 1)], csm0[(NIT + 1) * (NIT + 1)];
+      zd cvect[(NIT + 1) * (NIT + 1)], csm2[(NIT + 1) * (NIT + 1)];
+      double eigen[NIT + 1], work[NIT + 1];
+#define M2(a, i, j) a[((i)-1) + m * ((j)-1)]
+      for (int j = 0; j <= iiter; j++)
+        for (int i = 0; i <= j; i++) {
+          const zd s = zscal(c.dv, zdotc(n, PH(i), SPH(j))), r = zscal(c.dv, zdotc(n, RV(i), RV(j)));
+          M2(csm, i + 1, j + 1) = s; M2(csm, j + 1, i + 1) = zconj(s);
+          M2(crm, i + 1, j + 1) = r; M2(crm, j + 1, i + 1) = zconj(r);
+        }
+      const double tmp = 1.0 / sqrt(M2(csm, m, m).r);
+      for (long i = 0; i < n; i++) {
+        PH(iiter)[i] = zscal(tmp, PH(iiter)[i]); HPH(iiter)[i] = zscal(tmp, HPH(iiter)[i]);
+        SPH(iiter)[i] = zscal(tmp, SPH(iiter)[i]); RV(iiter)[i] = zscal(tmp, RV(iiter)[i]);
+      }
+      for (long i = 0; i < n1; i++) CPH(iiter)[i] = zscal(tmp, CPH(iiter)[i]);
+      for (int i = 1; i <= m; i++) { M2(csm, m, i) = zscal(tmp, M2(csm, m, i)); M2(crm, m, i) = zscal(tmp, M2(crm, m, i)); }
+      for (int i = 1; i <= m; i++) { M2(csm, i, m) = zscal(tmp, M2(csm, i, m)); M2(crm, i, m) = zscal(tmp, M2(crm, i, m)); }
+      memcpy(csm0, csm, sizeof csm);
+      /* overlap eigenvectors, descending (:1216-1225) */
+      memcpy(csm2, csm, sizeof csm);
+      if ((rc = orc_zheev(m, (double *)csm2, eigen))) break;
+      for (int i = 1; i <= m; i++) work[i - 1] = eigen[i - 1];
+      for (int j = 1; j <= m; j++) {
+        for (int i = 1; i <= m; i++) M2(cvect, i, j) = M2(csm2, i, m + 1 - j);
+        eigen[j - 1] = work[m + 1 - j - 1];
+      }
+      /* drop overlap eigenvalues below eps_eig_diis (:1227-1240) */
+      int nmat = 0, k = 0;
+      while (k < m) {
+        nmat++;
+        k++;
+        if (eigen[nmat - 1] < eps_eig_diis) {
+          for (int i = nmat; i <= iiter - 1; i++) {
+            for (int r = 1; r <= m; r++) M2(cvect, r, i) = M2(cvect, r, i + 1);
+            eigen[i - 1] = eigen[i];
+          }
+          nmat--;
+        }
+      }
+      zd coef[NIT + 1];
+      if (nmat > 1) {
+        for (int j = 1; j <= nmat; j++)
+          for (int i = 1; i <= m; i++) M2(cvect, i, j) = zscal(1.0 / sqrt(eigen[j - 1]), M2(cvect, i, j));
+        /* projected residual matrix and its lowest eigenvector (:1242-1275) */
+        zd t1[(NIT + 1) * (NIT + 1)], cr2[(NIT + 1) * (NIT + 1)], cv2[(NIT + 1) * (NIT + 1)];
+        for (int kk = 1; kk <= m; kk++)
+          for (int j = 1; j <= nmat; j++) {
+            zd s = {0.0, 0.0};
+            for (int i = 1; i <= m; i++) s = zadd(s, zmul(M2(crm, kk, i), M2(cvect, i, j)));
+            M2(t1, kk, j) = s;
+          }
+        for (int kk = 1; kk <= nmat; kk++)
+          for (int j = 1; j <= nmat; j++) {
+            zd s = {0.0, 0.0};
+            for (int i = 1; i <= m; i++) s = zadd(s, zmul(zconj(M2(cvect, i, kk)), M2(t1, i, j)));
+            cr2[(kk - 1) + nmat * (j - 1)] = s;
+          }
+        if ((rc = orc_zheev(nmat, (double *)cr2, eigen))) break;
+        for (int i = 1; i <= nmat; i++) work[i - 1] = eigen[i - 1];
+        for (int j = 1; j <= nmat; j++) {
+          for (int i = 1; i <= nmat; i++) cv2[(i - 1) + nmat * (j - 1)] = cr2[(i - 1) + nmat * (nmat - j)];
+          eigen[j - 1] = work[nmat - j];
+        }
+        residual_states[l - 1] = eigen[nmat - 1];
+        /* the combination and its S-normalisation (:1293-1310) */
+        for (int j = 1; j <= m; j++) {
+          zd s = {0.0, 0.0};
+          for (int i = 1; i <= nmat; i++) s = zadd(s, zmul(M2(cvect, j, i), cv2[(i - 1) + nmat * (nmat - 1)]));
+          coef[j - 1] = s;
+        }
+        double t2 = 0.0;
+        for (int j = 1; j <= m; j++)
+          for (int i = 1; i <= m; i++) t2 += zmul(zmul(zconj(coef[i - 1]), M2(csm0, i, j)), coef[j - 1]).r;
+        for (int j = 1; j <= m; j++) coef[j - 1] = zscal(1.0 / sqrt(t2), coef[j - 1]);
+      } else {
+        for (int j = 1; j <= m; j++) coef[j - 1] = (zd){0.0, 0.0};
+        coef[m - 1] = (zd){1.0, 0.0};
+      }
+#undef M2
+      /* phase: the incoming band's coefficient real and non-negative */
+      const double a0 = hypot(coef[0].r, coef[0].i);
+      if (a0 > 0.0) {
+        const zd ph = {coef[0].r / a0, -coef[0].i / a0};
+        for (int j = 0; j < m; j++) coef[j] = zmul(coef[j], ph);
+        coef[0].i = 0.0;
+      }
+      /* new band (:1313-1352; ksdiis_c_03 adds the terms in pairs) */
+      zd *outs[4] = {Psi + (size_t)n * (l - 1), SPsi + (size_t)n * (l - 1), Hps + (size_t)n * (l - 1), NULL};
+      const zd *srcs[3] = {phi, sphi, hphi};
+      for (int q = 0; q < 3; q++)
+        for (long i = 0; i < n; i++) {
+          zd s = zadd(zmul(coef[0], srcs[q][i]), zmul(coef[1], srcs[q][i + n]));
+          outs[q][i] = s;
+        }
+      for (long i = 0; i < n1; i++) {
+        zd s = {0.0, 0.0};
+        for (int j = 0; j < m; j++) s = zadd(s, zmul(coef[j], CPH(j)[i]));
+        Cps[(size_t)n1 * (l - 1) + i] = s;
+      }
+      if (iiter > *ksitmax) *ksitmax = iiter;
+      *ksconv = *ksconv && (sqrt(residual_states[l - 1]) <= epssd);
+    }
+    if (rc) break;
+    /* l_hsv: orthonormalise, then H psi of every band into hsvcm (:1416-1478) */
+    if (l_hsv) {
+      for (int i = 1; i <= northo && !rc; i++)
+        rc = zortho_c_01(&c, neigmx, 0, key_ortho_cmpt_innerproduct, key_ortho_cmpt_innerproduct,
+                         Psi, SPsi, Cps);
+      for (int l = 1; l <= neigmx && !rc; l++) {
+        memcpy(cp, Cps + (size_t)n1 * (l - 1), sizeof(zd) * n1);
+        rc = zapply_h(&c, Psi + (size_t)n * (l - 1), cp, Hps + (size_t)n * (l - 1));
+      }
+    }
+  }
+#undef PH
+#undef SPH
+#undef HPH
+#undef RV
+#undef CPH
+  free(phi); free(sphi); free(hphi); free(rvec); free(cphi);
+  free(rv); free(av); free(asv); free(cp);
+  zctx_free(&c);
+  return rc;
+}

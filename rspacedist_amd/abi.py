@@ -243,6 +243,22 @@ BLOCH_SIGNATURES = {
                ("natz", "I"), ("dij", "D"), ("dijsoc", "D"), ("veff", "D"), ("vnlocp", "D"),
                ("sss", "D"), ("ksconv", "oi"), ("ksitmax", "oi"), ("sval", "D"), ("cspsep", "Z"),
                ("sveccm", "Z"), ("ssvcm", "Z"), ("residual_states", "D"), ("hsvcm", "Z")],
+    "ksdiis_c": [("l_hsv", "i"), ("nso", "i"), ("natom", "i"), ("num_spe", "i"), ("neigmx", "i"),
+                 ("nums", "i"), ("ncol", "i"), ("nspv", "i"), ("numk", "i"), ("nperi", "i"),
+                 ("ndiismax", "i"), ("nprecon_diis", "i"), ("nk", "i"), ("ns1", "i"), ("nf", "i"),
+                 ("nprjmx", "i"), ("num_list", "i"), ("num_ppcell", "i"), ("ndisp", "i"),
+                 ("northo", "i"), ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("npxmax", "i"),
+                 ("npymax", "i"), ("npzmax", "i"), ("key_ortho_cmpt_innerproduct", "i"),
+                 ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("key_soc_calc", "i"),
+                 ("eps_eig_diis", "d"), ("alambda_diis", "d"), ("ratio_diis", "d"),
+                 ("alambda_min", "d"), ("alambda_max", "d"), ("epssd", "d"), ("xmax", "d"),
+                 ("ymax", "d"), ("zmax", "d"), ("skpx", "d"), ("skpy", "d"), ("skpz", "d"),
+                 ("indspe", "I"), ("natpri", "I"), ("naps", "I"), ("nprj", "I"), ("natinf", "I"),
+                 ("lstvec2", "I"), ("latom", "I"), ("natsoc", "I"), ("ntyppp", "I"), ("lstx", "I"),
+                 ("lsty", "I"), ("lstz", "I"), ("natx", "I"), ("naty", "I"), ("natz", "I"),
+                 ("dij", "D"), ("dijsoc", "D"), ("veff", "D"), ("vnlocp", "D"), ("sss", "D"),
+                 ("ksconv", "oi"), ("ksitmax", "oi"), ("cspsep", "Z"), ("sval", "D"),
+                 ("sveccm", "Z"), ("ssvcm", "Z"), ("residual_states", "D"), ("hsvcm", "Z")],
     "rayleighritz_c": [("nso", "i"), ("natom", "i"), ("num_ppcell", "i"), ("num_spe", "i"),
                        ("nprjmx", "i"), ("neigmx", "i"), ("numk", "i"), ("nums", "i"),
                        ("ncol", "i"), ("nspv", "i"), ("num_list", "i"), ("ncpx", "i"),

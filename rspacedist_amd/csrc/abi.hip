@@ -568,6 +568,9 @@ int kukan_scf_diag(int nrc, int nso, int nprjmx, int num_ppcell, int num_spe, in
     b.key_cmpt = key_ortho_cmpt_innerproduct; b.key_nocmpt = key_ortho_nocmpt_innerproduct;
     b.key_in = key_natpri_in; b.key_inps = key_natpri_inps; b.nkscg = nkscg; b.nsdmax = nsdmax;
     b.nprecon_cg = nprecon_cg; b.l_rrb = l_rrb; b.l_rra = l_rra; b.l_reo = l_reo;
+    b.ndiismax = ndiismax; b.nprecon_diis = nprecon_diis; b.l_hsv = l_hsv;
+    b.eps_eig_diis = eps_eig_diis; b.alambda_diis = alambda_diis; b.alambda_min = alambda_min;
+    b.alambda_max = alambda_max;
     b.dx = dx; b.dy = dy; b.dz = dz; b.epssd = epssd; b.skpxx = skpxx; b.skpyy = skpyy;
     b.skpzz = skpzz; b.dij = dij; b.veff = veff; b.vnlocp = vnlocp; b.sss = sss;
     b.indspe = indspe; b.natpri = natpri; b.naps = naps; b.nprj = nprj; b.natinf = natinf;

@@ -274,6 +274,53 @@ int kukan_kscg_c(int nso, int natom, int num_spe, int neigmx, int nums, int ncol
   API_END
 }
 
+int kukan_ksdiis_c(int l_hsv, int nso, int natom, int num_spe, int neigmx, int nums, int ncol,
+                   int nspv, int numk, int nperi, int ndiismax, int nprecon_diis, int nk, int ns1,
+                   int nf, int nprjmx, int num_list, int num_ppcell, int ndisp, int northo, int ncpx,
+                   int ncpy, int ncpz, int npxmax, int npymax, int npzmax,
+                   int key_ortho_cmpt_innerproduct, int key_natpri_in, int key_natpri_inps,
+                   int key_soc_calc, double eps_eig_diis, double alambda_diis, double ratio_diis,
+                   double alambda_min, double alambda_max, double epssd, double xmax, double ymax,
+                   double zmax, double skpx, double skpy, double skpz, const int *indspe,
+                   const int *natpri, const int *naps, const int *nprj, const int *natinf,
+                   const int *lstvec2, const int *latom, const int *natsoc, const int *ntyppp,
+                   const int *lstx, const int *lsty, const int *lstz, const int *natx,
+                   const int *naty, const int *natz, const double *dij, const double *dijsoc,
+                   const double *veff, const double *vnlocp, const double *sss, int *ksconv,
+                   int *ksitmax, void *cspsep, double *sval, void *sveccm, void *ssvcm,
+                   double *residual_states, void *hsvcm) {
+  (void)ndisp; (void)npxmax; (void)npymax; (void)npzmax; (void)key_soc_calc; (void)ratio_diis;
+  (void)latom; (void)natsoc; (void)ntyppp; (void)dijsoc;
+  API_BEGIN
+  need_collinear(ncol, nso, "ksdiis_c");
+  // the grid spacing follows from the cell, as in the reference (ksdiis8f.F90:812-814)
+  Grid g{ncpx, ncpy, ncpz, nf, nperi, 2.0 * xmax / ncpx, 2.0 * ymax / ncpy, 2.0 * zmax / ncpz};
+  ZCall zc(g, skpx, skpy, skpz);
+  const long n = g.n(), n1 = (long)nprjmx * num_ppcell, nall = (long)neigmx * nums * numk;
+  Arr Dij(dij, (size_t)nprjmx * nprjmx * nums * natom * D, Arr::In);
+  Arr Sss(sss, (size_t)nprjmx * nprjmx * num_spe * D, Arr::In);
+  Arr Ve(veff, (size_t)n * nspv * D, Arr::In);
+  Arr Cs(cspsep, (size_t)n1 * nall * Z, Arr::InOut);
+  Arr Sv(sval, (size_t)nall * D, Arr::InOut);
+  Arr Ps(sveccm, (size_t)n * nall * Z, Arr::InOut);
+  Arr Sp(ssvcm, (size_t)n * nall * Z, Arr::InOut);
+  Arr Re(residual_states, (size_t)neigmx * D, Arr::InOut);
+  Arr Hp(hsvcm, (size_t)n * neigmx * Z, Arr::InOut);
+  zc.plan(natom, nprjmx, num_spe, num_ppcell, num_list, nums, key_natpri_in, key_natpri_inps, nprj,
+          indspe, natinf, natpri, naps, vnlocp, lstvec2, lstx, lsty, lstz, natx, naty, natz,
+          Dij.d(), Sss.d(), sss);
+  zc.z.ns_h = ns1;
+  const ZDiisArgs da{neigmx, northo, ndiismax, nprecon_diis, key_ortho_cmpt_innerproduct, l_hsv != 0,
+                     eps_eig_diis, alambda_diis, alambda_min, alambda_max, epssd};
+  zksdiis_c(zc.z, da, Ve.d() + (size_t)n * (std::min(ns1, nspv) - 1),
+            (double2 *)Ps.v() + kofs(n, neigmx, nums, ns1, nk),
+            (double2 *)Sp.v() + kofs(n, neigmx, nums, ns1, nk),
+            (double2 *)Cs.v() + kofs(n1, neigmx, nums, ns1, nk), (double2 *)Hp.v(),
+            Sv.d() + kofs(1, neigmx, nums, ns1, nk), Re.d(), ksconv, ksitmax);
+  Cs.finish(); Sv.finish(); Ps.finish(); Sp.finish(); Re.finish(); Hp.finish();
+  API_END
+}
+
 int kukan_rayleighritz_c(int nso, int natom, int num_ppcell, int num_spe, int nprjmx, int neigmx,
                          int numk, int nums, int ncol, int nspv, int num_list, int ncpx, int ncpy,
                          int ncpz, int npxmax, int npymax, int npzmax, int nperi, int nk, int ns1,
@@ -343,11 +390,11 @@ int kukan_scf_charge_smt_c(int neigmx, int nums, int ncol, int nspv, int numk, i
 namespace kukan {
 
 // scf_diag's complex branch (scf_diag8f.F90:213-312): per k-point and spin, Rayleigh-Ritz
-// (l_rrb), band-sequential kscg_c, Rayleigh-Ritz (l_rra), then the reorder (l_reo).
+// (l_rrb), band-sequential kscg_c or RMM-DIIS ksdiis_c, Rayleigh-Ritz (l_rra), then the
+// reorder (l_reo).
 void scf_diag_bloch(const BlochDiagArgs &a) {
   need_collinear(a.ncol, a.nso, "scf_diag");
-  if (!(a.iscf <= a.ncgscf || a.iscf % a.nretcg == 0))
-    throw Error(5, "scf_diag: RMM-DIIS of the complex path (ksdiis_c) is not part of this library");
+  const bool use_cg = a.iscf <= a.ncgscf || a.iscf % a.nretcg == 0;
   const Grid g{a.ncpx, a.ncpy, a.ncpz, a.nf, a.nperi, a.dx, a.dy, a.dz};
   const long n = g.n(), n1 = (long)a.nprjmx * a.num_ppcell, nall = (long)a.neigmx * a.nums * a.numk;
   const std::vector<double> kx = host_doubles(a.skpxx, a.numk), ky = host_doubles(a.skpyy, a.numk),
@@ -383,8 +430,14 @@ void scf_diag_bloch(const BlochDiagArgs &a) {
         zc.c.q_zero = qz;
       };
       if (a.l_rrb) rr(true);
-      zkscg_c(zc.z, a.neigmx, a.northo, a.nkscg, a.nsdmax, a.nprecon_cg, a.key_cmpt, a.key_nocmpt,
-              a.epssd, vs, psi, spsi, cps, (double2 *)Hp.v(), sv, re, a.ksconv, a.ksitmax);
+      if (use_cg) {
+        zkscg_c(zc.z, a.neigmx, a.northo, a.nkscg, a.nsdmax, a.nprecon_cg, a.key_cmpt, a.key_nocmpt,
+                a.epssd, vs, psi, spsi, cps, (double2 *)Hp.v(), sv, re, a.ksconv, a.ksitmax);
+      } else {  // scf_diag8f.F90:241-259
+        const ZDiisArgs da{a.neigmx, a.northo, a.ndiismax, a.nprecon_diis, a.key_cmpt, a.l_hsv,
+                           a.eps_eig_diis, a.alambda_diis, a.alambda_min, a.alambda_max, a.epssd};
+        zksdiis_c(zc.z, da, vs, psi, spsi, cps, (double2 *)Hp.v(), sv, re, a.ksconv, a.ksitmax);
+      }
       if (a.l_rra) rr(false);
     }
     if (a.l_reo)

@@ -197,10 +197,13 @@ __global__ void k_zlap(ZLap L, const zt *u, long ldu, const double *veff, zt *ou
 
 // K u = 1/2 u + 1/12 sum of the six neighbours (ksprecondition8f.F90:42-75 order), wrapped or
 // zero-padded per axis
-__global__ void k_zprecond(int nx, int ny, int nz, int px, int py, int pz, const zt *u, zt *out) {
+__global__ void k_zprecond(int nx, int ny, int nz, int px, int py, int pz, const zt *u, long ldu,
+                           zt *out, long ldo) {
   const long n = (long)nx * ny * nz;
   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (p >= n) return;
+  u += ldu * blockIdx.y;
+  out += ldo * blockIdx.y;
   const int ix = (int)(p % nx), iy = (int)((p / nx) % ny), iz = (int)(p / ((long)nx * ny));
   const double w = 1.0 / 12.0;
   auto at = [&](int x, int y, int z) {
@@ -254,6 +257,16 @@ __global__ void k_zaxpby(long n, zt a, const zt *x, zt c, const zt *y, zt *out) 
 __global__ void k_zscale(long n, double s, zt *x) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i < n) x[i] = zsc(s, x[i]);
+}
+// per-band coefficients: out_b = ca[b] x_b + cb[b] y_b (y may be null); out may alias x or y
+__global__ void k_zcomb(long n, const zt *ca, const zt *x, long ldx, const zt *cb, const zt *y,
+                        long ldy, zt *out, long ldo) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (i >= n) return;
+  zt v = zmul(ca[b], x[ldx * b + i]);
+  if (y) v = zadd(v, zmul(cb[b], y[ldy * b + i]));
+  out[ldo * b + i] = v;
 }
 
 // rho += sum_b f_b |u_b|^2 (band order as scf_charge8f.F90:354-362)
@@ -447,10 +460,12 @@ void ZCtx::spsi(const zt *u, long ldu, int nb, const zt *cp, long ldcp, zt *out,
   if (!c.q_zero) expand(0, 1, cp, ldcp, nb, out, ldo);
 }
 
-void ZCtx::precond(const zt *u, zt *out) {
+void ZCtx::precond(const zt *u, zt *out, int nb, long ldu, long ldo) {
   const Grid &g = c.g;
-  hipLaunchKernelGGL(k_zprecond, dim3(nb_(n(), 256)), dim3(256), 0, c.s, g.nx, g.ny, g.nz,
-                     (int)(g.nperi >= 1), (int)(g.nperi >= 2), (int)(g.nperi >= 3), u, out);
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_zprecond, dim3(nb_(n(), 256), nb), dim3(256), 0, c.s, g.nx, g.ny, g.nz,
+                     (int)(g.nperi >= 1), (int)(g.nperi >= 2), (int)(g.nperi >= 3), u,
+                     ldu > 0 ? ldu : n(), out, ldo > 0 ? ldo : n());
   HIPCK(hipGetLastError());
 }
 
@@ -481,6 +496,22 @@ void ZCtx::axpby(long len, zc a, const zt *x, zc b, const zt *y, zt *out) {
 void ZCtx::scale(long len, double s, zt *x) {
   hipLaunchKernelGGL(k_zscale, dim3(nb_(len, 256)), dim3(256), 0, c.s, len, s, x);
   HIPCK(hipGetLastError());
+}
+
+void ZCtx::comb(long len, int nb, const zc *ca, const zt *x, long ldx, const zc *cb, const zt *y,
+                long ldy, zt *out, long ldo) {
+  if (nb <= 0 || len <= 0) return;
+  std::vector<zt> h(2 * (size_t)nb);
+  for (int b = 0; b < nb; b++) {
+    h[b] = zmk(ca[b].real(), ca[b].imag());
+    h[nb + b] = cb ? zmk(cb[b].real(), cb[b].imag()) : zmk(0.0, 0.0);
+  }
+  zt *d = zbuf("z.comb", 2 * (size_t)nb);
+  HIPCK(hipMemcpyAsync(d, h.data(), sizeof(zt) * 2 * nb, hipMemcpyHostToDevice, c.s));
+  hipLaunchKernelGGL(k_zcomb, dim3(nb_(len, 256), nb), dim3(256), 0, c.s, len, d, x, ldx, d + nb, y,
+                     ldy, out, ldo);
+  HIPCK(hipGetLastError());
+  sync();  // the host coefficients go out of scope
 }
 
 namespace {
@@ -715,6 +746,196 @@ void zkscg_c(ZCtx &z, int M, int northo, int nkscg, int nsdmax, int nprecon_cg, 
       *ksconv = *ksconv && (conv || iiter < nsdmax);
     }
     *ksitmax = std::max(*ksitmax, iiter);
+  }
+  HIPCK(hipMemcpyAsync(sval, hsv.data(), sizeof(double) * M, hipMemcpyHostToDevice, c.s));
+  HIPCK(hipMemcpyAsync(resid, hres.data(), sizeof(double) * M, hipMemcpyHostToDevice, c.s));
+  sync();
+}
+
+namespace {
+
+// eigenpairs of the Hermitian 2 x 2 [[a, b], [conj(b), d]] (one complex Jacobi rotation:
+// the phase of b is moved to the second component, then the real symmetric rotation),
+// eigenvalues in descending order, eigenvectors as columns v[0..1] (v[j][i] = component i)
+void herm2(double a, zc b, double d, double w[2], zc v[2][2]) {
+  const double ab = std::abs(b);
+  if (ab == 0.0) {
+    const bool sw = d > a;
+    w[0] = sw ? d : a;
+    w[1] = sw ? a : d;
+    v[0][0] = sw ? 0.0 : 1.0; v[0][1] = sw ? 1.0 : 0.0;
+    v[1][0] = sw ? 1.0 : 0.0; v[1][1] = sw ? 0.0 : 1.0;
+    return;
+  }
+  const zc e = std::conj(b) / ab;  // D = diag(1, conj(b)/|b|) makes the off-diagonal |b|
+  const double tau = (d - a) / (2.0 * ab);
+  const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+  const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = t * cs;
+  const double l1 = a - t * ab, l2 = d + t * ab;  // eigenvectors (cs, -sn) and (sn, cs)
+  zc v1[2] = {cs, -sn * e}, v2[2] = {sn, cs * e};
+  if (l1 >= l2) {
+    w[0] = l1; w[1] = l2;
+    v[0][0] = v1[0]; v[0][1] = v1[1]; v[1][0] = v2[0]; v[1][1] = v2[1];
+  } else {
+    w[0] = l2; w[1] = l1;
+    v[0][0] = v2[0]; v[0][1] = v2[1]; v[1][0] = v1[0]; v[1][1] = v1[1];
+  }
+}
+
+}  // namespace
+
+// ksdiis_c (ksdiis8f.F90:715-1509) for one spin / k-point.  With nitermax = 2 every sweep takes
+// one DIIS step per band from its incoming vector phi_0 and the trial phi_1 = phi_0 + lambda K r:
+// the bands of a block go through each stage together (H, K, projections and all band scalars
+// batched), the 2 x 2 subspace problems are solved on the host.  The combination's overall phase
+// (zheev's choice in the reference) is fixed by making the incoming band's coefficient real and
+// non-negative.
+void zksdiis_c(ZCtx &z, const ZDiisArgs &a, const double *veff, zt *Psi, zt *SPsi, zt *Cps, zt *H,
+               double *sval, double *resid, int *ksconv, int *ksitmax) {
+  Ctx &c = z.c;
+  const long n = z.n(), n1 = z.n1();
+  const int M = a.M;
+  const double dv = c.g.dv();
+  // bands per block: 7 complex grid vectors each, at most about 8 GiB of them
+  const long per_band = 7 * n * (long)sizeof(zt);
+  const int nbk = (int)std::max<long>(1, std::min<long>(M, (8L << 30) / std::max<long>(1, per_band)));
+  zt *PH1 = z.zbuf("zd.ph1", (size_t)n * nbk), *HPH1 = z.zbuf("zd.hph1", (size_t)n * nbk);
+  zt *SPH1 = z.zbuf("zd.sph1", (size_t)n * nbk), *R0 = z.zbuf("zd.r0", (size_t)n * nbk);
+  zt *RV = z.zbuf("zd.rv", (size_t)n * nbk), *AV = z.zbuf("zd.av", (size_t)n * nbk);
+  zt *ASV = z.zbuf("zd.asv", (size_t)n * nbk);
+  zt *CP = z.zbuf("zd.cp", (size_t)n1 * nbk), *CPH1 = z.zbuf("zd.cph1", (size_t)n1 * nbk);
+  zt *dd = z.zbuf("zd.dots", (size_t)6 * nbk);
+  std::vector<double> hsv(M), hres(M);
+  HIPCK(hipMemcpyAsync(hsv.data(), sval, sizeof(double) * M, hipMemcpyDeviceToHost, c.s));
+  HIPCK(hipMemcpyAsync(hres.data(), resid, sizeof(double) * M, hipMemcpyDeviceToHost, c.s));
+  sync();
+  std::vector<zt> hd(6 * (size_t)nbk);
+  auto fetch = [&](int cnt) {
+    HIPCK(hipMemcpyAsync(hd.data(), dd, sizeof(zt) * cnt, hipMemcpyDeviceToHost, c.s));
+    sync();
+  };
+  std::vector<zc> one(nbk, zc(1.0, 0.0)), ca(nbk), cb(nbk);
+  for (int jiter = 1; jiter < a.ndiismax; jiter++) {  // ksdiis8f.F90:819-821
+    for (int l0 = 0; l0 < M; l0 += nbk) {
+      const int nb = std::min(nbk, M - l0);
+      zt *P = Psi + n * l0, *SP = SPsi + n * l0, *C = Cps + n1 * l0, *Hh = H + n * l0;
+      // eigenvalue and residual (ksdiis8f.F90:868-917): H phi_0 with the stored <p|phi_0>
+      z.hpsi(veff, P, n, nb, C, n1, Hh, n);
+      z.dots(P, n, Hh, n, nb, dd, dv);
+      fetch(nb);
+      for (int b = 0; b < nb; b++) {
+        hsv[l0 + b] = hd[b].x;
+        cb[b] = zc(-hd[b].x, 0.0);
+      }
+      z.comb(n, nb, one.data(), Hh, n, cb.data(), SP, n, R0, n);
+      // K r, <p|K r>, H K r, S K r (:920-1063)
+      if (a.nprecon_diis == 1) z.precond(R0, RV, nb, n, n);
+      else HIPCK(hipMemcpyAsync(RV, R0, sizeof(zt) * n * nb, hipMemcpyDeviceToDevice, c.s));
+      z.proj(RV, n, nb, CP, n1);
+      z.hpsi(veff, RV, n, nb, CP, n1, AV, n);
+      z.spsi(RV, n, nb, CP, n1, ASV, n);
+      // step length from the trial step alambda_diis (:1065-1096)
+      for (int b = 0; b < nb; b++) ca[b] = zc(a.alambda_diis, 0.0);
+      z.comb(n, nb, one.data(), P, n, ca.data(), RV, n, PH1, n);
+      z.comb(n, nb, one.data(), Hh, n, ca.data(), AV, n, HPH1, n);
+      z.comb(n, nb, one.data(), SP, n, ca.data(), ASV, n, SPH1, n);
+      z.dots(RV, n, Hh, n, nb, dd, 1.0);
+      z.dots(RV, n, SP, n, nb, dd + nb, 1.0);
+      z.dots(PH1, n, HPH1, n, nb, dd + 2 * nb, 1.0);
+      z.dots(PH1, n, SPH1, n, nb, dd + 3 * nb, 1.0);
+      fetch(4 * nb);
+      std::vector<double> lam(nb);
+      for (int b = 0; b < nb; b++) {
+        const double sv = hsv[l0 + b], a1 = a.alambda_diis;
+        const double s1 = hd[b].x, s2 = hd[nb + b].x, s3 = hd[2 * nb + b].x, s4 = hd[3 * nb + b].x;
+        const double aaa = 2.0 * (s1 - sv * s2) * dv;
+        const double bbb = (s3 / s4 - sv - aaa * a1) / (a1 * a1);
+        double al = 0.5 * aaa / bbb;
+        if (al > a.alambda_max) al = a.alambda_max;
+        if (al < a.alambda_min) al = a.alambda_min;
+        lam[b] = al;
+        ca[b] = zc(al, 0.0);
+      }
+      // trial phi (:1099-1116) and the residual of phi_1 (:1140-1149; r_0 is unchanged)
+      z.comb(n, nb, one.data(), P, n, ca.data(), RV, n, PH1, n);
+      z.comb(n, nb, one.data(), Hh, n, ca.data(), AV, n, HPH1, n);
+      z.comb(n, nb, one.data(), SP, n, ca.data(), ASV, n, SPH1, n);
+      z.comb(n1, nb, one.data(), C, n1, ca.data(), CP, n1, CPH1, n1);
+      for (int b = 0; b < nb; b++) cb[b] = zc(-hsv[l0 + b], 0.0);
+      z.comb(n, nb, one.data(), HPH1, n, cb.data(), SPH1, n, RV, n);  // r_1 (K r no longer needed)
+      // DIIS matrices (:1154-1199)
+      z.dots(P, n, SP, n, nb, dd, dv);
+      z.dots(P, n, SPH1, n, nb, dd + nb, dv);
+      z.dots(PH1, n, SPH1, n, nb, dd + 2 * nb, dv);
+      z.dots(R0, n, R0, n, nb, dd + 3 * nb, dv);
+      z.dots(R0, n, RV, n, nb, dd + 4 * nb, dv);
+      z.dots(RV, n, RV, n, nb, dd + 5 * nb, dv);
+      fetch(6 * nb);
+      for (int b = 0; b < nb; b++) {
+        auto g = [&](int k) { return zc(hd[k * nb + b].x, hd[k * nb + b].y); };
+        const double tmp = 1.0 / std::sqrt(g(2).real());
+        // csm, crm with row and column 2 scaled by tmp (the normalisation of phi_1)
+        const double s00 = g(0).real(), s11 = g(2).real() * tmp * tmp;
+        const zc s01 = g(1) * tmp;
+        const double r00 = g(3).real(), r11 = g(5).real() * tmp * tmp;
+        const zc r01 = g(4) * tmp;
+        double w[2];
+        zc v[2][2];
+        herm2(s00, s01, s11, w, v);  // descending
+        zc cv[2][2] = {{v[0][0], v[0][1]}, {v[1][0], v[1][1]}};
+        const double ev[2] = {w[0], w[1]};
+        // overlap eigenvalues below eps_eig_diis are dropped (:1227-1240); with two of them in
+        // descending order that leaves the count of those at or above it
+        const int nmat = (ev[0] >= a.eps_eig_diis) + (ev[1] >= a.eps_eig_diis);
+        zc c0, c1;
+        if (nmat > 1) {
+          for (int j = 0; j < 2; j++)
+            for (int i = 0; i < 2; i++) cv[j][i] /= std::sqrt(ev[j]);
+          // projected residual matrix cr2 = cv^H crm cv (:1242-1260)
+          const zc crm[2][2] = {{r00, r01}, {std::conj(r01), r11}};  // crm[i][k] = row i, col k
+          zc t1[2][2], cr2[2][2];
+          for (int kk = 0; kk < 2; kk++)
+            for (int j = 0; j < 2; j++) t1[kk][j] = crm[kk][0] * cv[j][0] + crm[kk][1] * cv[j][1];
+          for (int kk = 0; kk < 2; kk++)
+            for (int j = 0; j < 2; j++)
+              cr2[kk][j] = std::conj(cv[kk][0]) * t1[0][j] + std::conj(cv[kk][1]) * t1[1][j];
+          double w2[2];
+          zc v2[2][2];
+          herm2(cr2[0][0].real(), cr2[0][1], cr2[1][1].real(), w2, v2);
+          hres[l0 + b] = w2[1];
+          // the lowest: coefficients on phi_0, phi_1 (:1293-1310), S-normalised with csm0
+          const zc x0 = cv[0][0] * v2[1][0] + cv[1][0] * v2[1][1];
+          const zc x1 = cv[0][1] * v2[1][0] + cv[1][1] * v2[1][1];
+          const double t2 = (std::conj(x0) * s00 * x0 + std::conj(x0) * s01 * x1 +
+                             std::conj(x1) * std::conj(s01) * x0 + std::conj(x1) * s11 * x1).real();
+          c0 = x0 / std::sqrt(t2);
+          c1 = x1 / std::sqrt(t2);
+        } else {
+          c0 = 0.0;
+          c1 = 1.0;
+        }
+        const double m0 = std::abs(c0);
+        if (m0 > 0.0) {
+          const zc ph = std::conj(c0) / m0;
+          c0 = zc(m0, 0.0);
+          c1 *= ph;
+        }
+        ca[b] = c0;
+        cb[b] = c1 * tmp;  // phi_1 is combined unnormalised
+        *ksitmax = std::max(*ksitmax, 1);
+        *ksconv = *ksconv && (std::sqrt(hres[l0 + b]) <= a.epssd);
+      }
+      // new band (:1313-1352)
+      z.comb(n, nb, ca.data(), P, n, cb.data(), PH1, n, P, n);
+      z.comb(n, nb, ca.data(), SP, n, cb.data(), SPH1, n, SP, n);
+      z.comb(n, nb, ca.data(), Hh, n, cb.data(), HPH1, n, Hh, n);
+      z.comb(n1, nb, ca.data(), C, n1, cb.data(), CPH1, n1, C, n1);
+    }
+    // l_hsv: orthonormalise, then H psi of every band (:1416-1478)
+    if (a.l_hsv) {
+      for (int i = 0; i < a.northo; i++) zortho_c_01(z, M, 0, a.key_cmpt, a.key_cmpt, Psi, SPsi, Cps);
+      z.hpsi(veff, Psi, n, M, Cps, n1, H, n);
+    }
   }
   HIPCK(hipMemcpyAsync(sval, hsv.data(), sizeof(double) * M, hipMemcpyHostToDevice, c.s));
   HIPCK(hipMemcpyAsync(resid, hres.data(), sizeof(double) * M, hipMemcpyHostToDevice, c.s));

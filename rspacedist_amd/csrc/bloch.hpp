@@ -32,7 +32,11 @@ struct ZCtx {
   // out_b = S u_b
   void spsi(const double2 *u, long ldu, int nb, const double2 *cp, long ldcp, double2 *out,
             long ldo);
-  void precond(const double2 *u, double2 *out);
+  // out_b = K u_b for nb bands (band strides default to n)
+  void precond(const double2 *u, double2 *out, int nb = 1, long ldu = 0, long ldo = 0);
+  // out_b = ca[b] x_b + cb[b] y_b with host coefficient arrays (y may be null)
+  void comb(long len, int nb, const std::complex<double> *ca, const double2 *x, long ldx,
+            const std::complex<double> *cb, const double2 *y, long ldy, double2 *out, long ldo);
   // res_b = scale sum conj(x_b) y_b (device)
   void dots(const double2 *x, long ldx, const double2 *y, long ldy, int nb, double2 *res,
             double scale);
@@ -50,6 +54,14 @@ void zrayleigh_ritz(ZCtx &z, int M, const double *veff, bool l_rrb, double2 *psi
 void zkscg_c(ZCtx &z, int M, int northo, int nkscg, int nsdmax, int nprecon_cg, int key_cmpt,
              int key_nocmpt, double epssd, const double *veff, double2 *Psi, double2 *SPsi,
              double2 *Cps, double2 *H, double *sval, double *resid, int *ksconv, int *ksitmax);
+// ksdiis_c for one spin / k-point (ksdiis8f.F90:715-1509)
+struct ZDiisArgs {
+  int M, northo, ndiismax, nprecon_diis, key_cmpt;
+  bool l_hsv;
+  double eps_eig_diis, alambda_diis, alambda_min, alambda_max, epssd;
+};
+void zksdiis_c(ZCtx &z, const ZDiisArgs &a, const double *veff, double2 *Psi, double2 *SPsi,
+               double2 *Cps, double2 *H, double *sval, double *resid, int *ksconv, int *ksitmax);
 void zreorder(ZCtx &z, int M, double *sval, double *resid, double2 *psi, double2 *spsi,
               double2 *cps, double2 *hpsi);
 void zcharge(long n, int nb, const void *u, long ldu, const double *f, double *rho, hipStream_t s);
@@ -63,9 +75,9 @@ void zprecond_leaf(const Grid &g, int nf, const void *vcm, void *avcm, int ncol,
 struct BlochDiagArgs {
   int nso, ncol, iscf, ncgscf, nretcg, natom, num_spe, neigmx, nums, nspv, numk, nperi, northo, nf;
   int num_list, num_ppcell, nprjmx, ncpx, ncpy, ncpz, key_cmpt, key_nocmpt, key_in, key_inps;
-  int nkscg, nsdmax, nprecon_cg;
-  bool l_rrb, l_rra, l_reo;
-  double dx, dy, dz, epssd;
+  int nkscg, nsdmax, nprecon_cg, ndiismax, nprecon_diis;
+  bool l_rrb, l_rra, l_reo, l_hsv;
+  double dx, dy, dz, epssd, eps_eig_diis, alambda_diis, alambda_min, alambda_max;
   const double *skpxx, *skpyy, *skpzz, *dij, *veff, *vnlocp, *sss;
   const int *indspe, *natpri, *naps, *nprj, *natinf, *lstvec2, *lstx, *lsty, *lstz, *natx, *naty,
       *natz;

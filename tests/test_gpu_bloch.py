@@ -239,3 +239,61 @@ def test_complex_entry_points_reject_noncollinear(kk):
                 ncpz=s.n, fnele=np.ones(2), sveccm=psi, rhosmt=np.zeros(2 * s.npts))
     with pytest.raises(kk.KukanError, match=r"\(5\)"):
         kk.call("kscg_c", **dict(synth.bloch_args(s, K), nso=1), **init_c_dummy(s))
+
+
+@pytest.mark.parametrize("q", QS)
+@pytest.mark.parametrize("l_hsv", [0, 1])
+def test_ksdiis_c(kk, orc, q, l_hsv):
+    """Two RMM-DIIS sweeps after four CG iterations per band.  Both sides fix the combination's
+    phase the same way (the incoming band's coefficient real and non-negative), so the bands
+    compare directly."""
+    s = synth.make_system(12, 0.45, 3, 8, seed=36, q_scale=q)
+    st = init_c(orc, s)
+    keys = ("sval", "cspsep", "sveccm", "ssvcm", "residual_states", "hsvcm")
+    orc.call("kscg_c", **synth.bloch_args(s, K, nsdmax=4), **{k: st[k] for k in keys})
+    g, o = both(kk, orc, "ksdiis_c", {k: st[k] for k in keys},
+                **synth.bloch_args(s, K, ndiismax=3, l_hsv=l_hsv))
+    np.testing.assert_allclose(g["sval"], o["sval"], rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(g["residual_states"], o["residual_states"], rtol=1e-7, atol=1e-15)
+    for k in ZK:
+        close(g, o, k, 1e-9)
+    assert g["_ret"]["ksitmax"] == o["_ret"]["ksitmax"]
+    assert g["_ret"]["ksconv"] == o["_ret"]["ksconv"]
+
+
+@pytest.mark.parametrize("q", QS)
+def test_scf_diag_bloch_diis_steps(kk, orc, q):
+    """Five scf_diag steps with nrc = 1: CG for two, RMM-DIIS afterwards, Rayleigh-Ritz every
+    other step (so l_hsv is set on the steps before one): eigenvalues and density."""
+    s = synth.make_system(10, 0.5, 3, 8, seed=38, q_scale=q)
+    st0 = init_c(orc, s)
+    res = []
+    for f in (kk.call, orc.call):
+        st = {k: v.copy(order="F") for k, v in st0.items()}
+        kw = synth.bloch_args(s, K, ncgscf=2, nrrz=2, ndiismax=3)
+        for it in range(1, 6):
+            kw["iscf"] = it
+            f("scf_diag", **kw, **st, **{k: np.zeros(1) for k in REAL_DUMMIES})
+        res.append(st)
+    g, o = res
+    np.testing.assert_allclose(g["sval"], o["sval"], rtol=1e-10, atol=1e-10)
+    rho = [np.zeros(s.npts), np.zeros(s.npts)]
+    for st, rh, f in ((g, rho[0], kk.call), (o, rho[1], orc.call)):
+        f("scf_charge_smt_c", neigmx=s.nbands, nums=1, ncol=1, nspv=1, numk=1, ncpx=s.n, ncpy=s.n,
+          ncpz=s.n, fnele=np.ones(s.nbands), sveccm=st["sveccm"], rhosmt=rh)
+    np.testing.assert_allclose(rho[0], rho[1], rtol=0, atol=1e-9 * rho[1].max())
+
+
+@pytest.mark.parametrize("q", QS)
+def test_scf_diag_bloch_diis_converges_to_dense_eigenpairs(kk, orc, q):
+    s = synth.make_system(10, 0.5, 3, 6, seed=18, q_scale=q)
+    st = init_c(orc, s)
+    kw = synth.bloch_args(s, K, nsdmax=4, epssd=1e-12, ncgscf=10, ndiismax=6)
+    for it in range(1, 81):
+        kw["iscf"] = it
+        kk.call("scf_diag", **kw, **st, **{k: np.zeros(1) for k in REAL_DUMMIES})
+    Hd, Sd = dense_hk(s, K), dense_sk(s, K)
+    w = scipy.linalg.eigh(Hd, Sd, eigvals_only=True)[:6]
+    np.testing.assert_allclose(st["sval"], w, rtol=0, atol=1e-7)
+    Q = st["sveccm"]
+    np.testing.assert_allclose(np.conj(Q.T) @ Sd @ Q * s.dv, np.eye(6), rtol=0, atol=1e-12)

@@ -196,3 +196,57 @@ def test_scf_charge_smt_c_spin_mapping_equals_reference(orc, nspv):
     o, r = both(orc, "scf_charge_smt_c", {"rhosmt": np.full((s.npts, nspv), 7.0, order="F")}, neigmx=3,
                 nums=2, ncol=1, nspv=nspv, numk=2, ncpx=s.n, ncpy=s.n, ncpz=s.n, fnele=f, sveccm=psi)
     np.testing.assert_allclose(o["rhosmt"], r["rhosmt"], rtol=1e-14, atol=0)
+
+
+def _align_cols(o, r):
+    """r's bands times the unit phase that best matches o's (per band)."""
+    ph = np.einsum("ij,ij->j", np.conj(r), o)
+    return r * (ph / np.maximum(np.abs(ph), 1e-300))[None, :], ph / np.maximum(np.abs(ph), 1e-300)
+
+
+@pytest.mark.parametrize("q", QS)
+@pytest.mark.parametrize("l_hsv", [0, 1])
+def test_ksdiis_c_equals_reference(orc, q, l_hsv):
+    """Two RMM-DIIS sweeps (ndiismax = 3) after four CG iterations per band.  The combination's
+    overall phase per band is the eigensolver's choice; every later operation carries a band's
+    phase along, so the bands agree after one phase per band."""
+    s = synth.make_system(12, 0.45, 3, 8, seed=36, q_scale=q)
+    st = init_c(orc, s)
+    keys = ("sval", "cspsep", "sveccm", "ssvcm", "residual_states", "hsvcm")
+    orc.call("kscg_c", **synth.bloch_args(s, K, nsdmax=4), **{k: st[k] for k in keys})
+    o, r = both(orc, "ksdiis_c", {k: st[k] for k in keys},
+                **synth.bloch_args(s, K, ndiismax=3, l_hsv=l_hsv))
+    np.testing.assert_allclose(o["sval"], r["sval"], rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(o["residual_states"], r["residual_states"], rtol=1e-7, atol=1e-15)
+    _, ph = _align_cols(o["sveccm"], r["sveccm"])
+    for k in ("sveccm", "ssvcm", "hsvcm"):
+        np.testing.assert_allclose(o[k], r[k] * ph[None, :], rtol=0, atol=1e-9 * np.abs(r[k]).max(), err_msg=k)
+    cs = lambda a: a.reshape(-1, s.nbands, order="F")
+    np.testing.assert_allclose(cs(o["cspsep"]), cs(r["cspsep"]) * ph[None, :], rtol=0,
+                               atol=1e-9 * np.abs(r["cspsep"]).max())
+    assert o["_ret"]["ksitmax"] == r["_ret"]["ksitmax"]
+    assert o["_ret"]["ksconv"] == r["_ret"]["ksconv"]
+
+
+@pytest.mark.parametrize("q", QS)
+def test_scf_diag_bloch_diis_steps_equal_reference(orc, q):
+    """scf_diag with nrc = 1 taking the DIIS branch after two CG steps (ncgscf = 2), with the
+    Rayleigh-Ritz step before and the orthonormalisation of l_hsv: eigenvalues and densities."""
+    s = synth.make_system(10, 0.5, 3, 8, seed=38, q_scale=q)
+    st0 = init_c(orc, s)
+    dummies = {k: np.zeros(1) for k in ("rspsep", "svecre", "hsvre", "ssvre")}
+    res = []
+    for f in (orc.call, ref.call):
+        st = {k: v.copy(order="F") for k, v in st0.items()}
+        kw = synth.bloch_args(s, K, ncgscf=2, nrrz=2, ndiismax=3)
+        for it in range(1, 6):
+            kw["iscf"] = it
+            f("scf_diag", **kw, **st, **dummies)
+        res.append(st)
+    o, r = res
+    np.testing.assert_allclose(o["sval"], r["sval"], rtol=1e-10, atol=1e-10)
+    rho = [np.zeros(s.npts), np.zeros(s.npts)]
+    for st, rh, f in ((o, rho[0], orc.call), (r, rho[1], ref.call)):
+        f("scf_charge_smt_c", neigmx=s.nbands, nums=1, ncol=1, nspv=1, numk=1, ncpx=s.n, ncpy=s.n,
+          ncpz=s.n, fnele=np.ones(s.nbands), sveccm=st["sveccm"], rhosmt=rh)
+    np.testing.assert_allclose(rho[0], rho[1], rtol=0, atol=1e-9 * rho[1].max())

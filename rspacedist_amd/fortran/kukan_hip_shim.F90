@@ -1,7 +1,7 @@
 ! kukan_hip_shim.F90 -- Fortran drop-in for the RSPACE kukan v8f Kohn-Sham hot path on MI355X.
 !
 ! The modules below are named after the reference modules with a _hip suffix (so that both
-! can be linked into one program; the complex-path routines stay in the reference modules)
+! can be linked into one program)
 ! and carry the reference routine names with the reference argument lists
 ! (RSPACEes.v08.01/kukan/*.F90, cited per routine).  Each routine forwards
 ! its arrays by address and its scalars by value to libkukan_hip.so through ISO_C_BINDING
@@ -202,6 +202,105 @@ module kukan_hip_c
       import :: c_int, c_double
       integer(c_int), value :: neigmx, nums, nspv, numk, ncpx, ncpy, ncpz
       real(c_double) :: fnele(*), svecre(*), rho(*)
+    end function
+    integer(c_int) function c_kscg_c( &
+    nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, northo, ns1, nk, nf, nkscg, nsdmax, &
+        nprecon_cg, nprjmx, num_list, num_ppcell, ndisp, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, &
+        key_ortho_nocmpt_innerproduct, key_ortho_cmpt_innerproduct, key_natpri_in, key_natpri_inps, &
+        key_natpri_out, key_soc_calc, dx, dy, dz, skpx, skpy, skpz, epssd, indspe, natpri, naps, nprj, &
+        natinf, lstvec2, latom, natsoc, ntyppp, lstx, lsty, lstz, natx, naty, natz, dij, dijsoc, veff, &
+        vnlocp, sss, ksconv, ksitmax, sval, cspsep, sveccm, ssvcm, residual_states, hsvcm) &
+        bind(C, name='kukan_kscg_c')
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, northo, ns1
+      integer(c_int), value :: nk, nf, nkscg, nsdmax, nprecon_cg, nprjmx, num_list, num_ppcell, ndisp, ncpx
+      integer(c_int), value :: ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_nocmpt_innerproduct, key_ortho_cmpt_innerproduct
+      integer(c_int), value :: key_natpri_in, key_natpri_inps, key_natpri_out, key_soc_calc
+      real(c_double), value :: dx, dy, dz, skpx, skpy, skpz, epssd
+      integer(c_int) :: indspe(*), natpri(*), naps(*), nprj(*), natinf(*), lstvec2(*), latom(*)
+      integer(c_int) :: natsoc(*), ntyppp(*), lstx(*), lsty(*), lstz(*), natx(*), naty(*), natz(*)
+      real(c_double) :: dij(*), dijsoc(*), veff(*), vnlocp(*), sss(*), sval(*), residual_states(*)
+      complex(c_double_complex) :: cspsep(*), sveccm(*), ssvcm(*), hsvcm(*)
+      integer(c_int) :: ksconv, ksitmax
+    end function
+    integer(c_int) function c_ksdiis_c( &
+    l_hsv, nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, ndiismax, nprecon_diis, nk, ns1, &
+        nf, nprjmx, num_list, num_ppcell, ndisp, northo, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, &
+        key_ortho_cmpt_innerproduct, key_natpri_in, key_natpri_inps, key_soc_calc, eps_eig_diis, &
+        alambda_diis, ratio_diis, alambda_min, alambda_max, epssd, xmax, ymax, zmax, skpx, skpy, skpz, &
+        indspe, natpri, naps, nprj, natinf, lstvec2, latom, natsoc, ntyppp, lstx, lsty, lstz, natx, &
+        naty, natz, dij, dijsoc, veff, vnlocp, sss, ksconv, ksitmax, cspsep, sval, sveccm, ssvcm, &
+        residual_states, hsvcm) &
+        bind(C, name='kukan_ksdiis_c')
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: l_hsv, nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, ndiismax
+      integer(c_int), value :: nprecon_diis, nk, ns1, nf, nprjmx, num_list, num_ppcell, ndisp, northo, ncpx
+      integer(c_int), value :: ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct, key_natpri_in
+      integer(c_int), value :: key_natpri_inps, key_soc_calc
+      real(c_double), value :: eps_eig_diis, alambda_diis, ratio_diis, alambda_min, alambda_max, epssd
+      real(c_double), value :: xmax, ymax, zmax, skpx, skpy, skpz
+      integer(c_int) :: indspe(*), natpri(*), naps(*), nprj(*), natinf(*), lstvec2(*), latom(*)
+      integer(c_int) :: natsoc(*), ntyppp(*), lstx(*), lsty(*), lstz(*), natx(*), naty(*), natz(*)
+      real(c_double) :: dij(*), dijsoc(*), veff(*), vnlocp(*), sss(*), sval(*), residual_states(*)
+      complex(c_double_complex) :: cspsep(*), sveccm(*), ssvcm(*), hsvcm(*)
+      integer(c_int) :: ksconv, ksitmax
+    end function
+    integer(c_int) function c_orthogonalization_c( &
+    natom, neigmx, nprjmx, nums, ncol, numk, num_ppcell, num_spe, northo, num_list, nflag, ndisp, ncpx, &
+        ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct, key_natpri_in, &
+        key_natpri_inps, dx, dy, dz, nprj, indspe, natpri, naps, natinf, lstvec2, latom, ntyppp, lstx, &
+        lsty, lstz, natx, naty, natz, skpxx, skpyy, skpzz, vnlocp, sss, sveccm, ssvcm, cspsep) &
+        bind(C, name='kukan_orthogonalization_c')
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: natom, neigmx, nprjmx, nums, ncol, numk, num_ppcell, num_spe, northo, num_list
+      integer(c_int), value :: nflag, ndisp, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct
+      integer(c_int), value :: key_natpri_in, key_natpri_inps
+      real(c_double), value :: dx, dy, dz
+      integer(c_int) :: nprj(*), indspe(*), natpri(*), naps(*), natinf(*), lstvec2(*), latom(*)
+      integer(c_int) :: ntyppp(*), lstx(*), lsty(*), lstz(*), natx(*), naty(*), natz(*)
+      real(c_double) :: skpxx(*), skpyy(*), skpzz(*), vnlocp(*), sss(*)
+      complex(c_double_complex) :: sveccm(*), ssvcm(*), cspsep(*)
+    end function
+    integer(c_int) function c_orthogonalization_c_01( &
+    natom, neigmx, nprjmx, nums, ncol, numk, nk, ns1, l_inp, num_list, num_ppcell, num_spe, nflag, &
+        ndisp, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct, key_natpri_in, &
+        key_natpri_inps, dx, dy, dz, skpx, skpy, skpz, nprj, indspe, natpri, naps, natinf, lstvec2, &
+        latom, lstx, lsty, lstz, natx, naty, natz, vnlocp, sss, sveccm, ssvcm, cspsep) &
+        bind(C, name='kukan_orthogonalization_c_01')
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: natom, neigmx, nprjmx, nums, ncol, numk, nk, ns1, l_inp, num_list, num_ppcell
+      integer(c_int), value :: num_spe, nflag, ndisp, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct
+      integer(c_int), value :: key_natpri_in, key_natpri_inps
+      real(c_double), value :: dx, dy, dz, skpx, skpy, skpz
+      integer(c_int) :: nprj(*), indspe(*), natpri(*), naps(*), natinf(*), lstvec2(*), latom(*)
+      integer(c_int) :: lstx(*), lsty(*), lstz(*), natx(*), naty(*), natz(*)
+      real(c_double) :: vnlocp(*), sss(*)
+      complex(c_double_complex) :: sveccm(*), ssvcm(*), cspsep(*)
+    end function
+    integer(c_int) function c_rayleighritz_c( &
+    nso, natom, num_ppcell, num_spe, nprjmx, neigmx, numk, nums, ncol, nspv, num_list, ncpx, ncpy, &
+        ncpz, npxmax, npymax, npzmax, nperi, nk, ns1, nf, ndisp, key_natpri_in, key_natpri_inps, &
+        key_soc_calc, dx, dy, dz, xmax, ymax, zmax, skpxx, skpyy, skpzz, nprj, natinf, lstvec2, indspe, &
+        natpri, naps, natsoc, ntyppp, lstx, lsty, lstz, natx, naty, natz, dij, dijsoc, veff, vnlocp, &
+        l_rrb, cspsep, sveccm, ssvcm, hsvcm, sval) &
+        bind(C, name='kukan_rayleighritz_c')
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: nso, natom, num_ppcell, num_spe, nprjmx, neigmx, numk, nums, ncol, nspv
+      integer(c_int), value :: num_list, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, nperi, nk, ns1, nf, ndisp
+      integer(c_int), value :: key_natpri_in, key_natpri_inps, key_soc_calc, l_rrb
+      real(c_double), value :: dx, dy, dz, xmax, ymax, zmax
+      integer(c_int) :: nprj(*), natinf(*), lstvec2(*), indspe(*), natpri(*), naps(*), natsoc(*)
+      integer(c_int) :: ntyppp(*), lstx(*), lsty(*), lstz(*), natx(*), naty(*), natz(*)
+      real(c_double) :: skpxx(*), skpyy(*), skpzz(*), dij(*), dijsoc(*), veff(*), vnlocp(*), sval(*)
+      complex(c_double_complex) :: cspsep(*), sveccm(*), ssvcm(*), hsvcm(*)
+    end function
+    integer(c_int) function c_scf_charge_smt_c( &
+    neigmx, nums, ncol, nspv, numk, ncpx, ncpy, ncpz, fnele, sveccm, rhosmt) &
+        bind(C, name='kukan_scf_charge_smt_c')
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: neigmx, nums, ncol, nspv, numk, ncpx, ncpy, ncpz
+      real(c_double) :: fnele(*), rhosmt(*)
+      complex(c_double_complex) :: sveccm(*)
     end function
   end interface
 
@@ -405,6 +504,38 @@ contains
          dij, veff, vnlocp, sss, ic, it, sval, rspsep, svecre, ssvre, residual_states, hsvre))
     ksconv = ic /= 0; ksitmax = it
   end subroutine
+  ! kscg8f.F90:492-1032 (complex path, ncol = 1, nso = 0)
+  subroutine kscg_c( &
+      nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, northo, ns1, nk, nf, nkscg, nsdmax, &
+          nprecon_cg, nprjmx, num_list, num_ppcell, ndisp, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, &
+          key_ortho_nocmpt_innerproduct, key_ortho_cmpt_innerproduct, key_natpri_in, key_natpri_inps, &
+          key_natpri_out, key_soc_calc, dx, dy, dz, skpx, skpy, skpz, epssd, indspe, natpri, naps, &
+          nprj, natinf, lstvec2, latom, natsoc, ntyppp, lstx, lsty, lstz, natx, naty, natz, dij, &
+          dijsoc, veff, vnlocp, sss, ksconv, ksitmax, sval, cspsep, sveccm, ssvcm, residual_states, &
+          hsvcm)
+    integer, intent(in) :: nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, northo, ns1
+    integer, intent(in) :: nk, nf, nkscg, nsdmax, nprecon_cg, nprjmx, num_list, num_ppcell, ndisp, ncpx
+    integer, intent(in) :: ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_nocmpt_innerproduct, key_ortho_cmpt_innerproduct
+    integer, intent(in) :: key_natpri_in, key_natpri_inps, key_natpri_out, key_soc_calc
+    real*8, intent(in) :: dx, dy, dz, skpx, skpy, skpz, epssd
+    integer, intent(in) :: indspe(*), natpri(*), naps(*), nprj(*), natinf(*), lstvec2(*), latom(*)
+    integer, intent(in) :: natsoc(*), ntyppp(*), lstx(*), lsty(*), lstz(*), natx(*), naty(*), natz(*)
+    real*8, intent(in) :: dij(*), dijsoc(*), veff(*), vnlocp(*), sss(*)
+    real*8, intent(inout) :: sval(*), residual_states(*)
+    complex*16, intent(inout) :: cspsep(*), sveccm(*), ssvcm(*), hsvcm(*)
+    integer, intent(inout) :: ksitmax
+    logical, intent(inout) :: ksconv
+    integer(c_int) :: ic, it
+    ic = l2i(ksconv); it = ksitmax
+    call kukan_check(c_kscg_c(nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, northo, ns1, nk, nf, nkscg, &
+             nsdmax, nprecon_cg, nprjmx, num_list, num_ppcell, ndisp, ncpx, ncpy, ncpz, npxmax, npymax, &
+             npzmax, key_ortho_nocmpt_innerproduct, key_ortho_cmpt_innerproduct, key_natpri_in, &
+             key_natpri_inps, key_natpri_out, key_soc_calc, dx, dy, dz, skpx, skpy, skpz, epssd, &
+             indspe, natpri, naps, nprj, natinf, lstvec2, latom, natsoc, ntyppp, lstx, lsty, lstz, &
+             natx, naty, natz, dij, dijsoc, veff, vnlocp, sss, ic, it, sval, cspsep, sveccm, ssvcm, &
+             residual_states, hsvcm))
+    ksconv = ic /= 0; ksitmax = it
+  end subroutine
 end module mod_kscg_hip
 
 
@@ -444,6 +575,40 @@ contains
          ratio_diis, alambda_min, alambda_max, epssd, xmax, ymax, zmax, indspe, natpri, naps, nprj, &
          natinf, lstvec2, latom, ntyppp, dij, veff, vnlocp, sss, ic, it, rspsep, sval, svecre, ssvre, &
          residual_states, hsvre))
+    ksconv = ic /= 0; ksitmax = it
+  end subroutine
+  ! ksdiis8f.F90:715-1509 (complex path, ncol = 1, nso = 0)
+  subroutine ksdiis_c( &
+      l_hsv, nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, ndiismax, nprecon_diis, nk, &
+          ns1, nf, nprjmx, num_list, num_ppcell, ndisp, northo, ncpx, ncpy, ncpz, npxmax, npymax, &
+          npzmax, key_ortho_cmpt_innerproduct, key_natpri_in, key_natpri_inps, key_soc_calc, &
+          eps_eig_diis, alambda_diis, ratio_diis, alambda_min, alambda_max, epssd, xmax, ymax, zmax, &
+          skpx, skpy, skpz, indspe, natpri, naps, nprj, natinf, lstvec2, latom, natsoc, ntyppp, lstx, &
+          lsty, lstz, natx, naty, natz, dij, dijsoc, veff, vnlocp, sss, ksconv, ksitmax, cspsep, sval, &
+          sveccm, ssvcm, residual_states, hsvcm)
+    integer, intent(in) :: nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, ndiismax, nprecon_diis
+    integer, intent(in) :: nk, ns1, nf, nprjmx, num_list, num_ppcell, ndisp, northo, ncpx, ncpy, ncpz
+    integer, intent(in) :: npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct, key_natpri_in, key_natpri_inps
+    integer, intent(in) :: key_soc_calc
+    logical, intent(in) :: l_hsv
+    real*8, intent(in) :: eps_eig_diis, alambda_diis, ratio_diis, alambda_min, alambda_max, epssd
+    real*8, intent(in) :: xmax, ymax, zmax, skpx, skpy, skpz
+    integer, intent(in) :: indspe(*), natpri(*), naps(*), nprj(*), natinf(*), lstvec2(*), latom(*)
+    integer, intent(in) :: natsoc(*), ntyppp(*), lstx(*), lsty(*), lstz(*), natx(*), naty(*), natz(*)
+    real*8, intent(in) :: dij(*), dijsoc(*), veff(*), vnlocp(*), sss(*)
+    real*8, intent(inout) :: sval(*), residual_states(*)
+    complex*16, intent(inout) :: cspsep(*), sveccm(*), ssvcm(*), hsvcm(*)
+    integer, intent(inout) :: ksitmax
+    logical, intent(inout) :: ksconv
+    integer(c_int) :: ic, it
+    ic = l2i(ksconv); it = ksitmax
+    call kukan_check(c_ksdiis_c(l2i(l_hsv), nso, natom, num_spe, neigmx, nums, ncol, nspv, numk, nperi, ndiismax, &
+             nprecon_diis, nk, ns1, nf, nprjmx, num_list, num_ppcell, ndisp, northo, ncpx, ncpy, ncpz, &
+             npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct, key_natpri_in, key_natpri_inps, &
+             key_soc_calc, eps_eig_diis, alambda_diis, ratio_diis, alambda_min, alambda_max, epssd, &
+             xmax, ymax, zmax, skpx, skpy, skpz, indspe, natpri, naps, nprj, natinf, lstvec2, latom, &
+             natsoc, ntyppp, lstx, lsty, lstz, natx, naty, natz, dij, dijsoc, veff, vnlocp, sss, ic, &
+             it, cspsep, sval, sveccm, ssvcm, residual_states, hsvcm))
     ksconv = ic /= 0; ksitmax = it
   end subroutine
 end module mod_ksdiis_hip
@@ -494,6 +659,44 @@ contains
          key_natpri_inps, dx, dy, dz, nprj, indspe, natpri, naps, natinf, lstvec2, latom, vnlocp, sss, &
          svecre, ssvre, rspsep))
   end subroutine
+  ! orthogonalization8f.F90:47-90, 379-724 (complex path, ncol = 1)
+  subroutine orthogonalization_c( &
+      natom, neigmx, nprjmx, nums, ncol, numk, num_ppcell, num_spe, northo, num_list, nflag, ndisp, &
+          ncpx, ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct, key_natpri_in, &
+          key_natpri_inps, dx, dy, dz, nprj, indspe, natpri, naps, natinf, lstvec2, latom, ntyppp, &
+          lstx, lsty, lstz, natx, naty, natz, skpxx, skpyy, skpzz, vnlocp, sss, sveccm, ssvcm, cspsep)
+    integer, intent(in) :: natom, neigmx, nprjmx, nums, ncol, numk, num_ppcell, num_spe, northo, num_list
+    integer, intent(in) :: nflag, ndisp, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct
+    integer, intent(in) :: key_natpri_in, key_natpri_inps
+    real*8, intent(in) :: dx, dy, dz
+    integer, intent(in) :: nprj(*), indspe(*), natpri(*), naps(*), natinf(*), lstvec2(*), latom(*)
+    integer, intent(in) :: ntyppp(*), lstx(*), lsty(*), lstz(*), natx(*), naty(*), natz(*)
+    real*8, intent(in) :: skpxx(*), skpyy(*), skpzz(*), vnlocp(*), sss(*)
+    complex*16, intent(inout) :: sveccm(*), ssvcm(*), cspsep(*)
+    call kukan_check(c_orthogonalization_c(natom, neigmx, nprjmx, nums, ncol, numk, num_ppcell, num_spe, northo, num_list, nflag, ndisp, &
+             ncpx, ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct, key_natpri_in, &
+             key_natpri_inps, dx, dy, dz, nprj, indspe, natpri, naps, natinf, lstvec2, latom, ntyppp, &
+             lstx, lsty, lstz, natx, naty, natz, skpxx, skpyy, skpzz, vnlocp, sss, sveccm, ssvcm, cspsep))
+  end subroutine
+  subroutine orthogonalization_c_01( &
+      natom, neigmx, nprjmx, nums, ncol, numk, nk, ns1, l_inp, num_list, num_ppcell, num_spe, nflag, &
+          ndisp, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct, key_natpri_in, &
+          key_natpri_inps, dx, dy, dz, skpx, skpy, skpz, nprj, indspe, natpri, naps, natinf, lstvec2, &
+          latom, lstx, lsty, lstz, natx, naty, natz, vnlocp, sss, sveccm, ssvcm, cspsep)
+    integer, intent(in) :: natom, neigmx, nprjmx, nums, ncol, numk, nk, ns1, l_inp, num_list, num_ppcell
+    integer, intent(in) :: num_spe, nflag, ndisp, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct
+    integer, intent(in) :: key_natpri_in, key_natpri_inps
+    real*8, intent(in) :: dx, dy, dz, skpx, skpy, skpz
+    integer, intent(in) :: nprj(*), indspe(*), natpri(*), naps(*), natinf(*), lstvec2(*), latom(*)
+    integer, intent(in) :: lstx(*), lsty(*), lstz(*), natx(*), naty(*), natz(*)
+    real*8, intent(in) :: vnlocp(*), sss(*)
+    complex*16, intent(inout) :: sveccm(*), ssvcm(*), cspsep(*)
+    call kukan_check(c_orthogonalization_c_01(natom, neigmx, nprjmx, nums, ncol, numk, nk, ns1, l_inp, num_list, num_ppcell, num_spe, nflag, &
+             ndisp, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, key_ortho_cmpt_innerproduct, &
+             key_natpri_in, key_natpri_inps, dx, dy, dz, skpx, skpy, skpz, nprj, indspe, natpri, naps, &
+             natinf, lstvec2, latom, lstx, lsty, lstz, natx, naty, natz, vnlocp, sss, sveccm, ssvcm, &
+             cspsep))
+  end subroutine
 end module mod_orthogonalization_hip
 
 
@@ -529,6 +732,29 @@ contains
          num_list, ncpx, ncpy, ncpz, nperi, nk, ns, nf, ndisp, key_natpri_in, key_natpri_inps, dx, dy, &
          dz, nprj, natinf, lstvec2, indspe, natpri, naps, ntyppp, dij, veff, vnlocp, l2i(l_rrb), rspsep, &
          svecre, ssvre, hsvre, sval))
+  end subroutine
+  ! rayleigh-ritz8f.F90:128-242 (complex path, ncol = 1, nso = 0)
+  subroutine rayleighritz_c( &
+      nso, natom, num_ppcell, num_spe, nprjmx, neigmx, numk, nums, ncol, nspv, num_list, ncpx, ncpy, &
+          ncpz, npxmax, npymax, npzmax, nperi, nk, ns1, nf, ndisp, key_natpri_in, key_natpri_inps, &
+          key_soc_calc, dx, dy, dz, xmax, ymax, zmax, skpxx, skpyy, skpzz, nprj, natinf, lstvec2, &
+          indspe, natpri, naps, natsoc, ntyppp, lstx, lsty, lstz, natx, naty, natz, dij, dijsoc, veff, &
+          vnlocp, l_rrb, cspsep, sveccm, ssvcm, hsvcm, sval)
+    integer, intent(in) :: nso, natom, num_ppcell, num_spe, nprjmx, neigmx, numk, nums, ncol, nspv
+    integer, intent(in) :: num_list, ncpx, ncpy, ncpz, npxmax, npymax, npzmax, nperi, nk, ns1, nf, ndisp
+    integer, intent(in) :: key_natpri_in, key_natpri_inps, key_soc_calc
+    logical, intent(in) :: l_rrb
+    real*8, intent(in) :: dx, dy, dz, xmax, ymax, zmax
+    integer, intent(in) :: nprj(*), natinf(*), lstvec2(*), indspe(*), natpri(*), naps(*), natsoc(*)
+    integer, intent(in) :: ntyppp(*), lstx(*), lsty(*), lstz(*), natx(*), naty(*), natz(*)
+    real*8, intent(in) :: skpxx(*), skpyy(*), skpzz(*), dij(*), dijsoc(*), veff(*), vnlocp(*)
+    real*8, intent(inout) :: sval(*)
+    complex*16, intent(inout) :: cspsep(*), sveccm(*), ssvcm(*), hsvcm(*)
+    call kukan_check(c_rayleighritz_c(nso, natom, num_ppcell, num_spe, nprjmx, neigmx, numk, nums, ncol, nspv, num_list, ncpx, ncpy, &
+             ncpz, npxmax, npymax, npzmax, nperi, nk, ns1, nf, ndisp, key_natpri_in, key_natpri_inps, &
+             key_soc_calc, dx, dy, dz, xmax, ymax, zmax, skpxx, skpyy, skpzz, nprj, natinf, lstvec2, &
+             indspe, natpri, naps, natsoc, ntyppp, lstx, lsty, lstz, natx, naty, natz, dij, dijsoc, &
+             veff, vnlocp, l2i(l_rrb), cspsep, sveccm, ssvcm, hsvcm, sval))
   end subroutine
 end module mod_rayleighritz_hip
 
@@ -651,5 +877,14 @@ contains
     real*8, intent(in) :: svecre(ncpx, ncpy, ncpz, neigmx, nums, numk)
     real*8, intent(out) :: rhosmt(ncpx, ncpy, ncpz, nspv)
     call kukan_check(c_charge_smt_r(neigmx, nums, nspv, numk, ncpx, ncpy, ncpz, fnele, svecre, rhosmt))
+  end subroutine
+  ! scf_charge8f.F90:324-381
+  subroutine scf_charge_smt_c( &
+      neigmx, nums, ncol, nspv, numk, ncpx, ncpy, ncpz, fnele, sveccm, rhosmt)
+    integer, intent(in) :: neigmx, nums, ncol, nspv, numk, ncpx, ncpy, ncpz
+    real*8, intent(in) :: fnele(*)
+    real*8, intent(inout) :: rhosmt(*)
+    complex*16, intent(in) :: sveccm(*)
+    call kukan_check(c_scf_charge_smt_c(neigmx, nums, ncol, nspv, numk, ncpx, ncpy, ncpz, fnele, sveccm, rhosmt))
   end subroutine
 end module mod_kukan_charge

@@ -469,6 +469,75 @@ int kukan_rayleighritz_c(int nso, int natom, int num_ppcell, int num_spe, int np
 int kukan_scf_charge_smt_c(int neigmx, int nums, int ncol, int nspv, int numk, int ncpx, int ncpy,
                            int ncpz, const double *fnele, const void *sveccm, double *rhosmt);
 
+/* ===================================================================== potential assembly
+ * The coarse <-> dense grid transfers around the Hartree solve (one rank).  Halo'd coarse
+ * arrays span -(nf-1)..n+nf per axis, halo'd dense arrays -(nf nmesh-1)..n nmesh+nf nmesh. */
+
+/* interpolation8f.F90:8-225 interpolation: is = 1, v_dd = order-2nf Lagrange prolongation of
+ * the halo'd v_cc (v_dd's halo zeroed); is = 2, v_cc = its transpose over the whole v_dd
+ * divided by nmesh^3 (v_dd's halo along the periodic axes is zeroed first, as in the
+ * reference). */
+int kukan_interpolation(int ncpx, int ncpy, int ncpz, int nmesh, double *v_dd, double *v_cc,
+                        int nperi, int nf, int is, int ndisp);
+
+/* overlap_interpolation8f.F90:8-892 overlap_interpolation on one rank: nflag = 1 fills the
+ * halo of each periodic axis by the wrap, nflag = 2 adds it onto the layers it wraps to. */
+int kukan_overlap_interpolation(int ndisp, int nperi, int ncpa, int ncpb, int ncpc, int nf, int nf1,
+                                double *v_cc, int nflag);
+
+/* trans8f.F90:8-47 trans_c2d_smtcharge: rhosmt (coarse, nspv) -> rho_dense. */
+int kukan_trans_c2d_smtcharge(int nmesh, int nspv, int nperi, int ndisp, int nf, int ncpx, int ncpy,
+                              int ncpz, const double *rhosmt, double *rho_dense);
+
+/* trans8f.F90:50-90 trans_d2c_smtcharge: rho_aug_dense -> rho_coarse. */
+int kukan_trans_d2c_smtcharge(int nmesh, int nperi, int ndisp, int nf, int ncpx, int ncpy, int ncpz,
+                              const double *rho_aug_dense, double *rho_coarse);
+
+/* trans8f.F90:93-412 trans_d2c_vh: vh_dense -> vh_coarse, the dense halo of each isolated
+ * face holding the potential of the atoms' point multipoles (atmpole(10, natom)). */
+int kukan_trans_d2c_vh(int natom, int num_spe, int nmesh, int nperi, int ndisp, int nf, int ncpx,
+                       int ncpy, int ncpz, double xmax, double ymax, double zmax, const int *indspe,
+                       const double *atmpole, const double *cp, const double *atx, const double *aty,
+                       const double *atz, double *vh_coarse, const double *vh_dense);
+
+/* trans8f.F90:415-508 trans_d2c_vxc: vx_dense (nspv) -> vx. */
+int kukan_trans_d2c_vxc(int nmesh, int nspv, int nperi, int ndisp, int nf, int ncpx, int ncpy,
+                        int ncpz, double *vx, const double *vx_dense);
+
+/* scf_rhoaugdense8f.F90:8-48 scf_rhoaugdense: rho_aug_dense = spin sum of rho_dense plus the
+ * augmentation charge of the PAW atoms at their dense list points. */
+int kukan_scf_rhoaugdense(int key_natpri_inps, int key_pp_paw, int natom, int num_spe, int nspv,
+                          int ncpx_d, int ncpy_d, int ncpz_d, int num_list_d, int num_ppcell_d,
+                          const int *indspe, const int *natprid, const int *napsd,
+                          const int *natinfd, const int *ntyppp, const int *lstvecd2,
+                          const double *rho_dense, const double *rhoaug3d, double *rho_aug_dense);
+
+/* scf_potentials8f.F90:255-289 (the V_eff sum inside scf_potentials; no routine of its own
+ * there): veff = vx + vloc + vh + bias . r for the first two spin channels. */
+int kukan_scf_veff(int nspv, int ncpx, int ncpy, int ncpz, double xmax, double ymax, double zmax,
+                   double dx, double dy, double dz, double biasx, double biasy, double biasz,
+                   const double *vloc_coarse, const double *vh_coarse, const double *vx,
+                   double *veff);
+
+/* scf_potentials8f.F90:115-289, the grid chain of scf_potentials in one call (no routine of
+ * its own in the reference): trans_c2d_smtcharge, scf_rhoaugdense, scf_hartree on the dense
+ * grid (tools_correctpotential for a periodic cell), trans_d2c_vh and the V_eff sum.  The PAW
+ * one-centre terms (rhoaug3d, atmpole, boundx/y/z) and the exchange-correlation potential vx
+ * are inputs.  vh_dense is the Hartree solver's starting guess and returns the solution. */
+int kukan_scf_potentials_smooth(int key_natpri_inps, int key_pp_paw, int nmesh, int ncpx, int ncpy,
+                                int ncpz, int natom, int num_spe, int num_list_d, int num_ppcell_d,
+                                int nspv, int ndisp, int nperi, int nf, int nfh, int ncgres,
+                                int ncgmin, int ncgmax, const int *indspe, const int *natprid,
+                                const int *napsd, const int *natinfd, const int *ntyppp,
+                                const int *lstvecd2, double epsvh, double xmax, double ymax,
+                                double zmax, double dx, double dy, double dz, double biasx,
+                                double biasy, double biasz, const double *cp, const double *atx,
+                                const double *aty, const double *atz, const double *atmpole,
+                                const double *rhosmt, const double *rhoaug3d,
+                                const double *vloc_coarse, const double *vx, const double *boundx,
+                                const double *boundy, const double *boundz, double *rho_aug_dense,
+                                double *vh_dense, double *vh_coarse, double *veff);
+
 #ifdef __cplusplus
 }
 #endif

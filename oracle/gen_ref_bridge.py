@@ -58,6 +58,13 @@ ROUTINES = {
     "orthogonalization_c_01": ("mod_orthogonalization", [], []),
     "kscg_c": ("mod_kscg", [], []),
     "ksdiis_c": ("mod_ksdiis", [], []),
+    "interpolation": ("mod_interpolation", [], []),
+    "overlap_interpolation": ("mod_overlap_interpolation", [], []),
+    "trans_c2d_smtcharge": ("mod_trans", [], []),
+    "trans_d2c_smtcharge": ("mod_trans", [], []),
+    "trans_d2c_vh": ("mod_trans", [], []),
+    "trans_d2c_vxc": ("mod_trans", [], []),
+    "scf_rhoaugdense": ("mod_scf_rhoaugdense", [], []),
     "rayleighritz_c": ("mod_rayleighritz", [], []),
     "scf_charge_smt_c": ("mod_scf_charge", [], []),
 }

@@ -242,6 +242,29 @@ int orc_scf_charge_smt_c(int neigmx, int nums, int ncol, int nspv, int numk, int
                          int ncpz, const double *fnele, const double *sveccm, double *rhosmt);
 /* Hermitian eigensolver (cyclic Jacobi; upper triangle read), test helper */
 int orc_zheev(int n, double *a, double *w);
+/* kukan_oracle_trans.c: potential-assembly grid transfers */
+int orc_overlap_interpolation(int ndisp, int nperi, int ncpa, int ncpb, int ncpc, int nf, int nf1,
+                              double *v_cc, int nflag);
+int orc_interpolation(int ncpx, int ncpy, int ncpz, int nmesh, double *v_dd, double *v_cc, int nperi,
+                      int nf, int is, int ndisp);
+int orc_trans_c2d_smtcharge(int nmesh, int nspv, int nperi, int ndisp, int nf, int ncpx, int ncpy, int ncpz,
+                            const double *rhosmt, double *rho_dense);
+int orc_trans_d2c_smtcharge(int nmesh, int nperi, int ndisp, int nf, int ncpx, int ncpy, int ncpz,
+                            const double *rho_aug_dense, double *rho_coarse);
+int orc_trans_d2c_vh(int natom, int num_spe, int nmesh, int nperi, int ndisp, int nf, int ncpx,
+                     int ncpy, int ncpz, double xmax, double ymax, double zmax, const int *indspe,
+                     const double *atmpole, const double *cp, const double *atx, const double *aty,
+                     const double *atz, double *vh_coarse, const double *vh_dense);
+int orc_trans_d2c_vxc(int nmesh, int nspv, int nperi, int ndisp, int nf, int ncpx, int ncpy, int ncpz,
+                      double *vx, const double *vx_dense);
+int orc_scf_rhoaugdense(int key_natpri_inps, int key_pp_paw, int natom, int num_spe, int nspv,
+                        int ncpx_d, int ncpy_d, int ncpz_d, int num_list_d, int num_ppcell_d,
+                        const int *indspe, const int *natprid, const int *napsd, const int *natinfd,
+                        const int *ntyppp, const int *lstvecd2, const double *rho_dense,
+                        const double *rhoaug3d, double *rho_aug_dense);
+int orc_scf_veff(int nspv, int ncpx, int ncpy, int ncpz, double xmax, double ymax, double zmax,
+                 double dx, double dy, double dz, double biasx, double biasy, double biasz,
+                 const double *vloc_coarse, const double *vh_coarse, const double *vx, double *veff);
 int orc_ksdiis_c(int l_hsv, int nso, int natom, int num_spe, int neigmx, int nums, int ncol,
                  int nspv, int numk, int nperi, int ndiismax, int nprecon_diis, int nk, int ns1,
                  int nf, int nprjmx, int num_list, int num_ppcell, int ndisp, int northo, int ncpx,

@@ -279,6 +279,50 @@ BLOCH_SIGNATURES = {
 }
 SIGNATURES.update(BLOCH_SIGNATURES)
 
+# potential assembly: the coarse <-> dense grid transfers and the augmented dense density
+# (reference argument lists), and the V_eff sum of scf_potentials
+TRANS_SIGNATURES = {
+    "interpolation": [("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("nmesh", "i"), ("v_dd", "D"),
+                      ("v_cc", "D"), ("nperi", "i"), ("nf", "i"), ("is", "i"), ("ndisp", "i")],
+    "overlap_interpolation": [("ndisp", "i"), ("nperi", "i"), ("ncpa", "i"), ("ncpb", "i"),
+                              ("ncpc", "i"), ("nf", "i"), ("nf1", "i"), ("v_cc", "D"), ("nflag", "i")],
+    "trans_c2d_smtcharge": [("nmesh", "i"), ("nspv", "i"), ("nperi", "i"), ("ndisp", "i"), ("nf", "i"),
+                            ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("rhosmt", "D"),
+                            ("rho_dense", "D")],
+    "trans_d2c_smtcharge": [("nmesh", "i"), ("nperi", "i"), ("ndisp", "i"), ("nf", "i"), ("ncpx", "i"),
+                            ("ncpy", "i"), ("ncpz", "i"), ("rho_aug_dense", "D"), ("rho_coarse", "D")],
+    "trans_d2c_vh": [("natom", "i"), ("num_spe", "i"), ("nmesh", "i"), ("nperi", "i"), ("ndisp", "i"),
+                     ("nf", "i"), ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("xmax", "d"),
+                     ("ymax", "d"), ("zmax", "d"), ("indspe", "I"), ("atmpole", "D"), ("cp", "D"),
+                     ("atx", "D"), ("aty", "D"), ("atz", "D"), ("vh_coarse", "D"), ("vh_dense", "D")],
+    "trans_d2c_vxc": [("nmesh", "i"), ("nspv", "i"), ("nperi", "i"), ("ndisp", "i"), ("nf", "i"),
+                      ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("vx", "D"), ("vx_dense", "D")],
+    "scf_rhoaugdense": [("key_natpri_inps", "i"), ("key_pp_paw", "i"), ("natom", "i"), ("num_spe", "i"),
+                        ("nspv", "i"), ("ncpx_d", "i"), ("ncpy_d", "i"), ("ncpz_d", "i"),
+                        ("num_list_d", "i"), ("num_ppcell_d", "i"), ("indspe", "I"), ("natprid", "I"),
+                        ("napsd", "I"), ("natinfd", "I"), ("ntyppp", "I"), ("lstvecd2", "I"),
+                        ("rho_dense", "D"), ("rhoaug3d", "D"), ("rho_aug_dense", "D")],
+    # no reference routine of its own: the last loop of scf_potentials (scf_potentials8f.F90:255-289)
+    "scf_veff": [("nspv", "i"), ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"), ("xmax", "d"),
+                 ("ymax", "d"), ("zmax", "d"), ("dx", "d"), ("dy", "d"), ("dz", "d"), ("biasx", "d"),
+                 ("biasy", "d"), ("biasz", "d"), ("vloc_coarse", "D"), ("vh_coarse", "D"), ("vx", "D"),
+                 ("veff", "D")],
+    # no reference routine of its own: scf_potentials' grid chain from the smooth density to
+    # V_eff (scf_potentials8f.F90:115-289 without the PAW one-centre and XC terms)
+    "scf_potentials_smooth": [
+        ("key_natpri_inps", "i"), ("key_pp_paw", "i"), ("nmesh", "i"), ("ncpx", "i"), ("ncpy", "i"),
+        ("ncpz", "i"), ("natom", "i"), ("num_spe", "i"), ("num_list_d", "i"), ("num_ppcell_d", "i"),
+        ("nspv", "i"), ("ndisp", "i"), ("nperi", "i"), ("nf", "i"), ("nfh", "i"), ("ncgres", "i"),
+        ("ncgmin", "i"), ("ncgmax", "i"), ("indspe", "I"), ("natprid", "I"), ("napsd", "I"),
+        ("natinfd", "I"), ("ntyppp", "I"), ("lstvecd2", "I"), ("epsvh", "d"), ("xmax", "d"),
+        ("ymax", "d"), ("zmax", "d"), ("dx", "d"), ("dy", "d"), ("dz", "d"), ("biasx", "d"),
+        ("biasy", "d"), ("biasz", "d"), ("cp", "D"), ("atx", "D"), ("aty", "D"), ("atz", "D"),
+        ("atmpole", "D"), ("rhosmt", "D"), ("rhoaug3d", "D"), ("vloc_coarse", "D"), ("vx", "D"),
+        ("boundx", "D"), ("boundy", "D"), ("boundz", "D"), ("rho_aug_dense", "D"),
+        ("vh_dense", "D"), ("vh_coarse", "D"), ("veff", "D")],
+}
+SIGNATURES.update(TRANS_SIGNATURES)
+
 _CT = {"i": ctypes.c_int, "d": ctypes.c_double, "I": ctypes.c_void_p, "D": ctypes.c_void_p,
        "Z": ctypes.c_void_p, "oi": ctypes.POINTER(ctypes.c_int),
        "od": ctypes.POINTER(ctypes.c_double)}

@@ -538,6 +538,44 @@ int kukan_scf_potentials_smooth(int key_natpri_inps, int key_pp_paw, int nmesh, 
                                 const double *boundy, const double *boundz, double *rho_aug_dense,
                                 double *vh_dense, double *vh_coarse, double *veff);
 
+/* ===================================================================== checkpoint files
+ * rw_wfpot8f.F90 on one rank: wfc.* (version 5), rho.* / r_o.* / ele.* and vht.* (version 3),
+ * Fortran sequential unformatted records.  chdir is a NUL-terminated directory name (blank or
+ * empty: the working directory); a negative rank argument means rank 0.  Arrays may be host or
+ * device pointers; device arrays stream through pinned staging buffers, never a whole host
+ * copy. */
+
+/* rw_wfpot8f.F90:352-399 write_wf */
+int kukan_write_wf(int nr1In, int nr2In, const char *chdir, int nrc, int nums, int ncol, int neigmx,
+                   int numk, int ncpx, int ncpy, int ncpz, int nrxIn, int nryIn, int nrzIn, int ksym,
+                   const double *svecre, const void *sveccm, const double *sval);
+
+/* rw_wfpot8f.F90:7-185 read_wf, with the reference's mappings from a file of other spin,
+ * noncollinear, real / complex or half-k-point layout. */
+int kukan_read_wf(int nr1In, int nr2In, const char *chdir, int nrc, int nums, int ncol, int neigmx,
+                  int numk, int ncpx, int ncpy, int ncpz, int ksym, int key_ksym_inv2, double *svecre,
+                  void *sveccm, double *sval);
+
+/* rw_wfpot8f.F90:188-302 read_rho */
+int kukan_read_rho(int nrIn, const char *chdir, int nevhist, int nspv, int ncpx, int ncpy, int ncpz,
+                   int nradmx, int npoint, int num_atcell, int *lbrydn, int *na_atcell,
+                   int *natcell_inf, double *rhosmt, double *rhotrur, double *rhosmtr);
+
+/* rw_wfpot8f.F90:305-349 read_vht */
+int kukan_read_vht(int nrIn, const char *chdir, int nevhist, int ncpx, int ncpy, int ncpz, int nmesh,
+                   double *vh_dense);
+
+/* rw_wfpot8f.F90:402-505 write_rhovht */
+int kukan_write_rhovht(int nrIn, const char *chdir, int nrho, int nele, int npre, int nevhist, int nspv,
+                       int ncpx, int ncpy, int ncpz, int nmesh, int nradmx, int npoint, int num_atcell,
+                       int nrxIn, int nryIn, int nrzIn, int natom, int key_natpri_in, const int *natpri,
+                       const double *rhosmt_i, const double *rhotrur_i, const double *rhosmtr_i,
+                       const double *rhosmt_o, const double *rhotrur_o, const double *rhosmtr_o,
+                       const double *vh_dense);
+
+/* rw_wfpot8f.F90:506-532 filenumbers_rhovht: how many rho.* and vht.* files exist. */
+int kukan_filenumbers_rhovht(const char *chdir, int *nrrho, int *nrpot);
+
 #ifdef __cplusplus
 }
 #endif

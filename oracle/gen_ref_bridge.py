@@ -65,6 +65,12 @@ ROUTINES = {
     "trans_d2c_vh": ("mod_trans", [], []),
     "trans_d2c_vxc": ("mod_trans", [], []),
     "scf_rhoaugdense": ("mod_scf_rhoaugdense", [], []),
+    "write_wf": ("mod_rw_wfpot", [], []),
+    "read_wf": ("mod_rw_wfpot", [], []),
+    "read_rho": ("mod_rw_wfpot", [], []),
+    "read_vht": ("mod_rw_wfpot", [], []),
+    "write_rhovht": ("mod_rw_wfpot", [], []),
+    "filenumbers_rhovht": ("mod_rw_wfpot", [], []),
     "rayleighritz_c": ("mod_rayleighritz", [], []),
     "scf_charge_smt_c": ("mod_scf_charge", [], []),
 }
@@ -98,6 +104,16 @@ def wrapper(name):
         elif t == "D":
             decl.append(f"real(c_double) :: {a}(*)")
             actual.append(a)
+        elif t == "S":  # character(*) in the reference: a NUL-terminated C string
+            decl.append(f"character(kind=c_char) :: {a}(*)")
+            decl.append(f"character(len=1024) :: s__{a}")
+            decl.append(f"integer :: i__{a}")
+            conv_in.append(f"s__{a} = ' '")
+            conv_in.append(f"do i__{a} = 1, 1024")
+            conv_in.append(f"  if ({a}(i__{a}) == c_null_char) exit")
+            conv_in.append(f"  s__{a}(i__{a}:i__{a}) = {a}(i__{a})")
+            conv_in.append("end do")
+            actual.append(f"trim(s__{a})")
         elif t == "Z":  # complex*16 in the reference
             decl.append(f"complex(c_double_complex) :: {a}(*)")
             actual.append(a)

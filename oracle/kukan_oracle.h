@@ -242,6 +242,26 @@ int orc_scf_charge_smt_c(int neigmx, int nums, int ncol, int nspv, int numk, int
                          int ncpz, const double *fnele, const double *sveccm, double *rhosmt);
 /* Hermitian eigensolver (cyclic Jacobi; upper triangle read), test helper */
 int orc_zheev(int n, double *a, double *w);
+/* kukan_oracle_ckpt.c: checkpoint files */
+int orc_ckpt_path(const char *chdir, const char *chnam, int nr1, int nr2, char *out, int len);
+int orc_write_wf(int nr1In, int nr2In, const char *chdir, int nrc, int nums, int ncol, int neigmx,
+                 int numk, int ncpx, int ncpy, int ncpz, int nrxIn, int nryIn, int nrzIn, int ksym,
+                 const double *svecre, const double *sveccm, const double *sval);
+int orc_read_wf(int nr1In, int nr2In, const char *chdir, int nrc, int nums, int ncol, int neigmx,
+                int numk, int ncpx, int ncpy, int ncpz, int ksym, int key_ksym_inv2, double *svecre,
+                double *sveccm, double *sval);
+int orc_read_rho(int nrIn, const char *chdir, int nevhist, int nspv, int ncpx, int ncpy, int ncpz,
+                 int nradmx, int npoint, int num_atcell, int *lbrydn, int *na_atcell,
+                 int *natcell_inf, double *rhosmt, double *rhotrur, double *rhosmtr);
+int orc_read_vht(int nrIn, const char *chdir, int nevhist, int ncpx, int ncpy, int ncpz, int nmesh,
+                 double *vh_dense);
+int orc_write_rhovht(int nrIn, const char *chdir, int nrho, int nele, int npre, int nevhist, int nspv,
+                     int ncpx, int ncpy, int ncpz, int nmesh, int nradmx, int npoint, int num_atcell,
+                     int nrxIn, int nryIn, int nrzIn, int natom, int key_natpri_in, const int *natpri,
+                     const double *rhosmt_i, const double *rhotrur_i, const double *rhosmtr_i,
+                     const double *rhosmt_o, const double *rhotrur_o, const double *rhosmtr_o,
+                     const double *vh_dense);
+int orc_filenumbers_rhovht(const char *chdir, int *nrrho, int *nrpot);
 /* kukan_oracle_trans.c: potential-assembly grid transfers */
 int orc_overlap_interpolation(int ndisp, int nperi, int ncpa, int ncpb, int ncpc, int nf, int nf1,
                               double *v_cc, int nflag);

@@ -323,9 +323,37 @@ TRANS_SIGNATURES = {
 }
 SIGNATURES.update(TRANS_SIGNATURES)
 
+# checkpoint files wfc.* / rho.* / r_o.* / vht.* (rw_wfpot8f.F90; S = character, passed as a
+# NUL-terminated string)
+CKPT_SIGNATURES = {
+    "write_wf": [("nr1In", "i"), ("nr2In", "i"), ("chdir", "S"), ("nrc", "i"), ("nums", "i"),
+                 ("ncol", "i"), ("neigmx", "i"), ("numk", "i"), ("ncpx", "i"), ("ncpy", "i"),
+                 ("ncpz", "i"), ("nrxIn", "i"), ("nryIn", "i"), ("nrzIn", "i"), ("ksym", "i"),
+                 ("svecre", "D"), ("sveccm", "Z"), ("sval", "D")],
+    "read_wf": [("nr1In", "i"), ("nr2In", "i"), ("chdir", "S"), ("nrc", "i"), ("nums", "i"),
+                ("ncol", "i"), ("neigmx", "i"), ("numk", "i"), ("ncpx", "i"), ("ncpy", "i"),
+                ("ncpz", "i"), ("ksym", "i"), ("key_ksym_inv2", "i"), ("svecre", "D"),
+                ("sveccm", "Z"), ("sval", "D")],
+    "read_rho": [("nrIn", "i"), ("chdir", "S"), ("nevhist", "i"), ("nspv", "i"), ("ncpx", "i"),
+                 ("ncpy", "i"), ("ncpz", "i"), ("nradmx", "i"), ("npoint", "i"),
+                 ("num_atcell", "i"), ("lbrydn", "oi"), ("na_atcell", "oi"), ("natcell_inf", "I"),
+                 ("rhosmt", "D"), ("rhotrur", "D"), ("rhosmtr", "D")],
+    "read_vht": [("nrIn", "i"), ("chdir", "S"), ("nevhist", "i"), ("ncpx", "i"), ("ncpy", "i"),
+                 ("ncpz", "i"), ("nmesh", "i"), ("vh_dense", "D")],
+    "write_rhovht": [("nrIn", "i"), ("chdir", "S"), ("nrho", "i"), ("nele", "i"), ("npre", "i"),
+                     ("nevhist", "i"), ("nspv", "i"), ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"),
+                     ("nmesh", "i"), ("nradmx", "i"), ("npoint", "i"), ("num_atcell", "i"),
+                     ("nrxIn", "i"), ("nryIn", "i"), ("nrzIn", "i"), ("natom", "i"),
+                     ("key_natpri_in", "i"), ("natpri", "I"), ("rhosmt_i", "D"), ("rhotrur_i", "D"),
+                     ("rhosmtr_i", "D"), ("rhosmt_o", "D"), ("rhotrur_o", "D"), ("rhosmtr_o", "D"),
+                     ("vh_dense", "D")],
+    "filenumbers_rhovht": [("chdir", "S"), ("nrrho", "oi"), ("nrpot", "oi")],
+}
+SIGNATURES.update(CKPT_SIGNATURES)
+
 _CT = {"i": ctypes.c_int, "d": ctypes.c_double, "I": ctypes.c_void_p, "D": ctypes.c_void_p,
        "Z": ctypes.c_void_p, "oi": ctypes.POINTER(ctypes.c_int),
-       "od": ctypes.POINTER(ctypes.c_double)}
+       "od": ctypes.POINTER(ctypes.c_double), "S": ctypes.c_char_p}
 
 
 def bind(lib, prefix, table=None):
@@ -376,6 +404,9 @@ def call(lib, prefix, name, err=None, _table=None, **kw):
             if an not in kw:
                 raise TypeError(f"{name}: missing scalar {an}")
             cargs.append(kw[an])
+        elif t == "S":  # a character argument: NUL-terminated bytes
+            v = kw.get(an, "")
+            cargs.append(v.encode() if isinstance(v, str) else bytes(v))
         elif t in ("oi", "od"):
             box = (ctypes.c_int if t == "oi" else ctypes.c_double)(kw.get(an, 0))
             outs[an] = box

@@ -18,8 +18,9 @@ state instead of the random start).  From that same state:
 Beside the headline it reports the other workloads of the suite at their sizes: 40^3 x 32
 bands (H psi + a band-sequential CG sweep + Hartree, GPU and CPU oracle), 96^3 x 512 bands in
 FP64 (ten RR + CG + Gram-Schmidt iterations and the Hartree solve), the 96^3 x 256-band
-energy with its adjoint (forward + backward), and the 48-system batch (batched vs per-system
-loop).  Prints one JSON line.
+energy with its adjoint (forward + backward), the 48-system batch (batched vs per-system loop),
+and the projector pipeline at the headline cell (list vectors, vnlocp and forces for 128 atoms).
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -288,6 +289,118 @@ def config4(kk):
             "device_ms_per_step": {t: round(v[0] / reps, 2) for t, v in tim.items() if v[1] > 0}}
 
 
+def config_pp(kk, cpu=True):
+    """The projector pipeline at the headline cell: 128^3 h=0.2, 128 atoms of two species (9 and
+    18 projectors), nfdg 4, npmesh 2, nmesh 2, two spins: list vectors (pseudocalc_01b), vnlocp
+    (pseudocalc_07) and force() with every array in HBM.  The CPU oracle is timed on a sample
+    (the lists and vnlocp of 4 atoms, the forces of 2 atoms with one spin) and scaled by atoms
+    and spins; it is a reference figure, not the target."""
+    from rspacedist_amd import pp_synth as PS
+    d, x = PS.cell(seed=128, nperi=3, nxmax=64, nymax=64, nzmax=64, h=0.2, nmesh=2, nfdg=4, npmesh=2,
+                   natom=128, nprj=(9, 18), rc=(1.0, 0.8), nqmx=24)
+    npx = int(np.ceil(d["radial"].max() * d["psctoff"] / d["dx"])) + d["nfdg"] + 1
+    d.update(npxmax=npx, npymax=npx, npzmax=npx, num_list=(2 * npx) ** 3, num_list_d=(2 * npx * 2) ** 3)
+    dev = lambda a: kk.DeviceArray.from_numpy(a)
+    L = {k: dev(v) for k, v in PS.list_outputs(d).items()}
+    w = dev(np.zeros(2 * d["nfdg"] * x["npmesh"]))
+    kk.call("pseudocalc_01a", nfdg=d["nfdg"], npmesh=x["npmesh"], wxyz=w)
+    args01b = {**d, **L}
+    n = d["ncpx"] * d["ncpy"] * d["ncpz"]
+    nd, na, nspv = n * 8, d["natom"], 2
+    host_lists = None
+
+    def run(timed):
+        nonlocal host_lists
+        t = {}
+        kk.synchronize()
+        t0 = time.perf_counter()
+        kk.call("pseudocalc_01b", **args01b)
+        kk.synchronize()
+        t["lists_s"] = time.perf_counter() - t0
+        if host_lists is None:
+            host_lists = {k: v.numpy() for k, v in L.items()}
+        a = PS.pp07_args(d, x, host_lists, w)
+        a.update({k: L[k] for k in ("natpri", "naps", "natinf", "natx", "naty", "natz", "lstx", "lsty", "lstz")})
+        a.update(vnlocp=vnl, vtmp_pp=dummy, vtmp_pp1=dummy, vtmp_pp2=dummy, vnlspd=dummy)
+        t0 = time.perf_counter()
+        kk.call("pseudocalc_07", **a)
+        kk.synchronize()
+        t["vnlocp_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        kk.call("force", **fargs)
+        kk.synchronize()
+        t["force_s"] = time.perf_counter() - t0
+        return t
+
+    vnl = kk.DeviceArray((d["num_list"], x["nprjmx"], d["num_ppcell"]))
+    dummy = np.zeros(1)
+    # force() inputs: the big fields seeded on the device; the small tables from the host
+    fargs = PS.force_args(dict(d, natom=1, num_atcell=1, num_ppcell_d=1),
+                          x, {k: np.zeros(1, dtype=np.int32) for k in ("natpri", "natpri_inf", "natprid", "napsd",
+                                                                       "natinfd", "natinfd_vloc", "lstvecd2",
+                                                                       "lstdx", "lstdy", "lstdz", "ndatx", "ndaty",
+                                                                       "ndatz")}, nspv=nspv, new_pw=3, new_rs=1)
+    fargs.update({k: d[k] for k in ("natom", "num_atcell", "num_ppcell_d", "indspe", "atx", "aty", "atz")})
+    fargs.update({k: L[k] for k in ("natpri", "natpri_inf", "natprid", "napsd", "natinfd", "natinfd_vloc",
+                                    "lstvecd2", "lstdx", "lstdy", "lstdz", "ndatx", "ndaty", "ndatz")})
+    nld = d["num_list_d"] * d["num_ppcell_d"]
+    big = {}
+    for i, (k, shape) in enumerate([(f"dvlocd{c}_scw", (n, na)) for c in "xyz"] +
+                                   [(f"dvlocd{c}_hdp", (nld,)) for c in "xyz"] +
+                                   [(f"drhoaug3dd{c}", (nld,)) for c in "xyz"] +
+                                   [("rho_coarse", (n,)), ("rho_aug_dense", (nd,)), ("vloc_dense", (nd,)),
+                                    ("vh_dense", (nd,)), ("vx_dense", (nd, nspv))]):
+        big[k] = kk.DeviceArray(shape).fill_normal(900 + i)
+    fargs.update(big)
+    sh = (d["nradmx"], fargs["npoint"], nspv, na)
+    fargs.update(rhotrur=np.zeros(sh, order="F"), rhosmtr=np.zeros(sh, order="F"),
+                 rhoaugr=np.zeros((d["nradmx"], fargs["npoint"], na), order="F"))
+    for c in ("fatx", "faty", "fatz"):
+        fargs[c] = dev(np.zeros(na))
+    run(False)  # first call: buffers
+    kk.timing_reset()
+    kk.set_param("timing", 1)
+    t = run(True)
+    kk.set_param("timing", 0)
+    tim = kk.timing()
+    out = {"workload": f"128^3 h=0.2, {na} atoms (9 and 18 projectors), nfdg 4, npmesh 2, nmesh 2, {nspv} spins: "
+                       "pseudocalc_01b lists + pseudocalc_07 vnlocp + force()",
+           "natinf_mean": float(np.mean(host_lists["natinf"])), "num_list": d["num_list"],
+           **{k: round(v, 4) for k, v in t.items()},
+           "device_ms": {k: round(v[0], 3) for k, v in tim.items() if v[1] > 0 and not k.startswith("phase_")},
+           "force_local": roofline_entry(tim, "force_local")}
+    for v in list(big.values()) + [vnl, w] + list(L.values()):
+        v.free()
+    if cpu:
+        from oracle import oracle as orc
+        oracle_threads()
+        s4 = dict(d, natom=4, num_atcell=4, num_ppcell=4, num_ppcell_d=4, indspe=d["indspe"][:4].copy(),
+                  atx=d["atx"][:4].copy(), aty=d["aty"][:4].copy(), atz=d["atz"][:4].copy())
+        o = {**s4, **PS.list_outputs(s4)}
+        t0 = time.perf_counter()
+        orc.call("pseudocalc_01b", **o)
+        t1 = time.perf_counter()
+        wh = np.zeros(2 * d["nfdg"] * x["npmesh"])
+        orc.call("pseudocalc_01a", nfdg=d["nfdg"], npmesh=x["npmesh"], wxyz=wh)
+        a = PS.pp07_args(s4, x, o, wh)
+        t2 = time.perf_counter()
+        orc.call("pseudocalc_07", **a)
+        t3 = time.perf_counter()
+        s2 = dict(s4, natom=2, num_atcell=2, num_ppcell=2, num_ppcell_d=2, indspe=d["indspe"][:2].copy(),
+                  atx=d["atx"][:2].copy(), aty=d["aty"][:2].copy(), atz=d["atz"][:2].copy())
+        o2 = {**s2, **PS.list_outputs(s2)}
+        orc.call("pseudocalc_01b", **o2)
+        fa = PS.force_args(s2, x, o2, nspv=1, new_pw=3, new_rs=1)
+        t4 = time.perf_counter()
+        orc.call("force", **fa)
+        t5 = time.perf_counter()
+        cpu_s = (t1 - t0 + t3 - t2) * na / 4 + (t5 - t4) * na / 2 * nspv
+        out["cpu_oracle_s"] = round(cpu_s, 2)
+        out["cpu_sample"] = (f"oracle (threads {oracle_threads()}) lists + vnlocp of 4 atoms ({t1 - t0 + t3 - t2:.2f} s) "
+                             f"x {na // 4}; force of 2 atoms, 1 spin ({t5 - t4:.2f} s) x {na // 2 * nspv}")
+    return out
+
+
 def cpu_baseline(cfg):
     """The FP64 oracle (a C/OpenMP restatement of the reference's band-sequential path) on
     this host, as an extrapolation to the full step (a full step at 128^3 x 1024 bands is about
@@ -455,7 +568,8 @@ def main():
         for name, fn in (("c0_40cube_32bands", lambda: config0(kk, synth)),
                          ("c1_96cube_512bands_fp64", lambda: config1(kk)),
                          ("c3_96cube_256bands_adjoint", lambda: config3(kk)),
-                         ("c4_48_systems_batch", lambda: config4(kk))):
+                         ("c4_48_systems_batch", lambda: config4(kk)),
+                         ("pp_128cube_128atoms", lambda: config_pp(kk, cpu=not args.no_cpu_baseline))):
             log(f"[bench] {name}")
             t0 = time.perf_counter()
             suite[name] = fn()

@@ -576,6 +576,72 @@ int kukan_write_rhovht(int nrIn, const char *chdir, int nrho, int nele, int npre
 /* rw_wfpot8f.F90:506-532 filenumbers_rhovht: how many rho.* and vht.* files exist. */
 int kukan_filenumbers_rhovht(const char *chdir, int *nrrho, int *nrpot);
 
+/* ---------------------------------------------------------------------------------------------
+ * Projector tables, list vectors and forces (pseudocalc8f.F90, force8f.F90), one rank.
+ * Arrays may be host or device memory, as everywhere in this library.
+ * ------------------------------------------------------------------------------------------- */
+
+/* pseudocalc8f.F90:343-629 pseudocalc_01a: double-grid weights wxyz(-nfdg*npmesh:nfdg*npmesh-1)
+ * (nfdg 1..8; the first element is not used for odd npmesh and is set to zero). */
+int kukan_pseudocalc_01a(int nfdg, int npmesh, double *wxyz);
+
+/* pseudocalc8f.F90:633-1159 pseudocalc_01b: atom information (natx.., ndatx.., natpri, natprid,
+ * natpri_inf, naps, napsd) and the list vectors of the projector spheres (lstvec2, lstx/y/z,
+ * natinf) and of the compensation-charge spheres on the dense grid (lstvecd2, lstdx/y/z, natinfd,
+ * natinfd_vloc), bit-identical to the reference on one rank (ncpx = 2 nxmax ...).  List columns
+ * of atoms that are not listed keep their contents.  Errors: the reference's messages for
+ * num_atcell, num_ppcell, num_ppcell_d, num_list and num_list_d too small. */
+int kukan_pseudocalc_01b(int nperi, int natom, int num_spe, int nradmx, int nmesh, int nfdg, int ndisp,
+                         int num_atcell, int num_ppcell, int num_ppcell_d, int num_list, int num_list_d,
+                         int nxmax, int nymax, int nzmax, int ncpx, int ncpy, int ncpz, int npxmax, int npymax,
+                         int npzmax, int key_natpri_in, int key_natpri_inps, int key_natpri_out, double psctoff,
+                         double xmax, double ymax, double zmax, double dx, double dy, double dz,
+                         const int *indspe, const int *nradct, const double *radial, const double *atx,
+                         const double *aty, const double *atz, int *natx, int *naty, int *natz, int *ndatx,
+                         int *ndaty, int *ndatz, int *lstx, int *lsty, int *lstz, int *lstdx, int *lstdy,
+                         int *lstdz, int *natinf, int *natinfd, int *natinfd_vloc, int *lstvec2, int *lstvecd2,
+                         int *natpri, int *natprid, int *natpri_inf, int *naps, int *napsd);
+
+/* pseudocalc8f.F90:2515-3702 pseudocalc_07: the nonlocal projector table vnlocp(num_list, nprjmx,
+ * num_ppcell) from the Bessel expansions coef(0:nqmx, 0:7, num_spe) on the fine grid dx/npmesh,
+ * filtered onto the coarse list points with wxyz.  vtmp_pp, vtmp_pp1, vtmp_pp2 and vnlspd are the
+ * reference's work arrays; the library works in its own device memory and leaves them as they
+ * are.  nxspd.. as the reference sizes them (pseudocalc8f.F90:227-229) or larger. */
+int kukan_pseudocalc_07(int natom, int num_spe, int npmesh, int nprjmx, int nfdg, int nfiltyp, int num_list,
+                        int num_ppcell, int nradmx, int nqmx, int nxmax, int nymax, int nzmax, int npxmax,
+                        int npymax, int npzmax, int nxspd, int nyspd, int nzspd, int key_natpri_in,
+                        int key_natpri_inps, double eps, double psftrad, double psctoff, double filpp, double dx,
+                        double dy, double dz, const int *indspe, const int *nprj, const int *nradct,
+                        const int *natpri, const int *naps, const int *natinf, const int *nqct, const int *natx,
+                        const int *naty, const int *natz, const int *lstx, const int *lsty, const int *lstz,
+                        const double *wxyz, const double *radial, const double *coef, const double *atx,
+                        const double *aty, const double *atz, double *vnlocp, double *vtmp_pp, double *vtmp_pp1,
+                        double *vtmp_pp2, double *vnlspd);
+
+/* force8f.F90:1617-2013 force (with force_01 :2016-2109 and force_02 :2112-2190): adds to
+ * fatx/faty/fatz the local-potential and compensation-charge forces, the partial-core-correction
+ * force, the jellium slab force (jelcalc == key_jel_calc), the core-core interaction (Coulomb for
+ * nperi = 0, Ewald sums for 1..3 periodic directions), the bias field, and removes the net force;
+ * fjel is the jellium's force.  One rank: ncpx = 2 nxmax ..., ncpx_d = ncpx * nmesh. */
+int kukan_force(int natom, int num_spe, int num_atcell, int num_ppcell_d, int num_list_d, int nqmx, int nradmx,
+                int npoint, int nmesh, int nspv, int nperi, int nint1dmax, int nzmax, int jelcalc, int ncpx,
+                int ncpy, int ncpz, int ncpx_d, int ncpy_d, int ncpz_d, int new_pwx, int new_pwy, int new_pwz,
+                int new_rsx, int new_rsy, int new_rsz, int key_jel_calc, int key_natpri_in, int key_natpri_inps,
+                const int *indspe, const int *natpri, const int *natpri_inf, const int *nradct,
+                const int *natprid, const int *napsd, const int *natinfd, const int *natinfd_vloc,
+                const int *lstvecd2, const int *nqctpcc, const int *lstdx, const int *lstdy, const int *lstdz,
+                double veta, double psctoff, double psftrad, const double *radial, const double *dradial,
+                const double *point, const double *wt, double xmax, double ymax, double zmax, double biasx,
+                double biasy, double biasz, const double *cp, const double *coef, const double *atx,
+                const double *aty, const double *atz, const double *dvlocdx_scw, const double *dvlocdy_scw,
+                const double *dvlocdz_scw, const double *dvlocdx_hdp, const double *dvlocdy_hdp,
+                const double *dvlocdz_hdp, const double *rhotrur, const double *rhosmtr, const double *rhoaugr,
+                const double *drhoaug3ddx, const double *drhoaug3ddy, const double *drhoaug3ddz,
+                const double *rho_coarse, const double *rho_aug_dense, const double *vloc_dense,
+                const double *vh_dense, const double *vx_dense, double chrjel, double strjel, double endjel,
+                const int *ndatx, const int *ndaty, const int *ndatz, double *fatx, double *faty,
+                double *fatz, double *fjel);
+
 #ifdef __cplusplus
 }
 #endif

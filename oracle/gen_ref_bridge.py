@@ -73,6 +73,10 @@ ROUTINES = {
     "filenumbers_rhovht": ("mod_rw_wfpot", [], []),
     "rayleighritz_c": ("mod_rayleighritz", [], []),
     "scf_charge_smt_c": ("mod_scf_charge", [], []),
+    "pseudocalc_01a": ("mod_pseudocalc", [], []),
+    "pseudocalc_01b": ("mod_pseudocalc", [], []),
+    "pseudocalc_07": ("mod_pseudocalc", [], []),
+    "force": ("mod_force", [], []),
 }
 # C int arguments that are Fortran logicals in the reference
 LOGICALS = {"ksconv", "l_hsv", "l_rrb"}

@@ -300,6 +300,53 @@ int orc_ksdiis_c(int l_hsv, int nso, int natom, int num_spe, int neigmx, int num
                  const double *vnlocp, const double *sss, int *ksconv, int *ksitmax, double *cspsep,
                  double *sval, double *sveccm, double *ssvcm, double *residual_states,
                  double *hsvcm);
+/* kukan_oracle_pp.c: projector tables, list vectors and forces */
+int orc_pseudocalc_01a(int nfdg, int npmesh, double *wxyz);
+int orc_pseudocalc_01b(int nperi, int natom, int num_spe, int nradmx, int nmesh, int nfdg,
+                       int ndisp, int num_atcell, int num_ppcell, int num_ppcell_d, int num_list,
+                       int num_list_d, int nxmax, int nymax, int nzmax, int ncpx, int ncpy,
+                       int ncpz, int npxmax, int npymax, int npzmax, int key_natpri_in,
+                       int key_natpri_inps, int key_natpri_out, double psctoff, double xmax,
+                       double ymax, double zmax, double dx, double dy, double dz,
+                       const int *indspe, const int *nradct, const double *radial,
+                       const double *atx, const double *aty, const double *atz, int *natx,
+                       int *naty, int *natz, int *ndatx, int *ndaty, int *ndatz, int *lstx,
+                       int *lsty, int *lstz, int *lstdx, int *lstdy, int *lstdz, int *natinf,
+                       int *natinfd, int *natinfd_vloc, int *lstvec2, int *lstvecd2, int *natpri,
+                       int *natprid, int *natpri_inf, int *naps, int *napsd);
+int orc_pseudocalc_07(int natom, int num_spe, int npmesh, int nprjmx, int nfdg, int nfiltyp,
+                      int num_list, int num_ppcell, int nradmx, int nqmx, int nxmax, int nymax,
+                      int nzmax, int npxmax, int npymax, int npzmax, int nxspd, int nyspd,
+                      int nzspd, int key_natpri_in, int key_natpri_inps, double eps,
+                      double psftrad, double psctoff, double filpp, double dx, double dy,
+                      double dz, const int *indspe, const int *nprj, const int *nradct,
+                      const int *natpri, const int *naps, const int *natinf, const int *nqct,
+                      const int *natx, const int *naty, const int *natz, const int *lstx,
+                      const int *lsty, const int *lstz, const double *wxyz, const double *radial,
+                      const double *coef, const double *atx, const double *aty,
+                      const double *atz, double *vnlocp, double *vtmp_pp, double *vtmp_pp1,
+                      double *vtmp_pp2, double *vnlspd);
+int orc_force(int natom, int num_spe, int num_atcell, int num_ppcell_d, int num_list_d, int nqmx,
+              int nradmx, int npoint, int nmesh, int nspv, int nperi, int nint1dmax, int nzmax,
+              int jelcalc, int ncpx, int ncpy, int ncpz, int ncpx_d, int ncpy_d, int ncpz_d,
+              int new_pwx, int new_pwy, int new_pwz, int new_rsx, int new_rsy, int new_rsz,
+              int key_jel_calc, int key_natpri_in, int key_natpri_inps, const int *indspe,
+              const int *natpri, const int *natpri_inf, const int *nradct, const int *natprid,
+              const int *napsd, const int *natinfd, const int *natinfd_vloc,
+              const int *lstvecd2, const int *nqctpcc, const int *lstdx, const int *lstdy,
+              const int *lstdz, double veta, double psctoff, double psftrad,
+              const double *radial, const double *dradial, const double *point,
+              const double *wt, double xmax, double ymax, double zmax, double biasx,
+              double biasy, double biasz, const double *cp, const double *coef,
+              const double *atx, const double *aty, const double *atz,
+              const double *dvlocdx_scw, const double *dvlocdy_scw, const double *dvlocdz_scw,
+              const double *dvlocdx_hdp, const double *dvlocdy_hdp, const double *dvlocdz_hdp,
+              const double *rhotrur, const double *rhosmtr, const double *rhoaugr,
+              const double *drhoaug3ddx, const double *drhoaug3ddy, const double *drhoaug3ddz,
+              const double *rho_coarse, const double *rho_aug_dense, const double *vloc_dense,
+              const double *vh_dense, const double *vx_dense, double chrjel, double strjel,
+              double endjel, const int *ndatx, const int *ndaty, const int *ndatz,
+              double *fatx, double *faty, double *fatz, double *fjel);
 
 #ifdef __cplusplus
 }

@@ -351,6 +351,54 @@ CKPT_SIGNATURES = {
 }
 SIGNATURES.update(CKPT_SIGNATURES)
 
+# projector tables, list vectors and forces (pseudocalc8f.F90, force8f.F90)
+PP_SIGNATURES = {
+    "pseudocalc_01a": [("nfdg", "i"), ("npmesh", "i"), ("wxyz", "D")],
+    "pseudocalc_01b": [
+        ("nperi", "i"), ("natom", "i"), ("num_spe", "i"), ("nradmx", "i"), ("nmesh", "i"), ("nfdg", "i"),
+        ("ndisp", "i"), ("num_atcell", "i"), ("num_ppcell", "i"), ("num_ppcell_d", "i"), ("num_list", "i"),
+        ("num_list_d", "i"), ("nxmax", "i"), ("nymax", "i"), ("nzmax", "i"), ("ncpx", "i"), ("ncpy", "i"),
+        ("ncpz", "i"), ("npxmax", "i"), ("npymax", "i"), ("npzmax", "i"), ("key_natpri_in", "i"),
+        ("key_natpri_inps", "i"), ("key_natpri_out", "i"), ("psctoff", "d"), ("xmax", "d"), ("ymax", "d"),
+        ("zmax", "d"), ("dx", "d"), ("dy", "d"), ("dz", "d"), ("indspe", "I"), ("nradct", "I"),
+        ("radial", "D"), ("atx", "D"), ("aty", "D"), ("atz", "D"), ("natx", "I"), ("naty", "I"),
+        ("natz", "I"), ("ndatx", "I"), ("ndaty", "I"), ("ndatz", "I"), ("lstx", "I"), ("lsty", "I"),
+        ("lstz", "I"), ("lstdx", "I"), ("lstdy", "I"), ("lstdz", "I"), ("natinf", "I"), ("natinfd", "I"),
+        ("natinfd_vloc", "I"), ("lstvec2", "I"), ("lstvecd2", "I"), ("natpri", "I"), ("natprid", "I"),
+        ("natpri_inf", "I"), ("naps", "I"), ("napsd", "I")],
+    "pseudocalc_07": [
+        ("natom", "i"), ("num_spe", "i"), ("npmesh", "i"), ("nprjmx", "i"), ("nfdg", "i"), ("nfiltyp", "i"),
+        ("num_list", "i"), ("num_ppcell", "i"), ("nradmx", "i"), ("nqmx", "i"), ("nxmax", "i"),
+        ("nymax", "i"), ("nzmax", "i"), ("npxmax", "i"), ("npymax", "i"), ("npzmax", "i"), ("nxspd", "i"),
+        ("nyspd", "i"), ("nzspd", "i"), ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("eps", "d"),
+        ("psftrad", "d"), ("psctoff", "d"), ("filpp", "d"), ("dx", "d"), ("dy", "d"), ("dz", "d"),
+        ("indspe", "I"), ("nprj", "I"), ("nradct", "I"), ("natpri", "I"), ("naps", "I"), ("natinf", "I"),
+        ("nqct", "I"), ("natx", "I"), ("naty", "I"), ("natz", "I"), ("lstx", "I"), ("lsty", "I"),
+        ("lstz", "I"), ("wxyz", "D"), ("radial", "D"), ("coef", "D"), ("atx", "D"), ("aty", "D"),
+        ("atz", "D"), ("vnlocp", "D"), ("vtmp_pp", "D"), ("vtmp_pp1", "D"), ("vtmp_pp2", "D"),
+        ("vnlspd", "D")],
+    "force": [
+        ("natom", "i"), ("num_spe", "i"), ("num_atcell", "i"), ("num_ppcell_d", "i"), ("num_list_d", "i"),
+        ("nqmx", "i"), ("nradmx", "i"), ("npoint", "i"), ("nmesh", "i"), ("nspv", "i"), ("nperi", "i"),
+        ("nint1dmax", "i"), ("nzmax", "i"), ("jelcalc", "i"), ("ncpx", "i"), ("ncpy", "i"), ("ncpz", "i"),
+        ("ncpx_d", "i"), ("ncpy_d", "i"), ("ncpz_d", "i"), ("new_pwx", "i"), ("new_pwy", "i"),
+        ("new_pwz", "i"), ("new_rsx", "i"), ("new_rsy", "i"), ("new_rsz", "i"), ("key_jel_calc", "i"),
+        ("key_natpri_in", "i"), ("key_natpri_inps", "i"), ("indspe", "I"), ("natpri", "I"),
+        ("natpri_inf", "I"), ("nradct", "I"), ("natprid", "I"), ("napsd", "I"), ("natinfd", "I"),
+        ("natinfd_vloc", "I"), ("lstvecd2", "I"), ("nqctpcc", "I"), ("lstdx", "I"), ("lstdy", "I"),
+        ("lstdz", "I"), ("veta", "d"), ("psctoff", "d"), ("psftrad", "d"), ("radial", "D"),
+        ("dradial", "D"), ("point", "D"), ("wt", "D"), ("xmax", "d"), ("ymax", "d"), ("zmax", "d"),
+        ("biasx", "d"), ("biasy", "d"), ("biasz", "d"), ("cp", "D"), ("coef", "D"), ("atx", "D"),
+        ("aty", "D"), ("atz", "D"), ("dvlocdx_scw", "D"), ("dvlocdy_scw", "D"), ("dvlocdz_scw", "D"),
+        ("dvlocdx_hdp", "D"), ("dvlocdy_hdp", "D"), ("dvlocdz_hdp", "D"), ("rhotrur", "D"),
+        ("rhosmtr", "D"), ("rhoaugr", "D"), ("drhoaug3ddx", "D"), ("drhoaug3ddy", "D"),
+        ("drhoaug3ddz", "D"), ("rho_coarse", "D"), ("rho_aug_dense", "D"), ("vloc_dense", "D"),
+        ("vh_dense", "D"), ("vx_dense", "D"), ("chrjel", "d"), ("strjel", "d"), ("endjel", "d"),
+        ("ndatx", "I"), ("ndaty", "I"), ("ndatz", "I"), ("fatx", "D"), ("faty", "D"), ("fatz", "D"),
+        ("fjel", "od")],
+}
+SIGNATURES.update(PP_SIGNATURES)
+
 _CT = {"i": ctypes.c_int, "d": ctypes.c_double, "I": ctypes.c_void_p, "D": ctypes.c_void_p,
        "Z": ctypes.c_void_p, "oi": ctypes.POINTER(ctypes.c_int),
        "od": ctypes.POINTER(ctypes.c_double), "S": ctypes.c_char_p}

@@ -14,7 +14,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 SOURCES = ["common.hip", "kernels.hip", "gemm_mixed.hip", "solvers.hip", "batch.hip", "abi.hip",
-           "noalloc.hip", "bloch.hip", "abi_bloch.hip", "transfer.hip", "abi_trans.hip", "checkpoint.hip"]
+           "noalloc.hip", "bloch.hip", "abi_bloch.hip", "transfer.hip", "abi_trans.hip", "checkpoint.hip", "projector.hip",
+           "force.hip"]
 CFLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-result",
           # packed FP32 instructions stay off (see the design notes on gfx950 hazards)
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -302,6 +302,74 @@ module kukan_hip_c
       real(c_double) :: fnele(*), rhosmt(*)
       complex(c_double_complex) :: sveccm(*)
     end function
+    integer(c_int) function c_pseudocalc_01a(nfdg, npmesh, wxyz) &
+        bind(C, name='kukan_pseudocalc_01a')
+      import :: c_int, c_double
+      integer(c_int), value :: nfdg, npmesh
+      real(c_double) :: wxyz(*)
+    end function
+    integer(c_int) function c_pseudocalc_01b(nperi, natom, num_spe, nradmx, nmesh, nfdg, ndisp, &
+        num_atcell, num_ppcell, num_ppcell_d, num_list, num_list_d, nxmax, nymax, nzmax, ncpx, ncpy, &
+        ncpz, npxmax, npymax, npzmax, key_natpri_in, key_natpri_inps, key_natpri_out, psctoff, xmax, &
+        ymax, zmax, dx, dy, dz, indspe, nradct, radial, atx, aty, atz, natx, naty, natz, ndatx, &
+        ndaty, ndatz, lstx, lsty, lstz, lstdx, lstdy, lstdz, natinf, natinfd, natinfd_vloc, lstvec2, &
+        lstvecd2, natpri, natprid, natpri_inf, naps, napsd) &
+        bind(C, name='kukan_pseudocalc_01b')
+      import :: c_int, c_double
+      integer(c_int), value :: nperi, natom, num_spe, nradmx, nmesh, nfdg, ndisp, num_atcell, &
+        num_ppcell, num_ppcell_d, num_list, num_list_d, nxmax, nymax, nzmax, ncpx, ncpy, ncpz, &
+        npxmax, npymax, npzmax, key_natpri_in, key_natpri_inps, key_natpri_out
+      real(c_double), value :: psctoff, xmax, ymax, zmax, dx, dy, dz
+      integer(c_int) :: indspe(*), nradct(*), natx(*), naty(*), natz(*), ndatx(*), ndaty(*), &
+        ndatz(*), lstx(*), lsty(*), lstz(*), lstdx(*), lstdy(*), lstdz(*), natinf(*), natinfd(*), &
+        natinfd_vloc(*), lstvec2(*), lstvecd2(*), natpri(*), natprid(*), natpri_inf(*), naps(*), &
+        napsd(*)
+      real(c_double) :: radial(*), atx(*), aty(*), atz(*)
+    end function
+    integer(c_int) function c_pseudocalc_07(natom, num_spe, npmesh, nprjmx, nfdg, nfiltyp, num_list, &
+        num_ppcell, nradmx, nqmx, nxmax, nymax, nzmax, npxmax, npymax, npzmax, nxspd, nyspd, nzspd, &
+        key_natpri_in, key_natpri_inps, eps, psftrad, psctoff, filpp, dx, dy, dz, indspe, nprj, &
+        nradct, natpri, naps, natinf, nqct, natx, naty, natz, lstx, lsty, lstz, wxyz, radial, coef, &
+        atx, aty, atz, vnlocp, vtmp_pp, vtmp_pp1, vtmp_pp2, vnlspd) &
+        bind(C, name='kukan_pseudocalc_07')
+      import :: c_int, c_double
+      integer(c_int), value :: natom, num_spe, npmesh, nprjmx, nfdg, nfiltyp, num_list, num_ppcell, &
+        nradmx, nqmx, nxmax, nymax, nzmax, npxmax, npymax, npzmax, nxspd, nyspd, nzspd, &
+        key_natpri_in, key_natpri_inps
+      real(c_double), value :: eps, psftrad, psctoff, filpp, dx, dy, dz
+      integer(c_int) :: indspe(*), nprj(*), nradct(*), natpri(*), naps(*), natinf(*), nqct(*), &
+        natx(*), naty(*), natz(*), lstx(*), lsty(*), lstz(*)
+      real(c_double) :: wxyz(*), radial(*), coef(*), atx(*), aty(*), atz(*), vnlocp(*), vtmp_pp(*), &
+        vtmp_pp1(*), vtmp_pp2(*), vnlspd(*)
+    end function
+    integer(c_int) function c_force(natom, num_spe, num_atcell, num_ppcell_d, num_list_d, nqmx, &
+        nradmx, npoint, nmesh, nspv, nperi, nint1dmax, nzmax, jelcalc, ncpx, ncpy, ncpz, ncpx_d, &
+        ncpy_d, ncpz_d, new_pwx, new_pwy, new_pwz, new_rsx, new_rsy, new_rsz, key_jel_calc, &
+        key_natpri_in, key_natpri_inps, indspe, natpri, natpri_inf, nradct, natprid, napsd, natinfd, &
+        natinfd_vloc, lstvecd2, nqctpcc, lstdx, lstdy, lstdz, veta, psctoff, psftrad, radial, &
+        dradial, point, wt, xmax, ymax, zmax, biasx, biasy, biasz, cp, coef, atx, aty, atz, &
+        dvlocdx_scw, dvlocdy_scw, dvlocdz_scw, dvlocdx_hdp, dvlocdy_hdp, dvlocdz_hdp, rhotrur, &
+        rhosmtr, rhoaugr, drhoaug3ddx, drhoaug3ddy, drhoaug3ddz, rho_coarse, rho_aug_dense, &
+        vloc_dense, vh_dense, vx_dense, chrjel, strjel, endjel, ndatx, ndaty, ndatz, fatx, faty, &
+        fatz, fjel) &
+        bind(C, name='kukan_force')
+      import :: c_int, c_double
+      integer(c_int), value :: natom, num_spe, num_atcell, num_ppcell_d, num_list_d, nqmx, nradmx, &
+        npoint, nmesh, nspv, nperi, nint1dmax, nzmax, jelcalc, ncpx, ncpy, ncpz, ncpx_d, ncpy_d, &
+        ncpz_d, new_pwx, new_pwy, new_pwz, new_rsx, new_rsy, new_rsz, key_jel_calc, key_natpri_in, &
+        key_natpri_inps
+      real(c_double), value :: veta, psctoff, psftrad, xmax, ymax, zmax, biasx, biasy, biasz, chrjel, &
+        strjel, endjel
+      integer(c_int) :: indspe(*), natpri(*), natpri_inf(*), nradct(*), natprid(*), napsd(*), &
+        natinfd(*), natinfd_vloc(*), lstvecd2(*), nqctpcc(*), lstdx(*), lstdy(*), lstdz(*), ndatx(*), &
+        ndaty(*), ndatz(*)
+      real(c_double) :: radial(*), dradial(*), point(*), wt(*), cp(*), coef(*), atx(*), aty(*), &
+        atz(*), dvlocdx_scw(*), dvlocdy_scw(*), dvlocdz_scw(*), dvlocdx_hdp(*), dvlocdy_hdp(*), &
+        dvlocdz_hdp(*), rhotrur(*), rhosmtr(*), rhoaugr(*), drhoaug3ddx(*), drhoaug3ddy(*), &
+        drhoaug3ddz(*), rho_coarse(*), rho_aug_dense(*), vloc_dense(*), vh_dense(*), vx_dense(*), &
+        fatx(*), faty(*), fatz(*)
+      real(c_double) :: fjel
+    end function
   end interface
 
 contains
@@ -888,3 +956,105 @@ contains
     call kukan_check(c_scf_charge_smt_c(neigmx, nums, ncol, nspv, numk, ncpx, ncpy, ncpz, fnele, sveccm, rhosmt))
   end subroutine
 end module mod_kukan_charge
+
+! -----------------------------------------------------------------------------
+! mod_pseudocalc (pseudocalc8f.F90): the weights, list vectors and projector table
+module mod_pseudocalc_hip
+  use kukan_hip_c
+  implicit none
+contains
+  ! pseudocalc8f.F90:343-629
+  subroutine pseudocalc_01a(nfdg, npmesh, wxyz)
+    integer, intent(in) :: nfdg, npmesh
+    real*8 :: wxyz(*)
+    call kukan_check(c_pseudocalc_01a(nfdg, npmesh, wxyz))
+  end subroutine
+
+  ! pseudocalc8f.F90:633-1159
+  subroutine pseudocalc_01b(nperi, natom, num_spe, nradmx, nmesh, nfdg, ndisp, num_atcell, &
+      num_ppcell, num_ppcell_d, num_list, num_list_d, nxmax, nymax, nzmax, ncpx, ncpy, ncpz, npxmax, &
+      npymax, npzmax, key_natpri_in, key_natpri_inps, key_natpri_out, psctoff, xmax, ymax, zmax, dx, &
+      dy, dz, indspe, nradct, radial, atx, aty, atz, natx, naty, natz, ndatx, ndaty, ndatz, lstx, &
+      lsty, lstz, lstdx, lstdy, lstdz, natinf, natinfd, natinfd_vloc, lstvec2, lstvecd2, natpri, &
+      natprid, natpri_inf, naps, napsd)
+    integer, intent(in) :: nperi, natom, num_spe, nradmx, nmesh, nfdg, ndisp, num_atcell, num_ppcell, &
+      num_ppcell_d, num_list, num_list_d, nxmax, nymax, nzmax, ncpx, ncpy, ncpz, npxmax, npymax, &
+      npzmax, key_natpri_in, key_natpri_inps, key_natpri_out
+    real*8, intent(in) :: psctoff, xmax, ymax, zmax, dx, dy, dz
+    integer :: indspe(*), nradct(*), natx(*), naty(*), natz(*), ndatx(*), ndaty(*), ndatz(*), &
+      lstx(*), lsty(*), lstz(*), lstdx(*), lstdy(*), lstdz(*), natinf(*), natinfd(*), &
+      natinfd_vloc(*), lstvec2(*), lstvecd2(*), natpri(*), natprid(*), natpri_inf(*), naps(*), &
+      napsd(*)
+    real*8 :: radial(*), atx(*), aty(*), atz(*)
+    call kukan_check(c_pseudocalc_01b(nperi, natom, num_spe, nradmx, nmesh, nfdg, ndisp, num_atcell, &
+      num_ppcell, num_ppcell_d, num_list, num_list_d, nxmax, nymax, nzmax, ncpx, ncpy, ncpz, npxmax, &
+      npymax, npzmax, key_natpri_in, key_natpri_inps, key_natpri_out, psctoff, xmax, ymax, zmax, dx, &
+      dy, dz, indspe, nradct, radial, atx, aty, atz, natx, naty, natz, ndatx, ndaty, ndatz, lstx, &
+      lsty, lstz, lstdx, lstdy, lstdz, natinf, natinfd, natinfd_vloc, lstvec2, lstvecd2, natpri, &
+      natprid, natpri_inf, naps, napsd))
+  end subroutine
+
+  ! pseudocalc8f.F90:2515-3702
+  subroutine pseudocalc_07(natom, num_spe, npmesh, nprjmx, nfdg, nfiltyp, num_list, num_ppcell, &
+      nradmx, nqmx, nxmax, nymax, nzmax, npxmax, npymax, npzmax, nxspd, nyspd, nzspd, key_natpri_in, &
+      key_natpri_inps, eps, psftrad, psctoff, filpp, dx, dy, dz, indspe, nprj, nradct, natpri, naps, &
+      natinf, nqct, natx, naty, natz, lstx, lsty, lstz, wxyz, radial, coef, atx, aty, atz, vnlocp, &
+      vtmp_pp, vtmp_pp1, vtmp_pp2, vnlspd)
+    integer, intent(in) :: natom, num_spe, npmesh, nprjmx, nfdg, nfiltyp, num_list, num_ppcell, &
+      nradmx, nqmx, nxmax, nymax, nzmax, npxmax, npymax, npzmax, nxspd, nyspd, nzspd, key_natpri_in, &
+      key_natpri_inps
+    real*8, intent(in) :: eps, psftrad, psctoff, filpp, dx, dy, dz
+    integer :: indspe(*), nprj(*), nradct(*), natpri(*), naps(*), natinf(*), nqct(*), natx(*), &
+      naty(*), natz(*), lstx(*), lsty(*), lstz(*)
+    real*8 :: wxyz(*), radial(*), coef(*), atx(*), aty(*), atz(*), vnlocp(*), vtmp_pp(*), &
+      vtmp_pp1(*), vtmp_pp2(*), vnlspd(*)
+    call kukan_check(c_pseudocalc_07(natom, num_spe, npmesh, nprjmx, nfdg, nfiltyp, num_list, &
+      num_ppcell, nradmx, nqmx, nxmax, nymax, nzmax, npxmax, npymax, npzmax, nxspd, nyspd, nzspd, &
+      key_natpri_in, key_natpri_inps, eps, psftrad, psctoff, filpp, dx, dy, dz, indspe, nprj, nradct, &
+      natpri, naps, natinf, nqct, natx, naty, natz, lstx, lsty, lstz, wxyz, radial, coef, atx, aty, &
+      atz, vnlocp, vtmp_pp, vtmp_pp1, vtmp_pp2, vnlspd))
+  end subroutine
+end module mod_pseudocalc_hip
+
+! -----------------------------------------------------------------------------
+! mod_force (force8f.F90): the forces beyond the band-structure term
+module mod_force_hip
+  use kukan_hip_c
+  implicit none
+contains
+  ! force8f.F90:1617-2190
+  subroutine force(natom, num_spe, num_atcell, num_ppcell_d, num_list_d, nqmx, nradmx, npoint, nmesh, &
+      nspv, nperi, nint1dmax, nzmax, jelcalc, ncpx, ncpy, ncpz, ncpx_d, ncpy_d, ncpz_d, new_pwx, &
+      new_pwy, new_pwz, new_rsx, new_rsy, new_rsz, key_jel_calc, key_natpri_in, key_natpri_inps, &
+      indspe, natpri, natpri_inf, nradct, natprid, napsd, natinfd, natinfd_vloc, lstvecd2, nqctpcc, &
+      lstdx, lstdy, lstdz, veta, psctoff, psftrad, radial, dradial, point, wt, xmax, ymax, zmax, &
+      biasx, biasy, biasz, cp, coef, atx, aty, atz, dvlocdx_scw, dvlocdy_scw, dvlocdz_scw, &
+      dvlocdx_hdp, dvlocdy_hdp, dvlocdz_hdp, rhotrur, rhosmtr, rhoaugr, drhoaug3ddx, drhoaug3ddy, &
+      drhoaug3ddz, rho_coarse, rho_aug_dense, vloc_dense, vh_dense, vx_dense, chrjel, strjel, endjel, &
+      ndatx, ndaty, ndatz, fatx, faty, fatz, fjel)
+    integer, intent(in) :: natom, num_spe, num_atcell, num_ppcell_d, num_list_d, nqmx, nradmx, &
+      npoint, nmesh, nspv, nperi, nint1dmax, nzmax, jelcalc, ncpx, ncpy, ncpz, ncpx_d, ncpy_d, &
+      ncpz_d, new_pwx, new_pwy, new_pwz, new_rsx, new_rsy, new_rsz, key_jel_calc, key_natpri_in, &
+      key_natpri_inps
+    real*8, intent(in) :: veta, psctoff, psftrad, xmax, ymax, zmax, biasx, biasy, biasz, chrjel, &
+      strjel, endjel
+    integer :: indspe(*), natpri(*), natpri_inf(*), nradct(*), natprid(*), napsd(*), natinfd(*), &
+      natinfd_vloc(*), lstvecd2(*), nqctpcc(*), lstdx(*), lstdy(*), lstdz(*), ndatx(*), ndaty(*), &
+      ndatz(*)
+    real*8 :: radial(*), dradial(*), point(*), wt(*), cp(*), coef(*), atx(*), aty(*), atz(*), &
+      dvlocdx_scw(*), dvlocdy_scw(*), dvlocdz_scw(*), dvlocdx_hdp(*), dvlocdy_hdp(*), dvlocdz_hdp(*), &
+      rhotrur(*), rhosmtr(*), rhoaugr(*), drhoaug3ddx(*), drhoaug3ddy(*), drhoaug3ddz(*), &
+      rho_coarse(*), rho_aug_dense(*), vloc_dense(*), vh_dense(*), vx_dense(*), fatx(*), faty(*), &
+      fatz(*)
+    real*8, intent(out) :: fjel
+    call kukan_check(c_force(natom, num_spe, num_atcell, num_ppcell_d, num_list_d, nqmx, nradmx, &
+      npoint, nmesh, nspv, nperi, nint1dmax, nzmax, jelcalc, ncpx, ncpy, ncpz, ncpx_d, ncpy_d, &
+      ncpz_d, new_pwx, new_pwy, new_pwz, new_rsx, new_rsy, new_rsz, key_jel_calc, key_natpri_in, &
+      key_natpri_inps, indspe, natpri, natpri_inf, nradct, natprid, napsd, natinfd, natinfd_vloc, &
+      lstvecd2, nqctpcc, lstdx, lstdy, lstdz, veta, psctoff, psftrad, radial, dradial, point, wt, &
+      xmax, ymax, zmax, biasx, biasy, biasz, cp, coef, atx, aty, atz, dvlocdx_scw, dvlocdy_scw, &
+      dvlocdz_scw, dvlocdx_hdp, dvlocdy_hdp, dvlocdz_hdp, rhotrur, rhosmtr, rhoaugr, drhoaug3ddx, &
+      drhoaug3ddy, drhoaug3ddz, rho_coarse, rho_aug_dense, vloc_dense, vh_dense, vx_dense, chrjel, &
+      strjel, endjel, ndatx, ndaty, ndatz, fatx, faty, fatz, fjel))
+  end subroutine
+end module mod_force_hip

@@ -70,3 +70,54 @@ def test_subspace_gemm_wrapper_marshals_arguments():
         assert not C.any()
     else:  # a GPU is present
         np.testing.assert_allclose(C, A.T @ A)
+
+
+def _shim_interfaces():
+    """bind(C) interfaces of the Fortran drop-in: C symbol -> [(argument, kind)], kind one of
+    'i'/'d' (integer/real by value), 'I'/'D'/'Z'/'S' (integer/real/complex/character array),
+    'oi'/'od' (integer/real by reference)."""
+    src = open(os.path.join(ROOT, "rspacedist_amd", "fortran", "kukan_hip_shim.F90")).read()
+    src = re.sub(r"&\s*\n\s*", "", src)  # join continuation lines
+    out = {}
+    for m in re.finditer(r"function\s+c_\w+\(([^)]*)\)\s*bind\(C,\s*name='(kukan_\w+)'\)(.*?)end function", src,
+                         flags=re.S | re.I):
+        names = [a.strip() for a in m.group(1).split(",") if a.strip()]
+        decl = {}
+        for line in m.group(3).splitlines():
+            line = line.strip()
+            if "::" not in line:
+                continue
+            typ, vars_ = line.split("::", 1)
+            val = "value" in typ
+            for v in re.split(r",\s*(?![^(]*\))", vars_):
+                v = v.strip()
+                arr = "(" in v
+                nm = v.split("(")[0].strip()
+                if typ.lower().startswith("integer"):
+                    k = "I" if arr else ("i" if val else "oi")
+                elif typ.lower().startswith("real"):
+                    k = "D" if arr else ("d" if val else "od")
+                elif typ.lower().startswith("complex"):
+                    k = "Z"
+                else:
+                    k = "S"
+                decl[nm] = k
+        out[m.group(2)] = [(a, decl.get(a)) for a in names]
+    return out
+
+
+def test_fortran_shim_interfaces_match_signature_table():
+    """Every bind(C) interface of the Fortran drop-in passes its arguments in the order and
+    with the kinds of the C-ABI (rspacedist_amd/abi.py, checked against the header above)."""
+    from rspacedist_amd import abi
+    ifs = _shim_interfaces()
+    assert len(ifs) > 20
+    for sym, args in ifs.items():
+        name = sym[len("kukan_"):]
+        if name not in abi.SIGNATURES:
+            continue  # library utilities (init, set_param, bind, ...)
+        want = [t for _, t in abi.SIGNATURES[name]]
+        got = [k for _, k in args]
+        assert got == want, (sym, [(a, k, w) for (a, k), w in zip(args, want) if k != w])
+    for name in ("pseudocalc_01a", "pseudocalc_01b", "pseudocalc_07", "force", "scf_diag", "kscg_c"):
+        assert "kukan_" + name in ifs

@@ -488,6 +488,8 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--precision", choices=["mixed", "fp64", "both"], default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--param", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a library parameter (kukan_set_param) before the run")
     ap.add_argument("--no-suite", action="store_true",
                     help="skip the 40^3, 96^3 FP64, adjoint and batch workloads")
     ap.add_argument("--cpu-config1", action="store_true",
@@ -499,6 +501,9 @@ def main():
         sys.exit(2)
     from rspacedist_amd import kukan as kk, synth, workload
     kk.init(0)
+    for kv in args.param:
+        k, v = kv.split("=", 1)
+        kk.set_param(k, float(v))
     log(f"[bench] config {args.config}: building synthetic system")
     cell = workload.Cell(cfg=args.config)
     s = cell.s

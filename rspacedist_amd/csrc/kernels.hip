@@ -1651,6 +1651,31 @@ void launch_canon_sign(int n, double *z, hipStream_t s) {
 }
 
 // ----------------------------------------------------------------------------- plan build
+namespace {
+// order-dependent checksum of an int array (wrap-around integer sums: the same for any schedule)
+__global__ void k_int_checksum(const int *a, long n, unsigned long long *out) {
+  unsigned long long h = 0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    h += (unsigned long long)(unsigned)a[i] * (2654435761ull * (unsigned long long)(i + 1) + 97531ull);
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_down(h, o);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, h);
+}
+
+// a built plan and what it was built from: the atom tables, the sizes, lstvec2's address and
+// checksum, and the scratch buffers it lives in (a reallocated buffer invalidates it)
+struct CachedPlan {
+  std::vector<long> key;
+  const int *lst = nullptr;
+  unsigned long long sum = 0;
+  std::vector<void *> bufs;
+  NonlocalPlan pl;
+};
+std::map<std::string, CachedPlan> &plan_cache() {
+  static std::map<std::string, CachedPlan> m;
+  return m;
+}
+}  // namespace
+
 void build_plan(NonlocalPlan &pl, const Grid &g, int natom, int nprjmx, int num_spe,
                 int num_ppcell, int num_list, int nums, int key_in, int key_inps,
                 const int *nprj, const int *indspe, const int *natinf, const int *natpri,
@@ -1662,6 +1687,41 @@ void build_plan(NonlocalPlan &pl, const Grid &g, int natom, int nprjmx, int num_
   std::vector<int> h_nprj = host_ints(nprj, num_spe), h_ind = host_ints(indspe, natom),
                    h_inf = host_ints(natinf, natom), h_pri = host_ints(natpri, natom),
                    h_aps = host_ints(naps, natom);
+  // the plan of the previous call with the same inputs is reused (param "plan_cache", default
+  // on): same sizes and atom tables, same lstvec2 address and contents
+  const bool use_cache = param("plan_cache", 1.0) >= 1.0 && !lib().capturing;
+  std::vector<long> key = {g.n(), g.nx, g.ny, g.nz, natom, nprjmx, num_spe, num_ppcell, num_list, nums, key_in,
+                           key_inps};
+  for (const std::vector<int> *v : {&h_nprj, &h_ind, &h_inf, &h_pri, &h_aps}) key.insert(key.end(), v->begin(), v->end());
+  unsigned long long sum = 0;
+  if (use_cache) {
+    unsigned long long *d = (unsigned long long *)scratch(tag + ".sum", sizeof(unsigned long long));
+    HIPCK(hipMemsetAsync(d, 0, sizeof(unsigned long long), s));
+    const long nl = (long)num_list * num_ppcell;
+    hipLaunchKernelGGL(k_int_checksum, dim3((unsigned)std::min<long>(1024, std::max<long>(1, nblk(nl, 256)))),
+                       dim3(256), 0, s, lstvec2_dev, nl, d);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(&sum, d, sizeof(sum), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    auto it = plan_cache().find(tag);
+    if (it != plan_cache().end() && it->second.key == key && it->second.lst == lstvec2_dev && it->second.sum == sum) {
+      const CachedPlan &c = it->second;
+      const NonlocalPlan &q = c.pl;
+      const long n = g.n();
+      const std::vector<void *> now = {
+          scratch(tag + ".act", sizeof(int) * std::max<size_t>(1, q.act_h.size())),
+          scratch(tag + ".iprj", sizeof(int) * std::max(1, num_ppcell)),
+          scratch(tag + ".rowptr", sizeof(int) * (n + 1)),
+          scratch(tag + ".ent", sizeof(int2) * std::max<long>(1, q.nent)),
+          scratch(tag + ".nzrows", sizeof(int) * std::max<long>(1, q.nnz))};
+      if (now == c.bufs) {
+        pl = q;
+        pl.vnl = vnlocp_dev;
+        pl.lst = lstvec2_dev;
+        return;
+      }
+    }
+  }
   pl.act_h.clear();
   std::vector<int> iprj_slot(num_ppcell > 0 ? num_ppcell : 1, 0);
   pl.max_iprj = 0;
@@ -1730,6 +1790,14 @@ void build_plan(NonlocalPlan &pl, const Grid &g, int natom, int nprjmx, int num_
   pl.nzrows = (int *)scratch(tag + ".nzrows", sizeof(int) * std::max<long>(1, pl.nnz));
   hipLaunchKernelGGL(k_compact_rows, dim3(nblk(n, 256)), dim3(256), 0, s, flag, pos, n, pl.nzrows);
   HIPCK(hipGetLastError());
+  if (use_cache) {
+    CachedPlan &c = plan_cache()[tag];
+    c.key = std::move(key);
+    c.lst = lstvec2_dev;
+    c.sum = sum;
+    c.pl = pl;
+    c.bufs = {pl.act, pl.iprj_slot, pl.row_ptr, pl.ent, pl.nzrows};
+  }
 }
 
 }  // namespace kukan

@@ -364,6 +364,11 @@ __global__ void __launch_bounds__(TB) k_dense_lists(ListArgs a) {
 struct PPAtom {
   int offx, offy, offz;  // npat - npmesh * nat: fine index of box point 0 in vnlspd
   int sp, np, nq, mask, iaps, natinf, lcol;
+  // the part of the fine work array (0-based array indices, start and extent per axis) where
+  // the projectors can be nonzero: the atom's box cut to its sphere.  Every pass reads only
+  // this part and treats the rest as the zeros it holds in the reference: skipping a zero term
+  // leaves a sum unchanged bit for bit.
+  int f0x, f0y, f0z, fwx, fwy, fwz;
   double ax, ay, az;  // atom minus the fine point it rounds to, plus half a fine step
   double rc, thr, dqq, tmp;
 };
@@ -407,11 +412,12 @@ inline int pp_mask(int nprj) {
 __global__ void __launch_bounds__(256) k_fine_projectors(FineArgs a, int slot0) {
   const PPAtom &t = a.at[slot0 + blockIdx.y];
   double *out = a.out + (long)blockIdx.y * a.nprjmx * a.plane;
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= a.plane || t.np == 0) return;
+  const long u = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= (long)t.fwx * t.fwy * t.fwz || t.np == 0) return;
+  const int i0 = t.f0x + (int)(u % t.fwx), j0 = t.f0y + (int)((u / t.fwx) % t.fwy), k0 = t.f0z + (int)(u / ((long)t.fwx * t.fwy));
   const int sx = 2 * a.nxspd, sy = 2 * a.nyspd;
-  const int kx = (int)(q % sx) - a.nxspd + 1, ky = (int)((q / sx) % sy) - a.nyspd + 1,
-            kz = (int)(q / ((long)sx * sy)) - a.nzspd + 1;
+  const long q = i0 + (long)sx * (j0 + (long)sy * k0);
+  const int kx = i0 - a.nxspd + 1, ky = j0 - a.nyspd + 1, kz = k0 - a.nzspd + 1;
   const int ix = kx - t.offx, iy = ky - t.offy, iz = kz - t.offz;
   const bool inbox = ix > -a.npxd && ix <= a.npxd && iy > -a.npyd && iy <= a.npyd && iz > -a.npzd && iz <= a.npzd;
   const double x = ix * a.dpx - t.ax, y = iy * a.dpy - t.ay, z = iz * a.dpz - t.az;
@@ -500,35 +506,46 @@ struct FilterArgs {
   double *t1, *t2;
 };
 
-// z pass: t1(ix, iy, iz) over fine x, y and coarse z (pseudocalc8f.F90:3662-3671)
+// z pass: t1(ix, iy, iz) over fine x, y and coarse z (pseudocalc8f.F90:3662-3671), on the
+// columns of the atom's fine part
 __global__ void k_filter_z(FilterArgs a, int slot0) {
   const PPAtom &t = a.at[slot0 + blockIdx.y];
   const int la = blockIdx.z;
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (la >= t.np || q >= a.plane1) return;
+  const long u = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int cz = 2 * a.npzmax;
+  if (la >= t.np || u >= (long)t.fwx * t.fwy * cz) return;
   const int sx = 2 * a.nxspd, sy = 2 * a.nyspd;
-  const int ixy = (int)(q % ((long)sx * sy)), iz = (int)(q / ((long)sx * sy)) - a.npzmax + 1;
+  const int i0 = t.f0x + (int)(u % t.fwx), j0 = t.f0y + (int)((u / t.fwx) % t.fwy);
+  const int izc = (int)(u / ((long)t.fwx * t.fwy)), iz = izc - a.npzmax + 1;
+  const long ixy = i0 + (long)sx * j0;
   const double *src = a.fine + ((long)blockIdx.y * a.nprjmx + la) * a.plane + ixy;
   const int kkz = iz * a.npmesh - a.half;
   double acc = 0.0;
-  for (int jz = a.js; jz <= a.je; jz++)
-    acc = acc + src[(long)(kkz + jz + a.nzspd - 1) * sx * sy] * a.w[jz - a.wlo];
-  a.t1[((long)blockIdx.y * a.nprjmx + la) * a.plane1 + q] = acc;
+  for (int jz = a.js; jz <= a.je; jz++) {
+    const int k0 = kkz + jz + a.nzspd - 1;
+    if (k0 >= t.f0z && k0 < t.f0z + t.fwz) acc = acc + src[(long)k0 * sx * sy] * a.w[jz - a.wlo];
+  }
+  a.t1[((long)blockIdx.y * a.nprjmx + la) * a.plane1 + ixy + (long)sx * sy * izc] = acc;
 }
 
-// y pass: t2(ix, iy, iz) over fine x and coarse y, z (:3673-3682)
+// y pass: t2(ix, iy, iz) over fine x and coarse y, z (:3673-3682), on the atom's fine x range
 __global__ void k_filter_y(FilterArgs a, int slot0) {
   const PPAtom &t = a.at[slot0 + blockIdx.y];
   const int la = blockIdx.z;
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (la >= t.np || q >= a.plane2) return;
-  const int sx = 2 * a.nxspd, sy = 2 * a.nyspd, cy = 2 * a.npymax;
-  const int ix = (int)(q % sx), iy = (int)((q / sx) % cy) - a.npymax + 1, izc = (int)(q / ((long)sx * cy));
-  const double *src = a.t1 + ((long)blockIdx.y * a.nprjmx + la) * a.plane1 + (long)izc * sx * sy + ix;
+  const long u = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int cy = 2 * a.npymax, cz = 2 * a.npzmax;
+  if (la >= t.np || u >= (long)t.fwx * cy * cz) return;
+  const int sx = 2 * a.nxspd, sy = 2 * a.nyspd;
+  const int i0 = t.f0x + (int)(u % t.fwx), iyc = (int)((u / t.fwx) % cy), izc = (int)(u / ((long)t.fwx * cy));
+  const int iy = iyc - a.npymax + 1;
+  const double *src = a.t1 + ((long)blockIdx.y * a.nprjmx + la) * a.plane1 + (long)izc * sx * sy + i0;
   const int kky = iy * a.npmesh - a.half;
   double acc = 0.0;
-  for (int jy = a.js; jy <= a.je; jy++) acc = acc + src[(long)(kky + jy + a.nyspd - 1) * sx] * a.w[jy - a.wlo];
-  a.t2[((long)blockIdx.y * a.nprjmx + la) * a.plane2 + q] = acc;
+  for (int jy = a.js; jy <= a.je; jy++) {
+    const int j0 = kky + jy + a.nyspd - 1;
+    if (j0 >= t.f0y && j0 < t.f0y + t.fwy) acc = acc + src[(long)j0 * sx] * a.w[jy - a.wlo];
+  }
+  a.t2[((long)blockIdx.y * a.nprjmx + la) * a.plane2 + i0 + (long)sx * (iyc + (long)cy * izc)] = acc;
 }
 
 // x pass at the atom's list points, into vnlocp(:, la, iaps) (:3684-3698)
@@ -545,7 +562,10 @@ __global__ void k_filter_x_gather(FilterArgs a, int slot0, const int *lstx, cons
                       ((long)(iz + a.npzmax - 1) * cy + (iy + a.npymax - 1)) * sx;
   const int kkx = ix * a.npmesh - a.half;
   double acc = 0.0;
-  for (int jx = a.js; jx <= a.je; jx++) acc = acc + src[kkx + jx + a.nxspd - 1] * a.w[jx - a.wlo];
+  for (int jx = a.js; jx <= a.je; jx++) {
+    const int i0 = kkx + jx + a.nxspd - 1;
+    if (i0 >= t.f0x && i0 < t.f0x + t.fwx) acc = acc + src[i0] * a.w[jx - a.wlo];
+  }
   vnlocp[(long)num_list * (la + (long)a.nprjmx * (t.iaps - 1)) + p] = acc;
 }
 
@@ -710,6 +730,16 @@ int kukan_pseudocalc_07(int natom, int num_spe, int npmesh, int nprjmx, int nfdg
     t.thr = t.rc * psctoff;
     t.dqq = pi / (t.rc * psftrad);
     t.tmp = fourpi * t.dqq * twopicbin;
+    // the fine part: box points within the sphere's bounding box (one point of margin)
+    auto part = [&](double c, double h, int nbox, int off, int nspd, int &f0, int &fw) {
+      const int lo = std::max(-nbox + 1, (int)std::floor((c - t.thr) / h) - 1);
+      const int hi = std::min(nbox, (int)std::ceil((c + t.thr) / h) + 1);
+      f0 = lo + off + nspd - 1;
+      fw = std::max(0, hi - lo + 1);
+    };
+    part(t.ax, dpx, npxmax * npmesh, t.offx, nxspd, t.f0x, t.fwx);
+    part(t.ay, dpy, npymax * npmesh, t.offy, nyspd, t.f0y, t.fwy);
+    part(t.az, dpz, npzmax * npmesh, t.offz, nzspd, t.f0z, t.fwz);
     at.push_back(t);
   }
   const size_t vb = (size_t)num_list * nprjmx * num_ppcell * D;
@@ -752,17 +782,26 @@ int kukan_pseudocalc_07(int natom, int num_spe, int npmesh, int nprjmx, int nfdg
     FilterArgs ga{atd, W.d(), (npmesh % 2 == 1) ? wlo + 1 : wlo, nfdg * npmesh - 1, wlo, npmesh, half, nxspd,
                   nyspd, nzspd, npxmax, npymax, npzmax, nprjmx, plane, plane1, plane2, fine, t1, t2};
     int maxinf = 0;
-    for (const PPAtom &t : at) maxinf = std::max(maxinf, t.natinf);
+    long maxf = 0, maxz = 0, maxy = 0;
+    for (const PPAtom &t : at) {
+      maxinf = std::max(maxinf, t.natinf);
+      maxf = std::max(maxf, (long)t.fwx * t.fwy * t.fwz);
+      maxz = std::max(maxz, (long)t.fwx * t.fwy * 2 * npzmax);
+      maxy = std::max(maxy, (long)t.fwx * 2 * npymax * 2 * npzmax);
+    }
     for (int a0 = 0; a0 < nact; a0 += batch) {
       const int nb = std::min(batch, nact - a0);  // <= 65535 (grid.y): batch holds ~1 GiB
       {
         Timed t("pp_fine", 0.0);
-        hipLaunchKernelGGL(k_fine_projectors, dim3((unsigned)((plane + 255) / 256), nb), dim3(256), 0, s, fa, a0);
+        if (maxf > 0)
+          hipLaunchKernelGGL(k_fine_projectors, dim3((unsigned)((maxf + 255) / 256), nb), dim3(256), 0, s, fa, a0);
       }
       {
         Timed t("pp_filter", 0.0);
-        hipLaunchKernelGGL(k_filter_z, dim3((unsigned)((plane1 + 255) / 256), nb, nprjmx), dim3(256), 0, s, ga, a0);
-        hipLaunchKernelGGL(k_filter_y, dim3((unsigned)((plane2 + 255) / 256), nb, nprjmx), dim3(256), 0, s, ga, a0);
+        if (maxz > 0)
+          hipLaunchKernelGGL(k_filter_z, dim3((unsigned)((maxz + 255) / 256), nb, nprjmx), dim3(256), 0, s, ga, a0);
+        if (maxy > 0)
+          hipLaunchKernelGGL(k_filter_y, dim3((unsigned)((maxy + 255) / 256), nb, nprjmx), dim3(256), 0, s, ga, a0);
         if (maxinf > 0)
           hipLaunchKernelGGL(k_filter_x_gather, dim3((unsigned)((maxinf + 255) / 256), nb, nprjmx), dim3(256), 0, s,
                              ga, a0, Lx.i(), Ly.i(), Lz.i(), V.d(), num_list);

@@ -91,6 +91,35 @@ def test_nonlocal_r01(kk, orc):
     np.testing.assert_allclose(g["rpsep"], o["rpsep"], rtol=1e-12, atol=1e-13)
 
 
+def test_projector_plan_follows_lstvec2_in_place(kk, orc):
+    """The library reuses the projector plan of the previous call when the atom tables and
+    lstvec2 (address and contents) are the same; changing lstvec2 in place at the same device
+    address, or reordering it, must give the new result."""
+    import ctypes
+    s, rng = _nl_system()
+    avre0 = rng.standard_normal(s.npts)
+    rp = np.asfortranarray(rng.standard_normal((9, s.natom)))
+    args = synth.abi_args(s)
+    lst = kk.DeviceArray.from_numpy(np.asfortranarray(args["lstvec2"]))
+    args = dict(args, lstvec2=lst, nhs=1)
+    h = np.asfortranarray(args["natinf"]).copy()
+    for trial in range(3):
+        host = np.asfortranarray(synth.abi_args(s)["lstvec2"]).copy()
+        if trial == 1:  # the same points in another order (same checksum class of entries)
+            for na in range(s.natom):
+                host[: h[na], na] = host[: h[na], na][::-1]
+        if trial == 2:  # shifted points
+            host[: h[0], 0] = (host[: h[0], 0] % s.npts) + 1
+        kk.lib().kukan_memcpy(ctypes.c_void_p(lst.device_ptr), host.ctypes.data, host.nbytes, 1)
+        kk.synchronize()
+        g = {"rpsep": rp.copy(order="F"), "dijsss": s.dij, "avre": avre0.copy(), "avr": np.zeros(s.num_list)}
+        kk.call("nonlocaloperation_r_02", **args, **g)
+        o = {"rpsep": rp.copy(order="F"), "dijsss": s.dij, "avre": avre0.copy(), "avr": np.zeros(s.num_list)}
+        orc.call("nonlocaloperation_r_02", **dict(args, lstvec2=host), **o)
+        np.testing.assert_allclose(g["avre"], o["avre"], rtol=1e-12, atol=1e-12, err_msg=str(trial))
+    lst.free()
+
+
 @pytest.mark.parametrize("nhs", [1, 0])
 def test_nonlocal_r02(kk, orc, nhs):
     s, rng = _nl_system()

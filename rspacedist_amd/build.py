@@ -21,7 +21,7 @@ CFLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I", os.path.join(os.path.dirname(HERE), "include")]
 LDFLAGS = ["-shared", f"--offload-arch={ARCH}", "-L", os.path.join(ROCM, "lib"), "-lrocblas",
-           "-lrocsolver", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"]
+           "-lrocsolver", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"]
 
 
 def _compile(src):

@@ -2,6 +2,8 @@
 #include <cstdio>
 #include <cstring>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "kukan_internal.hpp"
 
 namespace kukan {
@@ -96,6 +98,10 @@ std::map<std::string, TimingAcc> &timing_table() {
 }
 
 Timed::Timed(const char *tag, double work) : tag_(tag), work_(work) {
+  if (std::strncmp(tag, "phase_", 6) == 0) {
+    roctxRangePushA(tag);
+    range_ = true;
+  }
   if (param("timing", 0.0) < 1.0 || lib().capturing) return;
   a_ = get_event();
   b_ = get_event();
@@ -103,6 +109,7 @@ Timed::Timed(const char *tag, double work) : tag_(tag), work_(work) {
 }
 
 Timed::~Timed() {
+  if (range_) roctxRangePop();
   if (!a_) return;
   (void)hipEventRecord(b_, lib().stream);
   pending().push_back({tag_, a_, b_, work_});

@@ -563,11 +563,14 @@ template <int IPM, int BB>
 __global__ __launch_bounds__(256) void k_gather(const int *act, int num_list, int nprjmx,
                                                 const double *vnl, const int *lst,
                                                 const double *psi, long ld, int nb,
-                                                double *rp, long ld_rp, int accumulate) {
+                                                double *rp, long ld_rp, int accumulate, int band_major) {
   __shared__ double red[4][IPM * BB];
-  const int a = blockIdx.x;
+  // band_major: the band groups of one atom are consecutive work-groups, so the atom's
+  // projector table is read from HBM once per XCD and then served by that XCD's L2 (2.60
+  // against 2.81 ms per launch at 128^3 x 1024 bands x 128 atoms)
+  const int a = band_major ? blockIdx.y : blockIdx.x;
   const int slot = act[5 * a], natinf = act[5 * a + 1], iprj = act[5 * a + 2];
-  const int b0 = blockIdx.y * BB;
+  const int b0 = (band_major ? blockIdx.x : blockIdx.y) * BB;
   double acc[BB][IPM];
 #pragma unroll
   for (int q = 0; q < BB; q++)
@@ -662,13 +665,16 @@ __global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz
                                                      double *part, long part_ld, int part_off,
                                                      const double *in2, const double *ca,
                                                      const double *cb, double *part2,
-                                                     double *maxpart) {
+                                                     double *maxpart, int band_major) {
   // coef is [slot][j][b] (bands fastest): the EB coefficients of one (slot, j) are contiguous
-  // and shared by all rows of the same atom, so they come from L1.
+  // and shared by all rows of the same atom, so they come from L1.  band_major: the band
+  // groups of one chunk of rows are consecutive work-groups, so the chunk's entries and
+  // projector values are read from HBM about once per XCD instead of once per band group.
   __shared__ double red[EB][4];
   __shared__ double redm[EB][4];
-  const long i = blockIdx.x * 256L + threadIdx.x;
-  const int b0 = blockIdx.y * EB;
+  const unsigned rchunk = band_major ? blockIdx.y : blockIdx.x, nrchunk = band_major ? gridDim.y : gridDim.x;
+  const long i = rchunk * 256L + threadIdx.x;
+  const int b0 = (band_major ? blockIdx.x : blockIdx.y) * EB;
   const bool full = b0 + EB <= nb;
   double acc[EB];
 #pragma unroll
@@ -710,7 +716,7 @@ __global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz
     __syncthreads();
     if (threadIdx.x < EB && b0 + (int)threadIdx.x < nb) {
       const int q = threadIdx.x;
-      maxpart[(long)(b0 + q) * gridDim.x + blockIdx.x] =
+      maxpart[(long)(b0 + q) * nrchunk + rchunk] =
           fmax(fmax(redm[q][0], redm[q][1]), fmax(redm[q][2], redm[q][3]));
     }
   }
@@ -729,7 +735,7 @@ __global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz
     __syncthreads();
     if (threadIdx.x < EB && b0 + (int)threadIdx.x < nb) {
       const int q = threadIdx.x;
-      part[(long)(b0 + q) * part_ld + part_off + blockIdx.x] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
+      part[(long)(b0 + q) * part_ld + part_off + rchunk] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
     }
     if (in2) {  // <in2|added>
       __syncthreads();
@@ -742,7 +748,7 @@ __global__ __launch_bounds__(256) void k_expand_rows(const int *nzrows, long nnz
       __syncthreads();
       if (threadIdx.x < EB && b0 + (int)threadIdx.x < nb) {
         const int q = threadIdx.x;
-        part2[(long)(b0 + q) * part_ld + part_off + blockIdx.x] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
+        part2[(long)(b0 + q) * part_ld + part_off + rchunk] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
       }
     }
   }
@@ -1478,14 +1484,16 @@ void launch_gather(const NonlocalPlan &pl, const double *psi, long ld, int nb, d
                    long ld_rp, bool accumulate, hipStream_t s) {
   if (pl.nact == 0 || nb == 0) return;
   Timed t("nl_gather", 8.0 * ((double)nb * pl.nent + (double)pl.nprjmx * pl.nent));
+  const bool bm = param("nl_gather_band_major", 1.0) >= 1.0 && pl.nact <= 65535;
+  auto grid = [&](unsigned groups) { return bm ? dim3(groups, pl.nact) : dim3(pl.nact, groups); };
   if (pl.max_iprj <= 9) {
-    hipLaunchKernelGGL((k_gather<9, 4>), dim3(pl.nact, nblk(nb, 4)), dim3(256), 0, s, pl.act,
+    hipLaunchKernelGGL((k_gather<9, 4>), grid(nblk(nb, 4)), dim3(256), 0, s, pl.act,
                        pl.num_list, pl.nprjmx, pl.vnl, pl.lst, psi, ld, nb, rp, ld_rp,
-                       accumulate ? 1 : 0);
+                       accumulate ? 1 : 0, bm ? 1 : 0);
   } else if (pl.max_iprj <= 32) {
-    hipLaunchKernelGGL((k_gather<32, 1>), dim3(pl.nact, nb), dim3(256), 0, s, pl.act,
+    hipLaunchKernelGGL((k_gather<32, 1>), grid((unsigned)nb), dim3(256), 0, s, pl.act,
                        pl.num_list, pl.nprjmx, pl.vnl, pl.lst, psi, ld, nb, rp, ld_rp,
-                       accumulate ? 1 : 0);
+                       accumulate ? 1 : 0, bm ? 1 : 0);
   } else {
     throw Error(1, "nonlocal projectors: more than 32 per atom are not supported");
   }
@@ -1538,9 +1546,14 @@ void launch_expand_rows(const NonlocalPlan &pl, const double *coef, int nb, doub
   }
   Timed t("nl_expand", 8.0 * (2.0 * (double)pl.nnz * nb + (double)pl.nprjmx * pl.nent));
   constexpr int EB = 16;
-  hipLaunchKernelGGL(k_expand_rows<EB>, dim3(np, nblk(nb, EB)), dim3(256), 0, s, pl.nzrows, pl.nnz,
+  // band groups fastest (param "nl_expand_band_major"; measured 4.82 against 4.65 ms per launch
+  // at 128^3 x 1024 bands, so off by default) unless the row chunks exceed grid.y's range
+  const bool bm = param("nl_expand_band_major", 0.0) >= 1.0 && np <= 65535;
+  const dim3 grid = bm ? dim3(nblk(nb, EB), np) : dim3(np, nblk(nb, EB));
+  hipLaunchKernelGGL(k_expand_rows<EB>, grid, dim3(256), 0, s, pl.nzrows, pl.nnz,
                      pl.row_ptr, pl.ent, pl.iprj_slot, pl.vnl, pl.num_list, pl.nprjmx, coef, nb,
-                     out, ld_out, psi, ld_psi, part, part_ld, part_off, in2, ca, cb, part2, maxpart);
+                     out, ld_out, psi, ld_psi, part, part_ld, part_off, in2, ca, cb, part2, maxpart,
+                     bm ? 1 : 0);
   HIPCK(hipGetLastError());
 }
 

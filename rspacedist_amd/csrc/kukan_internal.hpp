@@ -83,7 +83,9 @@ bool is_device_ptr(const void *p);
 // ----------------------------------------------------------------------------- timing
 // With param "timing" = 1, Timed regions record HIP events on the library stream; the
 // accumulated device time per tag (and the algorithmic work the caller attributes to it)
-// is read back through kukan_timing_get.
+// is read back through kukan_timing_get.  Regions whose tag starts with "phase_" (the SCF
+// step's phases: Rayleigh-Ritz, CG, orthonormalisation, reorder, Hartree) are also roctx
+// ranges, always, so that a profiler's marker trace cuts the timeline by phase.
 class Timed {
  public:
   Timed(const char *tag, double work);
@@ -93,6 +95,7 @@ class Timed {
   const char *tag_;
   double work_;
   hipEvent_t a_ = nullptr, b_ = nullptr;
+  bool range_ = false;
 };
 void timing_collect();
 // adds work (no time, no launch) to a tag's accumulator while timing is on: a second measure

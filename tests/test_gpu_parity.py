@@ -104,7 +104,7 @@ def test_projector_plan_follows_lstvec2_in_place(kk, orc):
     args = dict(args, lstvec2=lst, nhs=1)
     h = np.asfortranarray(args["natinf"]).copy()
     for trial in range(3):
-        host = np.asfortranarray(synth.abi_args(s)["lstvec2"]).copy()
+        host = np.asfortranarray(synth.abi_args(s)["lstvec2"]).copy(order="F")
         if trial == 1:  # the same points in another order (same checksum class of entries)
             for na in range(s.natom):
                 host[: h[na], na] = host[: h[na], na][::-1]
@@ -112,6 +112,7 @@ def test_projector_plan_follows_lstvec2_in_place(kk, orc):
             host[: h[0], 0] = (host[: h[0], 0] % s.npts) + 1
         kk.lib().kukan_memcpy(ctypes.c_void_p(lst.device_ptr), host.ctypes.data, host.nbytes, 1)
         kk.synchronize()
+        assert np.array_equal(lst.numpy(), host)
         g = {"rpsep": rp.copy(order="F"), "dijsss": s.dij, "avre": avre0.copy(), "avr": np.zeros(s.num_list)}
         kk.call("nonlocaloperation_r_02", **args, **g)
         o = {"rpsep": rp.copy(order="F"), "dijsss": s.dij, "avre": avre0.copy(), "avr": np.zeros(s.num_list)}

@@ -1,8 +1,9 @@
 // batch.hip — batching pattern: many independent periodic systems of different grid sizes
 // advanced together.  Per system and step: H psi for all bands, one preconditioned
 // steepest-descent band update with a 2x2 Rayleigh-Ritz per band (the first, reset iteration
-// of kscg_r, kscg8f.F90:98-472 and :1197-1230, without the Gram-Schmidt against lower
-// bands), then the Hartree CG solve of rho = sum_l psi_l^2 (scf_hartree8f.F90).
+// of kscg_r, kscg8f.F90:98-472 and :1197-1230), the orthonormalisation of the bands (FP64
+// Cholesky-QR, the Q of orthogonalization8f.F90:93-376), then the Hartree CG solve of
+// rho = sum_l psi_l^2 (scf_hartree8f.F90).
 //
 // Batched mode: every stencil pass of all systems is one persistent launch (the items of all
 // systems in one work list), per-band arithmetic runs as launches over a global band table,
@@ -272,6 +273,133 @@ inline unsigned nb_(long n, int t) { return (unsigned)((n + t - 1) / t); }
 }  // namespace
 
 // ============================================================================= Batch
+// ---------------------------------------------------------------- orthonormalisation
+// One FP64 Cholesky-QR per system after the band update (S = 1): G = dV Psi^T Psi = R^T R,
+// Psi <- Psi R^-1, the same Q as the reference's Gram-Schmidt (orthogonalization8f.F90:93-376)
+// in exact arithmetic.  The work of all systems runs in one launch per kernel over unit
+// tables (system-major), so loop mode launches the same kernels on one system's units.
+constexpr int GT = 32;          // band tile of the Gram partials
+constexpr long GCH = 16384;     // points per Gram work-group
+constexpr int OMAX = 128;       // bands per system (the Cholesky runs in 128 KB of LDS)
+struct OSys {
+  double *psi;
+  long n;
+  int M, ntile, nchunk, ntp;
+  double dv;
+  double *G, *Rinv, *part;  // M x M, M x M, [tile pair][chunk][GT x GT]
+  int *flag;                // 1: a pivot was not positive
+};
+struct GUnit {
+  int sys, tp, ti, tj, chunk;
+};
+struct AUnit {
+  int sys;
+  long p0;  // first point of the 64-point tile
+};
+
+// Gram partial of band tiles (ti, tj) over one chunk of points: 4 outputs per thread
+__global__ __launch_bounds__(256) void k_bgram_part(const OSys *os, const GUnit *units) {
+  __shared__ double A[GT][65], Bt[GT][65];
+  const GUnit u = units[blockIdx.x];
+  const OSys S = os[u.sys];
+  const long c0 = (long)u.chunk * GCH, c1 = min(S.n, c0 + GCH);
+  const int t = threadIdx.x, i = t & 31, j0 = t >> 5;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long p0 = c0; p0 < c1; p0 += 64) {
+    for (int e = t; e < GT * 64; e += 256) {
+      const int b = e >> 6, q = e & 63;
+      const long pt = p0 + q;
+      const int ba = u.ti * GT + b, bb = u.tj * GT + b;
+      A[b][q] = (ba < S.M && pt < c1) ? S.psi[(long)ba * S.n + pt] : 0.0;
+      Bt[b][q] = (bb < S.M && pt < c1) ? S.psi[(long)bb * S.n + pt] : 0.0;
+    }
+    __syncthreads();
+    for (int q = 0; q < 64; q++) {
+      const double a = A[i][q];
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc[k] += a * Bt[j0 + 8 * k][q];
+    }
+    __syncthreads();
+  }
+  double *o = S.part + ((long)u.tp * S.nchunk + u.chunk) * (GT * GT);
+#pragma unroll
+  for (int k = 0; k < 4; k++) o[i + GT * (j0 + 8 * k)] = acc[k];
+}
+
+// G tile (ti, tj) = dV sum over chunks in order
+__global__ __launch_bounds__(256) void k_bgram_reduce(const OSys *os, const GUnit *units) {
+  const GUnit u = units[blockIdx.x];  // chunk field unused: one unit per tile pair
+  const OSys S = os[u.sys];
+  for (int e = threadIdx.x; e < GT * GT; e += 256) {
+    const int i = e % GT, j = e / GT, gi = u.ti * GT + i, gj = u.tj * GT + j;
+    double s = 0.0;
+    for (int c = 0; c < S.nchunk; c++) s += S.part[((long)u.tp * S.nchunk + c) * (GT * GT) + e];
+    if (gi < S.M && gj < S.M) S.G[gi + (long)S.M * gj] = S.dv * s;
+  }
+}
+
+// upper Cholesky factor of G in LDS, then its inverse (one column per thread), per system
+__global__ __launch_bounds__(256) void k_bchol_inv(const OSys *os) {
+  __shared__ double R[OMAX * OMAX];
+  __shared__ int bad;
+  const OSys S = os[blockIdx.x];
+  const int M = S.M, t = threadIdx.x;
+  for (int e = t; e < M * M; e += 256) R[e] = S.G[e];
+  if (t == 0) bad = 0;
+  __syncthreads();
+  for (int k = 0; k < M; k++) {
+    if (t == 0) {
+      const double d = R[k + M * k];
+      if (!(d > 0.0)) bad = 1;
+      R[k + M * k] = d > 0.0 ? sqrt(d) : 1.0;
+    }
+    __syncthreads();
+    const double rkk = R[k + M * k];
+    for (int j = k + 1 + t; j < M; j += 256) R[k + M * j] /= rkk;
+    __syncthreads();
+    const int m = M - k - 1;  // the trailing upper triangle, (i, j) with k < i <= j
+    for (int e = t; e < m * m; e += 256) {
+      const int i = k + 1 + e % m, j = k + 1 + e / m;
+      if (i <= j) R[i + M * j] -= R[k + M * i] * R[k + M * j];
+    }
+    __syncthreads();
+  }
+  // X = R^-1 (upper): column j by back substitution, written to Rinv (zero below the diagonal)
+  for (int j = t; j < M; j += 256) {
+    double *X = S.Rinv + (long)M * j;
+    for (int i = j + 1; i < M; i++) X[i] = 0.0;
+    X[j] = 1.0 / R[j + M * j];
+    for (int i = j - 1; i >= 0; i--) {
+      double s = 0.0;
+      for (int k = i + 1; k <= j; k++) s += R[i + M * k] * X[k];
+      X[i] = -s / R[i + M * i];
+    }
+  }
+  if (t == 0) *S.flag = bad;
+}
+
+// Psi <- Psi R^-1 on a tile of 64 points: the tile's bands staged in LDS, in place
+__global__ __launch_bounds__(256) void k_bortho_apply(const OSys *os, const AUnit *units) {
+  __shared__ double X[OMAX][64];
+  const AUnit u = units[blockIdx.x];
+  const OSys S = os[u.sys];
+  const int t = threadIdx.x, q = t & 63;
+  const long pt = u.p0 + q;
+  for (int e = t; e < S.M * 64; e += 256) {
+    const int b = e >> 6, qq = e & 63;
+    const long p = u.p0 + qq;
+    X[b][qq] = p < S.n ? S.psi[(long)b * S.n + p] : 0.0;
+  }
+  __syncthreads();
+  if (pt >= S.n) return;
+  for (int j = t >> 6; j < S.M; j += 4) {
+    const double *rc = S.Rinv + (long)S.M * j;
+    double y = 0.0;
+    for (int i = 0; i <= j; i++) y += X[i][q] * rc[i];
+    S.psi[(long)j * S.n + pt] = y;
+  }
+}
+
 struct BatchSystem {
   int n = 0, M = 0, natom = 0, num_list = 0, band0 = 0;
   double h = 0;
@@ -313,6 +441,14 @@ struct Batch {
   std::vector<StencilBatchPlan> plan[5];
   hipGraphExec_t g_band = nullptr, g_hart = nullptr;
   int hart_blk = 16;
+  // orthonormalisation: per-system descriptors and unit tables (system-major) with each
+  // system's first unit, so loop mode launches one system's range
+  bool ortho = true;
+  OSys *osys = nullptr;
+  GUnit *gunits = nullptr, *runits = nullptr;
+  AUnit *aunits = nullptr;
+  std::vector<int> g0, r0, a0;  // ns + 1 offsets each
+  int *oflag = nullptr;
   ~Batch() {
     if (g_band) (void)hipGraphExecDestroy(g_band);
     if (g_hart) (void)hipGraphExecDestroy(g_hart);
@@ -428,6 +564,14 @@ void band_step(Batch &B, int s0, int s1, bool batched, hipStream_t s) {
   hipLaunchKernelGGL(k_band_rr, dim3(nb_(nb, 64)), dim3(64), 0, s, nb, B.eps + b0, B.pap + b0,
                      B.sap + b0, B.ab + 2L * b0, B.sval + b0);
   hipLaunchKernelGGL(k_band_update, gu, dim3(256), 0, s, B.bands + b0, B.ab + 2L * b0);
+  // Cholesky-QR of each system's bands
+  if (B.ortho) {
+    const int ng = B.g0[s1] - B.g0[s0], nr = B.r0[s1] - B.r0[s0], na = B.a0[s1] - B.a0[s0];
+    if (ng > 0) hipLaunchKernelGGL(k_bgram_part, dim3(ng), dim3(256), 0, s, B.osys, B.gunits + B.g0[s0]);
+    if (nr > 0) hipLaunchKernelGGL(k_bgram_reduce, dim3(nr), dim3(256), 0, s, B.osys, B.runits + B.r0[s0]);
+    hipLaunchKernelGGL(k_bchol_inv, dim3(s1 - s0), dim3(256), 0, s, B.osys + s0);
+    if (na > 0) hipLaunchKernelGGL(k_bortho_apply, dim3(na), dim3(256), 0, s, B.osys, B.aunits + B.a0[s0]);
+  }
   HIPCK(hipGetLastError());
 }
 
@@ -604,6 +748,51 @@ void prepare(Batch &B) {
   }
   HIPCK(hipMemcpy(B.bands, br.data(), sizeof(BandRef) * nbt, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(B.hs, hs.data(), sizeof(HSys) * ns, hipMemcpyHostToDevice));
+  // orthonormalisation tables (param "batch_ortho", default 1, read once here)
+  B.ortho = param("batch_ortho", 1.0) >= 1.0;
+  if (B.ortho) {
+    std::vector<OSys> os(ns);
+    std::vector<GUnit> gu, ru;
+    std::vector<AUnit> au;
+    B.oflag = B.alloc<int>(ns);
+    B.g0.assign(1, 0);
+    B.r0.assign(1, 0);
+    B.a0.assign(1, 0);
+    for (int i = 0; i < ns; i++) {
+      BatchSystem &S = *B.sys[i];
+      if (S.M > OMAX) throw Error(1, "batch: at most 128 bands per system");
+      OSys &o = os[i];
+      o.psi = S.psi;
+      o.n = S.g.n();
+      o.M = S.M;
+      o.ntile = (S.M + GT - 1) / GT;
+      o.nchunk = (int)((o.n + GCH - 1) / GCH);
+      o.ntp = o.ntile * (o.ntile + 1) / 2;
+      o.dv = S.g.dv();
+      o.G = S.alloc<double>((size_t)S.M * S.M);
+      o.Rinv = S.alloc<double>((size_t)S.M * S.M);
+      o.part = S.alloc<double>((size_t)o.ntp * o.nchunk * GT * GT);
+      o.flag = B.oflag + i;
+      int tp = 0;
+      for (int tj = 0; tj < o.ntile; tj++)
+        for (int ti = 0; ti <= tj; ti++, tp++) {
+          for (int c = 0; c < o.nchunk; c++) gu.push_back({i, tp, ti, tj, c});
+          ru.push_back({i, tp, ti, tj, 0});
+        }
+      for (long p0 = 0; p0 < o.n; p0 += 64) au.push_back({i, p0});
+      B.g0.push_back((int)gu.size());
+      B.r0.push_back((int)ru.size());
+      B.a0.push_back((int)au.size());
+    }
+    B.osys = B.alloc<OSys>(ns);
+    B.gunits = B.alloc<GUnit>(gu.size());
+    B.runits = B.alloc<GUnit>(ru.size());
+    B.aunits = B.alloc<AUnit>(au.size());
+    HIPCK(hipMemcpy(B.osys, os.data(), sizeof(OSys) * ns, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(B.gunits, gu.data(), sizeof(GUnit) * gu.size(), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(B.runits, ru.data(), sizeof(GUnit) * ru.size(), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(B.aunits, au.data(), sizeof(AUnit) * au.size(), hipMemcpyHostToDevice));
+  }
   build_plans(B);
   B.ready = true;
 }
@@ -705,6 +894,12 @@ void batch_step(void *h, int batched, int ncgmax, double epsvh) {
     }
   }
   sync();
+  if (B->ortho) {
+    std::vector<int> f(ns);
+    HIPCK(hipMemcpy(f.data(), B->oflag, sizeof(int) * ns, hipMemcpyDeviceToHost));
+    for (int i = 0; i < ns; i++)
+      if (f[i]) throw Error(2, "batch: orthonormalisation of system " + std::to_string(i) + ": wf has negative norm");
+  }
 }
 
 void batch_get(void *h, int i, double *psi, double *sval, double *vh, int *iters) {

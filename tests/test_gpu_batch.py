@@ -1,6 +1,7 @@
 """Batching pattern on the GPU: the persistent, graph-replayed batched step against the same
 kernels launched system by system (bit-identical), and each system against the oracle's
-one-system restatement (orc_band_sd_step, then orc_scf_hartree)."""
+one-system restatement (orc_band_sd_step, the orthonormalisation orc_orthogonalization_r, then
+orc_scf_hartree); the bands leave each step orthonormal to FP64 rounding."""
 import time
 
 import numpy as np
@@ -48,6 +49,10 @@ def _oracle_step(orc, s, psi0):
     orc.call("band_sd_step", n=s.n, h=s.h, nbands=s.nbands, natom=s.natom, num_list=s.num_list,
              lstvec2=s.lstvec2, natinf=s.natinf, vnlocp=s.vnlocp, dij=dij,
              veff=np.asfortranarray(s.veff[..., 0]), svecre=psi, sval=sval)
+    # the batched step orthonormalises the updated bands (Cholesky-QR; the reference's
+    # Gram-Schmidt gives the same Q)
+    orc.call("orthogonalization_r", **synth.abi_args(s), nflag=1, svecre=psi, ssvre=psi.copy(order="F"),
+             rspsep=np.zeros((9, s.natom, s.nbands), order="F"))
     rho = (psi ** 2).sum(axis=1)
     z = np.zeros(1)
     vo = np.zeros(s.npts)
@@ -71,6 +76,8 @@ def test_batched_and_loop_match_oracle(kk, orc):
                   f"|dsval| {np.abs(sg - sval).max():.2e} |dvh| {np.abs(vg - vo).max():.2e}")
             np.testing.assert_allclose(sg, sval, rtol=1e-11, atol=1e-12)
             np.testing.assert_allclose(pg, psi, rtol=0, atol=1e-10 * np.abs(psi).max())
+            gram = s.dv * (pg.T @ pg)
+            assert np.abs(gram - np.eye(s.nbands)).max() < 1e-12
             np.testing.assert_allclose(vg, vo, rtol=0, atol=1e-8 * np.abs(vo).max())
         b.close()
 
@@ -94,7 +101,7 @@ def test_batched_equals_loop_bitwise(kk):
 def test_batching_config_48_systems(kk):
     """The 48-system configuration (sizes in {32,48,64,80}^3, 16..128 bands, 2..24 atoms):
     batched (persistent stencil launches + graph replays) vs the per-system launch loop, same
-    results bit for bit; throughput printed."""
+    results bit for bit, bands orthonormal after the steps; throughput printed."""
     from rspacedist_amd import batch as BT
     cfgs = BT.batch_configs(48, seed=4)
     systems = []
@@ -130,5 +137,8 @@ def test_batching_config_48_systems(kk):
         np.testing.assert_array_equal(sb, sl)
         np.testing.assert_array_equal(vb, vl)
         assert ib == il
+        s = systems[i][0]
+        gram = s.dv * (pb.T @ pb)
+        assert np.abs(gram - np.eye(s.nbands)).max() < 1e-11, i
     bb.close()
     bl.close()

@@ -272,8 +272,11 @@ int kukan_scf_charge_smt_r(int neigmx, int nums, int nspv, int numk, int ncpx, i
 /* Batching pattern (no reference counterpart): independent periodic cubic systems of different
  * sizes advanced together.  A step is, per system, H psi for all bands, one preconditioned
  * steepest-descent band update with a 2x2 Rayleigh-Ritz per band (the reset iteration of
- * kscg_r, kscg8f.F90:98-472, without Gram-Schmidt against lower bands), then the Hartree CG
- * of rho = sum psi^2 to epsvh (scf_hartree8f.F90, warm-started from the previous step).
+ * kscg_r, kscg8f.F90:98-472, without Gram-Schmidt against lower bands), the
+ * orthonormalisation of each system's bands (the result of orthogonalization_r,
+ * orthogonalization8f.F90:8-44, by FP64 Cholesky-QR; at most 128 bands per system; a non-positive
+ * pivot returns "wf has negative norm"), then the Hartree CG of rho = sum psi^2 to epsvh
+ * (scf_hartree8f.F90, warm-started from the previous step).
  * batched = 1: one persistent stencil launch per pass over all systems, band and Hartree work
  * over global tables, HIP-graph replays; batched = 0: the same kernels system by system.  Both
  * give bit-identical results.  Systems: grid n^3 (8 <= n <= 256), spacing h, nf = nfh = 4,

@@ -400,15 +400,37 @@ __device__ inline void wave_lds_sync() {
 }
 
 constexpr int NN_NS = 5;
-typedef _Float16 FragBuf[NN_NS][32][512];
+// ring of NS stages for a work-group of 2 WR waves over 64 WR rows x 256 columns: per stage
+// 2 WR A fragments (hi), 2 WR (lo), then 8 B fragments (hi) and 8 (lo), 1 KB each
+template <int WR, int NS>
+using FragRing = _Float16[NS][4 * WR + 16][512];
 
-// The fragment-GEMM main loop shared by NN and pre-split TN: acc (wave w: rows 64 (w & 3),
-// columns 128 (w >> 2) of a 256 x 256 tile) += the products of nst 16-deep stages.  The tile's
-// A fragments of stage st are at a[h|l] + (idx KC + st) 512 (idx = row block 0..7), the B
-// fragments likewise; the caller offsets a* / b* to its first row / column block and first
-// stage.  Each wave issues 4 LDS-DMA loads per stage (buffer_load ... lds through one buffer
-// descriptor per fragment row: the MUBUF form, unlike the global one, leaves the compiler's
-// LDS-read waits graded), NN_NS - 1 stages ahead of the one it computes.  Stage st + 1's
+// wait until this wave's loads of all but the `later` most recent stages (LPS loads each) have
+// landed; later <= MAXL
+template <int LPS, int MAXL>
+__device__ inline void wait_stages(int later) {
+  static_assert(LPS * MAXL < 16, "vmcnt immediate");
+  if constexpr (MAXL >= 3) {
+    if (later >= 3) { wait_vm<3 * LPS>(); return; }
+  }
+  if constexpr (MAXL >= 2) {
+    if (later >= 2) { wait_vm<2 * LPS>(); return; }
+  }
+  if constexpr (MAXL >= 1) {
+    if (later >= 1) { wait_vm<LPS>(); return; }
+  }
+  wait_vm<0>();
+}
+
+// The fragment-GEMM main loop shared by NN and pre-split TN, for a work-group of 2 WR waves:
+// acc (wave w: rows 64 (w % WR), columns 128 (w / WR) of a 64 WR x 256 tile) += the products
+// of nst 16-deep stages.  The tile's A fragments of stage st are at a[h|l] + (idx KC + st) 512
+// (idx = row block 0 .. 2 WR - 1), the B fragments likewise (idx 0..7); the caller offsets
+// a* / b* to its first row / column block and first stage.  Each wave issues LPS = 2 + 8 / WR
+// LDS-DMA loads per stage (buffer_load ... lds through one buffer descriptor per fragment row:
+// the MUBUF form, unlike the global one, leaves the compiler's LDS-read waits graded), NS - 1
+// stages ahead of the one it computes.  WR = 4, NS = 5 (160 KB, one work-group per CU) and
+// WR = 2, NS = 3 (72 KB, two per CU) are used.  Stage st + 1's
 // fragments are read from LDS while stage st multiplies: the next A fragments after the first
 // MFMA group, each B fragment as soon as its last MFMA of stage st is issued (at most 12 LDS
 // reads outstanding, within the 4-bit lgkm counter).  Before reading stage st + 1 a wave waits
@@ -422,35 +444,53 @@ typedef _Float16 FragBuf[NN_NS][32][512];
 // of stage nst - 1 (its fragments are read during stage nst - 2), and their reads of stage
 // nst - 2's slot, issued during stage nst - 3, may still be outstanding when they pass that
 // barrier.  The slots of stages nst - 3 and earlier were read into registers and consumed by
-// MFMAs before every wave reached the barrier, so those three slots are free.
+// MFMAs before every wave reached the barrier, so NS - 2 slots are free.
 // Fragment rows are addressed through 32-bit buffer offsets: KC 16-deep stages of 1 KB each.
 static void frag_rows_ok(long KC, const char *who) {
   if (KC * 1024 > 0x7fffffffL) throw Error(1, std::string(who) + ": operand too long for the split-precision product");
 }
 
-template <int NT>
-__device__ inline void frag_gemm(const _Float16 *ah_g, const _Float16 *al_g, const _Float16 *bh_g,
-                                 const _Float16 *bl_g, long KC, int nst, FragBuf &buf,
-                                 f16v (&acc)[2][4]) {
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
-  // this wave's 4 fragments per stage: f = 4 w + i; f / 8 = A hi, A lo, B hi, B lo.  One
-  // buffer descriptor per fragment row (uniform per wave); a row spans KC * 1 KB, which the
-  // host keeps below 2^31 bytes (frag_rows_ok)
-  const int ws = __builtin_amdgcn_readfirstlane(w);
-  __amdgpu_buffer_rsrc_t rs[4];
+// ring slot of stage st: NF fragments of 512 halves
+template <int NF, int NS>
+__device__ inline _Float16 (*frag_slot(_Float16 *ring, int st))[512] {
+  return reinterpret_cast<_Float16(*)[512]>(ring + (long)(st % NS) * NF * 512);
+}
+
+// this wave's LPS LDS-DMA loads of stage st (fragments LPS ws .. LPS ws + LPS - 1)
+template <int LPS, int NF, int NS>
+__device__ inline void frag_dma(_Float16 *ring, const __amdgpu_buffer_rsrc_t *rs, int ws, int lane,
+                                int st) {
+  _Float16(*d)[512] = frag_slot<NF, NS>(ring, st);
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int f = 4 * ws + i, op = f >> 3, idx = f & 7;
-    const _Float16 *base = op == 0 ? ah_g : op == 1 ? al_g : op == 2 ? bh_g : bl_g;
+  for (int i = 0; i < LPS; i++)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs[i], NN_LDS(&d[LPS * ws + i][0]), 16, st * 1024 + 16 * lane, 0, 0, 0);
+}
+
+template <int NT, int WR, int NS>
+__device__ inline void frag_gemm(const _Float16 *ah_g, const _Float16 *al_g, const _Float16 *bh_g,
+                                 const _Float16 *bl_g, long KC, int nst, _Float16 *ring,
+                                 f16v (&acc)[2][4]) {
+  static_assert((WR == 2 || WR == 4) && NS >= 3, "ring shape");
+  constexpr int NA = 2 * WR, LPS = 2 + 8 / WR;  // A fragments per half; loads per wave and stage
+  constexpr int NF = 2 * NA + 16;               // fragments per stage (ring slot)
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w % WR, wj = w / WR;
+  // this wave's LPS fragments per stage: f = LPS w + i; f < NA: A hi, < 2 NA: A lo, then 8
+  // B hi and 8 B lo.  One buffer descriptor per fragment row (uniform per wave); a row spans
+  // KC * 1 KB, which the host keeps below 2^31 bytes (frag_rows_ok)
+  const int ws = __builtin_amdgcn_readfirstlane(w);
+  __amdgpu_buffer_rsrc_t rs[LPS];
+#pragma unroll
+  for (int i = 0; i < LPS; i++) {
+    const int f = LPS * ws + i;
+    const _Float16 *base;
+    int idx;
+    if (f < NA) { base = ah_g; idx = f; }
+    else if (f < 2 * NA) { base = al_g; idx = f - NA; }
+    else if (f < 2 * NA + 8) { base = bh_g; idx = f - 2 * NA; }
+    else { base = bl_g; idx = f - 2 * NA - 8; }
     rs[i] = __builtin_amdgcn_make_buffer_rsrc((void *)(base + (long)idx * KC * 512), (short)0,
                                               (int)(KC * 1024), 0x00020000);
   }
-  auto dma = [&](int st) {
-    _Float16(*d)[512] = buf[st % NN_NS];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs[i], NN_LDS(&d[4 * ws + i][0]), 16, st * 1024 + 16 * lane, 0, 0, 0);
-  };
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -458,48 +498,39 @@ __device__ inline void frag_gemm(const _Float16 *ah_g, const _Float16 *al_g, con
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
 #pragma unroll
-  for (int i = 0; i < NN_NS - 1; i++)
-    if (i < nst) dma(i);
-  {
-    const int later = min(NN_NS - 2, nst - 1);
-    if (later >= 3) wait_vm<12>();
-    else if (later == 2) wait_vm<8>();
-    else if (later == 1) wait_vm<4>();
-    else wait_vm<0>();
-  }
+  for (int i = 0; i < NS - 1; i++)
+    if (i < nst) frag_dma<LPS, NF, NS>(ring, rs, ws, lane, i);
+  wait_stages<LPS, NS - 2>(min(NS - 2, nst - 1));
   asm volatile("" ::: "memory");
   __builtin_amdgcn_s_barrier();
   asm volatile("" ::: "memory");
   __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
   half8 ah[2], al[2], bh[4], bl[4];
   {
-    const _Float16(*C)[512] = buf[0];
+    const _Float16(*C)[512] = frag_slot<NF, NS>(ring, 0);
 #pragma unroll
     for (int a = 0; a < 2; a++) {
       ah[a] = *(const half8 *)&C[2 * wi + a][8 * lane];
-      al[a] = *(const half8 *)&C[8 + 2 * wi + a][8 * lane];
+      al[a] = *(const half8 *)&C[NA + 2 * wi + a][8 * lane];
     }
 #pragma unroll
     for (int b = 0; b < 4; b++) {
-      bh[b] = *(const half8 *)&C[16 + 4 * wj + b][8 * lane];
-      bl[b] = *(const half8 *)&C[24 + 4 * wj + b][8 * lane];
+      bh[b] = *(const half8 *)&C[2 * NA + 4 * wj + b][8 * lane];
+      bl[b] = *(const half8 *)&C[2 * NA + 8 + 4 * wj + b][8 * lane];
     }
   }
   auto stage = [&](auto more_t, int st) {
     constexpr bool more = decltype(more_t)::value;
     half8 ahn[2], aln[2];
-    const _Float16(*C1)[512] = buf[(st + 1) % NN_NS];
+    const _Float16(*C1)[512] = frag_slot<NF, NS>(ring, st + 1);
     if (more) {
       // stage st + 1 landed (own loads) and published; stages after it stay in flight
-      const int later = min(NN_NS - 3, nst - 2 - st);
-      if (later >= 2) wait_vm<8>();
-      else if (later == 1) wait_vm<4>();
-      else wait_vm<0>();
+      wait_stages<LPS, NS - 3>(min(NS - 3, nst - 2 - st));
       asm volatile("" ::: "memory");
       __builtin_amdgcn_s_barrier();
       asm volatile("" ::: "memory");
       // the buffer refilled held stage st - 1, whose fragments every wave has consumed
-      if (st + NN_NS - 1 < nst) dma(st + NN_NS - 1);
+      if (st + NS - 1 < nst) frag_dma<LPS, NF, NS>(ring, rs, ws, lane, st + NS - 1);
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -516,13 +547,13 @@ __device__ inline void frag_gemm(const _Float16 *ah_g, const _Float16 *al_g, con
 #pragma unroll
         for (int a = 0; a < 2; a++) {
           ahn[a] = *(const half8 *)&C1[2 * wi + a][8 * lane];
-          aln[a] = *(const half8 *)&C1[8 + 2 * wi + a][8 * lane];
+          aln[a] = *(const half8 *)&C1[NA + 2 * wi + a][8 * lane];
         }
       }
-      if (more) bl[b] = *(const half8 *)&C1[24 + 4 * wj + b][8 * lane];
+      if (more) bl[b] = *(const half8 *)&C1[2 * NA + 8 + 4 * wj + b][8 * lane];
 #pragma unroll
       for (int a = 0; a < 2; a++) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
-      if (more) bh[b] = *(const half8 *)&C1[16 + 4 * wj + b][8 * lane];
+      if (more) bh[b] = *(const half8 *)&C1[2 * NA + 4 * wj + b][8 * lane];
       __builtin_amdgcn_sched_barrier(0);
     }
     if (more) {
@@ -538,48 +569,66 @@ __device__ inline void frag_gemm(const _Float16 *ah_g, const _Float16 *al_g, con
   wait_vm<0>();
 }
 
-// Y tile 256 rows x 256 columns (frag_gemm over all of K).  The col_tiles of one row block
-// run consecutively on one XCD so that X'' is fetched from HBM once.  The epilogue goes
-// through LDS so that each column of Y is written (and, for beta != 0, read) as 512-byte runs.
-template <int NT>
-__global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
-  __shared__ __attribute__((aligned(16))) FragBuf buf;
+// Y tile 64 WR rows x 256 columns (frag_gemm over all of K): WR = 4 (256 rows, 160 KB ring,
+// one work-group per CU) or WR = 2 (128 rows, 72 KB ring, two work-groups per CU, so one's
+// epilogue and load waits overlap the other's MFMAs).  Every output gets the same MFMA
+// sequence in both shapes, so Y is the same bit for bit.  The col_tiles of one row block run
+// consecutively on one XCD so that X'' is fetched from HBM once.  The epilogue goes through
+// LDS so that each column of Y is written (and, for beta != 0, read) as runs of 64 rows.
+template <int NT, int WR, int NS>
+__global__ __launch_bounds__(128 * WR, 2) void k_gemm_nn(NNArgs p) {
+  constexpr int NF = 4 * WR + 16;  // fragments per ring slot
+  __shared__ __attribute__((aligned(16))) _Float16 ring[NS * NF * 512];
   const int L = blockIdx.x, xcd = L & 7, jj = L >> 3;
   const long rb = (long)(jj / p.col_tiles) * 8 + xcd;
   const int ct = jj % p.col_tiles;
   if (rb >= p.row_blocks) return;
-  const long r0 = rb * NN_BM;
+  const long r0 = rb * 64 * WR;
   const int c0 = ct * NN_BN;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w % WR, wj = w / WR;
   const long KC = p.Kp / 16;
   // with C upper triangular, rows k >= c0 + 256 of C are zero for these columns: their
   // products would add exact zeros, so the stages are skipped
   const int nst = p.cupper ? (int)min(KC, (long)(c0 + NN_BN) / 16) : (int)KC;
   f16v acc[2][4];
-  frag_gemm<NT>(p.xh + (r0 / 32) * KC * 512, p.xl + (r0 / 32) * KC * 512,
-                p.ch + (long)(c0 / 32) * KC * 512, p.cl + (long)(c0 / 32) * KC * 512, KC, nst,
-                buf, acc);
+  frag_gemm<NT, WR, NS>(p.xh + (r0 / 32) * KC * 512, p.xl + (r0 / 32) * KC * 512,
+                        p.ch + (long)(c0 / 32) * KC * 512, p.cl + (long)(c0 / 32) * KC * 512, KC,
+                        nst, ring, acc);
   // epilogue: per 32-column block b, each wave parks its 64 x 32 piece in LDS as
   // [column][row] (pitch 65 floats: conflict-free) and writes it back column by column.
   // For beta != 0 the block's 32 Y values per lane are loaded before the parking, as
   // independent loads whose latency overlaps the LDS round trip.  Each wave's area (and its
   // column sums) is its own, so the waves synchronise only with themselves until the column
   // sums are combined.
-  // The areas lie in the three ring slots of stages nst - 3, nst - 4 and nst - 5 (slots last +
-  // 3, + 2, + 1), which no wave reads after the last barrier (stage nst - 1's slot may still
-  // be read, and stage nst - 2's may still have reads in flight): three 8320-byte wave areas
-  // per 32 KB slot, waves 6 and 7 with the column sums behind them in the third, so a wave
-  // starts its epilogue without waiting for the others.
+  // WR = 4: the areas lie in the three ring slots of stages nst - 3, nst - 4 and nst - 5
+  // (slots last + 3, + 2, + 1), which no wave reads after the last barrier (stage nst - 1's
+  // slot may still be read, and stage nst - 2's may still have reads in flight): three
+  // 8320-byte wave areas per 32 KB slot, waves 6 and 7 with the column sums behind them in the
+  // third, so a wave starts its epilogue without waiting for the others.  WR = 2: the ring has
+  // one free slot only; the work-group synchronises once and the areas start at the ring's
+  // base (the other work-group on the CU keeps the SIMDs busy meanwhile).
   constexpr int EP = 65;
-  static_assert(NN_NS == 5 && 3 * 32 * EP * sizeof(float) <= sizeof(buf[0]) &&
-                    2 * 32 * EP * sizeof(float) + 1024 * sizeof(double) <= sizeof(buf[0]),
-                "epilogue areas: three free ring slots, three wave areas per slot");
-  const int last = (nst - 1) % NN_NS;  // nst >= 1
-  float *W = reinterpret_cast<float *>(&buf[(last + 1 + w / 3) % NN_NS][0][0]) + (w % 3) * 32 * EP;
-  // column sums of squares of the parked values, [row group wi][tile column], after the two
-  // wave areas of slot last + 3 (2 * 32 * EP floats = 16640 bytes, 8-byte aligned)
-  double *cs = reinterpret_cast<double *>(reinterpret_cast<float *>(&buf[(last + 3) % NN_NS][0][0]) +
-                                          2 * 32 * EP);
+  float *W;
+  double *cs;  // column sums of squares of the parked values, [row group wi][tile column]
+  if constexpr (WR == 4) {
+    constexpr size_t slot_bytes = NF * 512 * sizeof(_Float16);
+    static_assert(NS == 5 && 3 * 32 * EP * sizeof(float) <= slot_bytes &&
+                      2 * 32 * EP * sizeof(float) + 1024 * sizeof(double) <= slot_bytes,
+                  "epilogue areas: three free ring slots, three wave areas per slot");
+    const int last = (nst - 1) % NS;  // nst >= 1
+    W = reinterpret_cast<float *>(ring + (long)((last + 1 + w / 3) % NS) * NF * 512) + (w % 3) * 32 * EP;
+    // after the two wave areas of slot last + 3 (2 * 32 * EP floats = 16640 bytes, 8-byte
+    // aligned)
+    cs = reinterpret_cast<double *>(reinterpret_cast<float *>(ring + (long)((last + 3) % NS) * NF * 512) +
+                                    2 * 32 * EP);
+  } else {
+    static_assert(2 * WR * 32 * EP * sizeof(float) + WR * 256 * sizeof(double) <= sizeof(ring),
+                  "epilogue areas: the whole ring");
+    __syncthreads();  // every wave's last fragment reads are done
+    float *base = reinterpret_cast<float *>(ring);
+    W = base + w * 32 * EP;
+    cs = reinterpret_cast<double *>(base + 2 * WR * 32 * EP);  // 8-byte aligned
+  }
   const long m = r0 + 64 * wi + lane;
   const bool mrow = m < p.R;
 #pragma unroll
@@ -623,8 +672,10 @@ __global__ __launch_bounds__(512) void k_gemm_nn(NNArgs p) {
   if (p.colsq) __syncthreads();  // column sums of all waves
   if (p.colsq && t < 256 && c0 + t < p.M2) {  // the values written are ys W exactly
     const double y2 = p.ys[c0 + t] * p.ys[c0 + t];
-    p.colsq[(long)(c0 + t) * p.row_blocks + rb] =
-        y2 * (((cs[t] + cs[256 + t]) + cs[512 + t]) + cs[768 + t]);
+    double sum = cs[t];
+#pragma unroll
+    for (int g = 1; g < WR; g++) sum += cs[g * 256 + t];
+    p.colsq[(long)(c0 + t) * p.row_blocks + rb] = y2 * sum;
   }
 }
 
@@ -762,7 +813,7 @@ struct TNFArgs {
 // A and B fragments come from HBM once.
 template <int NT>
 __global__ __launch_bounds__(512) void k_gemm_tnf(TNFArgs p) {
-  __shared__ __attribute__((aligned(16))) FragBuf buf;
+  __shared__ __attribute__((aligned(16))) FragRing<4, NN_NS> buf;
   const int L = blockIdx.x, xcd = L & 7, jj = L >> 3;
   const int tiles = p.tri ? p.tiles_i * (p.tiles_i + 1) / 2 : p.tiles_i * p.tiles_j;
   const int split = (jj / tiles) * 8 + xcd;
@@ -785,7 +836,7 @@ __global__ __launch_bounds__(512) void k_gemm_tnf(TNFArgs p) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wi = w & 3, wj = w >> 2;
   f16v acc[2][4];
   const long ao = ((long)(i0 / 32) * p.KC + st0) * 512, bo = ((long)(j0 / 32) * p.KC + st0) * 512;
-  frag_gemm<NT>(p.ah + ao, p.al + ao, p.bh + bo, p.bl + bo, p.KC, nst, buf, acc);
+  frag_gemm<NT, 4, NN_NS>(p.ah + ao, p.al + ao, p.bh + bo, p.bl + bo, p.KC, nst, &buf[0][0][0], acc);
   float *out = p.part + (long)split * p.M1 * p.M2;
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -965,7 +1016,10 @@ void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ld
   p.R = R; p.M2 = M2; p.Kp = Kp; p.xh = xh; p.xl = xl; p.ch = ch; p.cl = cl;
   p.ys = ys; p.beta = beta; p.Y = Y; p.ldy = ldy;
   p.cupper = c_upper ? 1 : 0;
-  p.row_blocks = (int)(Rp / NN_BM);
+  // rows per work-group: 256 (one work-group per CU, the faster: 13.9 against 16.3 ms at
+  // 2.1M x 1024 x 1024, beta = 0) or 128 (two per CU); Rp is a multiple of both
+  const int rows = param("nn_rows", 256.0) >= 256.0 ? 256 : 128;
+  p.row_blocks = (int)(Rp / rows);
   p.col_tiles = M2p / NN_BN;
   const long groups = (p.row_blocks + 7) / 8;
   const unsigned grid = (unsigned)(groups * 8 * p.col_tiles);
@@ -985,8 +1039,13 @@ void gemm_mixed_nn(long R, int K, int M2, double alpha, const double *X, long ld
     // algorithmic flops 2 R M2 K; the f16 MFMA flops of the stages computed beside them
     timing_add_work("gemm_mixed_nn.mfma", frac * 2.0 * (nterms >= 4 ? 4 : 3) * (double)R * M2 * K);
     Timed t("gemm_mixed_nn", 2.0 * (double)R * M2 * K);
-    if (nterms >= 4) hipLaunchKernelGGL(k_gemm_nn<4>, dim3(grid), dim3(512), 0, s, p);
-    else hipLaunchKernelGGL(k_gemm_nn<3>, dim3(grid), dim3(512), 0, s, p);
+    if (rows == 256) {
+      if (nterms >= 4) hipLaunchKernelGGL((k_gemm_nn<4, 4, NN_NS>), dim3(grid), dim3(512), 0, s, p);
+      else hipLaunchKernelGGL((k_gemm_nn<3, 4, NN_NS>), dim3(grid), dim3(512), 0, s, p);
+    } else {
+      if (nterms >= 4) hipLaunchKernelGGL((k_gemm_nn<4, 2, 3>), dim3(grid), dim3(256), 0, s, p);
+      else hipLaunchKernelGGL((k_gemm_nn<3, 2, 3>), dim3(grid), dim3(256), 0, s, p);
+    }
     HIPCK(hipGetLastError());
   }
   if (colsq) launch_reduce_partials(part, p.row_blocks, M2, dv, colsq, s);

@@ -115,6 +115,26 @@ def test_mixed_nn(mixed, R, K, n, beta):
     assert np.all(err <= bound)
 
 
+@pytest.mark.parametrize("R,K,n,beta", [(1000, 37, 53, 0.0), (70001, 130, 300, 0.5),
+                                        (300000, 256, 257, -1.0)])
+def test_nn_tile_shapes_bitwise(mixed, R, K, n, beta):
+    """128-row work-groups (two per CU) and 256-row work-groups (one per CU) give every
+    output the same MFMA sequence: the products are equal bit for bit, ragged rows, columns
+    and K included."""
+    rng = np.random.default_rng(R + n)
+    X = rng.standard_normal((R, K))
+    Cm = rng.standard_normal((K, n))
+    Y0 = rng.standard_normal((R, n))
+    out = []
+    try:
+        for rows in (256, 128):
+            mixed.set_param("nn_rows", rows)
+            out.append(_nn(mixed, X, Cm, 0.75, beta, Y0))
+    finally:
+        mixed.set_param("nn_rows", 256)
+    np.testing.assert_array_equal(out[0], out[1])
+
+
 def test_mixed_terms_four_is_tighter(mixed):
     rng = np.random.default_rng(9)
     A = rng.standard_normal((200000, 64))

@@ -480,6 +480,44 @@ def cpu_config1_sweep():
     return {"threads": threads, "one_iteration_s": round(time.perf_counter() - t0, 2)}
 
 
+def product_only(kk, args):
+    """One split-FP16 subspace product alone at the headline shape (2,097,152 x 1024 x 1024;
+    NN: Y = X C, TN: C = X^T X), --warmup untimed launches then --steps timed ones, with the
+    operands resident in HBM.  For profiling the product without the rest of the step."""
+    R, M = 128 ** 3, 1024
+    X = kk.DeviceArray((R, M)).fill_normal(3)
+    C = kk.DeviceArray((M, M)).fill_normal(4)
+    Y = kk.DeviceArray((R, M)) if args.product == "nn" else None
+    kk.set_param("precision", 1)
+    kk.set_param("mixed_min_rows", 0)
+    tag = "gemm_mixed_" + args.product
+
+    def launch():
+        if args.product == "nn":
+            kk.subspace_gemm(0, R, M, M, 1.0, X, R, C, M, 0.0, Y, R)
+        else:
+            kk.subspace_gemm(1, M, M, R, 1.0, X, R, X, R, 0.0, C, M)
+
+    for _ in range(args.warmup):
+        launch()
+    kk.synchronize()
+    kk.timing_reset()
+    kk.set_param("timing", 1)
+    for _ in range(args.steps):
+        launch()
+    kk.synchronize()
+    kk.set_param("timing", 0)
+    tim = kk.timing()
+    ent = roofline_entry(tim, tag)
+    print(json.dumps({"metric": f"split-FP16 {args.product.upper()} product, {R} x {M} x {M}",
+                      "value": round(ent["avg_launch_ms"], 4), "unit": "ms/product", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "higher_is_better": False,
+                      "dtype": "f64 via split-f16", "data": "synthetic", "roofline": ent}), flush=True)
+    for a in (X, C, Y):
+        if a is not None:
+            a.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -492,6 +530,8 @@ def main():
                     help="set a library parameter (kukan_set_param) before the run")
     ap.add_argument("--no-suite", action="store_true",
                     help="skip the 40^3, 96^3 FP64, adjoint and batch workloads")
+    ap.add_argument("--product", choices=["nn", "tn"],
+                    help="time one split-FP16 product alone at the headline shape, nothing else")
     ap.add_argument("--cpu-config1", action="store_true",
                     help="also time one oracle iteration at 96^3 x 512 bands (minutes)")
     args = ap.parse_args()
@@ -504,6 +544,9 @@ def main():
     for kv in args.param:
         k, v = kv.split("=", 1)
         kk.set_param(k, float(v))
+    if args.product:
+        product_only(kk, args)
+        return
     log(f"[bench] config {args.config}: building synthetic system")
     cell = workload.Cell(cfg=args.config)
     s = cell.s

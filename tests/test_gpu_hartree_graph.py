@@ -14,9 +14,11 @@ pytestmark = pytest.mark.gpu
 def kk():
     from rspacedist_amd import kukan
     kukan.init(0)
+    kukan.set_param("hartree_persistent", 0)  # the iteration blocks (the persistent CG has its own tests)
     yield kukan
     kukan.set_param("hartree_graph", 1)
     kukan.set_param("hartree_block", 16)
+    kukan.set_param("hartree_persistent", 0)
 
 
 def _solve(kk, n, h, rho, vh0, graph, ncgmax=800, eps=1e-10, nperi=3):

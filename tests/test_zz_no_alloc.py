@@ -1,8 +1,10 @@
 """kukan_scf_hartree_noalloc (the Hartree solve on the library's fixed per-GPU workspace, for
 callers that cannot allocate) against kukan_scf_hartree on the same device inputs: two calls
 in a row, the second on a smaller grid (the workspace keeps the first call's contents), a
-Dirichlet (isolated) case, and the size limit.  Both entry points run the same kernels in the
-same order, so the potentials and iteration counts agree exactly."""
+Dirichlet (isolated) case, and the size limit.  The no-alloc entry point always runs the
+Hartree CG as iteration blocks; with the allocating entry point's opt-in persistent CG off
+(hartree_persistent 0, the default) both run the same kernels in the same order, so the
+potentials and iteration counts agree exactly."""
 import numpy as np
 import pytest
 
@@ -13,7 +15,9 @@ pytestmark = pytest.mark.gpu
 def kk():
     from rspacedist_amd import kukan
     kukan.init(0)
-    return kukan
+    kukan.set_param("hartree_persistent", 0)
+    yield kukan
+    kukan.set_param("hartree_persistent", 0)
 
 
 def _solve(kk, name, n, rho, vh0, nperi=3, nd=1, bounds=None, epsvh=1e-10):

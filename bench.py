@@ -281,7 +281,7 @@ def config4(kk):
     # lower bound on the step's HBM fraction: only H psi's algorithmic bytes (psi read and
     # H psi written once, 16 bytes per point-band) over the whole batched step time
     hbm = 16.0 * pts / tb / 1e9
-    return {"workload": f"48 systems {{32,48,64,80}}^3 h=0.25, {bands} bands, {pts / 1e6:.0f} M point-bands: H psi + band update + Hartree CG per system",
+    return {"workload": f"48 systems {{32,48,64,80}}^3 h=0.25, {bands} bands, {pts / 1e6:.0f} M point-bands: H psi + band update + FP64 Cholesky-QR orthonormalisation + Hartree CG per system",
             "batched_ms_per_step": round(tb * 1e3, 2), "loop_ms_per_step": round(tl * 1e3, 2),
             "systems_per_s": round(48 / tb, 1), "speedup_vs_loop": round(tl / tb, 2),
             "hbm_frac_lower_bound": round(hbm / PEAK_HBM_GBS, 4),

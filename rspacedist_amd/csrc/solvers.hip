@@ -529,7 +529,7 @@ __global__ void k_hcg_ctl(double *sc, double dvo, double eps, int ncgmin, int nc
 // hartree_persistent 1).  Work-group g owns a 16 x 16 (x, y) patch times 16 z-planes; each
 // thread keeps its z-column's V, A V, A p and p in registers for the whole solve, so an
 // iteration moves only p and b through memory: each thread stores its new p, and the stencil
-// reads the patch's x / y halo rows (staged 8 planes at a time in LDS) and each column's z-halo
+// reads the patch's x / y halo tiles (8 planes at a time, by LDS-DMA) and each column's z-halo
 // from the global copy.  Per iteration one grid barrier (p complete) and two all-gathered sums
 // (<p|A p> and <r|r>: every work-group reads all work-groups' partials and folds them in the
 // same order, so all of them take the same alpha, beta and stopping decision (ncgmin, ncgmax,
@@ -552,6 +552,7 @@ __global__ void k_hcgp_init(HcgBar *bar, int nblocks, unsigned *slot_gens, int n
 }
 constexpr int HCGP_T = 16, HCGP_P = 16;  // patch edge in x and y; z-points per thread
 constexpr long HCGP_SPINS = 1L << 22;    // bound of every wait (seconds; a real wait is microseconds)
+constexpr int HCGP_AUX_SC1 = 16;         // buffer cache policy: sc1 (agent-coherent), as the atomics
 // Data that crosses work-groups inside the solve (p, the partial sums) is written and read only
 // with agent-scope atomics (coherent across the XCDs' L2 caches, no cache-wide write-back or
 // invalidate); the barrier orders them by waiting for each wave's stores to complete before the
@@ -699,6 +700,8 @@ __global__ __launch_bounds__(256, 2) void k_hcg_persistent(HPArgs a) {
     HPArgs ai = a;
     asm volatile("" : "+s"(ai.p));
     const double *pc = ai.p + (long)x + (long)a.nx * y;
+    const __amdgpu_buffer_rsrc_t prs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)ai.p, (short)0, (int)(plane * a.nz * 8), 0x00020000);
     for (int q = 0; q < 2 * H; q++) {  // z-planes of the halo: wrapped, or 0 outside an isolated cell
       int zq = q < H ? z0 - H + q : z0 + P + q - H;
       const bool out = zq < 0 || zq >= a.nz;
@@ -709,14 +712,34 @@ __global__ __launch_bounds__(256, 2) void k_hcg_persistent(HPArgs a) {
 #pragma unroll
     for (int j0 = 0; j0 < P; j0 += PG) {
       if (j0 > 0) __syncthreads();  // the previous phase's planes are read
-#pragma unroll 6
-      for (int idx = t; idx < PG * W * W; idx += 256) {  // unrolled: several loads in flight
-        const int jj = idx / (W * W), rem = idx - jj * (W * W), ly = rem / W, lx = rem - ly * W;
-        if (lx >= H && lx < H + T && ly >= H && ly < H + T) continue;  // the owner writes it
-        tile[jj][rem] = hcgp_p(ai, x0 + lx - H, y0 + ly - H, z0 + j0 + jj);
+      // the 8 planes' tiles straight into LDS (buffer_load ... lds, one dword per lane, agent-
+      // coherent): every load of the phase in flight at once, none through registers.  Tile
+      // dword d of a plane is double d / 2 (row-major W x W, own points included: the global
+      // copy equals the registers here); a wave's 64 lanes fill 64 consecutive dwords.  Points
+      // outside an isolated cell get an offset past the buffer's end, which reads 0.
+      int zi = z0 + j0;
+      asm volatile("" : "+s"(zi));  // offsets formed here, not hoisted out of the CG loop
+#pragma unroll 1
+      for (int jj = 0; jj < PG; jj++) {
+        const long zoff = plane * (zi + jj);
+#pragma unroll 1
+        for (int d0 = 0; d0 < 2 * W * W; d0 += 256) {
+          const int d = d0 + t;
+          if (d < 2 * W * W) {
+            const int e2 = d >> 1, ly = e2 / W, lx = e2 - ly * W;
+            int X = x0 + lx - H, Y = y0 + ly - H;
+            bool zero = false;
+            if (X < 0 || X >= a.nx) { if (a.px) X = hcgp_wrap(X, a.nx); else zero = true; }
+            if (Y < 0 || Y >= a.ny) { if (a.py) Y = hcgp_wrap(Y, a.ny); else zero = true; }
+            const unsigned off = zero ? 0xFFFFFFF0u
+                                      : (unsigned)((X + (long)a.nx * Y + zoff) * 8 + (d & 1) * 4);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                prs, (__attribute__((address_space(3))) void *)((char *)&tile[jj][0] + 4 * (d0 + (t & ~63))),
+                4, off, 0, 0, HCGP_AUX_SC1);
+          }
+        }
       }
-#pragma unroll
-      for (int jj = 0; jj < PG; jj++) tile[jj][(ty + H) * W + tx + H] = e[j0 + jj];
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's tile loads have landed
       __syncthreads();
 #pragma unroll
       for (int jj = 0; jj < PG; jj++) {
@@ -1912,8 +1935,8 @@ int hartree(const HartreeArgs &a, const HartreeWork *ws) {
   st.in = p; st.out = ap; st.dot_partial = part; st.skip = state;
   const double dvo = g.dv() / omega;
   lib().params["hartree_persistent_used"] = 0.0;
-  // opt-in: at 128^3 it takes 49 us per iteration against 55 for the iteration blocks, at
-  // 96^3 and 64^3 48 / 42 against 44 / 35 (its iterations are bound by memory round trips)
+  // opt-in: at 128^3 it takes 47 us per iteration against 55 for the iteration blocks, at
+  // 96^3 and 64^3 45 / 38 against 43 / 34 (its iterations are bound by memory round trips)
   if (!ws && param("hartree_persistent", 0.0) >= 1.0) {
     const int loops = hartree_persistent(a, g, n, b, p, avhs, sc, dvo, state);
     if (loops > 0) {
